@@ -1,0 +1,232 @@
+#!/usr/bin/env python
+"""Benchmark: warm assembly throughput (tets/s) of the 3-D mixed Darcy system on one MI355X.
+
+Workload (the learned-preconditioner configuration of the benchmark suite):
+unit cube, 32^3 hexes x 6 Kuhn tets = 196,608 tets, RT1 flux / P0 pressure,
+log-normal permeability (Gaussian random field, squared-exponential, correlation
+length 0.05, sigma 2.0, sampled at centroids), unit source, homogeneous Dirichlet
+pressure on the 6 boundary faces.
+
+A step = one warm assembly of the whole system from inputs resident in HBM:
+element kernel (Jacobian + residual of every tet into its CSR slots), staged
+segment reduction, Dirichlet facet term. The JSON line also reports the cold
+setup, the time to solve the assembled system to an FP64 relative residual of
+1e-8 (FP32 MINRES + block preconditioner inside FP64 refinement), the roofline
+of the dominant kernel and the FP64 CPU restatement of the reference path timed
+on one host core.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+METRIC = "assembled tets/s"
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(seconds: float, n: int = 16):
+    """Per-element FP64 restatement of the reference assembly on one core."""
+    try:
+        from threadpoolctl import threadpool_limits
+    except Exception:  # pragma: no cover
+        threadpool_limits = None
+    import oracle.mesh as om
+    import oracle.spaces as osp
+    from oracle.assemble import TripletCollector
+    from oracle.forms import darcy_element
+    from nfempy_amd.fields import grf_host
+
+    pts, bl = om.structured_cube(n)
+    m = om.ConformalMesh(3, pts, bl)
+    ps = osp.ProductSpace({"u": (3, 1, "RT", 1, m), "p": (3, 1, "Lagrange", 0, m)}, {"p": True})
+    ps.build_structures({"u": [1], "p": [1]}, {"u": [2, 3, 4, 5, 6, 7]})
+    cells = np.array([m.cells[c].node_tags for c in ps.spaces["u"].element_ids])
+    cent = m.points[cells].mean(axis=1)
+    kappa = np.exp(grf_host(cent, 7, 0.05, 2.0))
+    fun = {"rhs": np.ones(len(cells)), "kappa": kappa}
+    coll = TripletCollector(ps.n_dof)
+    rg = np.zeros(ps.n_dof)
+    ctx = threadpool_limits(1) if threadpool_limits else None
+    done = 0
+    t0 = time.perf_counter()
+    try:
+        for i in range(len(cells)):
+            dest = ps.destination_indexes(i)
+            r, J = darcy_element(ps, i, fun)
+            rg[dest] += r
+            coll.add_element(dest, J)
+            done += 1
+            if time.perf_counter() - t0 > seconds:
+                break
+        coll.csr()
+    finally:
+        if ctx is not None:
+            ctx.unregister() if hasattr(ctx, "unregister") else None
+    el = time.perf_counter() - t0
+    return {
+        "value": done / el,
+        "unit": "tets/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"first {done} tets of a {n}^3x6 Kuhn cube, RT1-P0, same log-normal field model; "
+                  f"per-element FP64 restatement of LaplaceDualWeakForm + COO scatter, {el:.1f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=32, help="cube divisions per axis")
+    ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also launch a 256 MiB device copy (for counter calibration runs)")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+
+    import torch
+
+    from nfempy_amd import _native as nat
+    from nfempy_amd.device import CoefficientArg, DarcySystem
+    from nfempy_amd.fields import grf_device
+    from nfempy_amd.mesh import Mesh, unit_cube
+    from nfempy_amd.solvers import MinresIR, darcy_preconditioner
+    from nfempy_amd.spaces import ProductSpace
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    n = args.n
+    log(f"building mesh {n}^3x6 and spaces")
+    t0 = time.perf_counter()
+    pts, bl = unit_cube(n)
+    mesh = Mesh.from_blocks(3, pts, bl)
+    ps = ProductSpace({"u": (3, 1, "RT", 1, mesh), "p": (3, 1, "Lagrange", 0, mesh)})
+    ps.make_subspaces_discontinuous({"p": True})
+    ps.build_structures({"u": [1], "p": [1]}, {"u": [2, 3, 4, 5, 6, 7]})
+    t1 = time.perf_counter()
+    sysd = DarcySystem(ps)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    E, N, nnz = sysd.E, sysd.n, sysd.nnz
+    log(f"E={E} N={N} nnz={nnz} setup numbering {t1 - t0:.2f}s pattern+upload {t2 - t1:.2f}s")
+
+    cent = sysd.coords64[sysd.cells.long()].mean(dim=1)
+    logk = grf_device(cent, seed=7, ell=0.05, sigma=2.0)
+    kinv = CoefficientArg(torch.exp(-logk).contiguous(), 1, 0)
+    fsrc = CoefficientArg(torch.ones(1, dtype=torch.float64, device=dev), 0, 0)
+    pd = CoefficientArg(torch.zeros(sysd.F * sysd.nqf, dtype=torch.float64, device=dev), sysd.nqf, 1)
+    for c in (kinv, fsrc, pd):
+        c.get(False)
+        c.get(True)
+
+    def step(fp64=False):
+        return sysd.assemble(kinv, fsrc, pd, fp64=fp64)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    te = time.perf_counter()
+    ms = (te - ts) * 1e3 / args.steps
+    value = E / (ms * 1e-3)
+    log(f"warm assembly {ms:.4f} ms/step -> {value:.4e} tets/s")
+
+    # dominant kernel timed alone with events on the launch stream
+    s = nat.stream_handle()
+    reps = 20
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(reps):
+        nat.call("nfx_assemble_darcy", 3, 1, E, nat.ptr(sysd.cells), nat.ptr(sysd.coords), nat.ptr(sysd.tables32),
+                 nat.ptr(kinv.get(False)), kinv.se, kinv.sq, nat.ptr(fsrc.get(False)), fsrc.se, fsrc.sq, nat.ptr(sysd.slot),
+                 nat.ptr(sysd.rhs_row), 0, nat.ptr(sysd.vals32), nat.ptr(sysd.stage32), nat.ptr(sysd.rhs32), s)
+    ev1.record()
+    torch.cuda.synchronize()
+    k_ms = ev0.elapsed_time(ev1) / reps
+    alg_bytes_per_tet = 16 + 48 + 4 + 16 + 1 + 4.0 * nnz / E + 4.0 * N / E
+    achieved = alg_bytes_per_tet * E / (k_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": 8000.0, "unit": "GB/s",
+                "frac": round(achieved / 8000.0, 4), "traffic": None,
+                "kernel": "k_darcy<3,4,1,8,float>", "kernel_ms": round(k_ms, 5),
+                "alg_bytes_per_tet": round(alg_bytes_per_tet, 2)}
+    log(f"k_darcy {k_ms * 1e3:.2f} us/launch, {achieved:.1f} GB/s algorithmic")
+
+    if args.calibrate_copy:
+        a = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device=dev).fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("calibration copy: 3 x 256 MiB")
+
+    solve = None
+    if not args.no_solve:
+        log("solving (FP64 assembly, block preconditioner, MINRES + FP64 refinement)")
+        step(fp64=True)
+        step(fp64=False)
+        torch.cuda.synchronize()
+        tsolve0 = time.perf_counter()
+        sign, pinv = darcy_preconditioner(sysd.rowptr, sysd.col, sysd.vals64, N, sysd.n_u)
+        solver = MinresIR(sysd.rowptr, sysd.col, sysd.vals32, sysd.vals64, sign, pinv)
+        x, info = solver.solve(-sysd.rhs64, rtol=1e-8, inner_rtol=1e-5, max_inner=50000, max_outer=40)
+        torch.cuda.synchronize()
+        tsolve = time.perf_counter() - tsolve0
+        solve = {"time_to_solve_plain_s": round(tsolve, 4), "plain_iterations": info["iterations"],
+                 "plain_refinements": info["outer"], "plain_residual": info["residual"],
+                 "plain_converged": info["converged"], "time_to_solve_learned_s": None}
+        log(f"solve: {tsolve:.3f}s, {info['iterations']} MINRES iterations, {info['outer']} refinements, "
+            f"residual {info['residual']:.3e}")
+        solver.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log(f"cpu baseline ({args.cpu_seconds:.0f} s sample, 1 core)")
+        cpu = cpu_baseline(args.cpu_seconds)
+        log(f"cpu baseline {cpu['value']:.1f} tets/s")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "tets/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"darcy3d-rt1p0-kuhn{n}x6-lognormal(l=0.05,sigma=2)-warm-assembly",
+            "elements": E, "dofs": N, "nnz": nnz, "cold_setup_s": round(t2 - t0, 3),
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if solve:
+        out["solve"] = solve
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,184 @@
+/*
+ * nfx.h - C ABI of libnfx, the MI355X (gfx950) assembly-and-solve library behind
+ * the nfempy_amd drop-in for nfempy's weak-form / product-space / assembler path.
+ *
+ * Conventions
+ *  - Every entry point returns an int status: 0 on success, a negative NFX_E_*
+ *    code otherwise; nfx_last_error() returns a thread-local message. Nothing
+ *    throws across this boundary.
+ *  - Pointer arguments named d_* are device pointers (HBM); h_* are host pointers.
+ *    Sizes are plain integers. Work is enqueued on the stream passed in (a
+ *    hipStream_t, NULL = default stream) and is asynchronous unless the function
+ *    says otherwise.
+ *  - Indices are int32; values are FP32 (float) or FP64 (double) as selected by
+ *    an fp64 flag.
+ *  - Element kernels compute element matrices in the REFERENCE basis order
+ *    (field by field; inside a field basis function a, component c -> a*n_comp+c).
+ *    The reference's facet orientation handling (src/basis/permute_and_transform.py:8-149:
+ *    a signed permutation of facet DoFs per element) is folded by the host into the
+ *    destination codes, so kernels never branch on orientation.
+ *
+ * Slot / row codes (int32), one per element-local matrix entry or residual entry:
+ *    code < 0        dropped
+ *    bits 0..27      CSR value index (or stage index, or row)
+ *    bit 28          contribute 0.0 (explicit structural zero)
+ *    bit 29          index is into the stage array (multi-contributor slot)
+ *    bit 30          contribute the negated value (orientation sign)
+ *
+ * Each entry point cites the reference interface it replaces (paths relative to
+ * the OmarDuran/nfempy repository).
+ */
+#ifndef NFX_H
+#define NFX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NFX_OK 0
+#define NFX_E_INVALID -1       /* bad argument: size, pointer, shape */
+#define NFX_E_HIP -2           /* a HIP runtime call failed */
+#define NFX_E_UNSUPPORTED -3   /* element type / size not compiled in */
+#define NFX_E_NOT_CONVERGED -4 /* solver stopped above the requested tolerance */
+
+/* ---- library ------------------------------------------------------------ */
+
+/* Version as 10000*major + 100*minor + patch. */
+int nfx_version(void);
+/* Thread-local description of the last error on this thread. */
+const char* nfx_last_error(void);
+/* Number of visible devices. */
+int nfx_device_count(int* h_count);
+
+/* ---- assembly: mixed Darcy (RT_k - P_{k-1}) ----------------------------------
+ * Replaces LaplaceDualWeakForm.evaluate_form
+ * (src/weak_forms/laplace_dual_weak_form.py:11-87) evaluated for every element,
+ * together with the Darcy scatter of examples/md_elliptic_scalar/md_elliptic_scalar.py:250-265
+ * (src/assembly/ScatterFormData.py:5-25), i.e. the element loop of
+ * SequentialAssembler.scatter_forms (src/assembly/SequentialAssembler.py:69-86).
+ *
+ *  d_cells  (E, dim+1) int32 vertex ids
+ *  d_coords (V, 3) vertex coordinates (float or double as fp64)
+ *  d_tables reference tables (float or double, matching fp64):
+ *           w[nq] | phi[nq][nb][dim] | div[nq][nb] | psi[nq][np]
+ *  d_kinv, d_f  inverse permeability and source at quadrature points (float or
+ *           double as fp64), element e point q at index e*stride_e + q*stride_q
+ *           (stride_q = 0: one value per element; both 0: one value)
+ *  d_slot   (nl*nl, E) slot codes, entry-major, nl = nb + np
+ *  d_rhs_row (np, E) row codes of the pressure residual
+ *  outputs  d_vals (CSR values), d_stage (staged contributions), d_rhs (rg),
+ *           float or double as fp64.
+ * Supported: (dim, k) in {2, 3} x {1, 2}: RT1-P0 and RT2-P1.
+ */
+int nfx_assemble_darcy(int dim, int k, int n_elem, const int32_t* d_cells,
+                       const void* d_coords, const void* d_tables,
+                       const void* d_kinv, int kinv_stride_e, int kinv_stride_q,
+                       const void* d_f, int f_stride_e, int f_stride_q,
+                       const int32_t* d_slot, const int32_t* d_rhs_row, int fp64,
+                       void* d_vals, void* d_stage, void* d_rhs, void* stream);
+
+/* Dirichlet pressure facet term, LaplaceDualWeakFormBCDirichlet.evaluate_form
+ * (src/weak_forms/laplace_dual_weak_form.py:91-154) for every boundary facet:
+ * rg[row] = sum_q w |detJ_f| p_D(x_q) psi(x_q) . n with the flux basis traced from
+ * the neighbour cell and n the unit outward normal (src/geometry/compute_normal.py:7-24).
+ *  d_facet (F, dim) facet vertex ids in stored order; d_neigh (F) neighbour element;
+ *  d_combo (F) index of (local facet, facet vertex order) into d_trace;
+ *  d_trace (ncombo, nqf, fd, dim) reference flux basis of the facet DoFs at the facet
+ *  points mapped into the neighbour (src/basis/parametric_transformation.py:30-98);
+ *  d_fw (nqf) facet weights; d_pD (F, nqf) Dirichlet data (float or double as
+ *  fp64); d_rows (F, fd) row codes.
+ * Each boundary facet owns its rows (element kernels never write them).
+ */
+int nfx_assemble_darcy_bc(int dim, int k, int n_facets, const int32_t* d_facet,
+                          const int32_t* d_neigh, const int32_t* d_combo,
+                          const int32_t* d_cells, const void* d_coords,
+                          const void* d_trace, const void* d_fw, int nqf,
+                          const void* d_pD, const int32_t* d_rows, int fp64,
+                          void* d_rhs, void* stream);
+
+/* ---- scatter reduction ---------------------------------------------------
+ * Deterministic segmented sum of staged contributions into CSR slots - the
+ * multi-contributor part of the reference's serial setValue(addv=True) scatter
+ * (src/assembly/ScatterFormData.py:5-11): for segment m,
+ *   vals[d_target[m]] = sum of stage[d_segptr[m] .. d_segptr[m+1]) in element order. */
+int nfx_stage_reduce(int n_seg, const int32_t* d_segptr, const int32_t* d_target,
+                     int fp64, const void* d_stage, void* d_vals, void* stream);
+
+/* Fill a float / double array with a value (a kernel, safe under graph capture). */
+int nfx_fill(int fp64, int64_t n, double value, void* d_x, void* stream);
+
+/* ---- geometry -------------------------------------------------------------
+ * Physical quadrature points x_q = sum_a bary[q][a] X_a of every element in FP64:
+ * the points at which the reference evaluates its coefficient callables
+ * (src/geometry/mapping.py:39-41). d_out (E, nq, 3). */
+int nfx_quadrature_points(int dim, int n_elem, int nq, const int32_t* d_cells,
+                          const double* d_coords64, const double* d_bary,
+                          double* d_out, void* stream);
+
+/* ---- sparse kernels --------------------------------------------------------
+ * y = alpha A x + beta y, CSR FP32 (PETSc MatMult inside the reference's KSP,
+ * examples/md_elliptic_scalar/md_elliptic_scalar.py:351-364). */
+int nfx_spmv(int n_rows, const int32_t* d_rowptr, const int32_t* d_col,
+             const float* d_val, float alpha, const float* d_x, float beta,
+             float* d_y, void* stream);
+
+/* FP64 residual r = b - A x with FP64 values (the refinement residual). */
+int nfx_residual64(int n_rows, const int32_t* d_rowptr, const int32_t* d_col,
+                   const double* d_val, const double* d_x, const double* d_b,
+                   double* d_r, void* stream);
+
+/* FP64 row diagonal of a CSR matrix, and the FP64 Schur-complement diagonal
+ * S[k - n_u] = sum_{j < n_u} A[k, j] A[j, k] / A[j, j] for rows k >= n_u (the
+ * pressure part of the block preconditioner; pattern must be structurally
+ * symmetric with sorted columns). */
+int nfx_csr_diag64(int n_rows, const int32_t* d_rowptr, const int32_t* d_col,
+                   const double* d_val, double* d_diag, void* stream);
+int nfx_schur_diag64(int n_rows, int n_u, const int32_t* d_rowptr,
+                     const int32_t* d_col, const double* d_val,
+                     const double* d_diag, double* d_schur, void* stream);
+
+/* ---- Krylov --------------------------------------------------------------
+ * Preconditioned MINRES (Paige-Saunders) in FP32 on S = D A, D = diag(d_sign)
+ * (-1 on rows that turn the saddle point symmetric; NULL = identity), with the
+ * SPD diagonal preconditioner d_pinv (FP32 inverse diagonal), inside FP64
+ * iterative refinement on the as-assembled FP64 matrix. Replaces KSP minres +
+ * PCFIELDSPLIT (examples/cauchy_elasticity/mfe_elasticity_example_1.py:238-274)
+ * and KSP preonly + LU/MUMPS (examples/md_elliptic_scalar/md_elliptic_scalar.py:351-364)
+ * by an iterative solve to ||b - A x||_2 / ||b||_2 <= rtol evaluated in FP64.
+ * Synchronous. h_info[0] = refinement steps, h_info[1] = total MINRES iterations;
+ * h_resid[0] = final relative FP64 residual; h_hist (optional, hist_cap values) =
+ * relative residual before each refinement step. Returns NFX_E_NOT_CONVERGED if
+ * rtol is not reached within max_outer refinement steps.
+ * A solver object owns its Krylov workspace (11 FP32 vectors + 1 FP64 vector).
+ */
+typedef struct nfx_solver nfx_solver;
+int nfx_solver_create(nfx_solver** out, int n_rows, int nnz, void* stream);
+int nfx_solver_destroy(nfx_solver* s);
+int nfx_solve_minres_ir(nfx_solver* s, const int32_t* d_rowptr, const int32_t* d_col,
+                        const float* d_val32, const double* d_val64,
+                        const float* d_sign, const float* d_pinv,
+                        const double* d_b, double* d_x,
+                        double rtol, double inner_rtol, int max_inner, int max_outer,
+                        int* h_info, double* h_resid, double* h_hist, int hist_cap);
+
+/* ---- error norms (FP64) ---------------------------------------------------
+ * Squared error of one field summed over elements, as l2_error / div_error
+ * (src/postprocess/l2_error_post_processor.py:7-63,185-237). kind 0: vector
+ * (RT/BDM rows) L2, 1: scalar (Lagrange) L2, 2: vector div. d_rows (E, nb, n_comp)
+ * row codes (bit 30 = orientation sign); d_exact the exact field at the element
+ * quadrature points: (E, nq, n_comp, dim) for kind 0, (E, nq, n_comp) otherwise.
+ * d_tables64: w | phi | div (vector) or w | psi (scalar). Synchronous. */
+int nfx_field_error(int kind, int dim, int nb, int nq, int n_comp, int n_elem,
+                    const int32_t* d_cells, const double* d_coords64,
+                    const double* d_tables64, const int32_t* d_rows,
+                    const double* d_alpha, const double* d_exact, double* h_out,
+                    void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NFX_H */

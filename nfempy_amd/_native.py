@@ -1,0 +1,128 @@
+"""ctypes binding of libnfx (the C ABI declared in include/nfx.h).
+
+The library is built in-tree (``nfempy_amd/lib/libnfx.so``) by
+``nfempy_amd.build_lib``. There is no CPU fallback: every compute call goes to the
+HIP library, and importing a compute path without the library raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libnfx.so"
+
+NFX_OK = 0
+NFX_E_INVALID = -1
+NFX_E_HIP = -2
+NFX_E_UNSUPPORTED = -3
+NFX_E_NOT_CONVERGED = -4
+
+# Slot / row code flags (include/nfx.h)
+SLOT_INDEX_MASK = (1 << 28) - 1
+SLOT_ZERO = 1 << 28
+SLOT_STAGE = 1 << 29
+SLOT_NEG = 1 << 30
+
+_lib = None
+
+c_int = ctypes.c_int
+c_i64 = ctypes.c_int64
+c_double = ctypes.c_double
+c_float = ctypes.c_float
+c_void_p = ctypes.c_void_p
+c_char_p = ctypes.c_char_p
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "nfx_version": (c_int, []),
+    "nfx_last_error": (c_char_p, []),
+    "nfx_device_count": (c_int, [c_void_p]),
+    "nfx_assemble_darcy": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int,
+                                   c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
+                                   c_void_p, c_void_p]),
+    "nfx_assemble_darcy_bc": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                      c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p,
+                                      c_void_p]),
+    "nfx_stage_reduce": (c_int, [c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "nfx_fill": (c_int, [c_int, c_i64, c_double, c_void_p, c_void_p]),
+    "nfx_quadrature_points": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "nfx_spmv": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_float, c_void_p, c_float, c_void_p, c_void_p]),
+    "nfx_residual64": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "nfx_csr_diag64": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "nfx_schur_diag64": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "nfx_solver_create": (c_int, [c_void_p, c_int, c_int, c_void_p]),
+    "nfx_solver_destroy": (c_int, [c_void_p]),
+    "nfx_solve_minres_ir": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                    c_void_p, c_void_p, c_double, c_double, c_int, c_int, c_void_p, c_void_p,
+                                    c_void_p, c_int]),
+    "nfx_field_error": (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
+                                c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+}
+
+
+class NfxError(RuntimeError):
+    pass
+
+
+def library_available() -> bool:
+    return LIB_PATH.exists()
+
+
+def lib():
+    """Load libnfx once; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise NfxError(
+            f"libnfx not built: {LIB_PATH} is missing (run `python -m nfempy_amd.build_lib`)"
+        )
+    handle = ctypes.CDLL(str(LIB_PATH), mode=os.RTLD_NOW | getattr(os, "RTLD_GLOBAL", 0))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(handle, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = handle
+    return _lib
+
+
+def exported_symbols():
+    return list(SIGNATURES)
+
+
+def check(rc: int, what: str = "") -> int:
+    if rc == NFX_OK:
+        return rc
+    msg = lib().nfx_last_error().decode(errors="replace")
+    if rc == NFX_E_INVALID:
+        raise ValueError(f"{what}: {msg}")
+    if rc == NFX_E_NOT_CONVERGED:
+        raise NfxError(f"{what}: not converged: {msg}")
+    raise NfxError(f"{what}: {msg} (status {rc})")
+
+
+def call(name: str, *args) -> int:
+    rc = getattr(lib(), name)(*args)
+    if name in ("nfx_version",):
+        return rc
+    return check(rc, name)
+
+
+def ptr(t) -> int | None:
+    """Device (or host) address of a torch tensor / numpy array, or None."""
+    if t is None:
+        return None
+    if hasattr(t, "data_ptr"):
+        return t.data_ptr()
+    if hasattr(t, "ctypes"):
+        return t.ctypes.data
+    raise TypeError(type(t))
+
+
+def stream_handle(stream=None) -> int | None:
+    import torch
+
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream

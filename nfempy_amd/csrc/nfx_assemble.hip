@@ -1,0 +1,430 @@
+// Element assembly kernels for the mixed Darcy form (RT_k - P_{k-1}, k = 1, 2,
+// triangles and tetrahedra), its Dirichlet facet term, the deterministic
+// staged-contribution reduction and small fill / geometry kernels.
+//
+// Data layout (HBM):
+//   cells   int32 (E, DIM+1)        vertex ids, element-major
+//   coords  float (V, 3)            vertex coordinates
+//   slot    int32 (NL*NL, E)        entry-major slot codes (see nfx_common.h),
+//                                   reference-basis order a*NL + b; the facet
+//                                   orientation (signed permutation) of the
+//                                   reference's permute_and_transform is folded
+//                                   into these codes on the host
+//   rhsrow  int32 (NP, E)           pressure residual rows
+//   tables  R: w[NQ] | phi[NQ][NB][DIM] | div[NQ][NB] | psi[NQ][NP]
+// One thread assembles one element; the element matrix never leaves registers.
+#include "nfx_common.h"
+
+namespace nfx {
+
+template <int DIM, typename R>
+struct Geom {
+  R J[3][DIM];
+  R G[DIM][DIM];
+  R detJ;
+};
+
+template <int DIM, typename R>
+__device__ __forceinline__ void load_geometry(const int32_t* __restrict__ cells,
+                                              const R* __restrict__ coords,
+                                              int e, Geom<DIM, R>& g,
+                                              R (&X)[DIM + 1][3]) {
+#pragma unroll
+  for (int a = 0; a <= DIM; ++a) {
+    const int v = cells[e * (DIM + 1) + a];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) X[a][c] = coords[3 * v + c];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) g.J[c][d] = X[d + 1][c] - X[0][c];
+#pragma unroll
+  for (int i = 0; i < DIM; ++i)
+#pragma unroll
+    for (int j = 0; j < DIM; ++j) {
+      R s = 0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) s += g.J[c][i] * g.J[c][j];
+      g.G[i][j] = s;
+    }
+  if constexpr (DIM == 3) {
+    const R d = g.J[0][0] * (g.J[1][1] * g.J[2][2] - g.J[1][2] * g.J[2][1]) -
+                g.J[0][1] * (g.J[1][0] * g.J[2][2] - g.J[1][2] * g.J[2][0]) +
+                g.J[0][2] * (g.J[1][0] * g.J[2][1] - g.J[1][1] * g.J[2][0]);
+    g.detJ = d < R(0) ? -d : d;
+  } else {
+    // |J0 x J1|: the QR determinant of a 3x2 Jacobian (src/geometry/mapping.py:23-36)
+    const R cx = g.J[1][0] * g.J[2][1] - g.J[2][0] * g.J[1][1];
+    const R cy = g.J[2][0] * g.J[0][1] - g.J[0][0] * g.J[2][1];
+    const R cz = g.J[0][0] * g.J[1][1] - g.J[1][0] * g.J[0][1];
+    g.detJ = sqrt(cx * cx + cy * cy + cz * cz);
+  }
+}
+
+// Mixed Darcy element (LaplaceDualWeakForm.evaluate_form):
+//   M[a][b] = sum_q w kinv phi_a^T G phi_b / detJ
+//   B[k][b] = sum_q w div_b psi_k                 (detJ cancels)
+//   local (a, NB+k) = -B[k][a], (NB+k, b) = +B[k][b], (p, p) = 0
+//   rp[k]   = -sum_q w detJ f psi_k
+template <int DIM, int NB, int NP, int NQ, typename R>
+__global__ __launch_bounds__(256) void k_darcy(
+    int n_elem, const int32_t* __restrict__ cells, const R* __restrict__ coords,
+    const R* __restrict__ tab, const R* __restrict__ kinv, int ks_e, int ks_q,
+    const R* __restrict__ f, int fs_e, int fs_q, const int32_t* __restrict__ slot,
+    const int32_t* __restrict__ rhs_row, R* __restrict__ vals, R* __restrict__ stage,
+    R* __restrict__ rhs) {
+  constexpr int NL = NB + NP;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_elem) return;
+  const R* __restrict__ tw = tab;
+  const R* __restrict__ tphi = tab + NQ;
+  const R* __restrict__ tdiv = tphi + NQ * NB * DIM;
+  const R* __restrict__ tpsi = tdiv + NQ * NB;
+
+  Geom<DIM, R> g;
+  R X[DIM + 1][3];
+  load_geometry<DIM, R>(cells, coords, e, g, X);
+  const R invdet = R(1) / g.detJ;
+
+  R M[NB][NB];
+  R B[NP][NB];
+  R rp[NP];
+#pragma unroll
+  for (int a = 0; a < NB; ++a)
+#pragma unroll
+    for (int b = 0; b < NB; ++b) M[a][b] = 0;
+#pragma unroll
+  for (int k = 0; k < NP; ++k) {
+    rp[k] = 0;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) B[k][b] = 0;
+  }
+
+  for (int q = 0; q < NQ; ++q) {
+    const R w = tw[q];
+    const R ki = kinv[(int64_t)e * ks_e + q * ks_q];
+    const R fq = f[(int64_t)e * fs_e + q * fs_q];
+    const R cm = w * ki * invdet;
+    R gphi[NB][DIM];
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        R s = 0;
+#pragma unroll
+        for (int d2 = 0; d2 < DIM; ++d2) s += g.G[d][d2] * tphi[(q * NB + b) * DIM + d2];
+        gphi[b][d] = s * cm;
+      }
+#pragma unroll
+    for (int a = 0; a < NB; ++a)
+#pragma unroll
+      for (int b = a; b < NB; ++b) {
+        R s = 0;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) s += tphi[(q * NB + a) * DIM + d] * gphi[b][d];
+        M[a][b] += s;
+      }
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+      const R psi = tpsi[q * NP + k];
+      const R wp = w * psi;
+#pragma unroll
+      for (int b = 0; b < NB; ++b) B[k][b] += wp * tdiv[q * NB + b];
+      rp[k] -= wp * g.detJ * fq;
+    }
+  }
+
+  const int64_t E = n_elem;
+#pragma unroll
+  for (int a = 0; a < NB; ++a) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const R v = a <= b ? M[a][b] : M[b][a];
+      put_slot<R>(slot[(int64_t)(a * NL + b) * E + e], v, vals, stage);
+    }
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+      put_slot<R>(slot[(int64_t)(a * NL + NB + k) * E + e], -B[k][a], vals, stage);
+      put_slot<R>(slot[(int64_t)((NB + k) * NL + a) * E + e], B[k][a], vals, stage);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NP; ++k) {
+#pragma unroll
+    for (int l = 0; l < NP; ++l)
+      put_slot<R>(slot[(int64_t)((NB + k) * NL + NB + l) * E + e], R(0), vals, stage);
+    put_row<R>(rhs_row[(int64_t)k * E + e], rp[k], rhs);
+  }
+}
+
+// Dirichlet pressure facet term (LaplaceDualWeakFormBCDirichlet.evaluate_form).
+// trace: (ncombo, NQF, FD, DIM) reference flux basis at the facet points mapped
+// into the neighbour's reference cell, per (local facet, facet vertex order).
+template <int DIM, int FD, int NQF, typename R>
+__global__ __launch_bounds__(256) void k_darcy_bc(
+    int n_facets, const int32_t* __restrict__ facet_nodes, const int32_t* __restrict__ neigh,
+    const int32_t* __restrict__ combo, const int32_t* __restrict__ cells,
+    const R* __restrict__ coords, const R* __restrict__ trace, const R* __restrict__ fw,
+    const R* __restrict__ pD, const int32_t* __restrict__ rows, R* __restrict__ rhs) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n_facets) return;
+  const int e = neigh[f];
+  Geom<DIM, R> g;
+  R X[DIM + 1][3];
+  load_geometry<DIM, R>(cells, coords, e, g, X);
+  // facet geometry in stored node order
+  R Y[DIM][3];
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) {
+    const int v = facet_nodes[f * DIM + a];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) Y[a][c] = coords[3 * v + c];
+  }
+  R xc0[3], xc1[3], out[3], n[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    R s0 = 0, s1 = 0;
+#pragma unroll
+    for (int a = 0; a <= DIM; ++a) s0 += X[a][c];
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) s1 += Y[a][c];
+    xc0[c] = s0 / R(DIM + 1);
+    xc1[c] = s1 / R(DIM);
+    out[c] = xc1[c] - xc0[c];
+  }
+  R detf;
+  if constexpr (DIM == 2) {
+    R v[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v[c] = Y[1][c] - Y[0][c];
+    detf = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    // n = v x (v x out) / |...| rearranged: in-plane normal of the edge
+    R wv[3] = {v[1] * out[2] - v[2] * out[1], v[2] * out[0] - v[0] * out[2], v[0] * out[1] - v[1] * out[0]};
+    n[0] = v[1] * wv[2] - v[2] * wv[1];
+    n[1] = v[2] * wv[0] - v[0] * wv[2];
+    n[2] = v[0] * wv[1] - v[1] * wv[0];
+  } else {
+    R a1[3], a2[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      a1[c] = Y[1][c] - Y[0][c];
+      a2[c] = Y[2][c] - Y[0][c];
+    }
+    const R cx = a1[1] * a2[2] - a1[2] * a2[1];
+    const R cy = a1[2] * a2[0] - a1[0] * a2[2];
+    const R cz = a1[0] * a2[1] - a1[1] * a2[0];
+    detf = sqrt(cx * cx + cy * cy + cz * cz);
+    n[0] = cx;
+    n[1] = cy;
+    n[2] = cz;
+  }
+  {
+    const R nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    const R sgn = (n[0] * out[0] + n[1] * out[1] + n[2] * out[2]) < R(0) ? R(-1) : R(1);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) n[c] = sgn * n[c] / nn;
+  }
+  // J^T n / detJ (pushing the flux basis forward and dotting with n)
+  R jtn[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    R s = 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) s += g.J[c][d] * n[c];
+    jtn[d] = s / g.detJ;
+  }
+  const R* __restrict__ tr = trace + (int64_t)combo[f] * NQF * FD * DIM;
+  R val[FD];
+#pragma unroll
+  for (int a = 0; a < FD; ++a) val[a] = 0;
+#pragma unroll
+  for (int q = 0; q < NQF; ++q) {
+    const R c = fw[q] * detf * pD[(int64_t)f * NQF + q];
+#pragma unroll
+    for (int a = 0; a < FD; ++a) {
+      R s = 0;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) s += tr[(q * FD + a) * DIM + d] * jtn[d];
+      val[a] += c * s;
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < FD; ++a) put_row<R>(rows[(int64_t)f * FD + a], val[a], rhs);
+}
+
+// vals[target[m]] = sum of stage[segptr[m] .. segptr[m+1]) in order.
+template <typename R>
+__global__ __launch_bounds__(256) void k_stage_reduce(int n_seg, const int32_t* __restrict__ segptr,
+                                                      const int32_t* __restrict__ target,
+                                                      const R* __restrict__ stage,
+                                                      R* __restrict__ vals) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n_seg) return;
+  const int b = segptr[m], en = segptr[m + 1];
+  R s = stage[b];
+  for (int r = b + 1; r < en; ++r) s += stage[r];
+  vals[target[m]] = s;
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_fill(int64_t n, R v, R* __restrict__ x) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t j = i; j < n; j += stride) x[j] = v;
+}
+
+__global__ __launch_bounds__(256) void k_qpoints(int dim, int n_elem, int nq,
+                                                 const int32_t* __restrict__ cells,
+                                                 const double* __restrict__ coords,
+                                                 const double* __restrict__ bary,
+                                                 double* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n_elem * nq) return;
+  const int e = (int)(t / nq), q = (int)(t % nq);
+  double x[3] = {0, 0, 0};
+  for (int a = 0; a <= dim; ++a) {
+    const int v = cells[(int64_t)e * (dim + 1) + a];
+    const double s = bary[q * (dim + 1) + a];
+    for (int c = 0; c < 3; ++c) x[c] += s * coords[3 * v + c];
+  }
+  for (int c = 0; c < 3; ++c) out[t * 3 + c] = x[c];
+}
+
+template <int DIM, int NB, int NP, int NQ, typename R>
+static int launch_darcy(int n, const int32_t* cells, const void* coords, const void* tab,
+                        const void* kinv, int kse, int ksq, const void* f, int fse, int fsq,
+                        const int32_t* slot, const int32_t* rhs_row, void* vals, void* stage,
+                        void* rhs, hipStream_t s) {
+  hipLaunchKernelGGL((k_darcy<DIM, NB, NP, NQ, R>), dim3(grid_for(n, 256)), dim3(256), 0, s, n,
+                     cells, static_cast<const R*>(coords), static_cast<const R*>(tab),
+                     static_cast<const R*>(kinv), kse, ksq,
+                     static_cast<const R*>(f), fse, fsq, slot,
+                     rhs_row, static_cast<R*>(vals), static_cast<R*>(stage), static_cast<R*>(rhs));
+  NFX_LAUNCH_CHECK();
+  return NFX_OK;
+}
+
+template <int DIM, int FD, int NQF, typename R>
+static int launch_darcy_bc(int n, const int32_t* facet, const int32_t* neigh, const int32_t* combo,
+                           const int32_t* cells, const void* coords, const void* trace,
+                           const void* fw, const void* pD, const int32_t* rows, void* rhs,
+                           hipStream_t s) {
+  hipLaunchKernelGGL((k_darcy_bc<DIM, FD, NQF, R>), dim3(grid_for(n, 256)), dim3(256), 0, s, n,
+                     facet, neigh, combo, cells, static_cast<const R*>(coords),
+                     static_cast<const R*>(trace),
+                     static_cast<const R*>(fw), static_cast<const R*>(pD), rows, static_cast<R*>(rhs));
+  NFX_LAUNCH_CHECK();
+  return NFX_OK;
+}
+
+}  // namespace nfx
+
+using namespace nfx;
+
+extern "C" int nfx_assemble_darcy(int dim, int k, int n_elem, const int32_t* d_cells,
+                                  const void* d_coords, const void* d_tables, const void* d_kinv,
+                                  int kinv_stride_e, int kinv_stride_q, const void* d_f,
+                                  int f_stride_e, int f_stride_q, const int32_t* d_slot,
+                                  const int32_t* d_rhs_row, int fp64, void* d_vals,
+                                  void* d_stage, void* d_rhs, void* stream) {
+  NFX_REQUIRE(n_elem >= 0, "nfx_assemble_darcy: n_elem < 0");
+  if (n_elem == 0) return NFX_OK;
+  NFX_REQUIRE(d_cells && d_coords && d_tables && d_kinv && d_f && d_slot && d_rhs_row && d_vals &&
+                  d_rhs,
+              "nfx_assemble_darcy: null pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+#define NFX_DARCY(DIM, NB, NP, NQ)                                                                \
+  return fp64 ? launch_darcy<DIM, NB, NP, NQ, double>(n_elem, d_cells, d_coords, d_tables, d_kinv, \
+                                                      kinv_stride_e, kinv_stride_q, d_f,          \
+                                                      f_stride_e, f_stride_q, d_slot, d_rhs_row,  \
+                                                      d_vals, d_stage, d_rhs, s)                  \
+              : launch_darcy<DIM, NB, NP, NQ, float>(n_elem, d_cells, d_coords, d_tables, d_kinv,  \
+                                                     kinv_stride_e, kinv_stride_q, d_f,           \
+                                                     f_stride_e, f_stride_q, d_slot, d_rhs_row,   \
+                                                     d_vals, d_stage, d_rhs, s)
+  if (dim == 2 && k == 1) NFX_DARCY(2, 3, 1, 4);
+  if (dim == 2 && k == 2) NFX_DARCY(2, 8, 3, 9);
+  if (dim == 3 && k == 1) NFX_DARCY(3, 4, 1, 8);
+  if (dim == 3 && k == 2) NFX_DARCY(3, 15, 4, 27);
+#undef NFX_DARCY
+  set_error("nfx_assemble_darcy: unsupported (dim, k)");
+  return NFX_E_UNSUPPORTED;
+}
+
+extern "C" int nfx_assemble_darcy_bc(int dim, int k, int n_facets, const int32_t* d_facet,
+                                     const int32_t* d_neigh, const int32_t* d_combo,
+                                     const int32_t* d_cells, const void* d_coords,
+                                     const void* d_trace, const void* d_fw, int nqf,
+                                     const void* d_pD, const int32_t* d_rows, int fp64,
+                                     void* d_rhs, void* stream) {
+  NFX_REQUIRE(n_facets >= 0, "nfx_assemble_darcy_bc: n_facets < 0");
+  if (n_facets == 0) return NFX_OK;
+  NFX_REQUIRE(d_facet && d_neigh && d_combo && d_cells && d_coords && d_trace && d_fw && d_pD &&
+                  d_rows && d_rhs,
+              "nfx_assemble_darcy_bc: null pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+#define NFX_DBC(DIM, FD, NQF)                                                                     \
+  return fp64 ? launch_darcy_bc<DIM, FD, NQF, double>(n_facets, d_facet, d_neigh, d_combo,         \
+                                                      d_cells, d_coords, d_trace, d_fw, d_pD,      \
+                                                      d_rows, d_rhs, s)                            \
+              : launch_darcy_bc<DIM, FD, NQF, float>(n_facets, d_facet, d_neigh, d_combo, d_cells, \
+                                                     d_coords, d_trace, d_fw, d_pD, d_rows, d_rhs, \
+                                                     s)
+  if (dim == 2 && k == 1 && nqf == 2) NFX_DBC(2, 1, 2);
+  if (dim == 2 && k == 2 && nqf == 3) NFX_DBC(2, 2, 3);
+  if (dim == 3 && k == 1 && nqf == 4) NFX_DBC(3, 1, 4);
+  if (dim == 3 && k == 2 && nqf == 9) NFX_DBC(3, 3, 9);
+#undef NFX_DBC
+  set_error("nfx_assemble_darcy_bc: unsupported (dim, k, nqf)");
+  return NFX_E_UNSUPPORTED;
+}
+
+extern "C" int nfx_stage_reduce(int n_seg, const int32_t* d_segptr, const int32_t* d_target,
+                                int fp64, const void* d_stage, void* d_vals, void* stream) {
+  NFX_REQUIRE(n_seg >= 0, "nfx_stage_reduce: n_seg < 0");
+  if (n_seg == 0) return NFX_OK;
+  NFX_REQUIRE(d_segptr && d_target && d_stage && d_vals, "nfx_stage_reduce: null pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (fp64)
+    hipLaunchKernelGGL(k_stage_reduce<double>, dim3(grid_for(n_seg, 256)), dim3(256), 0, s, n_seg,
+                       d_segptr, d_target, static_cast<const double*>(d_stage),
+                       static_cast<double*>(d_vals));
+  else
+    hipLaunchKernelGGL(k_stage_reduce<float>, dim3(grid_for(n_seg, 256)), dim3(256), 0, s, n_seg,
+                       d_segptr, d_target, static_cast<const float*>(d_stage),
+                       static_cast<float*>(d_vals));
+  NFX_LAUNCH_CHECK();
+  return NFX_OK;
+}
+
+extern "C" int nfx_fill(int fp64, int64_t n, double value, void* d_x, void* stream) {
+  NFX_REQUIRE(n >= 0, "nfx_fill: n < 0");
+  if (n == 0) return NFX_OK;
+  NFX_REQUIRE(d_x, "nfx_fill: null pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int64_t blocks64 = (n + 255) / 256;
+  const int blocks = static_cast<int>(blocks64 < 65536 ? blocks64 : 65536);
+  if (fp64)
+    hipLaunchKernelGGL(k_fill<double>, dim3(blocks), dim3(256), 0, s, n, value,
+                       static_cast<double*>(d_x));
+  else
+    hipLaunchKernelGGL(k_fill<float>, dim3(blocks), dim3(256), 0, s, n, static_cast<float>(value),
+                       static_cast<float*>(d_x));
+  NFX_LAUNCH_CHECK();
+  return NFX_OK;
+}
+
+extern "C" int nfx_quadrature_points(int dim, int n_elem, int nq, const int32_t* d_cells,
+                                     const double* d_coords64, const double* d_bary,
+                                     double* d_out, void* stream) {
+  NFX_REQUIRE(dim == 1 || dim == 2 || dim == 3, "nfx_quadrature_points: dim");
+  NFX_REQUIRE(n_elem >= 0 && nq > 0, "nfx_quadrature_points: sizes");
+  if (n_elem == 0) return NFX_OK;
+  NFX_REQUIRE(d_cells && d_coords64 && d_bary && d_out, "nfx_quadrature_points: null pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_qpoints, dim3(grid_for((int64_t)n_elem * nq, 256)), dim3(256), 0, s, dim,
+                     n_elem, nq, d_cells, d_coords64, d_bary, d_out);
+  NFX_LAUNCH_CHECK();
+  return NFX_OK;
+}

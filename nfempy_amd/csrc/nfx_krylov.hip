@@ -1,0 +1,602 @@
+// Sparse kernels and the FP32 MINRES solver inside FP64 iterative refinement.
+//
+// Reductions are deterministic: every block writes its partial sum, an integer
+// ticket picks the last block to finish, and that block sums the partials in a
+// fixed order (no floating-point atomics). Scalars of the Krylov recurrence live
+// in device memory; the host only reads the residual estimate every few
+// iterations, so an iteration is two kernel launches:
+//   A(k): x/w update of iteration k-1, y = D A z / beta - (beta/oldb) r1, alpha = v.y
+//   B(k): y -= (alpha/beta) r2, z = P^-1 y, beta' = sqrt(y.z), Givens update
+#include <cmath>
+#include <vector>
+
+#include "nfx_common.h"
+
+namespace nfx {
+
+constexpr int kBlock = 256;
+constexpr int kMaxBlocks = 1024;
+
+struct MinresState {
+  double beta, oldb, alfa, beta1;
+  double dbar, epsln, oldeps, phibar, cs, sn;
+  double delta, denom, phi;
+  double s, ratio;  // 1/beta and beta/oldb for the next A kernel
+  double rnorm2;    // scratch for norms
+  double bnorm2;
+  int stop;
+  int pad;
+};
+
+__device__ __forceinline__ double warp_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// Deterministic block sum; result valid in thread 0.
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  v = warp_sum(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double r = 0;
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int i = 0; i < nw; ++i) r += sh[i];
+  }
+  __syncthreads();
+  return r;
+}
+
+// Last-block reduction of partial[0..gridDim.x) into *out (fixed order).
+// Returns true in the block that performed it (all threads).
+__device__ __forceinline__ bool last_block_reduce(double blocksum, double* partial,
+                                                  unsigned int* counter, double* out,
+                                                  double* sh) {
+  __shared__ bool is_last;
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = blocksum;
+    __threadfence();
+    const unsigned int t = atomicAdd(counter, 1u);
+    is_last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!is_last) return false;
+  __threadfence();
+  // partials were written by other blocks: device-scope loads bypass the CU cache
+  double v = 0;
+  for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x)
+    v += __hip_atomic_load(&partial[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  v = block_sum(v, sh);
+  if (threadIdx.x == 0) {
+    *out = v;
+    *counter = 0u;
+  }
+  __syncthreads();
+  return true;
+}
+
+template <int LANES>
+__device__ __forceinline__ float group_sum(float v) {
+#pragma unroll
+  for (int off = LANES / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, LANES);
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// plain SpMV: y = alpha A x + beta y (FP32, LANES lanes per row)
+template <int LANES>
+__global__ __launch_bounds__(kBlock) void k_spmv(int n, const int32_t* __restrict__ rp,
+                                                 const int32_t* __restrict__ ci,
+                                                 const float* __restrict__ va, float alpha,
+                                                 const float* __restrict__ x, float beta,
+                                                 float* __restrict__ y) {
+  const int lane = threadIdx.x % LANES;
+  const int rows_per_grid = gridDim.x * (kBlock / LANES);
+  for (int row = blockIdx.x * (kBlock / LANES) + threadIdx.x / LANES; row < n;
+       row += rows_per_grid) {
+    float s = 0.f;
+    for (int j = rp[row] + lane; j < rp[row + 1]; j += LANES) s += va[j] * x[ci[j]];
+    s = group_sum<LANES>(s);
+    if (lane == 0) y[row] = alpha * s + (beta == 0.f ? 0.f : beta * y[row]);
+  }
+}
+
+// FP64 residual r = b - A x and the two squared norms (last-block reduced).
+__global__ __launch_bounds__(kBlock) void k_residual64(
+    int n, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+    const double* __restrict__ va, const double* __restrict__ x, const double* __restrict__ b,
+    double* __restrict__ r, double* partial, unsigned int* counter, MinresState* st) {
+  __shared__ double sh[kBlock / 64];
+  double rr = 0, bb = 0;
+  for (int row = blockIdx.x * kBlock + threadIdx.x; row < n; row += gridDim.x * kBlock) {
+    double s = 0;
+    for (int j = rp[row]; j < rp[row + 1]; ++j) s += va[j] * x[ci[j]];
+    const double ri = b[row] - s;
+    r[row] = ri;
+    rr += ri * ri;
+    bb += b[row] * b[row];
+  }
+  const double brr = block_sum(rr, sh);
+  const double bbb = block_sum(bb, sh);
+  // two reductions through the same ticket: pack the second into partial[gridDim.x + i]
+  __shared__ bool is_last;
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = brr;
+    partial[gridDim.x + blockIdx.x] = bbb;
+    __threadfence();
+    const unsigned int t = atomicAdd(counter, 1u);
+    is_last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  double v1 = 0, v2 = 0;
+  for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) {
+    v1 += __hip_atomic_load(&partial[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    v2 += __hip_atomic_load(&partial[gridDim.x + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  v1 = block_sum(v1, sh);
+  v2 = block_sum(v2, sh);
+  if (threadIdx.x == 0) {
+    st->rnorm2 = v1;
+    st->bnorm2 = v2;
+    *counter = 0u;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_diag64(int n, const int32_t* __restrict__ rp,
+                                                   const int32_t* __restrict__ ci,
+                                                   const double* __restrict__ va,
+                                                   double* __restrict__ d) {
+  const int row = blockIdx.x * kBlock + threadIdx.x;
+  if (row >= n) return;
+  double v = 0;
+  for (int j = rp[row]; j < rp[row + 1]; ++j)
+    if (ci[j] == row) v = va[j];
+  d[row] = v;
+}
+
+// S_kk = sum_{j < nu} A[k, j] * A[j, k] / A[j, j]; A[j, k] found by a search in row j.
+__global__ __launch_bounds__(kBlock) void k_schur64(int n, int nu, const int32_t* __restrict__ rp,
+                                                    const int32_t* __restrict__ ci,
+                                                    const double* __restrict__ va,
+                                                    const double* __restrict__ d,
+                                                    double* __restrict__ out) {
+  const int k = nu + blockIdx.x * kBlock + threadIdx.x;
+  if (k >= n) return;
+  double s = 0;
+  for (int jj = rp[k]; jj < rp[k + 1]; ++jj) {
+    const int j = ci[jj];
+    if (j >= nu) continue;
+    double akj = va[jj], ajk = 0;
+    int lo = rp[j], hi = rp[j + 1];
+    while (lo < hi) {  // columns are sorted
+      const int mid = (lo + hi) >> 1;
+      if (ci[mid] < k) lo = mid + 1; else hi = mid;
+    }
+    if (lo < rp[j + 1] && ci[lo] == k) ajk = va[lo];
+    s += akj * ajk / d[j];
+  }
+  out[k - nu] = s;
+}
+
+// ---------------------------------------------------------------------------
+// MINRES kernels (FP32 vectors, FP64 scalars)
+
+// Initial step: r1 = r2 = rhs, z = P^-1 rhs, beta1 = sqrt(rhs . z), x = w* = 0.
+__global__ __launch_bounds__(kBlock) void k_minres_init(int n, const float* __restrict__ rhs,
+                                                        const float* __restrict__ pinv,
+                                                        float* r1, float* r2, float* z,
+                                                        float* x, float* w0, float* w1, float* w2,
+                                                        double* partial, unsigned int* counter,
+                                                        MinresState* st) {
+  __shared__ double sh[kBlock / 64];
+  double acc = 0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const float b = rhs[i];
+    const float zz = pinv[i] * b;
+    r1[i] = b;
+    r2[i] = b;
+    z[i] = zz;
+    x[i] = 0.f;
+    w0[i] = 0.f;
+    w1[i] = 0.f;
+    w2[i] = 0.f;
+    acc += (double)b * (double)zz;
+  }
+  const double bs = block_sum(acc, sh);
+  double tot;
+  if (last_block_reduce(bs, partial, counter, &tot, sh) && threadIdx.x == 0) {
+    const double beta1 = sqrt(tot > 0 ? tot : 0);
+    st->beta1 = beta1;
+    st->beta = beta1;
+    st->oldb = 0;
+    st->alfa = 0;
+    st->dbar = 0;
+    st->epsln = 0;
+    st->oldeps = 0;
+    st->phibar = beta1;
+    st->cs = -1;
+    st->sn = 0;
+    st->delta = 0;
+    st->denom = 0;
+    st->phi = 0;
+    st->s = beta1 > 0 ? 1.0 / beta1 : 0.0;
+    st->ratio = 0;
+    st->stop = beta1 > 0 ? 0 : 1;
+  }
+}
+
+// A(k): w/x update of iteration k-1 (when upd != 0), then y = s * D A z - ratio * r1,
+// alpha = sum v_i y_i with v = s z.
+template <int LANES>
+__global__ __launch_bounds__(kBlock) void k_minres_a(
+    int n, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+    const float* __restrict__ va, const float* __restrict__ sign, const float* __restrict__ z,
+    const float* __restrict__ zprev, const float* __restrict__ r1, float* __restrict__ y,
+    float* __restrict__ x, float* __restrict__ wnew, const float* __restrict__ wa,
+    const float* __restrict__ wb, int upd, int first, double* partial, unsigned int* counter,
+    MinresState* st) {
+  __shared__ double sh[kBlock / 64];
+  const float s = (float)st->s;
+  const float ratio = first ? 0.f : (float)st->ratio;
+  // scalars of the previous iteration's Givens step (computed in B(k-1))
+  const float sprev = (float)st->oldb > 0 ? (float)(1.0 / st->oldb) : 0.f;
+  const float oldeps = (float)st->oldeps, delta = (float)st->delta, denom = (float)st->denom,
+              phi = (float)st->phi;
+  const int lane = threadIdx.x % LANES;
+  double acc = 0;
+  const int rows_per_grid = gridDim.x * (kBlock / LANES);
+  for (int row = blockIdx.x * (kBlock / LANES) + threadIdx.x / LANES; row < n;
+       row += rows_per_grid) {
+    float sum = 0.f;
+    for (int j = rp[row] + lane; j < rp[row + 1]; j += LANES) sum += va[j] * z[ci[j]];
+    sum = group_sum<LANES>(sum);
+    if (lane == 0) {
+      if (upd) {
+        // v_{k-1} = zprev / beta_{k-1}; w = (v - oldeps * w1 - delta * w2) * denom
+        const float v = zprev[row] * sprev;
+        const float w = (v - oldeps * wa[row] - delta * wb[row]) * denom;
+        wnew[row] = w;
+        x[row] += phi * w;
+      }
+      const float sg = sign ? sign[row] : 1.f;
+      const float yi = s * sg * sum - ratio * r1[row];
+      y[row] = yi;
+      acc += (double)(s * z[row]) * (double)yi;
+    }
+  }
+  const double bs = block_sum(acc, sh);
+  double tot;
+  if (last_block_reduce(bs, partial, counter, &tot, sh) && threadIdx.x == 0) st->alfa = tot;
+}
+
+// B(k): y -= (alpha/beta) r2, z = P^-1 y, beta' = sqrt(y.z), Givens rotation.
+__global__ __launch_bounds__(kBlock) void k_minres_b(int n, float* __restrict__ y,
+                                                     const float* __restrict__ r2,
+                                                     const float* __restrict__ pinv,
+                                                     float* __restrict__ znew, double* partial,
+                                                     unsigned int* counter, MinresState* st) {
+  __shared__ double sh[kBlock / 64];
+  const float c = (float)(st->alfa / st->beta);
+  double acc = 0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const float yi = y[i] - c * r2[i];
+    y[i] = yi;
+    const float zi = pinv[i] * yi;
+    znew[i] = zi;
+    acc += (double)yi * (double)zi;
+  }
+  const double bs = block_sum(acc, sh);
+  double tot;
+  if (last_block_reduce(bs, partial, counter, &tot, sh) && threadIdx.x == 0) {
+    const double alfa = st->alfa;
+    const double oldb = st->beta;
+    const double beta = sqrt(tot > 0 ? tot : 0);
+    const double oldeps = st->epsln;
+    const double delta = st->cs * st->dbar + st->sn * alfa;
+    const double gbar = st->sn * st->dbar - st->cs * alfa;
+    const double epsln = st->sn * beta;
+    const double dbar = -st->cs * beta;
+    double gamma = hypot(gbar, beta);
+    if (gamma < 1e-300) gamma = 1e-300;
+    const double cs = gbar / gamma, sn = beta / gamma;
+    const double phi = cs * st->phibar;
+    st->phibar = sn * st->phibar;
+    st->oldb = oldb;
+    st->beta = beta;
+    st->oldeps = oldeps;
+    st->delta = delta;
+    st->epsln = epsln;
+    st->dbar = dbar;
+    st->cs = cs;
+    st->sn = sn;
+    st->phi = phi;
+    st->denom = 1.0 / gamma;
+    st->s = beta > 0 ? 1.0 / beta : 0.0;
+    st->ratio = oldb > 0 ? beta / oldb : 0.0;
+    if (!(beta > 0)) st->stop = 1;
+  }
+}
+
+// Final w/x update after the last B.
+__global__ __launch_bounds__(kBlock) void k_minres_final(int n, const float* __restrict__ zprev,
+                                                         float* __restrict__ x,
+                                                         float* __restrict__ wnew,
+                                                         const float* __restrict__ wa,
+                                                         const float* __restrict__ wb,
+                                                         const MinresState* st) {
+  const float sprev = st->oldb > 0 ? (float)(1.0 / st->oldb) : 0.f;
+  const float oldeps = (float)st->oldeps, delta = (float)st->delta, denom = (float)st->denom,
+              phi = (float)st->phi;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const float v = zprev[i] * sprev;
+    const float w = (v - oldeps * wa[i] - delta * wb[i]) * denom;
+    wnew[i] = w;
+    x[i] += phi * w;
+  }
+}
+
+// rf = sign * r / scale (FP64 -> FP32)
+__global__ __launch_bounds__(kBlock) void k_to_f32(int n, const double* __restrict__ r,
+                                                   const float* __restrict__ sign, double scale,
+                                                   float* __restrict__ rf) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double v = r[i] / scale;
+  rf[i] = (float)(sign ? sign[i] * v : v);
+}
+
+// x += scale * d (FP32 -> FP64)
+__global__ __launch_bounds__(kBlock) void k_axpy64(int n, double scale, const float* __restrict__ d,
+                                                   double* __restrict__ x) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  x[i] += scale * (double)d[i];
+}
+
+}  // namespace nfx
+
+using namespace nfx;
+
+struct nfx_solver {
+  int n = 0;
+  int nnz = 0;
+  hipStream_t stream = nullptr;
+  float* buf = nullptr;  // 11 vectors of n floats
+  double* r64 = nullptr;
+  double* partial = nullptr;
+  unsigned int* counter = nullptr;
+  MinresState* st = nullptr;
+  MinresState* h_st = nullptr;
+  int blocks = 1;
+  int lanes = 4;
+};
+
+extern "C" int nfx_spmv(int n_rows, const int32_t* d_rowptr, const int32_t* d_col,
+                        const float* d_val, float alpha, const float* d_x, float beta, float* d_y,
+                        void* stream) {
+  NFX_REQUIRE(n_rows >= 0, "nfx_spmv: n_rows < 0");
+  if (n_rows == 0) return NFX_OK;
+  NFX_REQUIRE(d_rowptr && d_col && d_val && d_x && d_y, "nfx_spmv: null pointer");
+  const int rows_per_block = kBlock / 4;
+  int blocks = grid_for(n_rows, rows_per_block);
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_spmv<4>, dim3(blocks), dim3(kBlock), 0, static_cast<hipStream_t>(stream),
+                     n_rows, d_rowptr, d_col, d_val, alpha, d_x, beta, d_y);
+  NFX_LAUNCH_CHECK();
+  return NFX_OK;
+}
+
+extern "C" int nfx_csr_diag64(int n_rows, const int32_t* d_rowptr, const int32_t* d_col,
+                              const double* d_val, double* d_diag, void* stream) {
+  NFX_REQUIRE(n_rows >= 0, "nfx_csr_diag64: n_rows < 0");
+  if (n_rows == 0) return NFX_OK;
+  NFX_REQUIRE(d_rowptr && d_col && d_val && d_diag, "nfx_csr_diag64: null pointer");
+  hipLaunchKernelGGL(k_diag64, dim3(grid_for(n_rows, kBlock)), dim3(kBlock), 0,
+                     static_cast<hipStream_t>(stream), n_rows, d_rowptr, d_col, d_val, d_diag);
+  NFX_LAUNCH_CHECK();
+  return NFX_OK;
+}
+
+extern "C" int nfx_schur_diag64(int n_rows, int n_u, const int32_t* d_rowptr, const int32_t* d_col,
+                                const double* d_val, const double* d_diag, double* d_schur,
+                                void* stream) {
+  NFX_REQUIRE(n_rows >= 0 && n_u >= 0 && n_u <= n_rows, "nfx_schur_diag64: sizes");
+  if (n_rows == n_u) return NFX_OK;
+  NFX_REQUIRE(d_rowptr && d_col && d_val && d_diag && d_schur, "nfx_schur_diag64: null pointer");
+  hipLaunchKernelGGL(k_schur64, dim3(grid_for(n_rows - n_u, kBlock)), dim3(kBlock), 0,
+                     static_cast<hipStream_t>(stream), n_rows, n_u, d_rowptr, d_col, d_val, d_diag,
+                     d_schur);
+  NFX_LAUNCH_CHECK();
+  return NFX_OK;
+}
+
+extern "C" int nfx_solver_create(nfx_solver** out, int n_rows, int nnz, void* stream) {
+  NFX_REQUIRE(out && n_rows > 0, "nfx_solver_create: bad arguments");
+  nfx_solver* s = new nfx_solver();
+  s->n = n_rows;
+  s->nnz = nnz;
+  s->stream = static_cast<hipStream_t>(stream);
+  int blocks = grid_for(n_rows, kBlock);
+  s->blocks = blocks < kMaxBlocks ? blocks : kMaxBlocks;
+  const double avg = n_rows > 0 ? (double)nnz / n_rows : 0;
+  s->lanes = avg > 24 ? 16 : (avg > 10 ? 8 : 4);
+  hipError_t e = hipMalloc(&s->buf, sizeof(float) * (size_t)n_rows * 11);
+  if (e == hipSuccess) e = hipMalloc(&s->r64, sizeof(double) * (size_t)n_rows);
+  if (e == hipSuccess) e = hipMalloc(&s->partial, sizeof(double) * 2 * kMaxBlocks);
+  if (e == hipSuccess) e = hipMalloc(&s->counter, sizeof(unsigned int));
+  if (e == hipSuccess) e = hipMalloc(&s->st, sizeof(MinresState));
+  if (e == hipSuccess) e = hipHostMalloc(&s->h_st, sizeof(MinresState), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMemsetAsync(s->counter, 0, sizeof(unsigned int), s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  if (e != hipSuccess) {
+    set_error(std::string("nfx_solver_create: ") + hipGetErrorString(e));
+    nfx_solver_destroy(s);
+    return NFX_E_HIP;
+  }
+  *out = s;
+  return NFX_OK;
+}
+
+extern "C" int nfx_solver_destroy(nfx_solver* s) {
+  if (!s) return NFX_OK;
+  if (s->buf) (void)hipFree(s->buf);
+  if (s->r64) (void)hipFree(s->r64);
+  if (s->partial) (void)hipFree(s->partial);
+  if (s->counter) (void)hipFree(s->counter);
+  if (s->st) (void)hipFree(s->st);
+  if (s->h_st) (void)hipHostFree(s->h_st);
+  delete s;
+  return NFX_OK;
+}
+
+static int launch_a(nfx_solver* s, const int32_t* rp, const int32_t* ci, const float* va,
+                    const float* sign, const float* z, const float* zprev, const float* r1,
+                    float* y, float* x, float* wnew, const float* wa, const float* wb, int upd,
+                    int first) {
+  const int n = s->n;
+  const int lanes = s->lanes;
+  int blocks = grid_for(n, kBlock / lanes);
+  if (blocks > kMaxBlocks) blocks = kMaxBlocks;
+#define NFX_A(L)                                                                                \
+  hipLaunchKernelGGL(k_minres_a<L>, dim3(blocks), dim3(kBlock), 0, s->stream, n, rp, ci, va,    \
+                     sign, z, zprev, r1, y, x, wnew, wa, wb, upd, first, s->partial, s->counter, \
+                     s->st)
+  if (lanes == 16)
+    NFX_A(16);
+  else if (lanes == 8)
+    NFX_A(8);
+  else
+    NFX_A(4);
+#undef NFX_A
+  NFX_LAUNCH_CHECK();
+  return NFX_OK;
+}
+
+// One FP32 MINRES solve of (D A) d = rhs (rhs already scaled), result in x.
+static int minres_inner(nfx_solver* s, const int32_t* rp, const int32_t* ci, const float* va,
+                        const float* sign, const float* pinv, const float* rhs, float* x,
+                        double tol, int maxit, int* iters) {
+  const int n = s->n;
+  float* V = s->buf;
+  float* R[3] = {V + (size_t)n * 0, V + (size_t)n * 1, V + (size_t)n * 2};
+  float* Z[2] = {V + (size_t)n * 3, V + (size_t)n * 4};
+  float* W[3] = {V + (size_t)n * 5, V + (size_t)n * 6, V + (size_t)n * 7};
+  hipLaunchKernelGGL(k_minres_init, dim3(s->blocks), dim3(kBlock), 0, s->stream, n, rhs, pinv,
+                     R[0], R[1], Z[1], x, W[0], W[1], W[2], s->partial, s->counter, s->st);
+  NFX_LAUNCH_CHECK();
+  // The init above put r1 = r2 = rhs into R[0], R[1] and z_1 into Z[1].
+  // Iteration k uses r1 = R[(k+2)%3], r2 = R[k%3] ... we choose the mapping below.
+  int k = 1;
+  const int check = 16;
+  double beta1 = 0;
+  for (; k <= maxit; ++k) {
+    // buffers for iteration k
+    float* r1 = R[(k + 2) % 3];
+    float* r2 = R[k % 3];
+    float* y = R[(k + 1) % 3];
+    float* z = Z[k % 2];
+    float* zprev = Z[(k + 1) % 2];
+    // w update for iteration k-1 writes W[(k-1)%3], reads W[k%3] (w1) and W[(k+1)%3] (w2)
+    float* wnew = W[(k + 2) % 3];
+    const float* wa = W[k % 3];
+    const float* wb = W[(k + 1) % 3];
+    int rc = launch_a(s, rp, ci, va, sign, z, zprev, r1, y, x, wnew, wa, wb, k >= 2, k == 1);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_minres_b, dim3(s->blocks), dim3(kBlock), 0, s->stream, n, y, r2, pinv,
+                       zprev, s->partial, s->counter, s->st);
+    NFX_LAUNCH_CHECK();
+    if (k % check == 0 || k == maxit) {
+      NFX_CHECK_HIP(hipMemcpyAsync(s->h_st, s->st, sizeof(MinresState), hipMemcpyDeviceToHost,
+                                   s->stream));
+      NFX_CHECK_HIP(hipStreamSynchronize(s->stream));
+      beta1 = s->h_st->beta1;
+      if (s->h_st->stop || !(beta1 > 0) || s->h_st->phibar <= tol * beta1) break;
+    }
+  }
+  if (k > maxit) k = maxit;
+  // final update for iteration k: v_k = z_k / beta_k; z_k is Z[k%2]
+  {
+    float* zk = Z[k % 2];
+    float* wnew = W[k % 3];
+    const float* wa = W[(k + 1) % 3];
+    const float* wb = W[(k + 2) % 3];
+    hipLaunchKernelGGL(k_minres_final, dim3(s->blocks), dim3(kBlock), 0, s->stream, n, zk, x, wnew,
+                       wa, wb, s->st);
+    NFX_LAUNCH_CHECK();
+  }
+  *iters = k;
+  return NFX_OK;
+}
+
+extern "C" int nfx_solve_minres_ir(nfx_solver* s, const int32_t* d_rowptr, const int32_t* d_col,
+                                   const float* d_val32, const double* d_val64,
+                                   const float* d_sign, const float* d_pinv, const double* d_b,
+                                   double* d_x, double rtol, double inner_rtol, int max_inner,
+                                   int max_outer, int* h_info, double* h_resid, double* h_hist,
+                                   int hist_cap) {
+  NFX_REQUIRE(s && d_rowptr && d_col && d_val32 && d_val64 && d_pinv && d_b && d_x,
+              "nfx_solve_minres_ir: null pointer");
+  NFX_REQUIRE(max_inner > 0 && max_outer > 0, "nfx_solve_minres_ir: iteration limits");
+  const int n = s->n;
+  float* rf = s->buf + (size_t)n * 8;
+  float* dx = s->buf + (size_t)n * 9;
+  int total = 0, outer = 0;
+  double rel = 0;
+  const int g = grid_for(n, kBlock);
+  for (; outer <= max_outer; ++outer) {
+    hipLaunchKernelGGL(k_residual64, dim3(s->blocks), dim3(kBlock), 0, s->stream, n, d_rowptr,
+                       d_col, d_val64, d_x, d_b, s->r64, s->partial, s->counter, s->st);
+    NFX_LAUNCH_CHECK();
+    NFX_CHECK_HIP(hipMemcpyAsync(s->h_st, s->st, sizeof(MinresState), hipMemcpyDeviceToHost,
+                                 s->stream));
+    NFX_CHECK_HIP(hipStreamSynchronize(s->stream));
+    const double rn = sqrt(s->h_st->rnorm2), bn = sqrt(s->h_st->bnorm2);
+    rel = bn > 0 ? rn / bn : rn;
+    if (h_hist && outer < hist_cap) h_hist[outer] = rel;
+    if (rel <= rtol || outer == max_outer || rn == 0) break;
+    hipLaunchKernelGGL(k_to_f32, dim3(g), dim3(kBlock), 0, s->stream, n, s->r64, d_sign, rn, rf);
+    NFX_LAUNCH_CHECK();
+    int it = 0;
+    int rc = minres_inner(s, d_rowptr, d_col, d_val32, d_sign, d_pinv, rf, dx, inner_rtol,
+                          max_inner, &it);
+    if (rc) return rc;
+    total += it;
+    hipLaunchKernelGGL(k_axpy64, dim3(g), dim3(kBlock), 0, s->stream, n, rn, dx, d_x);
+    NFX_LAUNCH_CHECK();
+  }
+  NFX_CHECK_HIP(hipStreamSynchronize(s->stream));
+  if (h_info) {
+    h_info[0] = outer;
+    h_info[1] = total;
+  }
+  if (h_resid) h_resid[0] = rel;
+  return rel <= rtol ? NFX_OK : NFX_E_NOT_CONVERGED;
+}
+
+extern "C" int nfx_residual64(int n_rows, const int32_t* d_rowptr, const int32_t* d_col,
+                              const double* d_val, const double* d_x, const double* d_b,
+                              double* d_r, void* stream) {
+  NFX_REQUIRE(n_rows > 0, "nfx_residual64: n_rows");
+  NFX_REQUIRE(d_rowptr && d_col && d_val && d_x && d_b && d_r, "nfx_residual64: null pointer");
+  // standalone residual: no norms; reuse the CSR kernel with a private scratch
+  double* partial = nullptr;
+  unsigned int* counter = nullptr;
+  MinresState* st = nullptr;
+  hipStream_t sm = static_cast<hipStream_t>(stream);
+  NFX_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&partial), sizeof(double) * 2 * kMaxBlocks, sm));
+  NFX_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&counter), sizeof(unsigned int), sm));
+  NFX_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&st), sizeof(MinresState), sm));
+  NFX_CHECK_HIP(hipMemsetAsync(counter, 0, sizeof(unsigned int), sm));
+  int blocks = grid_for(n_rows, kBlock);
+  if (blocks > kMaxBlocks) blocks = kMaxBlocks;
+  hipLaunchKernelGGL(k_residual64, dim3(blocks), dim3(kBlock), 0, sm, n_rows, d_rowptr, d_col,
+                     d_val, d_x, d_b, d_r, partial, counter, st);
+  NFX_LAUNCH_CHECK();
+  NFX_CHECK_HIP(hipFreeAsync(partial, sm));
+  NFX_CHECK_HIP(hipFreeAsync(counter, sm));
+  NFX_CHECK_HIP(hipFreeAsync(st, sm));
+  return NFX_OK;
+}

@@ -1,0 +1,171 @@
+// FP64 error norms: per-element squared L2 / div errors of a discrete field
+// against exact values at the element quadrature points, reduced
+// deterministically (block partials + last-block sum in fixed order).
+//
+// Vector (H(div)) field with n_comp rows:
+//   f_h[c][d](x_q) = sum_a alpha[row(a, c)] * sign * (J phi_a(q))_d / detJ
+//   err_e = sum_q w_q detJ sum_{c,d} (f_e - f_h)^2
+// Scalar (Lagrange) field: f_h[c] = sum_k alpha[row(k, c)] psi_k(q).
+// div: f_h[c] = sum_a alpha[row(a, c)] * sign * div_a(q) / detJ.
+// tables64: w[NQ] | phi[NQ][NB][DIM] | div[NQ][NB]   (vector)
+//           w[NQ] | psi[NQ][NB]                      (scalar)
+#include "nfx_common.h"
+
+namespace nfx {
+
+constexpr int kNB = 256;
+
+__device__ __forceinline__ double nrm_warp_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+__device__ double nrm_block_sum(double v, double* sh) {
+  v = nrm_warp_sum(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double r = 0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x + 63) / 64; ++i) r += sh[i];
+  __syncthreads();
+  return r;
+}
+
+// kind: 0 = vector L2, 1 = scalar L2, 2 = vector div
+__global__ __launch_bounds__(kNB) void k_field_error(
+    int kind, int dim, int nb, int nq, int nc, int n_elem, const int32_t* __restrict__ cells,
+    const double* __restrict__ coords, const double* __restrict__ tab,
+    const int32_t* __restrict__ rows, const double* __restrict__ alpha,
+    const double* __restrict__ exact, double* partial, unsigned int* counter, double* out) {
+  __shared__ double sh[kNB / 64];
+  double acc = 0;
+  for (int e = blockIdx.x * kNB + threadIdx.x; e < n_elem; e += gridDim.x * kNB) {
+    double X[4][3];
+    for (int a = 0; a <= dim; ++a) {
+      const int v = cells[(int64_t)e * (dim + 1) + a];
+      for (int c = 0; c < 3; ++c) X[a][c] = coords[3 * v + c];
+    }
+    double J[3][3];
+    for (int c = 0; c < 3; ++c)
+      for (int d = 0; d < dim; ++d) J[c][d] = X[d + 1][c] - X[0][c];
+    double detJ;
+    if (dim == 3) {
+      const double dd = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                        J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                        J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+      detJ = fabs(dd);
+    } else {
+      const double cx = J[1][0] * J[2][1] - J[2][0] * J[1][1];
+      const double cy = J[2][0] * J[0][1] - J[0][0] * J[2][1];
+      const double cz = J[0][0] * J[1][1] - J[1][0] * J[0][1];
+      detJ = sqrt(cx * cx + cy * cy + cz * cz);
+    }
+    const double* w = tab;
+    for (int q = 0; q < nq; ++q) {
+      double err = 0;
+      for (int c = 0; c < nc; ++c) {
+        if (kind == 1) {
+          const double* psi = tab + nq;
+          double fh = 0;
+          for (int k = 0; k < nb; ++k) {
+            const int code = rows[((int64_t)e * nb + k) * nc + c];
+            fh += alpha[code & kSlotIndexMask] * psi[q * nb + k];
+          }
+          const double d = exact[((int64_t)e * nq + q) * nc + c] - fh;
+          err += d * d;
+        } else if (kind == 0) {
+          const double* phi = tab + nq;
+          double ref[3] = {0, 0, 0};
+          for (int a = 0; a < nb; ++a) {
+            const int code = rows[((int64_t)e * nb + a) * nc + c];
+            double al = alpha[code & kSlotIndexMask];
+            if (code & kSlotNeg) al = -al;
+            for (int d = 0; d < dim; ++d) ref[d] += al * phi[(q * nb + a) * dim + d];
+          }
+          for (int d = 0; d < dim; ++d) {
+            double fh = 0;
+            for (int d2 = 0; d2 < dim; ++d2) fh += J[d][d2] * ref[d2];
+            fh /= detJ;
+            const double diff = exact[(((int64_t)e * nq + q) * nc + c) * dim + d] - fh;
+            err += diff * diff;
+          }
+        } else {
+          const double* dv = tab + nq + nq * nb * dim;
+          double fh = 0;
+          for (int a = 0; a < nb; ++a) {
+            const int code = rows[((int64_t)e * nb + a) * nc + c];
+            double al = alpha[code & kSlotIndexMask];
+            if (code & kSlotNeg) al = -al;
+            fh += al * dv[q * nb + a];
+          }
+          fh /= detJ;
+          const double d = exact[((int64_t)e * nq + q) * nc + c] - fh;
+          err += d * d;
+        }
+      }
+      acc += w[q] * detJ * err;
+    }
+  }
+  const double bs = nrm_block_sum(acc, sh);
+  __shared__ bool is_last;
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = bs;
+    __threadfence();
+    is_last = atomicAdd(counter, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  double v = 0;
+  for (int i = threadIdx.x; i < (int)gridDim.x; i += kNB)
+    v += __hip_atomic_load(&partial[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  v = nrm_block_sum(v, sh);
+  if (threadIdx.x == 0) {
+    *out = v;
+    *counter = 0u;
+  }
+}
+
+}  // namespace nfx
+
+using namespace nfx;
+
+extern "C" int nfx_field_error(int kind, int dim, int nb, int nq, int n_comp, int n_elem,
+                               const int32_t* d_cells, const double* d_coords64,
+                               const double* d_tables64, const int32_t* d_rows,
+                               const double* d_alpha, const double* d_exact, double* h_out,
+                               void* stream) {
+  NFX_REQUIRE(kind >= 0 && kind <= 2, "nfx_field_error: kind");
+  NFX_REQUIRE((dim == 2 || dim == 3) && nb > 0 && nq > 0 && n_comp > 0 && n_elem >= 0,
+              "nfx_field_error: sizes");
+  NFX_REQUIRE(h_out, "nfx_field_error: null output");
+  if (n_elem == 0) {
+    *h_out = 0;
+    return NFX_OK;
+  }
+  NFX_REQUIRE(d_cells && d_coords64 && d_tables64 && d_rows && d_alpha && d_exact,
+              "nfx_field_error: null pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int blocks = grid_for(n_elem, kNB);
+  if (blocks > 1024) blocks = 1024;
+  double* scratch = nullptr;
+  NFX_CHECK_HIP(hipMalloc(&scratch, sizeof(double) * (1024 + 2)));
+  unsigned int* counter = reinterpret_cast<unsigned int*>(scratch + 1025);
+  double* out = scratch + 1024;
+  hipError_t e = hipMemsetAsync(counter, 0, sizeof(unsigned int), s);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_field_error, dim3(blocks), dim3(kNB), 0, s, kind, dim, nb, nq, n_comp,
+                       n_elem, d_cells, d_coords64, d_tables64, d_rows, d_alpha, d_exact, scratch,
+                       counter, out);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(h_out, out, sizeof(double), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(scratch);
+  if (e != hipSuccess) {
+    set_error(std::string("nfx_field_error: ") + hipGetErrorString(e));
+    return NFX_E_HIP;
+  }
+  return NFX_OK;
+}

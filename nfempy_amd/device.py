@@ -1,0 +1,339 @@
+"""Device-resident systems: mesh/space arrays in HBM and the warm assembly calls.
+
+``DarcySystem`` holds, for one ``ProductSpace`` with an RT_k flux field and a
+discontinuous P_{k-1} pressure field (the spaces built by
+``examples/md_elliptic_scalar/md_elliptic_scalar.py:44-86``), everything the
+HIP kernels read:
+
+  cells (E, D+1) int32, coords (V, 3) f32/f64, reference tables (f32 and f64),
+  slot map (nl*nl, E) int32, pressure rows (np, E), CSR rowptr/col int32,
+  stage segments, and the boundary-facet arrays of the Dirichlet term.
+
+``assemble`` is the warm path: coefficient arrays in, CSR values + residual out
+(FP32 by default, FP64 on request), pattern reused.
+"""
+
+from __future__ import annotations
+
+from itertools import permutations
+
+import numpy as np
+
+from . import _native as nat
+from .basis import create_element
+from .mesh import TET_FACES, TRI_EDGES
+from .pattern import build_pattern, row_codes
+
+_TORCH = None
+
+
+def torch():
+    global _TORCH
+    if _TORCH is None:
+        import torch as _t
+
+        _TORCH = _t
+    return _TORCH
+
+
+def to_dev(a, dtype=None, device="cuda"):
+    t = torch()
+    if isinstance(a, t.Tensor):
+        return a.to(device=device, dtype=dtype) if dtype is not None else a.to(device)
+    arr = np.ascontiguousarray(a)
+    out = t.from_numpy(arr)
+    if dtype is not None:
+        out = out.to(dtype)
+    return out.to(device)
+
+
+def p1_bary(points):
+    pts = np.asarray(points, dtype=np.float64)
+    return np.concatenate([1.0 - pts.sum(axis=1, keepdims=True), pts], axis=1)
+
+
+def darcy_tables(u_el, p_el, points, weights):
+    """w | phi (nq, nb, dim) | div (nq, nb) | psi (nq, np) as one FP64 vector."""
+    tu = u_el.tabulate(points)
+    tp = p_el.tabulate(points)
+    phi = tu[0]
+    div = np.einsum("kqjk->qj", tu[1:])
+    psi = tp[0, :, :, 0]
+    return np.concatenate([weights, phi.ravel(), div.ravel(), psi.ravel()])
+
+
+def facet_combo_tables(el, D, fpoints):
+    """Reference flux basis of each facet's DoFs at facet points mapped into the cell.
+
+    Combo index = local_facet * D! + index of the facet-vertex order in
+    permutations(range(D)), following ``transform_lower_to_higher``
+    (``src/basis/parametric_transformation.py:30-98``). Returns (ncombo, nqf, fd, D).
+    """
+    local = TRI_EDGES if D == 2 else TET_FACES
+    verts = np.eye(D + 1, D, -1)
+    perms = list(permutations(range(D)))
+    s = np.asarray(fpoints, dtype=np.float64)
+    bary = p1_bary(s)  # (nqf, D)
+    out = []
+    for lf in range(D + 1):
+        dofs = el.entity_dofs[D - 1][lf]
+        for p in perms:
+            loc = local[lf][list(p)]
+            mapped = bary @ verts[loc]
+            tab = el.tabulate(mapped)[0]  # (nqf, nb, D)
+            out.append(tab[:, dofs, :])
+    return np.array(out)
+
+
+def facet_combo_index(mesh, D, facet_ids, neigh_cells, local_facet):
+    """Combo index of each boundary facet (see ``facet_combo_tables``)."""
+    local = TRI_EDGES if D == 2 else TET_FACES
+    perms = {p: i for i, p in enumerate(permutations(range(D)))}
+    fnodes = mesh.entities[D - 1].nodes[facet_ids - mesh.entities[D - 1].offset]  # (F, D)
+    cnodes = mesh.entities[D].nodes[neigh_cells]  # (F, D+1)
+    # local vertex index (in the cell) of each facet node
+    locv = np.argmax(cnodes[:, None, :] == fnodes[:, :, None], axis=2)  # (F, D)
+    base = local[local_facet]  # (F, D) basix order of that facet's local vertices
+    pos = np.argmax(base[:, None, :] == locv[:, :, None], axis=2)  # (F, D) position in base
+    pidx = np.array([perms[tuple(r)] for r in pos.tolist()], dtype=np.int64) if len(pos) else np.zeros(0, np.int64)
+    return local_facet * len(perms) + pidx
+
+
+class CoefficientArg:
+    """A coefficient at quadrature points: device array + (stride_e, stride_q).
+
+    Holds the values in FP64 and hands out an FP32 or FP64 copy matching the
+    kernel precision (both cached).
+    """
+
+    def __init__(self, arr, se, sq):
+        t = torch()
+        self.se, self.sq = se, sq
+        self._by = {arr.dtype: arr.contiguous()}
+        self.arr = arr
+
+    def get(self, fp64):
+        t = torch()
+        dt = t.float64 if fp64 else t.float32
+        if dt not in self._by:
+            self._by[dt] = self.arr.to(dt).contiguous()
+        return self._by[dt]
+
+
+def coefficient(value, xq, nq, comp=0, invert=False, device="cuda"):
+    """Turn a reference-style coefficient into a kernel argument.
+
+    value: callable f(x, y, z) (vectorised over points; returns a scalar, an
+    (n_comp, npts) array or an (npts,) array), a per-element array (E,) or
+    (E, nq), a torch tensor of the same shapes, or a number.
+    """
+    t = torch()
+    if callable(value):
+        x = xq.reshape(-1, 3)
+        v = np.asarray(value(x[:, 0], x[:, 1], x[:, 2]), dtype=np.float64)
+        if v.ndim == 0:
+            arr = np.array([float(v)])
+            se = sq = 0
+        else:
+            if v.ndim >= 2:
+                v = v.reshape(v.shape[0], -1)[comp]
+            arr = v.reshape(-1)
+            if arr.size == 1:
+                se = sq = 0
+            else:
+                se, sq = nq, 1
+        if invert:
+            arr = 1.0 / arr
+        return CoefficientArg(to_dev(arr.astype(np.float64), device=device), se, sq)
+    if isinstance(value, (int, float, np.floating)):
+        v = 1.0 / float(value) if invert else float(value)
+        return CoefficientArg(t.tensor([v], dtype=t.float64, device=device), 0, 0)
+    arr = value if isinstance(value, t.Tensor) else t.from_numpy(np.asarray(value, dtype=np.float64))
+    arr = arr.to(device=device, dtype=t.float64)
+    if invert:
+        arr = 1.0 / arr
+    if arr.dim() == 1:
+        return CoefficientArg(arr.contiguous(), 1, 0)
+    return CoefficientArg(arr.reshape(arr.shape[0], -1).contiguous(), arr.shape[1], 1)
+
+
+# (dim, k) -> (NB, NP, NQ, FD, NQF) compiled into nfx_assemble_darcy / _bc
+DARCY_KERNELS = {(2, 1): (3, 1, 4, 1, 2), (2, 2): (8, 3, 9, 2, 3), (3, 1): (4, 1, 8, 1, 4), (3, 2): (15, 4, 27, 3, 9)}
+
+
+def _check_codes(codes, n_direct, n_stage, what):
+    c = np.asarray(codes).astype(np.int64).reshape(-1)
+    live = c[c >= 0]
+    idx = live & nat.SLOT_INDEX_MASK
+    staged = (live & nat.SLOT_STAGE) != 0
+    if np.any(idx[~staged] >= n_direct) or np.any(idx[staged] >= max(n_stage, 0)) or (np.any(staged) and n_stage == 0):
+        raise ValueError(f"{what}: slot/row code out of range")
+
+
+def check_coefficient(arg, E, nq, what):
+    n = int(arg.arr.numel())
+    last = (E - 1) * arg.se + (nq - 1) * arg.sq
+    if E > 0 and last >= n:
+        raise ValueError(f"{what}: coefficient array too short ({n} <= {last})")
+
+
+class DarcySystem:
+    """Mixed Darcy RT_k - P_{k-1} on one product space, resident on the GPU."""
+
+    def __init__(self, space, flux="u", potential="p", device="cuda"):
+        t = torch()
+        self.space = space
+        self.device = device
+        u_sp = space.discrete_spaces[flux]
+        p_sp = space.discrete_spaces[potential]
+        self.u_sp, self.p_sp = u_sp, p_sp
+        mesh = u_sp.mesh
+        D = u_sp.dimension
+        self.dim = D
+        self.k = u_sp.k_order
+        self.u_el = u_sp.element
+        self.p_el = p_sp.element
+        self.nb = self.u_el.ndofs
+        self.np_ = self.p_el.ndofs
+        self.nl = self.nb + self.np_
+        self.E = len(u_sp.element_ids)
+        self.n = space.n_dof
+        self.n_u = u_sp.n_dof
+        points, weights = space.quadrature[D]
+        self.qpoints, self.qweights = points, weights
+        self.nq = len(weights)
+        cell_tab = mesh.entities[D]
+        cells = cell_tab.nodes[u_sp.topology.cell_index].astype(np.int32)
+        self.cells_np = cells
+        self.cells = to_dev(cells, device=device)
+        self.coords = to_dev(mesh.points.astype(np.float32), device=device)
+        self.coords64 = to_dev(mesh.points, device=device)
+        tab = darcy_tables(self.u_el, self.p_el, points, weights)
+        self.tables64 = to_dev(tab, device=device)
+        self.tables32 = to_dev(tab.astype(np.float32), device=device)
+        # x_q on the host (FP64): coefficient callables are evaluated there
+        X = mesh.points[cells]  # (E, D+1, 3)
+        self.xq = np.einsum("qa,eac->eqc", p1_bary(points), X)
+        ref_dest, ref_sign = space.reference_destinations()
+        self.ref_dest, self.ref_sign = ref_dest, ref_sign
+        self.pattern = build_pattern(self.n, ref_dest, ref_sign)
+        pat = self.pattern
+        self.rowptr = to_dev(pat.rowptr, device=device)
+        self.col = to_dev(pat.col, device=device)
+        self.slot = to_dev(pat.slot, device=device)
+        self.segptr = to_dev(pat.segptr, device=device)
+        self.target = to_dev(pat.target, device=device)
+        self.nnz = pat.nnz
+        self.rhs_row = to_dev(np.ascontiguousarray(row_codes(ref_dest[:, self.nb:]).T), device=device)
+        self.vals32 = t.zeros(self.nnz, dtype=t.float32, device=device)
+        self.vals64 = t.zeros(self.nnz, dtype=t.float64, device=device)
+        self.stage32 = t.zeros(max(pat.n_stage, 1), dtype=t.float32, device=device)
+        self.stage64 = t.zeros(max(pat.n_stage, 1), dtype=t.float64, device=device)
+        self.rhs32 = t.zeros(self.n, dtype=t.float32, device=device)
+        self.rhs64 = t.zeros(self.n, dtype=t.float64, device=device)
+        self._setup_bc(mesh, D, device)
+        self._validate(mesh)
+
+    def _validate(self, mesh):
+        """Host-side checks that every kernel index stays inside its buffer."""
+        key = (self.dim, self.k)
+        if key not in DARCY_KERNELS:
+            raise ValueError(f"no Darcy kernel for dim={self.dim}, k={self.k}")
+        NB, NP, NQ, FD, NQF = DARCY_KERNELS[key]
+        if (self.nb, self.np_, self.nq) != (NB, NP, NQ):
+            raise ValueError("element sizes do not match the compiled kernel")
+        V = len(mesh.points)
+        if self.cells_np.shape != (self.E, self.dim + 1) or (self.E and (self.cells_np.min() < 0 or self.cells_np.max() >= V)):
+            raise ValueError("cells out of range")
+        if self.tables64.numel() != NQ * (1 + NB * self.dim + NB + NP):
+            raise ValueError("table size mismatch")
+        pat = self.pattern
+        if pat.slot.shape != (self.nl * self.nl, self.E):
+            raise ValueError("slot map shape")
+        _check_codes(pat.slot, pat.nnz, pat.n_stage, "slot map")
+        _check_codes(self.ref_dest[:, self.nb:], self.n, 0, "pressure rows")
+        if len(pat.target) and (pat.target.max() >= pat.nnz or pat.segptr[-1] != pat.n_stage):
+            raise ValueError("stage segments out of range")
+        if self.F:
+            if self.nqf != NQF:
+                raise ValueError("facet quadrature does not match the compiled kernel")
+            fn = self.bc_facet.cpu().numpy()
+            if fn.min() < 0 or fn.max() >= V:
+                raise ValueError("facet nodes out of range")
+            nb_ = self.bc_neigh.cpu().numpy()
+            if nb_.min() < 0 or nb_.max() >= self.E:
+                raise ValueError("facet neighbours out of range")
+            cb = self.bc_combo.cpu().numpy()
+            if cb.min() < 0 or cb.max() >= self.bc_trace64.shape[0] or self.bc_trace64.shape[1:] != (NQF, FD, self.dim):
+                raise ValueError("facet trace tables out of range")
+            rows = self.bc_rows.cpu().numpy()
+            if rows.shape != (self.F, FD):
+                raise ValueError("facet rows shape")
+            _check_codes(rows, self.n, 0, "facet rows")
+
+    def _setup_bc(self, mesh, D, device):
+        u_sp = self.u_sp
+        self.F = len(u_sp.bc_element_ids)
+        if not self.F:
+            return
+        fpts, fw = self.space.bc_quadrature[D - 1]
+        self.nqf = len(fw)
+        neigh_cells = u_sp.topology.cell_index[u_sp.bc_neighbour]
+        combo = facet_combo_index(mesh, D, u_sp.bc_element_ids, neigh_cells, u_sp.bc_local)
+        trace = facet_combo_tables(self.u_el, D, fpts)
+        fnodes = mesh.entities[D - 1].nodes[u_sp.bc_element_ids - mesh.entities[D - 1].offset]
+        self.bc_facet = to_dev(fnodes.astype(np.int32), device=device)
+        self.bc_neigh = to_dev(u_sp.bc_neighbour.astype(np.int32), device=device)
+        self.bc_combo = to_dev(combo.astype(np.int32), device=device)
+        self.bc_trace64 = to_dev(trace, device=device)
+        self.bc_trace32 = to_dev(trace.astype(np.float32), device=device)
+        self.bc_fw64 = to_dev(fw, device=device)
+        self.bc_fw32 = to_dev(fw.astype(np.float32), device=device)
+        shift = self.space.strides[self.space.names[0]] + self.space.dof_shift
+        self.bc_rows = to_dev(row_codes(u_sp.bc_ref_dest + shift, u_sp.bc_ref_sign), device=device)
+        Xf = mesh.points[fnodes]  # (F, D, 3)
+        self.bc_xq = np.einsum("qa,fac->fqc", p1_bary(fpts), Xf)
+
+    # -- warm assembly ------------------------------------------------------------
+    def assemble(self, kappa, rhs, p_dirichlet=None, fp64=False, stream=None):
+        """Values + residual rg (the reference's ``rg``, so b = -rg)."""
+        t = torch()
+        s = nat.stream_handle(stream)
+        vals = self.vals64 if fp64 else self.vals32
+        stage = self.stage64 if fp64 else self.stage32
+        rg = self.rhs64 if fp64 else self.rhs32
+        tables = self.tables64 if fp64 else self.tables32
+        ka = kappa if isinstance(kappa, CoefficientArg) else coefficient(kappa, self.xq, self.nq, invert=True, device=self.device)
+        fa = rhs if isinstance(rhs, CoefficientArg) else coefficient(rhs, self.xq, self.nq, device=self.device)
+        check_coefficient(ka, self.E, self.nq, "kappa")
+        check_coefficient(fa, self.E, self.nq, "rhs")
+        nat.call("nfx_fill", int(fp64), self.n, 0.0, nat.ptr(rg), s)
+        coords = self.coords64 if fp64 else self.coords
+        nat.call("nfx_assemble_darcy", self.dim, self.k, self.E, nat.ptr(self.cells), nat.ptr(coords),
+                 nat.ptr(tables), nat.ptr(ka.get(fp64)), ka.se, ka.sq, nat.ptr(fa.get(fp64)), fa.se, fa.sq,
+                 nat.ptr(self.slot), nat.ptr(self.rhs_row), int(fp64), nat.ptr(vals), nat.ptr(stage),
+                 nat.ptr(rg), s)
+        nat.call("nfx_stage_reduce", len(self.pattern.target), nat.ptr(self.segptr), nat.ptr(self.target),
+                 int(fp64), nat.ptr(stage), nat.ptr(vals), s)
+        if p_dirichlet is not None and self.F:
+            pd = p_dirichlet if isinstance(p_dirichlet, CoefficientArg) else coefficient(
+                p_dirichlet, self.bc_xq, self.nqf, device=self.device)
+            pdv = pd.get(fp64)
+            if pd.se == 0:
+                pdv = pdv.expand(self.F * self.nqf).contiguous()
+            elif pd.sq == 0:
+                pdv = pdv.reshape(-1, 1).expand(self.F, self.nqf).contiguous()
+            if pdv.numel() != self.F * self.nqf:
+                raise ValueError("Dirichlet data must have one value per facet point")
+            nat.call("nfx_assemble_darcy_bc", self.dim, self.k, self.F, nat.ptr(self.bc_facet),
+                     nat.ptr(self.bc_neigh), nat.ptr(self.bc_combo), nat.ptr(self.cells), nat.ptr(coords),
+                     nat.ptr(self.bc_trace64 if fp64 else self.bc_trace32),
+                     nat.ptr(self.bc_fw64 if fp64 else self.bc_fw32), self.nqf, nat.ptr(pdv),
+                     nat.ptr(self.bc_rows), int(fp64), nat.ptr(rg), s)
+        return vals, rg
+
+    def csr_numpy(self, fp64=True):
+        import scipy.sparse as sps
+
+        vals = (self.vals64 if fp64 else self.vals32).cpu().numpy()
+        return sps.csr_matrix((vals, self.pattern.col, self.pattern.rowptr), shape=(self.n, self.n))

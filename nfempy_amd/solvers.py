@@ -1,0 +1,94 @@
+"""Krylov solves on the device: MINRES in FP32 inside FP64 iterative refinement.
+
+Replaces the reference's PETSc KSP blocks:
+- Darcy: ``preonly`` + LU(MUMPS) on ``[[M, -B^T], [B, 0]] x = -rg``
+  (``examples/md_elliptic_scalar/md_elliptic_scalar.py:351-364``);
+- elasticity: MINRES + PCFIELDSPLIT additive
+  (``examples/cauchy_elasticity/mfe_elasticity_example_1.py:238-274``).
+
+Every answer is judged by the FP64 residual ``||b - A x||_2 / ||b||_2`` of the
+as-assembled FP64 matrix.
+
+Block preconditioner for the Darcy saddle point (``darcy_preconditioner``): the
+pressure rows are negated so that ``D A`` is symmetric indefinite, and MINRES runs
+with the SPD block-diagonal preconditioner ``diag(M)^-1`` on the flux rows and
+``|S|^-1`` on the pressure rows, ``S = diag(B diag(M)^-1 B^T)`` evaluated in FP64.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _native as nat
+from .device import torch
+
+
+class MinresIR:
+    """FP32 MINRES + FP64 refinement on one CSR operator resident on the device."""
+
+    def __init__(self, rowptr, col, vals32, vals64, sign=None, pinv=None, stream=None):
+        t = torch()
+        self.n = int(rowptr.numel() - 1)
+        self.nnz = int(col.numel())
+        self.rowptr, self.col, self.vals32, self.vals64 = rowptr, col, vals32, vals64
+        self.sign = sign
+        self.pinv = pinv if pinv is not None else t.ones(self.n, dtype=t.float32, device=rowptr.device)
+        self.stream = stream
+        h = ctypes.c_void_p()
+        nat.call("nfx_solver_create", ctypes.byref(h), self.n, self.nnz, nat.stream_handle(stream))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            nat.lib().nfx_solver_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve(self, b, x=None, rtol=1e-8, inner_rtol=1e-5, max_inner=20000, max_outer=30, raise_on_fail=False):
+        t = torch()
+        b = b.to(dtype=t.float64).contiguous()
+        if x is None:
+            x = t.zeros(self.n, dtype=t.float64, device=b.device)
+        info = (ctypes.c_int * 2)()
+        resid = (ctypes.c_double * 1)()
+        hist = (ctypes.c_double * (max_outer + 1))()
+        rc = nat.lib().nfx_solve_minres_ir(
+            self._h, nat.ptr(self.rowptr), nat.ptr(self.col), nat.ptr(self.vals32), nat.ptr(self.vals64),
+            nat.ptr(self.sign), nat.ptr(self.pinv), nat.ptr(b), nat.ptr(x), rtol, inner_rtol, max_inner,
+            max_outer, info, resid, hist, max_outer + 1)
+        if rc not in (nat.NFX_OK, nat.NFX_E_NOT_CONVERGED) or (raise_on_fail and rc != nat.NFX_OK):
+            nat.check(rc, "nfx_solve_minres_ir")
+        out = {
+            "converged": rc == nat.NFX_OK,
+            "outer": info[0],
+            "iterations": info[1],
+            "residual": resid[0],
+            "history": [hist[i] for i in range(min(info[0] + 1, max_outer + 1))],
+        }
+        return x, out
+
+
+def darcy_preconditioner(rowptr, col, vals64, n, n_u, stream=None):
+    """(sign, pinv) device FP32 vectors of the Darcy block preconditioner."""
+    t = torch()
+    s = nat.stream_handle(stream)
+    dev = rowptr.device
+    diag = t.empty(n, dtype=t.float64, device=dev)
+    nat.call("nfx_csr_diag64", n, nat.ptr(rowptr), nat.ptr(col), nat.ptr(vals64), nat.ptr(diag), s)
+    schur = t.empty(max(n - n_u, 1), dtype=t.float64, device=dev)
+    nat.call("nfx_schur_diag64", n, n_u, nat.ptr(rowptr), nat.ptr(col), nat.ptr(vals64), nat.ptr(diag),
+             nat.ptr(schur), s)
+    pinv = t.empty(n, dtype=t.float64, device=dev)
+    pinv[:n_u] = 1.0 / diag[:n_u].abs()
+    sa = schur[: n - n_u].abs()
+    pinv[n_u:] = t.where(sa > 0, 1.0 / sa, t.ones_like(sa))
+    sign = t.ones(n, dtype=t.float32, device=dev)
+    sign[n_u:] = -1.0
+    return sign, pinv.to(t.float32)

@@ -78,7 +78,8 @@ class DiscreteSpace:
             self.bc_family, self.bc_k = "P", self.k - 1
         else:
             self.bc_family, self.bc_k = "P", self.k
-        self.bc_element = create_element(self.bc_family, D - 1, self.bc_k) if D > 1 else None
+        # the facet element only fixes the number of traced DoFs per facet
+        # (= DoFs of the cell element on that facet), see forms._bc_trace
 
     def destination_indices(self, cell_id):
         D = self.dimension

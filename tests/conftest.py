@@ -1,0 +1,17 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs through libnfx on cuda:0)")
+
+
+@pytest.fixture(scope="session")
+def fixtures_dir():
+    return os.path.join(ROOT, "tests", "golden")
