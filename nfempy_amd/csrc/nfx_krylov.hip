@@ -48,34 +48,6 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return r;
 }
 
-// Last-block reduction of partial[0..gridDim.x) into *out (fixed order).
-// Returns true in the block that performed it (all threads).
-__device__ __forceinline__ bool last_block_reduce(double blocksum, double* partial,
-                                                  unsigned int* counter, double* out,
-                                                  double* sh) {
-  __shared__ bool is_last;
-  if (threadIdx.x == 0) {
-    partial[blockIdx.x] = blocksum;
-    __threadfence();
-    const unsigned int t = atomicAdd(counter, 1u);
-    is_last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!is_last) return false;
-  __threadfence();
-  // partials were written by other blocks: device-scope loads bypass the CU cache
-  double v = 0;
-  for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x)
-    v += __hip_atomic_load(&partial[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  v = block_sum(v, sh);
-  if (threadIdx.x == 0) {
-    *out = v;
-    *counter = 0u;
-  }
-  __syncthreads();
-  return true;
-}
-
 template <int LANES>
 __device__ __forceinline__ float group_sum(float v) {
 #pragma unroll
@@ -102,11 +74,12 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int n, const int32_t* __restric
   }
 }
 
-// FP64 residual r = b - A x and the two squared norms (last-block reduced).
+// FP64 residual r = b - A x; block partials of r.r and b.b (summed on the host
+// in block order).
 __global__ __launch_bounds__(kBlock) void k_residual64(
     int n, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
     const double* __restrict__ va, const double* __restrict__ x, const double* __restrict__ b,
-    double* __restrict__ r, double* partial, unsigned int* counter, MinresState* st) {
+    double* __restrict__ r, double* __restrict__ partial) {
   __shared__ double sh[kBlock / 64];
   double rr = 0, bb = 0;
   for (int row = blockIdx.x * kBlock + threadIdx.x; row < n; row += gridDim.x * kBlock) {
@@ -119,29 +92,9 @@ __global__ __launch_bounds__(kBlock) void k_residual64(
   }
   const double brr = block_sum(rr, sh);
   const double bbb = block_sum(bb, sh);
-  // two reductions through the same ticket: pack the second into partial[gridDim.x + i]
-  __shared__ bool is_last;
   if (threadIdx.x == 0) {
-    partial[blockIdx.x] = brr;
-    partial[gridDim.x + blockIdx.x] = bbb;
-    __threadfence();
-    const unsigned int t = atomicAdd(counter, 1u);
-    is_last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
-  double v1 = 0, v2 = 0;
-  for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) {
-    v1 += __hip_atomic_load(&partial[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    v2 += __hip_atomic_load(&partial[gridDim.x + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  v1 = block_sum(v1, sh);
-  v2 = block_sum(v2, sh);
-  if (threadIdx.x == 0) {
-    st->rnorm2 = v1;
-    st->bnorm2 = v2;
-    *counter = 0u;
+    partial[2 * blockIdx.x] = brr;
+    partial[2 * blockIdx.x + 1] = bbb;
   }
 }
 
@@ -183,14 +136,30 @@ __global__ __launch_bounds__(kBlock) void k_schur64(int n, int nu, const int32_t
 
 // ---------------------------------------------------------------------------
 // MINRES kernels (FP32 vectors, FP64 scalars)
+//
+// Reductions: a producer kernel writes one partial per block; every block of the
+// consumer kernel sums all partials in the same fixed order in its prologue, so
+// all blocks derive identical scalars without atomics. Scalar state is double
+// buffered: A(k) reads S[(k+1)&1] and writes S[k&1]; B(k) adds alpha_k to S[k&1].
 
-// Initial step: r1 = r2 = rhs, z = P^-1 rhs, beta1 = sqrt(rhs . z), x = w* = 0.
+// Fixed-order sum of partial[0..np) (every block gets the same value).
+__device__ __forceinline__ double sum_partials(const double* __restrict__ partial, int np,
+                                               double* sh) {
+  double v = 0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) v += partial[i];
+  v = block_sum(v, sh);
+  __shared__ double bc;
+  if (threadIdx.x == 0) bc = v;
+  __syncthreads();
+  return bc;
+}
+
+// Initial step: r1 = r2 = rhs, z = P^-1 rhs, x = w* = 0, partials of rhs . z.
 __global__ __launch_bounds__(kBlock) void k_minres_init(int n, const float* __restrict__ rhs,
                                                         const float* __restrict__ pinv,
                                                         float* r1, float* r2, float* z,
                                                         float* x, float* w0, float* w1, float* w2,
-                                                        double* partial, unsigned int* counter,
-                                                        MinresState* st) {
+                                                        double* partial_b) {
   __shared__ double sh[kBlock / 64];
   double acc = 0;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
@@ -206,80 +175,119 @@ __global__ __launch_bounds__(kBlock) void k_minres_init(int n, const float* __re
     acc += (double)b * (double)zz;
   }
   const double bs = block_sum(acc, sh);
-  double tot;
-  if (last_block_reduce(bs, partial, counter, &tot, sh) && threadIdx.x == 0) {
-    const double beta1 = sqrt(tot > 0 ? tot : 0);
-    st->beta1 = beta1;
-    st->beta = beta1;
-    st->oldb = 0;
-    st->alfa = 0;
-    st->dbar = 0;
-    st->epsln = 0;
-    st->oldeps = 0;
-    st->phibar = beta1;
-    st->cs = -1;
-    st->sn = 0;
-    st->delta = 0;
-    st->denom = 0;
-    st->phi = 0;
-    st->s = beta1 > 0 ? 1.0 / beta1 : 0.0;
-    st->ratio = 0;
-    st->stop = beta1 > 0 ? 0 : 1;
-  }
+  if (threadIdx.x == 0) partial_b[blockIdx.x] = bs;
 }
 
-// A(k): w/x update of iteration k-1 (when upd != 0), then y = s * D A z - ratio * r1,
-// alpha = sum v_i y_i with v = s z.
+// Scalar step of A(k): beta_k from B(k-1)'s partials, Givens rotation of
+// iteration k-1 (Paige-Saunders, as in SciPy's minres).
+__device__ __forceinline__ void minres_scalars(const MinresState& in, MinresState& out,
+                                               double beta, int first) {
+  out = in;
+  if (first) {
+    out.beta1 = beta;
+    out.beta = beta;
+    out.oldb = 0;
+    out.alfa = 0;
+    out.dbar = 0;
+    out.epsln = 0;
+    out.oldeps = 0;
+    out.phibar = beta;
+    out.cs = -1;
+    out.sn = 0;
+    out.delta = 0;
+    out.denom = 0;
+    out.phi = 0;
+    out.s = beta > 0 ? 1.0 / beta : 0.0;
+    out.ratio = 0;
+    out.stop = beta > 0 ? 0 : 1;
+    return;
+  }
+  const double alfa = in.alfa;
+  const double oldb = in.beta;
+  const double oldeps = in.epsln;
+  const double delta = in.cs * in.dbar + in.sn * alfa;
+  const double gbar = in.sn * in.dbar - in.cs * alfa;
+  const double epsln = in.sn * beta;
+  const double dbar = -in.cs * beta;
+  double gamma = hypot(gbar, beta);
+  if (gamma < 1e-300) gamma = 1e-300;
+  out.oldb = oldb;
+  out.beta = beta;
+  out.oldeps = oldeps;
+  out.delta = delta;
+  out.epsln = epsln;
+  out.dbar = dbar;
+  out.cs = gbar / gamma;
+  out.sn = beta / gamma;
+  out.phi = out.cs * in.phibar;
+  out.phibar = out.sn * in.phibar;
+  out.denom = 1.0 / gamma;
+  out.s = beta > 0 ? 1.0 / beta : 0.0;
+  out.ratio = oldb > 0 ? beta / oldb : 0.0;
+  if (!(beta > 0)) out.stop = 1;
+}
+
+// A(k): scalars, w/x update of iteration k-1 (unless first), then (unless
+// final) y = s D A z - ratio r1 and partials of alpha = (s z) . y.
 template <int LANES>
 __global__ __launch_bounds__(kBlock) void k_minres_a(
     int n, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
     const float* __restrict__ va, const float* __restrict__ sign, const float* __restrict__ z,
     const float* __restrict__ zprev, const float* __restrict__ r1, float* __restrict__ y,
     float* __restrict__ x, float* __restrict__ wnew, const float* __restrict__ wa,
-    const float* __restrict__ wb, int upd, int first, double* partial, unsigned int* counter,
-    MinresState* st) {
+    const float* __restrict__ wb, int first, int final_only, const double* __restrict__ partial_b,
+    int np_b, double* __restrict__ partial_a, const MinresState* __restrict__ sin,
+    MinresState* __restrict__ sout) {
   __shared__ double sh[kBlock / 64];
-  const float s = (float)st->s;
-  const float ratio = first ? 0.f : (float)st->ratio;
-  // scalars of the previous iteration's Givens step (computed in B(k-1))
-  const float sprev = (float)st->oldb > 0 ? (float)(1.0 / st->oldb) : 0.f;
-  const float oldeps = (float)st->oldeps, delta = (float)st->delta, denom = (float)st->denom,
-              phi = (float)st->phi;
+  const double bsum = sum_partials(partial_b, np_b, sh);
+  MinresState st;
+  minres_scalars(*sin, st, sqrt(bsum > 0 ? bsum : 0), first);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *sout = st;
+  const float s = (float)st.s;
+  const float ratio = first ? 0.f : (float)st.ratio;
+  const float sprev = st.oldb > 0 ? (float)(1.0 / st.oldb) : 0.f;
+  const float oldeps = (float)st.oldeps, delta = (float)st.delta, denom = (float)st.denom,
+              phi = (float)st.phi;
   const int lane = threadIdx.x % LANES;
   double acc = 0;
   const int rows_per_grid = gridDim.x * (kBlock / LANES);
   for (int row = blockIdx.x * (kBlock / LANES) + threadIdx.x / LANES; row < n;
        row += rows_per_grid) {
     float sum = 0.f;
-    for (int j = rp[row] + lane; j < rp[row + 1]; j += LANES) sum += va[j] * z[ci[j]];
+    if (!final_only)
+      for (int j = rp[row] + lane; j < rp[row + 1]; j += LANES) sum += va[j] * z[ci[j]];
     sum = group_sum<LANES>(sum);
     if (lane == 0) {
-      if (upd) {
-        // v_{k-1} = zprev / beta_{k-1}; w = (v - oldeps * w1 - delta * w2) * denom
+      if (!first) {
         const float v = zprev[row] * sprev;
         const float w = (v - oldeps * wa[row] - delta * wb[row]) * denom;
         wnew[row] = w;
         x[row] += phi * w;
       }
-      const float sg = sign ? sign[row] : 1.f;
-      const float yi = s * sg * sum - ratio * r1[row];
-      y[row] = yi;
-      acc += (double)(s * z[row]) * (double)yi;
+      if (!final_only) {
+        const float sg = sign ? sign[row] : 1.f;
+        const float yi = s * sg * sum - ratio * r1[row];
+        y[row] = yi;
+        acc += (double)(s * z[row]) * (double)yi;
+      }
     }
   }
   const double bs = block_sum(acc, sh);
-  double tot;
-  if (last_block_reduce(bs, partial, counter, &tot, sh) && threadIdx.x == 0) st->alfa = tot;
+  if (threadIdx.x == 0) partial_a[blockIdx.x] = bs;
 }
 
-// B(k): y -= (alpha/beta) r2, z = P^-1 y, beta' = sqrt(y.z), Givens rotation.
+// B(k): alpha_k, y -= (alpha/beta) r2, z = P^-1 y, partials of y . z.
 __global__ __launch_bounds__(kBlock) void k_minres_b(int n, float* __restrict__ y,
                                                      const float* __restrict__ r2,
                                                      const float* __restrict__ pinv,
-                                                     float* __restrict__ znew, double* partial,
-                                                     unsigned int* counter, MinresState* st) {
+                                                     float* __restrict__ znew,
+                                                     const double* __restrict__ partial_a,
+                                                     int np_a, double* __restrict__ partial_b,
+                                                     MinresState* __restrict__ st) {
   __shared__ double sh[kBlock / 64];
-  const float c = (float)(st->alfa / st->beta);
+  const double alfa = sum_partials(partial_a, np_a, sh);
+  const double beta = st->beta;
+  const float c = (float)(alfa / beta);
   double acc = 0;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     const float yi = y[i] - c * r2[i];
@@ -289,53 +297,9 @@ __global__ __launch_bounds__(kBlock) void k_minres_b(int n, float* __restrict__ 
     acc += (double)yi * (double)zi;
   }
   const double bs = block_sum(acc, sh);
-  double tot;
-  if (last_block_reduce(bs, partial, counter, &tot, sh) && threadIdx.x == 0) {
-    const double alfa = st->alfa;
-    const double oldb = st->beta;
-    const double beta = sqrt(tot > 0 ? tot : 0);
-    const double oldeps = st->epsln;
-    const double delta = st->cs * st->dbar + st->sn * alfa;
-    const double gbar = st->sn * st->dbar - st->cs * alfa;
-    const double epsln = st->sn * beta;
-    const double dbar = -st->cs * beta;
-    double gamma = hypot(gbar, beta);
-    if (gamma < 1e-300) gamma = 1e-300;
-    const double cs = gbar / gamma, sn = beta / gamma;
-    const double phi = cs * st->phibar;
-    st->phibar = sn * st->phibar;
-    st->oldb = oldb;
-    st->beta = beta;
-    st->oldeps = oldeps;
-    st->delta = delta;
-    st->epsln = epsln;
-    st->dbar = dbar;
-    st->cs = cs;
-    st->sn = sn;
-    st->phi = phi;
-    st->denom = 1.0 / gamma;
-    st->s = beta > 0 ? 1.0 / beta : 0.0;
-    st->ratio = oldb > 0 ? beta / oldb : 0.0;
-    if (!(beta > 0)) st->stop = 1;
-  }
-}
-
-// Final w/x update after the last B.
-__global__ __launch_bounds__(kBlock) void k_minres_final(int n, const float* __restrict__ zprev,
-                                                         float* __restrict__ x,
-                                                         float* __restrict__ wnew,
-                                                         const float* __restrict__ wa,
-                                                         const float* __restrict__ wb,
-                                                         const MinresState* st) {
-  const float sprev = st->oldb > 0 ? (float)(1.0 / st->oldb) : 0.f;
-  const float oldeps = (float)st->oldeps, delta = (float)st->delta, denom = (float)st->denom,
-              phi = (float)st->phi;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-    const float v = zprev[i] * sprev;
-    const float w = (v - oldeps * wa[i] - delta * wb[i]) * denom;
-    wnew[i] = w;
-    x[i] += phi * w;
-  }
+  if (threadIdx.x == 0) partial_b[blockIdx.x] = bs;
+  // alfa is a field no block of this kernel reads (each sums the partials itself)
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->alfa = alfa;
 }
 
 // rf = sign * r / scale (FP64 -> FP32)
@@ -364,13 +328,15 @@ struct nfx_solver {
   int n = 0;
   int nnz = 0;
   hipStream_t stream = nullptr;
-  float* buf = nullptr;  // 11 vectors of n floats
+  float* buf = nullptr;  // 10 vectors of n floats
   double* r64 = nullptr;
-  double* partial = nullptr;
-  unsigned int* counter = nullptr;
-  MinresState* st = nullptr;
-  MinresState* h_st = nullptr;
-  int blocks = 1;
+  double* partial_a = nullptr;
+  double* partial_b = nullptr;
+  double* partial_r = nullptr;
+  double* h_partial = nullptr;  // pinned
+  MinresState* st = nullptr;    // 2 states
+  MinresState* h_st = nullptr;  // pinned
+  int blocks_a = 1, blocks_b = 1;
   int lanes = 4;
 };
 
@@ -414,23 +380,25 @@ extern "C" int nfx_schur_diag64(int n_rows, int n_u, const int32_t* d_rowptr, co
 }
 
 extern "C" int nfx_solver_create(nfx_solver** out, int n_rows, int nnz, void* stream) {
-  NFX_REQUIRE(out && n_rows > 0, "nfx_solver_create: bad arguments");
+  NFX_REQUIRE(out && n_rows > 0 && nnz >= 0, "nfx_solver_create: bad arguments");
   nfx_solver* s = new nfx_solver();
   s->n = n_rows;
   s->nnz = nnz;
   s->stream = static_cast<hipStream_t>(stream);
-  int blocks = grid_for(n_rows, kBlock);
-  s->blocks = blocks < kMaxBlocks ? blocks : kMaxBlocks;
-  const double avg = n_rows > 0 ? (double)nnz / n_rows : 0;
+  const double avg = (double)nnz / n_rows;
   s->lanes = avg > 24 ? 16 : (avg > 10 ? 8 : 4);
-  hipError_t e = hipMalloc(&s->buf, sizeof(float) * (size_t)n_rows * 11);
+  int ba = grid_for(n_rows, kBlock / s->lanes);
+  int bb = grid_for(n_rows, kBlock);
+  s->blocks_a = ba < kMaxBlocks ? ba : kMaxBlocks;
+  s->blocks_b = bb < kMaxBlocks ? bb : kMaxBlocks;
+  hipError_t e = hipMalloc(&s->buf, sizeof(float) * (size_t)n_rows * 10);
   if (e == hipSuccess) e = hipMalloc(&s->r64, sizeof(double) * (size_t)n_rows);
-  if (e == hipSuccess) e = hipMalloc(&s->partial, sizeof(double) * 2 * kMaxBlocks);
-  if (e == hipSuccess) e = hipMalloc(&s->counter, sizeof(unsigned int));
-  if (e == hipSuccess) e = hipMalloc(&s->st, sizeof(MinresState));
+  if (e == hipSuccess) e = hipMalloc(&s->partial_a, sizeof(double) * kMaxBlocks);
+  if (e == hipSuccess) e = hipMalloc(&s->partial_b, sizeof(double) * kMaxBlocks);
+  if (e == hipSuccess) e = hipMalloc(&s->partial_r, sizeof(double) * 2 * kMaxBlocks);
+  if (e == hipSuccess) e = hipHostMalloc(&s->h_partial, sizeof(double) * 2 * kMaxBlocks, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMalloc(&s->st, 2 * sizeof(MinresState));
   if (e == hipSuccess) e = hipHostMalloc(&s->h_st, sizeof(MinresState), hipHostMallocDefault);
-  if (e == hipSuccess) e = hipMemsetAsync(s->counter, 0, sizeof(unsigned int), s->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
   if (e != hipSuccess) {
     set_error(std::string("nfx_solver_create: ") + hipGetErrorString(e));
     nfx_solver_destroy(s);
@@ -444,8 +412,10 @@ extern "C" int nfx_solver_destroy(nfx_solver* s) {
   if (!s) return NFX_OK;
   if (s->buf) (void)hipFree(s->buf);
   if (s->r64) (void)hipFree(s->r64);
-  if (s->partial) (void)hipFree(s->partial);
-  if (s->counter) (void)hipFree(s->counter);
+  if (s->partial_a) (void)hipFree(s->partial_a);
+  if (s->partial_b) (void)hipFree(s->partial_b);
+  if (s->partial_r) (void)hipFree(s->partial_r);
+  if (s->h_partial) (void)hipHostFree(s->h_partial);
   if (s->st) (void)hipFree(s->st);
   if (s->h_st) (void)hipHostFree(s->h_st);
   delete s;
@@ -454,19 +424,16 @@ extern "C" int nfx_solver_destroy(nfx_solver* s) {
 
 static int launch_a(nfx_solver* s, const int32_t* rp, const int32_t* ci, const float* va,
                     const float* sign, const float* z, const float* zprev, const float* r1,
-                    float* y, float* x, float* wnew, const float* wa, const float* wb, int upd,
-                    int first) {
+                    float* y, float* x, float* wnew, const float* wa, const float* wb, int first,
+                    int final_only, const MinresState* sin, MinresState* sout) {
   const int n = s->n;
-  const int lanes = s->lanes;
-  int blocks = grid_for(n, kBlock / lanes);
-  if (blocks > kMaxBlocks) blocks = kMaxBlocks;
-#define NFX_A(L)                                                                                \
-  hipLaunchKernelGGL(k_minres_a<L>, dim3(blocks), dim3(kBlock), 0, s->stream, n, rp, ci, va,    \
-                     sign, z, zprev, r1, y, x, wnew, wa, wb, upd, first, s->partial, s->counter, \
-                     s->st)
-  if (lanes == 16)
+#define NFX_A(L)                                                                                  \
+  hipLaunchKernelGGL(k_minres_a<L>, dim3(s->blocks_a), dim3(kBlock), 0, s->stream, n, rp, ci, va, \
+                     sign, z, zprev, r1, y, x, wnew, wa, wb, first, final_only, s->partial_b,     \
+                     s->blocks_b, s->partial_a, sin, sout)
+  if (s->lanes == 16)
     NFX_A(16);
-  else if (lanes == 8)
+  else if (s->lanes == 8)
     NFX_A(8);
   else
     NFX_A(4);
@@ -476,6 +443,7 @@ static int launch_a(nfx_solver* s, const int32_t* rp, const int32_t* ci, const f
 }
 
 // One FP32 MINRES solve of (D A) d = rhs (rhs already scaled), result in x.
+// Buffers: r1, r2, y rotate with period 3; z with period 2; w, w1, w2 with period 3.
 static int minres_inner(nfx_solver* s, const int32_t* rp, const int32_t* ci, const float* va,
                         const float* sign, const float* pinv, const float* rhs, float* x,
                         double tol, int maxit, int* iters) {
@@ -484,50 +452,71 @@ static int minres_inner(nfx_solver* s, const int32_t* rp, const int32_t* ci, con
   float* R[3] = {V + (size_t)n * 0, V + (size_t)n * 1, V + (size_t)n * 2};
   float* Z[2] = {V + (size_t)n * 3, V + (size_t)n * 4};
   float* W[3] = {V + (size_t)n * 5, V + (size_t)n * 6, V + (size_t)n * 7};
-  hipLaunchKernelGGL(k_minres_init, dim3(s->blocks), dim3(kBlock), 0, s->stream, n, rhs, pinv,
-                     R[0], R[1], Z[1], x, W[0], W[1], W[2], s->partial, s->counter, s->st);
+  MinresState* S[2] = {s->st, s->st + 1};
+  // r1 = r2 = rhs in R[0], R[1]; z_1 in Z[1]; beta_1 partials in partial_b
+  hipLaunchKernelGGL(k_minres_init, dim3(s->blocks_b), dim3(kBlock), 0, s->stream, n, rhs, pinv,
+                     R[0], R[1], Z[1], x, W[0], W[1], W[2], s->partial_b);
   NFX_LAUNCH_CHECK();
-  // The init above put r1 = r2 = rhs into R[0], R[1] and z_1 into Z[1].
-  // Iteration k uses r1 = R[(k+2)%3], r2 = R[k%3] ... we choose the mapping below.
-  int k = 1;
   const int check = 16;
-  double beta1 = 0;
+  int k = 1;
   for (; k <= maxit; ++k) {
-    // buffers for iteration k
     float* r1 = R[(k + 2) % 3];
     float* r2 = R[k % 3];
     float* y = R[(k + 1) % 3];
     float* z = Z[k % 2];
     float* zprev = Z[(k + 1) % 2];
-    // w update for iteration k-1 writes W[(k-1)%3], reads W[k%3] (w1) and W[(k+1)%3] (w2)
-    float* wnew = W[(k + 2) % 3];
-    const float* wa = W[k % 3];
-    const float* wb = W[(k + 1) % 3];
-    int rc = launch_a(s, rp, ci, va, sign, z, zprev, r1, y, x, wnew, wa, wb, k >= 2, k == 1);
+    float* wnew = W[(k + 2) % 3];  // w_{k-1}
+    const float* wa = W[k % 3];    // w_{k-3}
+    const float* wb = W[(k + 1) % 3];  // w_{k-2}
+    int rc = launch_a(s, rp, ci, va, sign, z, zprev, r1, y, x, wnew, wa, wb, k == 1, 0,
+                      S[(k + 1) & 1], S[k & 1]);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_minres_b, dim3(s->blocks), dim3(kBlock), 0, s->stream, n, y, r2, pinv,
-                       zprev, s->partial, s->counter, s->st);
+    hipLaunchKernelGGL(k_minres_b, dim3(s->blocks_b), dim3(kBlock), 0, s->stream, n, y, r2, pinv,
+                       zprev, s->partial_a, s->blocks_a, s->partial_b, S[k & 1]);
     NFX_LAUNCH_CHECK();
     if (k % check == 0 || k == maxit) {
-      NFX_CHECK_HIP(hipMemcpyAsync(s->h_st, s->st, sizeof(MinresState), hipMemcpyDeviceToHost,
+      NFX_CHECK_HIP(hipMemcpyAsync(s->h_st, S[k & 1], sizeof(MinresState), hipMemcpyDeviceToHost,
                                    s->stream));
       NFX_CHECK_HIP(hipStreamSynchronize(s->stream));
-      beta1 = s->h_st->beta1;
+      const double beta1 = s->h_st->beta1;
+      // phibar here is the estimate after iteration k-1
       if (s->h_st->stop || !(beta1 > 0) || s->h_st->phibar <= tol * beta1) break;
     }
   }
   if (k > maxit) k = maxit;
-  // final update for iteration k: v_k = z_k / beta_k; z_k is Z[k%2]
+  // final: rotation of iteration k and its w/x update (an A kernel without SpMV)
   {
-    float* zk = Z[k % 2];
-    float* wnew = W[k % 3];
-    const float* wa = W[(k + 1) % 3];
-    const float* wb = W[(k + 2) % 3];
-    hipLaunchKernelGGL(k_minres_final, dim3(s->blocks), dim3(kBlock), 0, s->stream, n, zk, x, wnew,
-                       wa, wb, s->st);
-    NFX_LAUNCH_CHECK();
+    const int kk = k + 1;
+    float* z = Z[kk % 2];
+    float* zprev = Z[(kk + 1) % 2];
+    float* wnew = W[(kk + 2) % 3];
+    const float* wa = W[kk % 3];
+    const float* wb = W[(kk + 1) % 3];
+    int rc = launch_a(s, rp, ci, va, sign, z, zprev, nullptr, nullptr, x, wnew, wa, wb, 0, 1,
+                      S[(kk + 1) & 1], S[kk & 1]);
+    if (rc) return rc;
   }
   *iters = k;
+  return NFX_OK;
+}
+
+// r = b - A x in FP64; returns ||r||^2 and ||b||^2 (host sum of block partials).
+static int residual64(nfx_solver* s, const int32_t* rp, const int32_t* ci, const double* va,
+                      const double* x, const double* b, double* rr, double* bb) {
+  const int blocks = s->blocks_b;
+  hipLaunchKernelGGL(k_residual64, dim3(blocks), dim3(kBlock), 0, s->stream, s->n, rp, ci, va, x,
+                     b, s->r64, s->partial_r);
+  NFX_LAUNCH_CHECK();
+  NFX_CHECK_HIP(hipMemcpyAsync(s->h_partial, s->partial_r, sizeof(double) * 2 * blocks,
+                               hipMemcpyDeviceToHost, s->stream));
+  NFX_CHECK_HIP(hipStreamSynchronize(s->stream));
+  double a = 0, c = 0;
+  for (int i = 0; i < blocks; ++i) {
+    a += s->h_partial[2 * i];
+    c += s->h_partial[2 * i + 1];
+  }
+  *rr = a;
+  *bb = c;
   return NFX_OK;
 }
 
@@ -539,7 +528,7 @@ extern "C" int nfx_solve_minres_ir(nfx_solver* s, const int32_t* d_rowptr, const
                                    int hist_cap) {
   NFX_REQUIRE(s && d_rowptr && d_col && d_val32 && d_val64 && d_pinv && d_b && d_x,
               "nfx_solve_minres_ir: null pointer");
-  NFX_REQUIRE(max_inner > 0 && max_outer > 0, "nfx_solve_minres_ir: iteration limits");
+  NFX_REQUIRE(max_inner > 0 && max_outer >= 0, "nfx_solve_minres_ir: iteration limits");
   const int n = s->n;
   float* rf = s->buf + (size_t)n * 8;
   float* dx = s->buf + (size_t)n * 9;
@@ -547,21 +536,18 @@ extern "C" int nfx_solve_minres_ir(nfx_solver* s, const int32_t* d_rowptr, const
   double rel = 0;
   const int g = grid_for(n, kBlock);
   for (; outer <= max_outer; ++outer) {
-    hipLaunchKernelGGL(k_residual64, dim3(s->blocks), dim3(kBlock), 0, s->stream, n, d_rowptr,
-                       d_col, d_val64, d_x, d_b, s->r64, s->partial, s->counter, s->st);
-    NFX_LAUNCH_CHECK();
-    NFX_CHECK_HIP(hipMemcpyAsync(s->h_st, s->st, sizeof(MinresState), hipMemcpyDeviceToHost,
-                                 s->stream));
-    NFX_CHECK_HIP(hipStreamSynchronize(s->stream));
-    const double rn = sqrt(s->h_st->rnorm2), bn = sqrt(s->h_st->bnorm2);
+    double rr = 0, bb = 0;
+    int rc = residual64(s, d_rowptr, d_col, d_val64, d_x, d_b, &rr, &bb);
+    if (rc) return rc;
+    const double rn = sqrt(rr), bn = sqrt(bb);
     rel = bn > 0 ? rn / bn : rn;
     if (h_hist && outer < hist_cap) h_hist[outer] = rel;
     if (rel <= rtol || outer == max_outer || rn == 0) break;
     hipLaunchKernelGGL(k_to_f32, dim3(g), dim3(kBlock), 0, s->stream, n, s->r64, d_sign, rn, rf);
     NFX_LAUNCH_CHECK();
     int it = 0;
-    int rc = minres_inner(s, d_rowptr, d_col, d_val32, d_sign, d_pinv, rf, dx, inner_rtol,
-                          max_inner, &it);
+    rc = minres_inner(s, d_rowptr, d_col, d_val32, d_sign, d_pinv, rf, dx, inner_rtol, max_inner,
+                      &it);
     if (rc) return rc;
     total += it;
     hipLaunchKernelGGL(k_axpy64, dim3(g), dim3(kBlock), 0, s->stream, n, rn, dx, d_x);
@@ -581,22 +567,14 @@ extern "C" int nfx_residual64(int n_rows, const int32_t* d_rowptr, const int32_t
                               double* d_r, void* stream) {
   NFX_REQUIRE(n_rows > 0, "nfx_residual64: n_rows");
   NFX_REQUIRE(d_rowptr && d_col && d_val && d_x && d_b && d_r, "nfx_residual64: null pointer");
-  // standalone residual: no norms; reuse the CSR kernel with a private scratch
-  double* partial = nullptr;
-  unsigned int* counter = nullptr;
-  MinresState* st = nullptr;
   hipStream_t sm = static_cast<hipStream_t>(stream);
-  NFX_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&partial), sizeof(double) * 2 * kMaxBlocks, sm));
-  NFX_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&counter), sizeof(unsigned int), sm));
-  NFX_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&st), sizeof(MinresState), sm));
-  NFX_CHECK_HIP(hipMemsetAsync(counter, 0, sizeof(unsigned int), sm));
+  double* partial = nullptr;
   int blocks = grid_for(n_rows, kBlock);
   if (blocks > kMaxBlocks) blocks = kMaxBlocks;
+  NFX_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&partial), sizeof(double) * 2 * blocks, sm));
   hipLaunchKernelGGL(k_residual64, dim3(blocks), dim3(kBlock), 0, sm, n_rows, d_rowptr, d_col,
-                     d_val, d_x, d_b, d_r, partial, counter, st);
+                     d_val, d_x, d_b, d_r, partial);
   NFX_LAUNCH_CHECK();
   NFX_CHECK_HIP(hipFreeAsync(partial, sm));
-  NFX_CHECK_HIP(hipFreeAsync(counter, sm));
-  NFX_CHECK_HIP(hipFreeAsync(st, sm));
   return NFX_OK;
 }

@@ -1,6 +1,6 @@
 // FP64 error norms: per-element squared L2 / div errors of a discrete field
 // against exact values at the element quadrature points, reduced
-// deterministically (block partials + last-block sum in fixed order).
+// deterministically (block partials summed on the host in block order).
 //
 // Vector (H(div)) field with n_comp rows:
 //   f_h[c][d](x_q) = sum_a alpha[row(a, c)] * sign * (J phi_a(q))_d / detJ
@@ -9,6 +9,8 @@
 // div: f_h[c] = sum_a alpha[row(a, c)] * sign * div_a(q) / detJ.
 // tables64: w[NQ] | phi[NQ][NB][DIM] | div[NQ][NB]   (vector)
 //           w[NQ] | psi[NQ][NB]                      (scalar)
+#include <vector>
+
 #include "nfx_common.h"
 
 namespace nfx {
@@ -37,7 +39,7 @@ __global__ __launch_bounds__(kNB) void k_field_error(
     int kind, int dim, int nb, int nq, int nc, int n_elem, const int32_t* __restrict__ cells,
     const double* __restrict__ coords, const double* __restrict__ tab,
     const int32_t* __restrict__ rows, const double* __restrict__ alpha,
-    const double* __restrict__ exact, double* partial, unsigned int* counter, double* out) {
+    const double* __restrict__ exact, double* __restrict__ partial) {
   __shared__ double sh[kNB / 64];
   double acc = 0;
   for (int e = blockIdx.x * kNB + threadIdx.x; e < n_elem; e += gridDim.x * kNB) {
@@ -108,23 +110,7 @@ __global__ __launch_bounds__(kNB) void k_field_error(
     }
   }
   const double bs = nrm_block_sum(acc, sh);
-  __shared__ bool is_last;
-  if (threadIdx.x == 0) {
-    partial[blockIdx.x] = bs;
-    __threadfence();
-    is_last = atomicAdd(counter, 1u) == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
-  double v = 0;
-  for (int i = threadIdx.x; i < (int)gridDim.x; i += kNB)
-    v += __hip_atomic_load(&partial[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  v = nrm_block_sum(v, sh);
-  if (threadIdx.x == 0) {
-    *out = v;
-    *counter = 0u;
-  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = bs;
 }
 
 }  // namespace nfx
@@ -149,23 +135,22 @@ extern "C" int nfx_field_error(int kind, int dim, int nb, int nq, int n_comp, in
   hipStream_t s = static_cast<hipStream_t>(stream);
   int blocks = grid_for(n_elem, kNB);
   if (blocks > 1024) blocks = 1024;
-  double* scratch = nullptr;
-  NFX_CHECK_HIP(hipMalloc(&scratch, sizeof(double) * (1024 + 2)));
-  unsigned int* counter = reinterpret_cast<unsigned int*>(scratch + 1025);
-  double* out = scratch + 1024;
-  hipError_t e = hipMemsetAsync(counter, 0, sizeof(unsigned int), s);
-  if (e == hipSuccess) {
-    hipLaunchKernelGGL(k_field_error, dim3(blocks), dim3(kNB), 0, s, kind, dim, nb, nq, n_comp,
-                       n_elem, d_cells, d_coords64, d_tables64, d_rows, d_alpha, d_exact, scratch,
-                       counter, out);
-    e = hipGetLastError();
-  }
-  if (e == hipSuccess) e = hipMemcpyAsync(h_out, out, sizeof(double), hipMemcpyDeviceToHost, s);
+  double* partial = nullptr;
+  NFX_CHECK_HIP(hipMalloc(&partial, sizeof(double) * blocks));
+  hipLaunchKernelGGL(k_field_error, dim3(blocks), dim3(kNB), 0, s, kind, dim, nb, nq, n_comp, n_elem,
+                     d_cells, d_coords64, d_tables64, d_rows, d_alpha, d_exact, partial);
+  hipError_t e = hipGetLastError();
+  std::vector<double> h(blocks);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(h.data(), partial, sizeof(double) * blocks, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
-  (void)hipFree(scratch);
+  (void)hipFree(partial);
   if (e != hipSuccess) {
     set_error(std::string("nfx_field_error: ") + hipGetErrorString(e));
     return NFX_E_HIP;
   }
+  double acc = 0;
+  for (int i = 0; i < blocks; ++i) acc += h[i];  // block order: deterministic
+  *h_out = acc;
   return NFX_OK;
 }

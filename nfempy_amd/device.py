@@ -297,7 +297,13 @@ class DarcySystem:
     # -- warm assembly ------------------------------------------------------------
     def assemble(self, kappa, rhs, p_dirichlet=None, fp64=False, stream=None):
         """Values + residual rg (the reference's ``rg``, so b = -rg)."""
-        t = torch()
+        vals, rg = self.assemble_elements(kappa, rhs, fp64=fp64, stream=stream)
+        if p_dirichlet is not None:
+            self.assemble_dirichlet(p_dirichlet, fp64=fp64, stream=stream)
+        return vals, rg
+
+    def assemble_elements(self, kappa, rhs, fp64=False, stream=None):
+        """Element kernels + staged-segment reduction; zeroes rg first."""
         s = nat.stream_handle(stream)
         vals = self.vals64 if fp64 else self.vals32
         stage = self.stage64 if fp64 else self.stage32
@@ -307,30 +313,87 @@ class DarcySystem:
         fa = rhs if isinstance(rhs, CoefficientArg) else coefficient(rhs, self.xq, self.nq, device=self.device)
         check_coefficient(ka, self.E, self.nq, "kappa")
         check_coefficient(fa, self.E, self.nq, "rhs")
-        nat.call("nfx_fill", int(fp64), self.n, 0.0, nat.ptr(rg), s)
         coords = self.coords64 if fp64 else self.coords
+        nat.call("nfx_fill", int(fp64), self.n, 0.0, nat.ptr(rg), s)
         nat.call("nfx_assemble_darcy", self.dim, self.k, self.E, nat.ptr(self.cells), nat.ptr(coords),
                  nat.ptr(tables), nat.ptr(ka.get(fp64)), ka.se, ka.sq, nat.ptr(fa.get(fp64)), fa.se, fa.sq,
                  nat.ptr(self.slot), nat.ptr(self.rhs_row), int(fp64), nat.ptr(vals), nat.ptr(stage),
                  nat.ptr(rg), s)
         nat.call("nfx_stage_reduce", len(self.pattern.target), nat.ptr(self.segptr), nat.ptr(self.target),
                  int(fp64), nat.ptr(stage), nat.ptr(vals), s)
-        if p_dirichlet is not None and self.F:
-            pd = p_dirichlet if isinstance(p_dirichlet, CoefficientArg) else coefficient(
-                p_dirichlet, self.bc_xq, self.nqf, device=self.device)
-            pdv = pd.get(fp64)
-            if pd.se == 0:
-                pdv = pdv.expand(self.F * self.nqf).contiguous()
-            elif pd.sq == 0:
-                pdv = pdv.reshape(-1, 1).expand(self.F, self.nqf).contiguous()
-            if pdv.numel() != self.F * self.nqf:
-                raise ValueError("Dirichlet data must have one value per facet point")
-            nat.call("nfx_assemble_darcy_bc", self.dim, self.k, self.F, nat.ptr(self.bc_facet),
-                     nat.ptr(self.bc_neigh), nat.ptr(self.bc_combo), nat.ptr(self.cells), nat.ptr(coords),
-                     nat.ptr(self.bc_trace64 if fp64 else self.bc_trace32),
-                     nat.ptr(self.bc_fw64 if fp64 else self.bc_fw32), self.nqf, nat.ptr(pdv),
-                     nat.ptr(self.bc_rows), int(fp64), nat.ptr(rg), s)
         return vals, rg
+
+    def dirichlet_values(self, p_dirichlet):
+        """Dirichlet data at facet points as a CoefficientArg with one value per point."""
+        pd = p_dirichlet if isinstance(p_dirichlet, CoefficientArg) else coefficient(
+            p_dirichlet, self.bc_xq, self.nqf, device=self.device)
+        if pd.se == 0 or pd.sq == 0:
+            t = torch()
+            base = pd.get(True)
+            full = base.expand(self.F * self.nqf) if pd.se == 0 else base.reshape(-1, 1).expand(self.F, self.nqf)
+            pd = CoefficientArg(full.reshape(-1).contiguous(), self.nqf, 1)
+        if pd.arr.numel() != self.F * self.nqf:
+            raise ValueError("Dirichlet data must have one value per facet point")
+        return pd
+
+    def assemble_dirichlet(self, p_dirichlet, fp64=False, stream=None, rows=None, out=None):
+        """Facet term into rg (each boundary facet owns its rows)."""
+        if not self.F:
+            return
+        s = nat.stream_handle(stream)
+        pd = self.dirichlet_values(p_dirichlet)
+        rg = out if out is not None else (self.rhs64 if fp64 else self.rhs32)
+        coords = self.coords64 if fp64 else self.coords
+        nat.call("nfx_assemble_darcy_bc", self.dim, self.k, self.F, nat.ptr(self.bc_facet),
+                 nat.ptr(self.bc_neigh), nat.ptr(self.bc_combo), nat.ptr(self.cells), nat.ptr(coords),
+                 nat.ptr(self.bc_trace64 if fp64 else self.bc_trace32),
+                 nat.ptr(self.bc_fw64 if fp64 else self.bc_fw32), self.nqf, nat.ptr(pd.get(fp64)),
+                 nat.ptr(self.bc_rows if rows is None else rows), int(fp64), nat.ptr(rg), s)
+
+    # -- element-local matrices (the reference's evaluate_form API) ----------------
+    def local_matrices(self, kappa, rhs, fp64=True):
+        """All element matrices / residuals in the reference's element-local order.
+
+        Returns numpy (E, nl, nl) and (E, nl): what ``LaplaceDualWeakForm.evaluate_form(i, alpha)``
+        returns for each element i (``src/weak_forms/laplace_dual_weak_form.py:11-87``).
+        The same kernel runs with slot codes that address a dense per-element array.
+        """
+        from .pattern import local_slot_map
+
+        t = torch()
+        if not hasattr(self, "_local_slot"):
+            loc = self.local_index_of_reference()
+            self._local_slot = to_dev(local_slot_map(loc, self.ref_sign, self.nl), device=self.device)
+            rows = (np.arange(self.E)[:, None] * self.nl + loc[:, self.nb:])
+            self._local_rows = to_dev(np.ascontiguousarray(rows.T).astype(np.int32), device=self.device)
+        dt = t.float64 if fp64 else t.float32
+        out = t.zeros(self.E * self.nl * self.nl, dtype=dt, device=self.device)
+        res = t.zeros(self.E * self.nl, dtype=dt, device=self.device)
+        ka = kappa if isinstance(kappa, CoefficientArg) else coefficient(kappa, self.xq, self.nq, invert=True, device=self.device)
+        fa = rhs if isinstance(rhs, CoefficientArg) else coefficient(rhs, self.xq, self.nq, device=self.device)
+        check_coefficient(ka, self.E, self.nq, "kappa")
+        check_coefficient(fa, self.E, self.nq, "rhs")
+        s = nat.stream_handle()
+        nat.call("nfx_assemble_darcy", self.dim, self.k, self.E, nat.ptr(self.cells),
+                 nat.ptr(self.coords64 if fp64 else self.coords), nat.ptr(self.tables64 if fp64 else self.tables32),
+                 nat.ptr(ka.get(fp64)), ka.se, ka.sq, nat.ptr(fa.get(fp64)), fa.se, fa.sq,
+                 nat.ptr(self._local_slot), nat.ptr(self._local_rows), int(fp64), nat.ptr(out), nat.ptr(out),
+                 nat.ptr(res), s)
+        return (out.cpu().numpy().reshape(self.E, self.nl, self.nl), res.cpu().numpy().reshape(self.E, self.nl))
+
+    def local_index_of_reference(self):
+        """(E, nl): element-local index of every reference basis function (fields stacked)."""
+        parts = []
+        off = 0
+        for name in self.space.names:
+            sp = self.space.discrete_spaces[name]
+            nb = sp.element.ndofs
+            nc = sp.n_comp
+            inv = np.argsort(sp.local_src, axis=1)  # reference a -> local i
+            loc = (inv[:, :, None] * nc + np.arange(nc)[None, None, :]).reshape(len(inv), nb * nc) + off
+            parts.append(loc)
+            off += nb * nc
+        return np.concatenate(parts, axis=1)
 
     def csr_numpy(self, fp64=True):
         import scipy.sparse as sps

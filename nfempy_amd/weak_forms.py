@@ -1,0 +1,147 @@
+"""Weak forms: the drop-in surface of ``src/weak_forms`` for the hot path.
+
+``WeakForm(space)`` with ``.space``, ``.functions`` and
+``evaluate_form(element_index, alpha) -> (r_el, j_el)`` as in
+``src/weak_forms/weak_from.py:6-29``; ``alpha`` is accepted and ignored exactly
+where the reference ignores it (the mixed Darcy form resets it to zero,
+``laplace_dual_weak_form.py:46``; the forms are linear).
+
+Unlike the reference, the element kernels run for all elements at once on the
+GPU: ``evaluate_form`` serves element ``i`` from one cached batched evaluation,
+and ``scatter(jacobian, residual)`` (used by ``SequentialAssembler``) writes the
+whole CSR system in HBM through the sorted-segment scatter.
+
+Forms:
+- ``LaplaceDualWeakForm`` / ``LaplaceDualWeakFormBCDirichlet``
+  (``src/weak_forms/laplace_dual_weak_form.py:10-154``), RT_k - P_{k-1}, k = 1, 2.
+- ``LEDualWeakForm`` / ``LEDualWeakFormBCDirichlet`` / ``LERieszMapWeakForm``
+  (``src/weak_forms/le_dual_weak_form.py``, ``le_riesz_map_weak_form.py``),
+  BDM1 - P0 - P0 in 3-D (see ``nfempy_amd.elasticity``).
+"""
+
+from __future__ import annotations
+
+from abc import ABC, abstractmethod
+
+import numpy as np
+
+
+class WeakForm(ABC):
+    def __init__(self, space):
+        self.space = space
+        self._functions = None
+        self._cache = None
+
+    @property
+    def space(self):
+        return self._space
+
+    @space.setter
+    def space(self, space):
+        self._space = space
+        self._cache = None
+
+    @property
+    def functions(self):
+        return self._functions
+
+    @functions.setter
+    def functions(self, functions):
+        self._functions = functions
+        self._cache = None
+
+    @abstractmethod
+    def evaluate_form(self, element_index, alpha):
+        pass
+
+
+def _darcy_system(space):
+    from .device import DarcySystem
+
+    sysd = getattr(space, "_nfx_darcy_system", None)
+    if sysd is None:
+        sysd = DarcySystem(space)
+        space._nfx_darcy_system = sysd
+    return sysd
+
+
+class LaplaceDualWeakForm(WeakForm):
+    """Mixed Darcy element: ``functions = {"rhs": f, "kappa": K}``."""
+
+    def _check(self):
+        if self.space is None or self.functions is None:
+            raise ValueError
+
+    def evaluate_form(self, element_index, alpha=None):
+        self._check()
+        if self._cache is None:
+            sysd = _darcy_system(self.space)
+            J, r = sysd.local_matrices(self.functions["kappa"], self.functions["rhs"], fp64=True)
+            self._cache = (J, r)
+        J, r = self._cache
+        return r[element_index].copy(), J[element_index].copy()
+
+    def scatter(self, jacobian, residual):
+        """Assemble every element into ``jacobian`` (a ``Mat``) and ``residual`` (numpy rg).
+
+        FP64 and FP32 values are both produced (the FP64 copy judges the solve)."""
+        self._check()
+        sysd = _darcy_system(self.space)
+        sysd.assemble_elements(self.functions["kappa"], self.functions["rhs"], fp64=True)
+        sysd.assemble_elements(self.functions["kappa"], self.functions["rhs"], fp64=False)
+        if jacobian is not None:
+            jacobian._attach(sysd)
+        if residual is not None:
+            residual += sysd.rhs64.cpu().numpy()
+        return sysd
+
+
+class LaplaceDualWeakFormBCDirichlet(WeakForm):
+    """Dirichlet pressure facet term: ``functions = {"p": p_D, ...}``."""
+
+    def evaluate_form(self, element_index, alpha=None):
+        r = self.evaluate_all()[element_index]
+        return r.copy(), np.zeros((len(r), len(r)))
+
+    def scatter(self, jacobian, residual):
+        """Facet residuals into the device rg (both precisions) and the host ``residual``."""
+        sysd = _darcy_system(self.space)
+        if not sysd.F:
+            return
+        sysd.assemble_dirichlet(self.functions["p"], fp64=True)
+        sysd.assemble_dirichlet(self.functions["p"], fp64=False)
+        if residual is not None:
+            vals = self.evaluate_all()
+            u = self.space.discrete_spaces[self.space.names[0]]
+            shift = self.space.strides[self.space.names[0]] + self.space.dof_shift
+            for j in range(vals.shape[1]):
+                np.add.at(residual, u.bc_dest[:, j] + shift, vals[:, j])
+
+    def evaluate_all(self):
+        if self._cache is None:
+            self._cache = _bc_facet_values(self.space, self.functions["p"])
+        return self._cache
+
+
+def _bc_facet_values(space, p_dirichlet):
+    """(F, fd) facet residuals in the reference's local (transformed) facet order."""
+    import torch
+
+    from . import _native as nat
+    from .device import to_dev
+
+    sysd = _darcy_system(space)
+    if not sysd.F:
+        return np.zeros((0, 0))
+    fd = sysd.bc_rows.shape[1]
+    u = space.discrete_spaces[space.names[0]]
+    # local codes: facet f, reference facet dof a -> f*fd + j(a), with the orientation sign
+    bcd = u.bc_dest
+    ref = u.bc_ref_dest
+    j_of_a = np.argmax(ref[:, :, None] == bcd[:, None, :], axis=2)
+    codes = (np.arange(sysd.F)[:, None] * fd + j_of_a).astype(np.int64)
+    codes |= np.where(u.bc_ref_sign < 0, nat.SLOT_NEG, 0)
+    d_codes = to_dev(codes.astype(np.int32))
+    out = torch.zeros(sysd.F * fd, dtype=torch.float64, device="cuda")
+    sysd.assemble_dirichlet(p_dirichlet, fp64=True, rows=d_codes, out=out)
+    return out.cpu().numpy().reshape(sysd.F, fd)

@@ -260,7 +260,7 @@ def write_msh41(path, points, blocks):
     lines.append(f"1 {nv} 1 {nv}")
     lines.append(f"3 1 0 {nv}")
     lines.extend(str(t + 1) for t in range(nv))
-    lines.extend(f"{p[0]!r} {p[1]!r} {p[2]!r}" for p in points)
+    lines.extend(f"{float(p[0])!r} {float(p[1])!r} {float(p[2])!r}" for p in points)
     lines.append("$EndNodes")
     ne = sum(len(b.data) for b in blocks)
     lines.append("$Elements")

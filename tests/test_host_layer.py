@@ -1,0 +1,104 @@
+"""Host layer of the product against the oracle (CPU only).
+
+Entity numbering, DoF destinations (element and boundary facet), the CSR
+pattern of the Darcy scatter and the reference-order destination maps are
+integer work: they must agree bit for bit. Also checks that the launch-data
+validation of ``DarcySystem`` accepts every test case and rejects corrupted
+slot codes (the host checks that keep every kernel index inside its buffer).
+"""
+
+import numpy as np
+import pytest
+
+import oracle.assemble as oa
+from tests._cases import DARCY_CASES, case_id, oracle_darcy_space, product_darcy_space
+
+
+@pytest.mark.parametrize("case", DARCY_CASES, ids=[case_id(c) for c in DARCY_CASES])
+def test_destinations_and_pattern(case):
+    from nfempy_amd.pattern import build_pattern
+
+    ops = oracle_darcy_space(*case)
+    pps = product_darcy_space(*case)
+    assert ops.n_dof == pps.n_dof
+    E = len(ops.spaces["u"].element_ids)
+    assert np.array_equal(np.array(ops.spaces["u"].element_ids), pps.discrete_spaces["u"].element_ids)
+    dest_o = np.array([ops.destination_indexes(i) for i in range(E)])
+    assert np.array_equal(dest_o, pps.all_destination_indexes())
+    nb = len(ops.spaces["u"].bc_element_ids)
+    assert np.array_equal(np.array(ops.spaces["u"].bc_element_ids), pps.discrete_spaces["u"].bc_element_ids)
+    for j in range(nb):
+        assert np.array_equal(ops.bc_destination_indexes(j), pps.bc_destination_indexes(j))
+    ref_dest, ref_sign = pps.reference_destinations()
+    pat = build_pattern(pps.n_dof, ref_dest, ref_sign)
+    rows = np.repeat(dest_o, dest_o.shape[1], axis=1).ravel()
+    cols = np.tile(dest_o, (1, dest_o.shape[1])).ravel()
+    ip, ix, _ = oa.csr_from_triplets(ops.n_dof, rows, cols, np.ones(len(rows)))
+    assert np.array_equal(ip, pat.rowptr) and np.array_equal(ix, pat.col)
+    # every reference-order destination is a permutation of the local destinations
+    assert np.array_equal(np.sort(ref_dest, axis=1), np.sort(dest_o, axis=1))
+
+
+def test_structured_generators_match_oracle():
+    import oracle.mesh as om
+    from nfempy_amd.mesh import unit_cube, unit_square
+
+    for n in (1, 3):
+        a = om.structured_square(n)
+        b = unit_square(n)
+        assert np.allclose(a[0], b[0])
+        for x, y in zip(a[1], b[1]):
+            assert x.dim == y.dim and np.array_equal(x.data, y.data) and np.array_equal(x.physical, y.physical)
+        a = om.structured_cube(n)
+        b = unit_cube(n)
+        assert np.allclose(a[0], b[0])
+        for x, y in zip(a[1], b[1]):
+            assert x.dim == y.dim and np.array_equal(x.data, y.data) and np.array_equal(x.physical, y.physical)
+
+
+def test_entity_counts_of_benchmark_meshes():
+    from nfempy_amd.mesh import Mesh, unit_cube, unit_square
+
+    m = Mesh.from_blocks(2, *unit_square(32))
+    assert (m.n_vertices, m.n_edges, m.n_faces) == (1089, 3136, 2048)
+    m = Mesh.from_blocks(3, *unit_cube(16))
+    assert (m.n_vertices, m.n_faces, m.n_volumes) == (4913, 50688, 24576)
+
+
+def test_msh_reader_matches_oracle(fixtures_dir):
+    import os
+
+    import oracle.mesh as om
+    from nfempy_amd.mesh import read_msh41
+
+    for f in ("example_3_2d_l_1.msh", "example_3_3d_l_1.msh"):
+        a = om.read_msh41(os.path.join(fixtures_dir, "meshes", f))
+        b = read_msh41(os.path.join(fixtures_dir, "meshes", f))
+        assert np.allclose(a[0], b[0])
+        assert len(a[1]) == len(b[1])
+        for x, y in zip(a[1], b[1]):
+            assert x.dim == y.dim and np.array_equal(x.data, y.data) and np.array_equal(x.physical, y.physical)
+
+
+def test_oracle_msh_roundtrip(tmp_path):
+    import oracle.mesh as om
+
+    pts, bl = om.structured_cube(2)
+    p = tmp_path / "cube.msh"
+    om.write_msh41(str(p), pts, bl)
+    pts2, bl2 = om.read_msh41(str(p))
+    assert np.allclose(pts, pts2)
+    for x, y in zip(bl, bl2):
+        assert np.array_equal(x.data, y.data) and np.array_equal(x.physical, y.physical)
+
+
+@pytest.mark.parametrize("case", DARCY_CASES[:5], ids=[case_id(c) for c in DARCY_CASES[:5]])
+def test_launch_validation(case):
+    from nfempy_amd.device import DarcySystem
+
+    sysd = DarcySystem(product_darcy_space(*case), device="cpu")
+    bad = sysd.pattern.slot.copy()
+    bad[0, 0] = sysd.nnz + 5  # direct slot past the end of the CSR values
+    sysd.pattern.slot = bad
+    with pytest.raises(ValueError):
+        sysd._validate(sysd.u_sp.mesh)
