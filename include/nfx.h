@@ -80,6 +80,24 @@ int nfx_assemble_darcy(int dim, int k, int n_elem, const int32_t* d_cells,
                        const int32_t* d_slot, const int32_t* d_rhs_row, int fp64,
                        void* d_vals, void* d_stage, void* d_rhs, void* stream);
 
+/* Row-owner variant of nfx_assemble_darcy (same reference interfaces): one thread
+ * per CSR row gathers row a of each incident element matrix (at most two
+ * incidences, element order) and the block stores its contiguous slice of CSR
+ * values; the residual of every row is written (pressure rows: -sum w detJ f psi,
+ * flux rows: 0, before the Dirichlet facet term). No slot map, no staging.
+ *  d_inc    (2, n_rows) int32 incidences e*32 + a (reference DoF a), -1 = none
+ *  d_pos    (n_rows, 2, nl) uint8 position of reference DoF b's column in the row
+ *           (255 = not present)
+ *  d_rowptr CSR row pointer; d_signs (E) uint32, bit a = reference DoF a negated
+ * Other arguments as nfx_assemble_darcy. */
+int nfx_assemble_darcy_rows(int dim, int k, int n_rows, const int32_t* d_inc,
+                            const uint8_t* d_pos, const int32_t* d_rowptr,
+                            const uint32_t* d_signs, const int32_t* d_cells,
+                            const void* d_coords, const void* d_tables,
+                            const void* d_kinv, int kinv_stride_e, int kinv_stride_q,
+                            const void* d_f, int f_stride_e, int f_stride_q, int fp64,
+                            void* d_vals, void* d_rhs, void* stream);
+
 /* Dirichlet pressure facet term, LaplaceDualWeakFormBCDirichlet.evaluate_form
  * (src/weak_forms/laplace_dual_weak_form.py:91-154) for every boundary facet:
  * rg[row] = sum_q w |detJ_f| p_D(x_q) psi(x_q) . n with the flux basis traced from

@@ -253,6 +253,133 @@ __global__ __launch_bounds__(256) void k_darcy_bc(
   for (int a = 0; a < FD; ++a) put_row<R>(rows[(int64_t)f * FD + a], val[a], rhs);
 }
 
+// Row-owner ("gather") assembly of the mixed Darcy system.
+//
+// One thread owns one CSR row. Row r lists its incident (element, reference DoF)
+// pairs in element order (at most two: a facet DoF is shared by the two cells of
+// the facet, a cell DoF by one). For each incidence the thread evaluates row a of
+// that element's matrix in the reference basis, applies the orientation signs and
+// accumulates into the row's slice of an LDS buffer at the precomputed position of
+// every column (first incidence then second: the reference's addv order). The
+// block then stores its contiguous range of CSR values with coalesced writes.
+//   inc[2][n_rows]  packed incidence e << 5 | a, -1 = none
+//   pos[n_rows][2][NL] column position of reference DoF b inside the row (255 = none)
+//   signs[E]        bit b set: reference DoF b of element e carries a minus sign;
+//                   entry (a, b) enters with the sign (-1)^(bit a xor bit b)
+// Element row a < NB: M[a][b] (b < NB) and -B[k][a] (b = NB + k);
+// row a = NB + k: B[k][b] (b < NB), 0 (b >= NB), residual -sum_q w detJ f psi_k.
+constexpr int kRowsPerBlock = 128;
+
+template <int DIM, int NB, int NP, int NQ, typename R>
+__global__ __launch_bounds__(kRowsPerBlock) void k_darcy_rows(
+    int n_rows, const int32_t* __restrict__ inc, const uint8_t* __restrict__ pos,
+    const int32_t* __restrict__ rowptr, const uint32_t* __restrict__ signs,
+    const int32_t* __restrict__ cells, const R* __restrict__ coords, const R* __restrict__ tab,
+    const R* __restrict__ kinv, int ks_e, int ks_q, const R* __restrict__ f, int fs_e, int fs_q,
+    R* __restrict__ vals, R* __restrict__ rhs) {
+  constexpr int NL = NB + NP;
+  constexpr int NT = NQ * (1 + NB * DIM + NB + NP);
+  __shared__ R st[NT];
+  __shared__ R sv[kRowsPerBlock * 2 * NL];
+  const int r0 = blockIdx.x * kRowsPerBlock;
+  const int r = r0 + threadIdx.x;
+  const int rl = min(r0 + kRowsPerBlock, n_rows);
+  for (int i = threadIdx.x; i < NT; i += kRowsPerBlock) st[i] = tab[i];
+  const int base = rowptr[r0];
+  const int span = rowptr[rl] - base;
+  for (int i = threadIdx.x; i < span; i += kRowsPerBlock) sv[i] = R(0);
+  __syncthreads();
+  const R* tw = st;
+  const R* tphi = st + NQ;
+  const R* tdiv = tphi + NQ * NB * DIM;
+  const R* tpsi = tdiv + NQ * NB;
+  if (r < n_rows) {
+    const int off = rowptr[r] - base;
+    R res = 0;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int code = inc[(int64_t)j * n_rows + r];
+      if (code < 0) continue;
+      const int e = code >> 5;
+      const int a = code & 31;
+      const uint32_t sg = signs[e] ^ (((signs[e] >> a) & 1u) ? 0xffffffffu : 0u);
+      R row[NL];
+      if (a < NB) {
+        Geom<DIM, R> g;
+        R X[DIM + 1][3];
+        load_geometry<DIM, R>(cells, coords, e, g, X);
+        const R invdet = R(1) / g.detJ;
+#pragma unroll
+        for (int b = 0; b < NL; ++b) row[b] = 0;
+        for (int q = 0; q < NQ; ++q) {
+          const R w = tw[q];
+          const R cm = w * kinv[(int64_t)e * ks_e + q * ks_q] * invdet;
+          R ga[DIM];
+#pragma unroll
+          for (int d = 0; d < DIM; ++d) {
+            R s = 0;
+#pragma unroll
+            for (int d2 = 0; d2 < DIM; ++d2) s += g.G[d][d2] * tphi[(q * NB + a) * DIM + d2];
+            ga[d] = s * cm;
+          }
+#pragma unroll
+          for (int b = 0; b < NB; ++b) {
+            R s = 0;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) s += ga[d] * tphi[(q * NB + b) * DIM + d];
+            row[b] += s;
+          }
+          const R wd = w * tdiv[q * NB + a];
+#pragma unroll
+          for (int k = 0; k < NP; ++k) row[NB + k] -= wd * tpsi[q * NP + k];
+        }
+      } else {
+        const int k = a - NB;
+#pragma unroll
+        for (int b = 0; b < NL; ++b) row[b] = 0;
+        R detJ = 0;
+        {
+          Geom<DIM, R> g;
+          R X[DIM + 1][3];
+          load_geometry<DIM, R>(cells, coords, e, g, X);
+          detJ = g.detJ;
+        }
+        for (int q = 0; q < NQ; ++q) {
+          const R wp = tw[q] * tpsi[q * NP + k];
+#pragma unroll
+          for (int b = 0; b < NB; ++b) row[b] += wp * tdiv[q * NB + b];
+          res -= wp * detJ * f[(int64_t)e * fs_e + q * fs_q];
+        }
+      }
+      const uint8_t* pr = pos + ((int64_t)r * 2 + j) * NL;
+#pragma unroll
+      for (int b = 0; b < NL; ++b) {
+        const int p = pr[b];
+        if (p == 255) continue;
+        sv[off + p] += ((sg >> b) & 1u) ? -row[b] : row[b];
+      }
+    }
+    rhs[r] = res;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < span; i += kRowsPerBlock) vals[base + i] = sv[i];
+}
+
+template <int DIM, int NB, int NP, int NQ, typename R>
+static int launch_darcy_rows(int n_rows, const int32_t* inc, const uint8_t* pos,
+                             const int32_t* rowptr, const uint32_t* signs, const int32_t* cells,
+                             const void* coords, const void* tab, const void* kinv, int kse,
+                             int ksq, const void* f, int fse, int fsq, void* vals, void* rhs,
+                             hipStream_t s) {
+  hipLaunchKernelGGL((k_darcy_rows<DIM, NB, NP, NQ, R>), dim3(grid_for(n_rows, kRowsPerBlock)),
+                     dim3(kRowsPerBlock), 0, s, n_rows, inc, pos, rowptr, signs, cells,
+                     static_cast<const R*>(coords), static_cast<const R*>(tab),
+                     static_cast<const R*>(kinv), kse, ksq, static_cast<const R*>(f), fse, fsq,
+                     static_cast<R*>(vals), static_cast<R*>(rhs));
+  NFX_LAUNCH_CHECK();
+  return NFX_OK;
+}
+
 // vals[target[m]] = sum of stage[segptr[m] .. segptr[m+1]) in order.
 template <typename R>
 __global__ __launch_bounds__(256) void k_stage_reduce(int n_seg, const int32_t* __restrict__ segptr,
@@ -427,4 +554,33 @@ extern "C" int nfx_quadrature_points(int dim, int n_elem, int nq, const int32_t*
                      n_elem, nq, d_cells, d_coords64, d_bary, d_out);
   NFX_LAUNCH_CHECK();
   return NFX_OK;
+}
+
+extern "C" int nfx_assemble_darcy_rows(int dim, int k, int n_rows, const int32_t* d_inc,
+                                       const uint8_t* d_pos, const int32_t* d_rowptr,
+                                       const uint32_t* d_signs, const int32_t* d_cells,
+                                       const void* d_coords, const void* d_tables,
+                                       const void* d_kinv, int kinv_stride_e, int kinv_stride_q,
+                                       const void* d_f, int f_stride_e, int f_stride_q, int fp64,
+                                       void* d_vals, void* d_rhs, void* stream) {
+  NFX_REQUIRE(n_rows >= 0, "nfx_assemble_darcy_rows: n_rows < 0");
+  if (n_rows == 0) return NFX_OK;
+  NFX_REQUIRE(d_inc && d_pos && d_rowptr && d_signs && d_cells && d_coords && d_tables && d_kinv &&
+                  d_f && d_vals && d_rhs,
+              "nfx_assemble_darcy_rows: null pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+#define NFX_DROWS(DIM, NB, NP, NQ)                                                                 \
+  return fp64 ? launch_darcy_rows<DIM, NB, NP, NQ, double>(                                        \
+                    n_rows, d_inc, d_pos, d_rowptr, d_signs, d_cells, d_coords, d_tables, d_kinv,  \
+                    kinv_stride_e, kinv_stride_q, d_f, f_stride_e, f_stride_q, d_vals, d_rhs, s)   \
+              : launch_darcy_rows<DIM, NB, NP, NQ, float>(                                         \
+                    n_rows, d_inc, d_pos, d_rowptr, d_signs, d_cells, d_coords, d_tables, d_kinv,  \
+                    kinv_stride_e, kinv_stride_q, d_f, f_stride_e, f_stride_q, d_vals, d_rhs, s)
+  if (dim == 2 && k == 1) NFX_DROWS(2, 3, 1, 4);
+  if (dim == 2 && k == 2) NFX_DROWS(2, 8, 3, 9);
+  if (dim == 3 && k == 1) NFX_DROWS(3, 4, 1, 8);
+  if (dim == 3 && k == 2) NFX_DROWS(3, 15, 4, 27);
+#undef NFX_DROWS
+  set_error("nfx_assemble_darcy_rows: unsupported (dim, k)");
+  return NFX_E_UNSUPPORTED;
 }

@@ -232,7 +232,46 @@ class DarcySystem:
         self.rhs32 = t.zeros(self.n, dtype=t.float32, device=device)
         self.rhs64 = t.zeros(self.n, dtype=t.float64, device=device)
         self._setup_bc(mesh, D, device)
+        self._setup_rows(device)
         self._validate(mesh)
+
+    def _setup_rows(self, device):
+        """Row-owner gather data: incidences, column positions, sign masks."""
+        n, nl, E = self.n, self.nl, self.E
+        flat = self.ref_dest.reshape(-1)
+        order = np.argsort(flat, kind="stable")  # element order inside each row
+        rows_sorted = flat[order]
+        counts = np.bincount(rows_sorted, minlength=n)
+        if counts.min() < 1 or counts.max() > 2:
+            raise ValueError("row-owner assembly needs 1 or 2 incidences per row")
+        first = np.concatenate([[0], np.cumsum(counts)[:-1]])
+        inc = np.full((2, n), -1, np.int64)
+        inc[0] = order[first]
+        two = counts == 2
+        inc[1, two] = order[first[two] + 1]
+        e_of = np.where(inc >= 0, inc // nl, 0)
+        a_of = np.where(inc >= 0, inc % nl, 0)
+        packed = np.where(inc >= 0, e_of * 32 + a_of, -1)
+        # column positions of every reference DoF b of each incident element, inside row r
+        rowptr = self.pattern.rowptr.astype(np.int64)
+        keys = np.repeat(np.arange(n, dtype=np.int64), np.diff(rowptr)) * n + self.pattern.col
+        pos = np.full((n, 2, nl), 255, np.uint8)
+        if (np.diff(rowptr).max(initial=0)) >= 255:
+            raise ValueError("row too long for uint8 column positions")
+        for j in range(2):
+            live = inc[j] >= 0
+            r = np.flatnonzero(live)
+            cols = self.ref_dest[e_of[j, live]]  # (m, nl)
+            q = r[:, None] * n + cols
+            idx = np.searchsorted(keys, q)
+            if not np.array_equal(keys[np.minimum(idx, len(keys) - 1)], q):
+                raise ValueError("incident column missing from the pattern")
+            pos[r, j, :] = (idx - rowptr[r][:, None]).astype(np.uint8)
+        signs = ((self.ref_sign < 0).astype(np.uint64) << np.arange(nl, dtype=np.uint64)[None, :]).sum(axis=1)
+        self.row_inc_np, self.row_pos_np = packed.astype(np.int32), pos
+        self.row_inc = to_dev(np.ascontiguousarray(packed.astype(np.int32)), device=device)
+        self.row_pos = to_dev(pos.reshape(-1), device=device)
+        self.row_signs = to_dev(signs.astype(np.uint32).view(np.int32), device=device)
 
     def _validate(self, mesh):
         """Host-side checks that every kernel index stays inside its buffer."""
@@ -254,6 +293,16 @@ class DarcySystem:
         _check_codes(self.ref_dest[:, self.nb:], self.n, 0, "pressure rows")
         if len(pat.target) and (pat.target.max() >= pat.nnz or pat.segptr[-1] != pat.n_stage):
             raise ValueError("stage segments out of range")
+        inc = self.row_inc_np
+        live = inc >= 0
+        if np.any((inc[live] >> 5) >= self.E) or np.any((inc[live] & 31) >= self.nl) or not np.all(live[0]):
+            raise ValueError("row incidences out of range")
+        lens = np.diff(pat.rowptr)
+        if lens.max(initial=0) > 2 * self.nl:
+            raise ValueError("row longer than the row kernel's LDS slice (2 * n_loc)")
+        pp = self.row_pos_np
+        if np.any((pp != 255) & (pp >= lens[:, None, None])):
+            raise ValueError("row column positions out of range")
         if self.F:
             if self.nqf != NQF:
                 raise ValueError("facet quadrature does not match the compiled kernel")
@@ -302,8 +351,14 @@ class DarcySystem:
             self.assemble_dirichlet(p_dirichlet, fp64=fp64, stream=stream)
         return vals, rg
 
-    def assemble_elements(self, kappa, rhs, fp64=False, stream=None):
-        """Element kernels + staged-segment reduction; zeroes rg first."""
+    def assemble_elements(self, kappa, rhs, fp64=False, stream=None, method="rows"):
+        """Matrix values + element residual (rg of every row rewritten).
+
+        method "rows": row-owner gather kernel (one pass, coalesced stores);
+        method "element": element kernel + slot map + staged-segment reduction.
+        """
+        if method == "rows":
+            return self._assemble_rows(kappa, rhs, fp64, stream)
         s = nat.stream_handle(stream)
         vals = self.vals64 if fp64 else self.vals32
         stage = self.stage64 if fp64 else self.stage32
@@ -321,6 +376,21 @@ class DarcySystem:
                  nat.ptr(rg), s)
         nat.call("nfx_stage_reduce", len(self.pattern.target), nat.ptr(self.segptr), nat.ptr(self.target),
                  int(fp64), nat.ptr(stage), nat.ptr(vals), s)
+        return vals, rg
+
+    def _assemble_rows(self, kappa, rhs, fp64, stream):
+        s = nat.stream_handle(stream)
+        vals = self.vals64 if fp64 else self.vals32
+        rg = self.rhs64 if fp64 else self.rhs32
+        ka = kappa if isinstance(kappa, CoefficientArg) else coefficient(kappa, self.xq, self.nq, invert=True, device=self.device)
+        fa = rhs if isinstance(rhs, CoefficientArg) else coefficient(rhs, self.xq, self.nq, device=self.device)
+        check_coefficient(ka, self.E, self.nq, "kappa")
+        check_coefficient(fa, self.E, self.nq, "rhs")
+        nat.call("nfx_assemble_darcy_rows", self.dim, self.k, self.n, nat.ptr(self.row_inc), nat.ptr(self.row_pos),
+                 nat.ptr(self.rowptr), nat.ptr(self.row_signs), nat.ptr(self.cells),
+                 nat.ptr(self.coords64 if fp64 else self.coords), nat.ptr(self.tables64 if fp64 else self.tables32),
+                 nat.ptr(ka.get(fp64)), ka.se, ka.sq, nat.ptr(fa.get(fp64)), fa.se, fa.sq, int(fp64),
+                 nat.ptr(vals), nat.ptr(rg), s)
         return vals, rg
 
     def dirichlet_values(self, p_dirichlet):

@@ -54,13 +54,21 @@ def test_darcy_assembly_parity(case):
     # indices: bit-exact against the oracle's COO summation
     assert np.array_equal(sysd.pattern.rowptr, A.indptr)
     assert np.array_equal(sysd.pattern.col, A.indices)
-    for fp64, tol in ((True, 1e-12), (False, 1e-5)):
-        vals, r = sysd.assemble(fun["kappa"], fun["rhs"], ex["p"], fp64=fp64)
-        torch.cuda.synchronize()
-        G = sysd.csr_numpy(fp64=fp64)
-        assert _row_rel_err(A, G) <= tol, (fp64, _row_rel_err(A, G))
-        rr = r.double().cpu().numpy()
-        assert np.abs(rr - rg).max() <= tol * max(np.abs(rg).max(), 1e-300), fp64
+    for method in ("rows", "element"):
+        for fp64, tol in ((True, 1e-12), (False, 1e-5)):
+            sysd.vals64.zero_()
+            sysd.vals32.zero_()
+            vals, r = sysd.assemble_elements(fun["kappa"], fun["rhs"], fp64=fp64, method=method)
+            sysd.assemble_dirichlet(ex["p"], fp64=fp64)
+            torch.cuda.synchronize()
+            G = sysd.csr_numpy(fp64=fp64)
+            assert _row_rel_err(A, G) <= tol, (method, fp64, _row_rel_err(A, G))
+            rr = r.double().cpu().numpy()
+            assert np.abs(rr - rg).max() <= tol * max(np.abs(rg).max(), 1e-300), (method, fp64)
+    # run-to-run determinism: a second assembly is bitwise identical
+    v1 = sysd.assemble_elements(fun["kappa"], fun["rhs"], fp64=False)[0].clone()
+    v2 = sysd.assemble_elements(fun["kappa"], fun["rhs"], fp64=False)[0].clone()
+    assert torch.equal(v1, v2)
 
 
 @pytest.mark.parametrize("case", DARCY_CASES, ids=[case_id(c) for c in DARCY_CASES])
