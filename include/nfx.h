@@ -117,6 +117,42 @@ int nfx_assemble_darcy_bc(int dim, int k, int n_facets, const int32_t* d_facet,
                           const void* d_pD, const int32_t* d_rows, int fp64,
                           void* d_rhs, void* stream);
 
+/* ---- assembly: dual-mixed elasticity (BDM1 - P0 - P0, tetrahedra) --------
+ * mode 0 replaces LEDualWeakForm.evaluate_form_vectorized
+ * (src/weak_forms/le_dual_weak_form.py:354-510), mode 1
+ * LERieszMapWeakForm.evaluate_form_vectorized (src/weak_forms/le_riesz_map_weak_form.py:258-360),
+ * each with the SBAIJ scatter of examples/cauchy_elasticity/mfe_elasticity_example_1.py:128-192
+ * expressed by the slot codes (dropped entries = -1, explicit zero diagonals = bit 28).
+ *  d_tables  w[8] | phi[8][12][3] | div[8][12]  (BDM1 reference basis)
+ *  d_lam, d_mu  Lame parameters at quadrature points (stride_e, stride_q)
+ *  d_f       body force + rotation source, 6 components: f[e*fs_e + q*fs_q + c*fs_c]
+ *  d_slot    (42*42, E) slot codes; d_rhs_row (6, E) rows of r(u_c), r(t_c)
+ * One wavefront per element. */
+int nfx_assemble_elasticity(int mode, int n_elem, const int32_t* d_cells,
+                            const void* d_coords, const void* d_tables,
+                            const void* d_lam, const void* d_mu, int coef_stride_e,
+                            int coef_stride_q, const void* d_f, int f_stride_e,
+                            int f_stride_q, int f_stride_c, const int32_t* d_slot,
+                            const int32_t* d_rhs_row, int fp64, void* d_vals,
+                            void* d_stage, void* d_rhs, void* stream);
+
+/* Stress Dirichlet term of LEDualWeakFormBCDirichlet.evaluate_form
+ * (src/weak_forms/le_dual_weak_form.py:514-578):
+ * r[(a, c)] = -sum_q w |detJ_f| u_D,c(x_q) psi_a(x_q) . n for the 3 BDM1 DoFs of
+ * each boundary face; d_uD (F, 4, 3); d_rows (F, 9) row codes (entry a*3 + c).
+ * Other arguments as nfx_assemble_darcy_bc. */
+int nfx_assemble_elasticity_bc(int n_facets, const int32_t* d_facet,
+                               const int32_t* d_neigh, const int32_t* d_combo,
+                               const int32_t* d_cells, const void* d_coords,
+                               const void* d_trace, const void* d_fw, int nqf,
+                               const void* d_uD, const int32_t* d_rows, int fp64,
+                               void* d_rhs, void* stream);
+
+/* dst[i] = src[d_idx[i]] (float or double): the full symmetric matrix from the
+ * SBAIJ upper triangle for the Krylov solver. */
+int nfx_gather(int fp64, int64_t n, const int32_t* d_idx, const void* d_src, void* d_dst,
+               void* stream);
+
 /* ---- scatter reduction ---------------------------------------------------
  * Deterministic segmented sum of staged contributions into CSR slots - the
  * multi-contributor part of the reference's serial setValue(addv=True) scatter
@@ -160,9 +196,10 @@ int nfx_schur_diag64(int n_rows, int n_u, const int32_t* d_rowptr,
 
 /* ---- Krylov --------------------------------------------------------------
  * Preconditioned MINRES (Paige-Saunders) in FP32 on S = D A, D = diag(d_sign)
- * (-1 on rows that turn the saddle point symmetric; NULL = identity), with the
- * SPD diagonal preconditioner d_pinv (FP32 inverse diagonal), inside FP64
- * iterative refinement on the as-assembled FP64 matrix. Replaces KSP minres +
+ * (-1 on rows that turn the saddle point symmetric; NULL = identity), with an SPD
+ * preconditioner given as a sparse matrix P^-1 (CSR d_pc_*, FP32: a diagonal, or
+ * the block-diagonal inverse of a field split), inside FP64 iterative refinement
+ * on the as-assembled FP64 matrix. Replaces KSP minres +
  * PCFIELDSPLIT (examples/cauchy_elasticity/mfe_elasticity_example_1.py:238-274)
  * and KSP preonly + LU/MUMPS (examples/md_elliptic_scalar/md_elliptic_scalar.py:351-364)
  * by an iterative solve to ||b - A x||_2 / ||b||_2 <= rtol evaluated in FP64.
@@ -177,7 +214,8 @@ int nfx_solver_create(nfx_solver** out, int n_rows, int nnz, void* stream);
 int nfx_solver_destroy(nfx_solver* s);
 int nfx_solve_minres_ir(nfx_solver* s, const int32_t* d_rowptr, const int32_t* d_col,
                         const float* d_val32, const double* d_val64,
-                        const float* d_sign, const float* d_pinv,
+                        const float* d_sign, const int32_t* d_pc_rowptr,
+                        const int32_t* d_pc_col, const float* d_pc_val,
                         const double* d_b, double* d_x,
                         double rtol, double inner_rtol, int max_inner, int max_outer,
                         int* h_info, double* h_resid, double* h_hist, int hist_cap);

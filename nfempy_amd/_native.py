@@ -48,6 +48,12 @@ SIGNATURES = {
     "nfx_assemble_darcy_bc": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                       c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p,
                                       c_void_p]),
+    "nfx_assemble_elasticity": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                                        c_int, c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_int,
+                                        c_void_p, c_void_p, c_void_p, c_void_p]),
+    "nfx_assemble_elasticity_bc": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                           c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+    "nfx_gather": (c_int, [c_int, c_i64, c_void_p, c_void_p, c_void_p, c_void_p]),
     "nfx_stage_reduce": (c_int, [c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "nfx_fill": (c_int, [c_int, c_i64, c_double, c_void_p, c_void_p]),
     "nfx_quadrature_points": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
@@ -58,8 +64,8 @@ SIGNATURES = {
     "nfx_solver_create": (c_int, [c_void_p, c_int, c_int, c_void_p]),
     "nfx_solver_destroy": (c_int, [c_void_p]),
     "nfx_solve_minres_ir": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
-                                    c_void_p, c_void_p, c_double, c_double, c_int, c_int, c_void_p, c_void_p,
-                                    c_void_p, c_int]),
+                                    c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_double, c_int, c_int,
+                                    c_void_p, c_void_p, c_void_p, c_int]),
     "nfx_field_error": (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
                                 c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
 }

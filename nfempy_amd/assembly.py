@@ -72,15 +72,25 @@ class Mat:
         self.n = system.n
         self._sys = system
 
+    def _attach_elasticity(self, system, which):
+        self._attach(system)
+        self._which = which
+
     @property
     def device_csr(self):
         """(rowptr, col, vals32, vals64) torch tensors in HBM."""
         s = self._sys
         if s is None:
             raise RuntimeError("matrix has no device values (no weak form scattered into it)")
+        if getattr(self, "_which", None):
+            return s.full(self._which)
         return s.rowptr, s.col, s.vals32, s.vals64
 
     def getValuesCSR(self):
+        if self._sys is not None and getattr(self, "_which", None):
+            op = self._sys.ops[self._which]
+            pat = op["pattern"]
+            return pat.rowptr.astype(np.int64), pat.col.astype(np.int64), op["vals64"].cpu().numpy()
         if self._sys is not None:
             s = self._sys
             return (s.pattern.rowptr.astype(np.int64), s.pattern.col.astype(np.int64), s.vals64.cpu().numpy())
@@ -89,7 +99,10 @@ class Mat:
         return np.zeros(self.n + 1, np.int64), np.zeros(0, np.int64), np.zeros(0)
 
     def getInfo(self):
-        nnz = self._sys.nnz if self._sys is not None else (self._host.nnz if self._host is not None else 0)
+        if self._sys is not None and getattr(self, "_which", None):
+            nnz = self._sys.ops[self._which]["pattern"].nnz
+        else:
+            nnz = self._sys.nnz if self._sys is not None else (self._host.nnz if self._host is not None else 0)
         return {"nz_allocated": nnz, "nz_used": nnz, "nz_unneeded": 0}
 
     def createVecLeft(self):
@@ -99,7 +112,11 @@ class Mat:
         return Vec(self.n)
 
     def scale(self, a):
-        if self._sys is not None:
+        if self._sys is not None and getattr(self, "_which", None):
+            op = self._sys.ops[self._which]
+            op["vals32"].mul_(a)
+            op["vals64"].mul_(a)
+        elif self._sys is not None:
             self._sys.vals32.mul_(a)
             self._sys.vals64.mul_(a)
         elif self._host is not None:

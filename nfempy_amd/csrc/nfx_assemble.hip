@@ -158,15 +158,19 @@ __global__ __launch_bounds__(256) void k_darcy(
   }
 }
 
-// Dirichlet pressure facet term (LaplaceDualWeakFormBCDirichlet.evaluate_form).
+// Dirichlet facet term of the mixed forms: for facet DoF a and data component c,
+//   r[a, c] = scale * sum_q w |detJ_f| g_c(x_q) psi_a(x_q) . n
+// (LaplaceDualWeakFormBCDirichlet: g = p_D, NC = 1, scale = +1;
+//  LEDualWeakFormBCDirichlet: g = u_D, NC = 3, scale = -1).
 // trace: (ncombo, NQF, FD, DIM) reference flux basis at the facet points mapped
-// into the neighbour's reference cell, per (local facet, facet vertex order).
-template <int DIM, int FD, int NQF, typename R>
-__global__ __launch_bounds__(256) void k_darcy_bc(
+// into the neighbour's reference cell, per (local facet, facet vertex order);
+// g: (F, NQF, NC); rows: (F, FD * NC) row codes, entry a * NC + c.
+template <int DIM, int FD, int NQF, int NC, typename R>
+__global__ __launch_bounds__(256) void k_facet_bc(
     int n_facets, const int32_t* __restrict__ facet_nodes, const int32_t* __restrict__ neigh,
     const int32_t* __restrict__ combo, const int32_t* __restrict__ cells,
     const R* __restrict__ coords, const R* __restrict__ trace, const R* __restrict__ fw,
-    const R* __restrict__ pD, const int32_t* __restrict__ rows, R* __restrict__ rhs) {
+    const R* __restrict__ g_data, const int32_t* __restrict__ rows, R scale, R* __restrict__ rhs) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= n_facets) return;
   const int e = neigh[f];
@@ -235,22 +239,28 @@ __global__ __launch_bounds__(256) void k_darcy_bc(
     jtn[d] = s / g.detJ;
   }
   const R* __restrict__ tr = trace + (int64_t)combo[f] * NQF * FD * DIM;
-  R val[FD];
+  R val[FD][NC];
 #pragma unroll
-  for (int a = 0; a < FD; ++a) val[a] = 0;
+  for (int a = 0; a < FD; ++a)
+#pragma unroll
+    for (int c = 0; c < NC; ++c) val[a][c] = 0;
 #pragma unroll
   for (int q = 0; q < NQF; ++q) {
-    const R c = fw[q] * detf * pD[(int64_t)f * NQF + q];
+    const R wq = fw[q] * detf;
 #pragma unroll
     for (int a = 0; a < FD; ++a) {
       R s = 0;
 #pragma unroll
       for (int d = 0; d < DIM; ++d) s += tr[(q * FD + a) * DIM + d] * jtn[d];
-      val[a] += c * s;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) val[a][c] += wq * g_data[((int64_t)f * NQF + q) * NC + c] * s;
     }
   }
 #pragma unroll
-  for (int a = 0; a < FD; ++a) put_row<R>(rows[(int64_t)f * FD + a], val[a], rhs);
+  for (int a = 0; a < FD; ++a)
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+      put_row<R>(rows[(int64_t)f * FD * NC + a * NC + c], scale * val[a][c], rhs);
 }
 
 // Row-owner ("gather") assembly of the mixed Darcy system.
@@ -432,15 +442,15 @@ static int launch_darcy(int n, const int32_t* cells, const void* coords, const v
   return NFX_OK;
 }
 
-template <int DIM, int FD, int NQF, typename R>
+template <int DIM, int FD, int NQF, typename R, int NC = 1>
 static int launch_darcy_bc(int n, const int32_t* facet, const int32_t* neigh, const int32_t* combo,
                            const int32_t* cells, const void* coords, const void* trace,
                            const void* fw, const void* pD, const int32_t* rows, void* rhs,
-                           hipStream_t s) {
-  hipLaunchKernelGGL((k_darcy_bc<DIM, FD, NQF, R>), dim3(grid_for(n, 256)), dim3(256), 0, s, n,
+                           hipStream_t s, double scale = 1.0) {
+  hipLaunchKernelGGL((k_facet_bc<DIM, FD, NQF, NC, R>), dim3(grid_for(n, 256)), dim3(256), 0, s, n,
                      facet, neigh, combo, cells, static_cast<const R*>(coords),
-                     static_cast<const R*>(trace),
-                     static_cast<const R*>(fw), static_cast<const R*>(pD), rows, static_cast<R*>(rhs));
+                     static_cast<const R*>(trace), static_cast<const R*>(fw),
+                     static_cast<const R*>(pD), rows, static_cast<R>(scale), static_cast<R*>(rhs));
   NFX_LAUNCH_CHECK();
   return NFX_OK;
 }
@@ -583,4 +593,25 @@ extern "C" int nfx_assemble_darcy_rows(int dim, int k, int n_rows, const int32_t
 #undef NFX_DROWS
   set_error("nfx_assemble_darcy_rows: unsupported (dim, k)");
   return NFX_E_UNSUPPORTED;
+}
+
+extern "C" int nfx_assemble_elasticity_bc(int n_facets, const int32_t* d_facet,
+                                          const int32_t* d_neigh, const int32_t* d_combo,
+                                          const int32_t* d_cells, const void* d_coords,
+                                          const void* d_trace, const void* d_fw, int nqf,
+                                          const void* d_uD, const int32_t* d_rows, int fp64,
+                                          void* d_rhs, void* stream) {
+  NFX_REQUIRE(n_facets >= 0, "nfx_assemble_elasticity_bc: n_facets < 0");
+  if (n_facets == 0) return NFX_OK;
+  NFX_REQUIRE(nqf == 4, "nfx_assemble_elasticity_bc: facet rule must have 4 points");
+  NFX_REQUIRE(d_facet && d_neigh && d_combo && d_cells && d_coords && d_trace && d_fw && d_uD &&
+                  d_rows && d_rhs,
+              "nfx_assemble_elasticity_bc: null pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return fp64 ? launch_darcy_bc<3, 3, 4, double, 3>(n_facets, d_facet, d_neigh, d_combo, d_cells,
+                                                    d_coords, d_trace, d_fw, d_uD, d_rows, d_rhs,
+                                                    s, -1.0)
+              : launch_darcy_bc<3, 3, 4, float, 3>(n_facets, d_facet, d_neigh, d_combo, d_cells,
+                                                   d_coords, d_trace, d_fw, d_uD, d_rows, d_rhs, s,
+                                                   -1.0);
 }

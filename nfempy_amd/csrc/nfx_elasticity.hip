@@ -1,0 +1,235 @@
+// Dual-mixed linear elasticity, BDM1 stress rows - P0 displacement - P0 rotation,
+// tetrahedra: the element operator of LEDualWeakForm.evaluate_form_vectorized
+// (mode 0) and of LERieszMapWeakForm.evaluate_form_vectorized (mode 1).
+//
+// One wavefront assembles one element (4 elements per 256-thread block). The
+// pushed-forward stress basis psi_a(x_q) = J phi_a / detJ (12 functions x 8
+// points x 3 components), its divergence and the per-point weights are staged
+// in LDS; lanes then evaluate the 903 entries (I <= J) of the symmetric 42 x 42
+// matrix and store both (I, J) and (J, I) through the slot map.
+//
+// Reference local order: stress (a, c) -> 3a + c (a < 12), displacement u_c ->
+// 36 + c, rotation t_c -> 39 + c. Entries:
+//   (s_ac, s_bd) = [c == d] sum_q k1 psi_a . psi_b - sum_q k2 psi_ac psi_bd
+//                  (+ [c == d] sum_q w detJ div_a div_b for the Riesz map)
+//       k1 = w detJ / (2 mu), k2 = w detJ (1 / 2mu) lambda / (2mu + 3 lambda)
+//   (u_c, s_bd)  = [c == d] sum_q w divref_b                        (mode 0)
+//   (t_c, s_bd)  = eps(c, d, k) sum_q w detJ psi_bk                  (mode 0)
+//   (u_c, u_c), (t_c, t_c) = sum_q w detJ                            (mode 1)
+//   r(u_c) = -sum_q w detJ f_c,  r(t_c) = -sum_q w detJ f_{3+c}      (mode 0)
+#include "nfx_common.h"
+
+namespace nfx {
+
+constexpr int kEB = 12;        // BDM1 basis functions
+constexpr int kEQ = 8;         // quadrature points (degree 3)
+constexpr int kEL = 42;        // local DoFs
+constexpr int kWavesPerBlock = 4;
+
+template <int MODE, typename R>
+__global__ __launch_bounds__(64 * kWavesPerBlock) void k_elasticity(
+    int n_elem, const int32_t* __restrict__ cells, const R* __restrict__ coords,
+    const R* __restrict__ tab, const R* __restrict__ lam, const R* __restrict__ mu, int cs_e,
+    int cs_q, const R* __restrict__ f, int fs_e, int fs_q, int fs_c,
+    const int32_t* __restrict__ slot, const int32_t* __restrict__ rhs_row, R* __restrict__ vals,
+    R* __restrict__ stage, R* __restrict__ rhs) {
+  __shared__ R s_psi[kWavesPerBlock][kEB * kEQ * 3];
+  __shared__ R s_div[kWavesPerBlock][kEB * kEQ];
+  __shared__ R s_k1[kWavesPerBlock][kEQ], s_k2[kWavesPerBlock][kEQ], s_wd[kWavesPerBlock][kEQ];
+  __shared__ R s_A[kWavesPerBlock][kEB * 3];
+  __shared__ R s_U[kWavesPerBlock][kEB];
+  const int wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int e = blockIdx.x * kWavesPerBlock + wv;
+  const bool live = e < n_elem;
+  const int ee = live ? e : 0;
+  const R* tw = tab;
+  const R* tphi = tab + kEQ;
+  const R* tdiv = tphi + kEQ * kEB * 3;
+
+  // geometry (every lane, same addresses)
+  R X[4][3];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int v = cells[ee * 4 + a];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) X[a][c] = coords[3 * v + c];
+  }
+  R J[3][3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) J[c][d] = X[d + 1][c] - X[0][c];
+  R detJ = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+           J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+           J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+  detJ = detJ < R(0) ? -detJ : detJ;
+  const R invdet = R(1) / detJ;
+
+  for (int idx = lane; idx < kEB * kEQ; idx += 64) {
+    const int q = idx / kEB, a = idx % kEB;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      R s = 0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) s += J[c][d] * tphi[(q * kEB + a) * 3 + d];
+      s_psi[wv][(a * kEQ + q) * 3 + c] = s * invdet;
+    }
+    s_div[wv][a * kEQ + q] = tdiv[q * kEB + a];  // reference divergence
+  }
+  if (lane < kEQ) {
+    const int q = lane;
+    const R m = mu[(int64_t)ee * cs_e + q * cs_q];
+    const R l = lam[(int64_t)ee * cs_e + q * cs_q];
+    const R wd = tw[q] * detJ;
+    s_wd[wv][q] = wd;
+    s_k1[wv][q] = wd / (R(2) * m);
+    s_k2[wv][q] = wd * (R(1) / (R(2) * m)) * (l / (R(2) * m + R(3) * l));
+  }
+  __syncthreads();
+  if (MODE == 0) {
+    if (lane < kEB * 3) {
+      const int b = lane / 3, k = lane % 3;
+      R s = 0;
+      for (int q = 0; q < kEQ; ++q) s += s_wd[wv][q] * s_psi[wv][(b * kEQ + q) * 3 + k];
+      s_A[wv][b * 3 + k] = s;
+    } else if (lane < kEB * 3 + kEB) {
+      const int b = lane - kEB * 3;
+      R s = 0;
+      for (int q = 0; q < kEQ; ++q) s += tw[q] * s_div[wv][b * kEQ + q];
+      s_U[wv][b] = s;
+    }
+  }
+  __syncthreads();
+  if (!live) return;
+
+  const int64_t E = n_elem;
+  for (int p = lane; p < kEL * kEL; p += 64) {
+    const int I = p / kEL, Jj = p % kEL;
+    if (I > Jj) continue;
+    R v = 0;
+    if (Jj < 36) {
+      const int a = I / 3, c = I % 3, b = Jj / 3, d = Jj % 3;
+      const R* pa = &s_psi[wv][a * kEQ * 3];
+      const R* pb = &s_psi[wv][b * kEQ * 3];
+      R vol = 0, dot = 0, dd = 0;
+#pragma unroll
+      for (int q = 0; q < kEQ; ++q) {
+        vol += s_k2[wv][q] * pa[q * 3 + c] * pb[q * 3 + d];
+        if (c == d) {
+          dot += s_k1[wv][q] * (pa[q * 3] * pb[q * 3] + pa[q * 3 + 1] * pb[q * 3 + 1] +
+                                pa[q * 3 + 2] * pb[q * 3 + 2]);
+          if (MODE == 1)
+            dd += s_wd[wv][q] * s_div[wv][a * kEQ + q] * s_div[wv][b * kEQ + q] * invdet * invdet;
+        }
+      }
+      v = dot - vol + dd;
+    } else if (I < 36) {
+      if (MODE == 0) {
+        const int b = I / 3, d = I % 3;
+        if (Jj < 39) {
+          v = (Jj - 36 == d) ? s_U[wv][b] : R(0);
+        } else {
+          const int c = Jj - 39;
+          // eps(c, d, k) with k the third index
+          if (c != d) {
+            const int k = 3 - c - d;
+            const bool even = (c == 0 && d == 1) || (c == 1 && d == 2) || (c == 2 && d == 0);
+            v = even ? s_A[wv][b * 3 + k] : -s_A[wv][b * 3 + k];
+          }
+        }
+      }
+    } else {
+      if (MODE == 1 && I == Jj) {
+        R s = 0;
+#pragma unroll
+        for (int q = 0; q < kEQ; ++q) s += s_wd[wv][q];
+        v = s;
+      }
+    }
+    put_slot<R>(slot[(int64_t)(I * kEL + Jj) * E + e], v, vals, stage);
+    if (I != Jj) put_slot<R>(slot[(int64_t)(Jj * kEL + I) * E + e], v, vals, stage);
+  }
+  if (MODE == 0 && lane < 6) {
+    R s = 0;
+#pragma unroll
+    for (int q = 0; q < kEQ; ++q)
+      s -= s_wd[wv][q] * f[(int64_t)e * fs_e + q * fs_q + lane * fs_c];
+    put_row<R>(rhs_row[(int64_t)lane * E + e], s, rhs);
+  }
+}
+
+template <int MODE, typename R>
+static int launch_elasticity(int n, const int32_t* cells, const void* coords, const void* tab,
+                             const void* lam, const void* mu, int cse, int csq, const void* f,
+                             int fse, int fsq, int fsc, const int32_t* slot, const int32_t* rhs_row,
+                             void* vals, void* stage, void* rhs, hipStream_t s) {
+  hipLaunchKernelGGL((k_elasticity<MODE, R>), dim3(grid_for(n, kWavesPerBlock)),
+                     dim3(64 * kWavesPerBlock), 0, s, n, cells, static_cast<const R*>(coords),
+                     static_cast<const R*>(tab), static_cast<const R*>(lam),
+                     static_cast<const R*>(mu), cse, csq, static_cast<const R*>(f), fse, fsq, fsc,
+                     slot, rhs_row, static_cast<R*>(vals), static_cast<R*>(stage),
+                     static_cast<R*>(rhs));
+  NFX_LAUNCH_CHECK();
+  return NFX_OK;
+}
+
+// gather: dst[i] = src[idx[i]] (FP32 or FP64)
+template <typename R>
+__global__ __launch_bounds__(256) void k_gather(int64_t n, const int32_t* __restrict__ idx,
+                                                const R* __restrict__ src, R* __restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) dst[i] = src[idx[i]];
+}
+
+}  // namespace nfx
+
+using namespace nfx;
+
+extern "C" int nfx_assemble_elasticity(int mode, int n_elem, const int32_t* d_cells,
+                                       const void* d_coords, const void* d_tables,
+                                       const void* d_lam, const void* d_mu, int coef_stride_e,
+                                       int coef_stride_q, const void* d_f, int f_stride_e,
+                                       int f_stride_q, int f_stride_c, const int32_t* d_slot,
+                                       const int32_t* d_rhs_row, int fp64, void* d_vals,
+                                       void* d_stage, void* d_rhs, void* stream) {
+  NFX_REQUIRE(mode == 0 || mode == 1, "nfx_assemble_elasticity: mode");
+  NFX_REQUIRE(n_elem >= 0, "nfx_assemble_elasticity: n_elem < 0");
+  if (n_elem == 0) return NFX_OK;
+  NFX_REQUIRE(d_cells && d_coords && d_tables && d_lam && d_mu && d_slot && d_vals,
+              "nfx_assemble_elasticity: null pointer");
+  NFX_REQUIRE(mode == 1 || (d_f && d_rhs_row && d_rhs), "nfx_assemble_elasticity: residual args");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (mode == 0)
+    return fp64 ? launch_elasticity<0, double>(n_elem, d_cells, d_coords, d_tables, d_lam, d_mu,
+                                               coef_stride_e, coef_stride_q, d_f, f_stride_e,
+                                               f_stride_q, f_stride_c, d_slot, d_rhs_row, d_vals,
+                                               d_stage, d_rhs, s)
+                : launch_elasticity<0, float>(n_elem, d_cells, d_coords, d_tables, d_lam, d_mu,
+                                              coef_stride_e, coef_stride_q, d_f, f_stride_e,
+                                              f_stride_q, f_stride_c, d_slot, d_rhs_row, d_vals,
+                                              d_stage, d_rhs, s);
+  return fp64 ? launch_elasticity<1, double>(n_elem, d_cells, d_coords, d_tables, d_lam, d_mu,
+                                             coef_stride_e, coef_stride_q, nullptr, 0, 0, 0,
+                                             d_slot, nullptr, d_vals, d_stage, nullptr, s)
+              : launch_elasticity<1, float>(n_elem, d_cells, d_coords, d_tables, d_lam, d_mu,
+                                            coef_stride_e, coef_stride_q, nullptr, 0, 0, 0,
+                                            d_slot, nullptr, d_vals, d_stage, nullptr, s);
+}
+
+extern "C" int nfx_gather(int fp64, int64_t n, const int32_t* d_idx, const void* d_src, void* d_dst,
+                          void* stream) {
+  NFX_REQUIRE(n >= 0, "nfx_gather: n < 0");
+  if (n == 0) return NFX_OK;
+  NFX_REQUIRE(d_idx && d_src && d_dst, "nfx_gather: null pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int blocks = static_cast<int>((n + 255) / 256);
+  if (fp64)
+    hipLaunchKernelGGL(k_gather<double>, dim3(blocks), dim3(256), 0, s, n, d_idx,
+                       static_cast<const double*>(d_src), static_cast<double*>(d_dst));
+  else
+    hipLaunchKernelGGL(k_gather<float>, dim3(blocks), dim3(256), 0, s, n, d_idx,
+                       static_cast<const float*>(d_src), static_cast<float*>(d_dst));
+  NFX_LAUNCH_CHECK();
+  return NFX_OK;
+}

@@ -156,7 +156,9 @@ __device__ __forceinline__ double sum_partials(const double* __restrict__ partia
 
 // Initial step: r1 = r2 = rhs, z = P^-1 rhs, x = w* = 0, partials of rhs . z.
 __global__ __launch_bounds__(kBlock) void k_minres_init(int n, const float* __restrict__ rhs,
-                                                        const float* __restrict__ pinv,
+                                                        const int32_t* __restrict__ prp,
+                                                        const int32_t* __restrict__ pci,
+                                                        const float* __restrict__ pva,
                                                         float* r1, float* r2, float* z,
                                                         float* x, float* w0, float* w1, float* w2,
                                                         double* partial_b) {
@@ -164,7 +166,8 @@ __global__ __launch_bounds__(kBlock) void k_minres_init(int n, const float* __re
   double acc = 0;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     const float b = rhs[i];
-    const float zz = pinv[i] * b;
+    float zz = 0.f;
+    for (int j = prp[i]; j < prp[i + 1]; ++j) zz += pva[j] * rhs[pci[j]];
     r1[i] = b;
     r2[i] = b;
     z[i] = zz;
@@ -276,14 +279,15 @@ __global__ __launch_bounds__(kBlock) void k_minres_a(
   if (threadIdx.x == 0) partial_a[blockIdx.x] = bs;
 }
 
-// B(k): alpha_k, y -= (alpha/beta) r2, z = P^-1 y, partials of y . z.
-__global__ __launch_bounds__(kBlock) void k_minres_b(int n, float* __restrict__ y,
-                                                     const float* __restrict__ r2,
-                                                     const float* __restrict__ pinv,
-                                                     float* __restrict__ znew,
-                                                     const double* __restrict__ partial_a,
-                                                     int np_a, double* __restrict__ partial_b,
-                                                     MinresState* __restrict__ st) {
+// B(k): alpha_k, y' = y - (alpha/beta) r2 (written into the free buffer ynew),
+// z = P^-1 y' with P^-1 a sparse SPD matrix (CSR; y' of other rows is formed on
+// the fly from the unmodified y and r2), partials of y' . z.
+__global__ __launch_bounds__(kBlock) void k_minres_b(
+    int n, const float* __restrict__ y, const float* __restrict__ r2,
+    const int32_t* __restrict__ prp, const int32_t* __restrict__ pci,
+    const float* __restrict__ pva, float* __restrict__ ynew, float* __restrict__ znew,
+    const double* __restrict__ partial_a, int np_a, double* __restrict__ partial_b,
+    MinresState* __restrict__ st) {
   __shared__ double sh[kBlock / 64];
   const double alfa = sum_partials(partial_a, np_a, sh);
   const double beta = st->beta;
@@ -291,8 +295,12 @@ __global__ __launch_bounds__(kBlock) void k_minres_b(int n, float* __restrict__ 
   double acc = 0;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     const float yi = y[i] - c * r2[i];
-    y[i] = yi;
-    const float zi = pinv[i] * yi;
+    ynew[i] = yi;
+    float zi = 0.f;
+    for (int j = prp[i]; j < prp[i + 1]; ++j) {
+      const int col = pci[j];
+      zi += pva[j] * (y[col] - c * r2[col]);
+    }
     znew[i] = zi;
     acc += (double)yi * (double)zi;
   }
@@ -443,36 +451,38 @@ static int launch_a(nfx_solver* s, const int32_t* rp, const int32_t* ci, const f
 }
 
 // One FP32 MINRES solve of (D A) d = rhs (rhs already scaled), result in x.
-// Buffers: r1, r2, y rotate with period 3; z with period 2; w, w1, w2 with period 3.
+// Buffers: r1/r2 swap every iteration (P), y lives in F, z swaps (period 2),
+// w, w1, w2 rotate with period 3.
 static int minres_inner(nfx_solver* s, const int32_t* rp, const int32_t* ci, const float* va,
-                        const float* sign, const float* pinv, const float* rhs, float* x,
-                        double tol, int maxit, int* iters) {
+                        const float* sign, const int32_t* prp, const int32_t* pci,
+                        const float* pva, const float* rhs, float* x, double tol, int maxit,
+                        int* iters) {
   const int n = s->n;
   float* V = s->buf;
-  float* R[3] = {V + (size_t)n * 0, V + (size_t)n * 1, V + (size_t)n * 2};
+  float* P[2] = {V + (size_t)n * 0, V + (size_t)n * 1};
+  float* F = V + (size_t)n * 2;
   float* Z[2] = {V + (size_t)n * 3, V + (size_t)n * 4};
   float* W[3] = {V + (size_t)n * 5, V + (size_t)n * 6, V + (size_t)n * 7};
   MinresState* S[2] = {s->st, s->st + 1};
-  // r1 = r2 = rhs in R[0], R[1]; z_1 in Z[1]; beta_1 partials in partial_b
-  hipLaunchKernelGGL(k_minres_init, dim3(s->blocks_b), dim3(kBlock), 0, s->stream, n, rhs, pinv,
-                     R[0], R[1], Z[1], x, W[0], W[1], W[2], s->partial_b);
+  hipLaunchKernelGGL(k_minres_init, dim3(s->blocks_b), dim3(kBlock), 0, s->stream, n, rhs, prp,
+                     pci, pva, P[0], P[1], Z[1], x, W[0], W[1], W[2], s->partial_b);
   NFX_LAUNCH_CHECK();
   const int check = 16;
   int k = 1;
   for (; k <= maxit; ++k) {
-    float* r1 = R[(k + 2) % 3];
-    float* r2 = R[k % 3];
-    float* y = R[(k + 1) % 3];
+    float* r1 = P[(k + 1) % 2];
+    float* r2 = P[k % 2];
     float* z = Z[k % 2];
     float* zprev = Z[(k + 1) % 2];
-    float* wnew = W[(k + 2) % 3];  // w_{k-1}
-    const float* wa = W[k % 3];    // w_{k-3}
+    float* wnew = W[(k + 2) % 3];      // w_{k-1}
+    const float* wa = W[k % 3];        // w_{k-3}
     const float* wb = W[(k + 1) % 3];  // w_{k-2}
-    int rc = launch_a(s, rp, ci, va, sign, z, zprev, r1, y, x, wnew, wa, wb, k == 1, 0,
+    int rc = launch_a(s, rp, ci, va, sign, z, zprev, r1, F, x, wnew, wa, wb, k == 1, 0,
                       S[(k + 1) & 1], S[k & 1]);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_minres_b, dim3(s->blocks_b), dim3(kBlock), 0, s->stream, n, y, r2, pinv,
-                       zprev, s->partial_a, s->blocks_a, s->partial_b, S[k & 1]);
+    // y' goes into r1's buffer (r1 is dead after A(k)); z_{k+1} into zprev's buffer
+    hipLaunchKernelGGL(k_minres_b, dim3(s->blocks_b), dim3(kBlock), 0, s->stream, n, F, r2, prp,
+                       pci, pva, r1, zprev, s->partial_a, s->blocks_a, s->partial_b, S[k & 1]);
     NFX_LAUNCH_CHECK();
     if (k % check == 0 || k == maxit) {
       NFX_CHECK_HIP(hipMemcpyAsync(s->h_st, S[k & 1], sizeof(MinresState), hipMemcpyDeviceToHost,
@@ -522,11 +532,13 @@ static int residual64(nfx_solver* s, const int32_t* rp, const int32_t* ci, const
 
 extern "C" int nfx_solve_minres_ir(nfx_solver* s, const int32_t* d_rowptr, const int32_t* d_col,
                                    const float* d_val32, const double* d_val64,
-                                   const float* d_sign, const float* d_pinv, const double* d_b,
-                                   double* d_x, double rtol, double inner_rtol, int max_inner,
-                                   int max_outer, int* h_info, double* h_resid, double* h_hist,
-                                   int hist_cap) {
-  NFX_REQUIRE(s && d_rowptr && d_col && d_val32 && d_val64 && d_pinv && d_b && d_x,
+                                   const float* d_sign, const int32_t* d_pc_rowptr,
+                                   const int32_t* d_pc_col, const float* d_pc_val,
+                                   const double* d_b, double* d_x, double rtol, double inner_rtol,
+                                   int max_inner, int max_outer, int* h_info, double* h_resid,
+                                   double* h_hist, int hist_cap) {
+  NFX_REQUIRE(s && d_rowptr && d_col && d_val32 && d_val64 && d_pc_rowptr && d_pc_col &&
+                  d_pc_val && d_b && d_x,
               "nfx_solve_minres_ir: null pointer");
   NFX_REQUIRE(max_inner > 0 && max_outer >= 0, "nfx_solve_minres_ir: iteration limits");
   const int n = s->n;
@@ -546,8 +558,8 @@ extern "C" int nfx_solve_minres_ir(nfx_solver* s, const int32_t* d_rowptr, const
     hipLaunchKernelGGL(k_to_f32, dim3(g), dim3(kBlock), 0, s->stream, n, s->r64, d_sign, rn, rf);
     NFX_LAUNCH_CHECK();
     int it = 0;
-    rc = minres_inner(s, d_rowptr, d_col, d_val32, d_sign, d_pinv, rf, dx, inner_rtol, max_inner,
-                      &it);
+    rc = minres_inner(s, d_rowptr, d_col, d_val32, d_sign, d_pc_rowptr, d_pc_col, d_pc_val, rf, dx,
+                      inner_rtol, max_inner, &it);
     if (rc) return rc;
     total += it;
     hipLaunchKernelGGL(k_axpy64, dim3(g), dim3(kBlock), 0, s->stream, n, rn, dx, d_x);

@@ -1,0 +1,339 @@
+"""Dual-mixed elasticity (BDM1 - P0 - P0 on tetrahedra) resident on the GPU.
+
+``ElasticitySystem(space, functions)`` mirrors the elasticity driver
+``examples/cauchy_elasticity/mfe_elasticity_example_1.py:68-294``:
+
+- operator A from ``LEDualWeakForm.evaluate_form_vectorized`` and preconditioning
+  operator P from ``LERieszMapWeakForm.evaluate_form_vectorized``, both stored
+  like PETSc SBAIJ (upper triangle, entries of an element contribution kept only
+  when ``not isclose(v, 1e-16)``, explicit zero diagonals for A only,
+  ``:128-192``). The value filter is decided once, in FP64 on the device (cold);
+  warm assemblies reuse the pattern;
+- residual ``rg`` including the stress Dirichlet term
+  (``LEDualWeakFormBCDirichlet``, ``le_dual_weak_form.py:514-578``);
+- the solve: the reference's MINRES + PCFIELDSPLIT additive {sigma: Cholesky of
+  P_ss, (u, t): ICC} becomes FP32 MINRES with the block-diagonal inverse of P
+  (one 9 x 9 block per face for sigma - the three BDM1 DoFs of the face times the
+  three stress rows - and the diagonal mass of u, t) inside FP64 refinement.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from . import _native as nat
+from .device import (CoefficientArg, check_coefficient, coefficient, facet_combo_index, facet_combo_tables,
+                     p1_bary, to_dev, torch)
+from .pattern import build_pattern, row_codes
+
+NL = 42
+
+
+def elasticity_tables(s_el, points, weights):
+    tab = s_el.tabulate(points)
+    div = np.einsum("kqjk->qj", tab[1:])
+    return np.concatenate([weights, tab[0].ravel(), div.ravel()])
+
+
+def _vector_coefficient(value, xq, nq, ncomp, device):
+    """(E, nq, ncomp) FP64 device array from a callable returning (ncomp, npts) or an array."""
+    t = torch()
+    if callable(value):
+        x = xq.reshape(-1, 3)
+        v = np.asarray(value(x[:, 0], x[:, 1], x[:, 2]), dtype=np.float64)
+        v = np.broadcast_to(v.reshape(v.shape[0], -1), (v.shape[0], len(x)))[:ncomp]
+        arr = np.ascontiguousarray(v.T).reshape(xq.shape[0], nq, ncomp)
+        return to_dev(arr, device=device)
+    arr = value if isinstance(value, t.Tensor) else t.from_numpy(np.asarray(value, dtype=np.float64))
+    arr = arr.to(device=device, dtype=t.float64)
+    if arr.dim() == 2:  # (E, ncomp) per element
+        arr = arr[:, None, :].expand(arr.shape[0], nq, ncomp)
+    return arr.contiguous()
+
+
+def _sbaij_mask(vals, ref_dest, ref_sign, upper=True):
+    """Entries the reference inserts: not isclose(v, 1e-16) (numpy defaults), row <= col."""
+    v = vals * (ref_sign[:, :, None] * ref_sign[:, None, :])
+    keep = ~np.isclose(v, 1.0e-16)
+    if upper:
+        keep &= ref_dest[:, :, None] <= ref_dest[:, None, :]
+    return keep
+
+
+def full_from_upper(n, rowptr, col):
+    """Full symmetric CSR pattern from an upper-triangle pattern + gather map into it."""
+    r = np.repeat(np.arange(n, dtype=np.int64), np.diff(rowptr))
+    c = col.astype(np.int64)
+    idx = np.arange(len(c), dtype=np.int64)
+    off = r != c
+    rows = np.concatenate([r, c[off]])
+    cols = np.concatenate([c, r[off]])
+    src = np.concatenate([idx, idx[off]])
+    order = np.lexsort((cols, rows))
+    rows, cols, src = rows[order], cols[order], src[order]
+    fr = np.zeros(n + 1, np.int64)
+    np.add.at(fr, rows + 1, 1)
+    return np.cumsum(fr).astype(np.int32), cols.astype(np.int32), src.astype(np.int32)
+
+
+class ElasticitySystem:
+    def __init__(self, space, functions, device="cuda"):
+        t = torch()
+        self.space = space
+        self.device = device
+        names = space.names
+        self.s_sp, self.u_sp, self.t_sp = (space.discrete_spaces[n] for n in names)
+        s_sp = self.s_sp
+        mesh = s_sp.mesh
+        D = s_sp.dimension
+        if D != 3 or s_sp.element.family != "BDM" or s_sp.k_order != 1 or s_sp.n_comp != 3:
+            raise NotImplementedError("GPU elasticity path: BDM1 - P0 - P0 on tetrahedra")
+        self.dim = 3
+        self.E = len(s_sp.element_ids)
+        self.n = space.n_dof
+        self.n_s = s_sp.n_dof
+        points, weights = space.quadrature[3]
+        if len(weights) != 8:
+            raise ValueError("expected the degree-3 tetrahedral rule (8 points)")
+        self.nq = 8
+        cells = mesh.entities[3].nodes[s_sp.topology.cell_index].astype(np.int32)
+        self.cells_np = cells
+        if cells.min() < 0 or cells.max() >= len(mesh.points):
+            raise ValueError("cells out of range")
+        self.cells = to_dev(cells, device=device)
+        self.coords = to_dev(mesh.points.astype(np.float32), device=device)
+        self.coords64 = to_dev(mesh.points, device=device)
+        tab = elasticity_tables(s_sp.element, points, weights)
+        self.tables64 = to_dev(tab, device=device)
+        self.tables32 = to_dev(tab.astype(np.float32), device=device)
+        X = mesh.points[cells]
+        self.xq = np.einsum("qa,eac->eqc", p1_bary(points), X)
+        if self.E * NL * NL > nat.SLOT_INDEX_MASK:
+            raise ValueError("mesh too large for the dense cold pass (28-bit local codes)")
+        ref_dest, ref_sign = space.reference_destinations()
+        if ref_dest.shape[1] != NL:
+            raise ValueError("unexpected local size")
+        self.ref_dest, self.ref_sign = ref_dest, ref_sign
+        self.functions = functions
+        self._coef = self._coefficients(functions)
+        # cold: FP64 element matrices -> value filter -> SBAIJ patterns
+        loc_slot = np.ascontiguousarray(
+            (np.arange(self.E, dtype=np.int64)[:, None] * NL * NL + np.arange(NL * NL)[None, :]).T).astype(np.int32)
+        d_loc = to_dev(loc_slot, device=device)
+        KA = self._local(0, d_loc)
+        KP = self._local(1, d_loc)
+        self.pattern_A = build_pattern(self.n, ref_dest, ref_sign, keep=_sbaij_mask(KA, ref_dest, ref_sign),
+                                       zero_diag_dest=ref_dest, upper=True)
+        self.pattern_P = build_pattern(self.n, ref_dest, ref_sign, keep=_sbaij_mask(KP, ref_dest, ref_sign),
+                                       upper=True)
+        del KA, KP
+        self._upload_patterns()
+        self.rhs_row = to_dev(np.ascontiguousarray(row_codes(ref_dest[:, 36:]).T), device=device)
+        self.rhs32 = t.zeros(self.n, dtype=t.float32, device=device)
+        self.rhs64 = t.zeros(self.n, dtype=t.float64, device=device)
+        self._setup_bc(mesh, device)
+        self._validate(mesh)
+
+    # -- setup ---------------------------------------------------------------------
+    def _coefficients(self, functions):
+        lam = coefficient(functions["lambda"], self.xq, self.nq, device=self.device)
+        mu = coefficient(functions["mu"], self.xq, self.nq, device=self.device)
+        if (lam.se, lam.sq) != (mu.se, mu.sq):
+            t = torch()
+            E, nq = self.E, self.nq
+
+            def full(c):
+                a = c.get(True)
+                if c.se == 0:
+                    return a.expand(E * nq).contiguous()
+                if c.sq == 0:
+                    return a.reshape(-1, 1).expand(E, nq).reshape(-1).contiguous()
+                return a
+
+            lam = CoefficientArg(full(lam), nq, 1)
+            mu = CoefficientArg(full(mu), nq, 1)
+        check_coefficient(lam, self.E, self.nq, "lambda")
+        check_coefficient(mu, self.E, self.nq, "mu")
+        f = functions.get("rhs")
+        fv = _vector_coefficient(f, self.xq, self.nq, 6, self.device) if f is not None else \
+            torch().zeros((self.E, self.nq, 6), dtype=torch().float64, device=self.device)
+        if fv.shape != (self.E, self.nq, 6):
+            raise ValueError("rhs must give 6 components (3 force + 3 rotation) per point")
+        return {"lam": lam, "mu": mu, "f": CoefficientArg(fv.reshape(-1), self.nq * 6, 6)}
+
+    def _local(self, mode, d_loc):
+        t = torch()
+        out = t.zeros(self.E * NL * NL, dtype=t.float64, device=self.device)
+        rhs = t.zeros(max(self.E * NL, 1), dtype=t.float64, device=self.device)
+        rows = to_dev(np.ascontiguousarray((np.arange(self.E)[:, None] * NL + np.arange(36, 42)[None, :]).T)
+                      .astype(np.int32), device=self.device)
+        self._launch(mode, True, d_loc, rows, out, out, rhs)
+        self._local_rhs = rhs
+        return out.cpu().numpy().reshape(self.E, NL, NL)
+
+    def _launch(self, mode, fp64, slot, rhs_row, vals, stage, rhs, stream=None):
+        c = self._coef
+        lam, mu, f = c["lam"], c["mu"], c["f"]
+        nat.call("nfx_assemble_elasticity", mode, self.E, nat.ptr(self.cells),
+                 nat.ptr(self.coords64 if fp64 else self.coords), nat.ptr(self.tables64 if fp64 else self.tables32),
+                 nat.ptr(lam.get(fp64)), nat.ptr(mu.get(fp64)), lam.se, lam.sq, nat.ptr(f.get(fp64)),
+                 self.nq * 6, 6, 1, nat.ptr(slot), nat.ptr(rhs_row), int(fp64), nat.ptr(vals), nat.ptr(stage),
+                 nat.ptr(rhs), nat.stream_handle(stream))
+
+    def _upload_patterns(self):
+        t = torch()
+        dev = self.device
+        self.ops = {}
+        for key, pat in (("A", self.pattern_A), ("P", self.pattern_P)):
+            frp, fcol, fsrc = full_from_upper(self.n, pat.rowptr, pat.col)
+            self.ops[key] = {
+                "pattern": pat,
+                "rowptr": to_dev(pat.rowptr, device=dev), "col": to_dev(pat.col, device=dev),
+                "slot": to_dev(pat.slot, device=dev), "segptr": to_dev(pat.segptr, device=dev),
+                "target": to_dev(pat.target, device=dev),
+                "vals32": t.zeros(pat.nnz, dtype=t.float32, device=dev),
+                "vals64": t.zeros(pat.nnz, dtype=t.float64, device=dev),
+                "stage32": t.zeros(max(pat.n_stage, 1), dtype=t.float32, device=dev),
+                "stage64": t.zeros(max(pat.n_stage, 1), dtype=t.float64, device=dev),
+                "full_rowptr": to_dev(frp, device=dev), "full_col": to_dev(fcol, device=dev),
+                "full_src": to_dev(fsrc, device=dev), "full_nnz": int(len(fcol)),
+            }
+
+    def _setup_bc(self, mesh, device):
+        s_sp = self.s_sp
+        self.F = len(s_sp.bc_element_ids)
+        if not self.F:
+            return
+        fpts, fw = self.space.bc_quadrature[2]
+        self.nqf = len(fw)
+        neigh_cells = s_sp.topology.cell_index[s_sp.bc_neighbour]
+        combo = facet_combo_index(mesh, 3, s_sp.bc_element_ids, neigh_cells, s_sp.bc_local)
+        trace = facet_combo_tables(s_sp.element, 3, fpts)
+        fnodes = mesh.entities[2].nodes[s_sp.bc_element_ids - mesh.entities[2].offset]
+        self.bc_facet = to_dev(fnodes.astype(np.int32), device=device)
+        self.bc_neigh = to_dev(s_sp.bc_neighbour.astype(np.int32), device=device)
+        self.bc_combo = to_dev(combo.astype(np.int32), device=device)
+        self.bc_trace64 = to_dev(trace, device=device)
+        self.bc_trace32 = to_dev(trace.astype(np.float32), device=device)
+        self.bc_fw64 = to_dev(fw, device=device)
+        self.bc_fw32 = to_dev(fw.astype(np.float32), device=device)
+        shift = self.space.strides[self.space.names[0]] + self.space.dof_shift
+        self.bc_rows_np = row_codes(s_sp.bc_ref_dest + shift, s_sp.bc_ref_sign)
+        self.bc_rows = to_dev(self.bc_rows_np, device=device)
+        self.bc_xq = np.einsum("qa,fac->fqc", p1_bary(fpts), mesh.points[fnodes])
+
+    def _validate(self, mesh):
+        for key in ("A", "P"):
+            op = self.ops[key]
+            pat = op["pattern"]
+            if pat.slot.shape != (NL * NL, self.E):
+                raise ValueError("slot map shape")
+            c = pat.slot.astype(np.int64).ravel()
+            live = c[c >= 0]
+            idx = live & nat.SLOT_INDEX_MASK
+            st = (live & nat.SLOT_STAGE) != 0
+            if np.any(idx[~st] >= pat.nnz) or np.any(idx[st] >= max(pat.n_stage, 1)):
+                raise ValueError(f"slot codes of {key} out of range")
+        rr = self.ref_dest[:, 36:]
+        if rr.min() < 0 or rr.max() >= self.n:
+            raise ValueError("residual rows out of range")
+        if self.F:
+            if self.nqf != 4 or self.bc_trace64.shape[1:] != (4, 3, 3):
+                raise ValueError("facet tables do not match the compiled kernel")
+            if self.bc_rows_np.shape != (self.F, 9) or (self.bc_rows_np & nat.SLOT_INDEX_MASK).max() >= self.n:
+                raise ValueError("facet rows out of range")
+
+    # -- warm assembly ---------------------------------------------------------------
+    def assemble(self, which="A", fp64=False, stream=None, u_dirichlet=None):
+        op = self.ops[which]
+        s = nat.stream_handle(stream)
+        vals = op["vals64" if fp64 else "vals32"]
+        stage = op["stage64" if fp64 else "stage32"]
+        rg = self.rhs64 if fp64 else self.rhs32
+        mode = 0 if which == "A" else 1
+        if mode == 0:
+            nat.call("nfx_fill", int(fp64), self.n, 0.0, nat.ptr(rg), s)
+        self._launch(mode, fp64, op["slot"], self.rhs_row, vals, stage, rg, stream)
+        pat = op["pattern"]
+        nat.call("nfx_stage_reduce", len(pat.target), nat.ptr(op["segptr"]), nat.ptr(op["target"]), int(fp64),
+                 nat.ptr(stage), nat.ptr(vals), s)
+        if mode == 0 and u_dirichlet is not None and self.F:
+            self.assemble_dirichlet(u_dirichlet, fp64=fp64, stream=stream)
+        return vals, rg
+
+    def assemble_dirichlet(self, u_dirichlet, fp64=False, stream=None, rows=None, out=None):
+        uD = _vector_coefficient(u_dirichlet, self.bc_xq, self.nqf, 3, self.device)
+        if uD.shape != (self.F, self.nqf, 3):
+            raise ValueError("u_D must give 3 components per facet point")
+        rg = out if out is not None else (self.rhs64 if fp64 else self.rhs32)
+        g = uD if fp64 else uD.to(torch().float32)
+        nat.call("nfx_assemble_elasticity_bc", self.F, nat.ptr(self.bc_facet), nat.ptr(self.bc_neigh),
+                 nat.ptr(self.bc_combo), nat.ptr(self.cells), nat.ptr(self.coords64 if fp64 else self.coords),
+                 nat.ptr(self.bc_trace64 if fp64 else self.bc_trace32), nat.ptr(self.bc_fw64 if fp64 else self.bc_fw32),
+                 self.nqf, nat.ptr(g.contiguous()), nat.ptr(self.bc_rows if rows is None else rows), int(fp64),
+                 nat.ptr(rg), nat.stream_handle(stream))
+
+    def full(self, which="A"):
+        """Full symmetric (rowptr, col, vals32, vals64) for the Krylov solver."""
+        t = torch()
+        op = self.ops[which]
+        n = op["full_nnz"]
+        if "full32" not in op:
+            op["full32"] = t.empty(n, dtype=t.float32, device=self.device)
+            op["full64"] = t.empty(n, dtype=t.float64, device=self.device)
+        s = nat.stream_handle()
+        nat.call("nfx_gather", 0, n, nat.ptr(op["full_src"]), nat.ptr(op["vals32"]), nat.ptr(op["full32"]), s)
+        nat.call("nfx_gather", 1, n, nat.ptr(op["full_src"]), nat.ptr(op["vals64"]), nat.ptr(op["full64"]), s)
+        return op["full_rowptr"], op["full_col"], op["full32"], op["full64"]
+
+    def csr_numpy(self, which="A", fp64=True):
+        import scipy.sparse as sps
+
+        op = self.ops[which]
+        pat = op["pattern"]
+        vals = op["vals64" if fp64 else "vals32"].cpu().numpy()
+        return sps.csr_matrix((vals, pat.col, pat.rowptr), shape=(self.n, self.n))
+
+    # -- preconditioner -----------------------------------------------------------
+    def preconditioner(self, P=None):
+        """(sign, pc) for MinresIR: block-diagonal inverse of the Riesz operator P.
+
+        sigma: one 9 x 9 block per face (its 3 BDM1 DoFs x 3 stress rows are
+        consecutive in the DoF numbering); u, t: inverse diagonal."""
+        t = torch()
+        self.assemble("P", fp64=True)
+        rp, col, _, v64 = self.full("P")
+        rp_h, col_h, v_h = rp.cpu().numpy(), col.cpu().numpy(), v64.cpu().numpy()
+        import scipy.sparse as sps
+
+        Pm = sps.csr_matrix((v_h, col_h, rp_h), shape=(self.n, self.n))
+        ns = self.n_s
+        nb = ns // 9
+        blocks = np.zeros((nb, 9, 9))
+        Ps = Pm[:ns, :ns].tocoo()
+        same = (Ps.row // 9) == (Ps.col // 9)
+        blocks[Ps.row[same] // 9, Ps.row[same] % 9, Ps.col[same] % 9] = Ps.data[same]
+        inv = np.linalg.inv(blocks)
+        inv = 0.5 * (inv + inv.transpose(0, 2, 1))
+        dg = Pm.diagonal()[ns:]
+        rows_b = (np.arange(nb)[:, None, None] * 9 + np.arange(9)[None, :, None]).repeat(9, axis=2)
+        cols_b = (np.arange(nb)[:, None, None] * 9 + np.arange(9)[None, None, :]).repeat(9, axis=1)
+        r = np.concatenate([rows_b.ravel(), np.arange(ns, self.n)])
+        c = np.concatenate([cols_b.ravel(), np.arange(ns, self.n)])
+        v = np.concatenate([inv.ravel(), 1.0 / dg])
+        M = sps.csr_matrix((v, (r, c)), shape=(self.n, self.n))
+        M.sort_indices()
+        pc = (to_dev(M.indptr.astype(np.int32), device=self.device), to_dev(M.indices.astype(np.int32), device=self.device),
+              to_dev(M.data.astype(np.float32), device=self.device))
+        return None, pc
+
+    def solve(self, rtol=1e-8, inner_rtol=1e-5, max_inner=50000, max_outer=30):
+        from .solvers import MinresIR
+
+        rp, col, v32, v64 = self.full("A")
+        sign, pc = self.preconditioner()
+        solver = MinresIR(rp, col, v32, v64, sign=None, pc=pc)
+        x, info = solver.solve(-self.rhs64, rtol=rtol, inner_rtol=inner_rtol, max_inner=max_inner,
+                               max_outer=max_outer)
+        solver.close()
+        return x, info

@@ -87,10 +87,11 @@ class KSP:
         sysd = self.A._sys
         n = sysd.n
         if hasattr(sysd, "preconditioner"):
-            sign, pinv = sysd.preconditioner(self.P)
+            sign, pc = sysd.preconditioner(self.P)
+            solver = MinresIR(rowptr, col, v32, v64, sign, pc=pc)
         else:
             sign, pinv = darcy_preconditioner(rowptr, col, v64, n, sysd.n_u)
-        solver = MinresIR(rowptr, col, v32, v64, sign, pinv)
+            solver = MinresIR(rowptr, col, v32, v64, sign, pinv)
         bd = torch.as_tensor(np.asarray(b.array), dtype=torch.float64, device=rowptr.device)
         xd, info = solver.solve(bd, rtol=self.rtol, max_inner=self.max_it)
         solver.close()

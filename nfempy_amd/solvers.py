@@ -28,13 +28,20 @@ from .device import torch
 class MinresIR:
     """FP32 MINRES + FP64 refinement on one CSR operator resident on the device."""
 
-    def __init__(self, rowptr, col, vals32, vals64, sign=None, pinv=None, stream=None):
+    def __init__(self, rowptr, col, vals32, vals64, sign=None, pinv=None, stream=None, pc=None):
+        """pinv: diagonal preconditioner (FP32 inverse diagonal) or pc: (rowptr, col, val)
+        CSR of a sparse SPD preconditioner P^-1 (FP32 values)."""
         t = torch()
         self.n = int(rowptr.numel() - 1)
         self.nnz = int(col.numel())
         self.rowptr, self.col, self.vals32, self.vals64 = rowptr, col, vals32, vals64
         self.sign = sign
-        self.pinv = pinv if pinv is not None else t.ones(self.n, dtype=t.float32, device=rowptr.device)
+        if pc is None:
+            dev = rowptr.device
+            diag = pinv if pinv is not None else t.ones(self.n, dtype=t.float32, device=dev)
+            ar = t.arange(self.n + 1, dtype=t.int32, device=dev)
+            pc = (ar, ar[:-1].contiguous(), diag.to(t.float32).contiguous())
+        self.pc = pc
         self.stream = stream
         h = ctypes.c_void_p()
         nat.call("nfx_solver_create", ctypes.byref(h), self.n, self.nnz, nat.stream_handle(stream))
@@ -61,7 +68,8 @@ class MinresIR:
         hist = (ctypes.c_double * (max_outer + 1))()
         rc = nat.lib().nfx_solve_minres_ir(
             self._h, nat.ptr(self.rowptr), nat.ptr(self.col), nat.ptr(self.vals32), nat.ptr(self.vals64),
-            nat.ptr(self.sign), nat.ptr(self.pinv), nat.ptr(b), nat.ptr(x), rtol, inner_rtol, max_inner,
+            nat.ptr(self.sign), nat.ptr(self.pc[0]), nat.ptr(self.pc[1]), nat.ptr(self.pc[2]), nat.ptr(b),
+            nat.ptr(x), rtol, inner_rtol, max_inner,
             max_outer, info, resid, hist, max_outer + 1)
         if rc not in (nat.NFX_OK, nat.NFX_E_NOT_CONVERGED) or (raise_on_fail and rc != nat.NFX_OK):
             nat.check(rc, "nfx_solve_minres_ir")

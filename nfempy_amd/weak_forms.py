@@ -145,3 +145,132 @@ def _bc_facet_values(space, p_dirichlet):
     out = torch.zeros(sysd.F * fd, dtype=torch.float64, device="cuda")
     sysd.assemble_dirichlet(p_dirichlet, fp64=True, rows=d_codes, out=out)
     return out.cpu().numpy().reshape(sysd.F, fd)
+
+
+def _elasticity_system(space, functions):
+    from .elasticity import ElasticitySystem
+
+    sysd = getattr(space, "_nfx_elasticity_system", None)
+    if sysd is None or sysd.functions is not functions:
+        sysd = ElasticitySystem(space, functions)
+        space._nfx_elasticity_system = sysd
+    return sysd
+
+
+class LEDualWeakForm(WeakForm):
+    """Dual-mixed elasticity element: ``functions = {"rhs": f (6 comps), "lambda", "mu"}``.
+
+    ``evaluate_form_vectorized`` is the reference's live path
+    (``le_dual_weak_form.py:354-510``); ``evaluate_form`` is the same operator here
+    (the reference's AD variant is not usable in 3-D)."""
+
+    _mode = 0
+
+    def _local(self):
+        if self.space is None or self.functions is None:
+            raise ValueError
+        if self._cache is None:
+            import numpy as _np
+
+            from .elasticity import NL
+
+            sysd = _elasticity_system(self.space, self.functions)
+            d_loc = _dense_local_slots(sysd)
+            K = sysd._local(self._mode, d_loc)
+            r = sysd._local_rhs.cpu().numpy().reshape(sysd.E, NL)
+            self._cache = (K, r)
+        return self._cache
+
+    def evaluate_form_vectorized(self, element_index, alpha=None):
+        K, r = self._local()
+        return r[element_index].copy(), K[element_index].copy()
+
+    evaluate_form = evaluate_form_vectorized
+
+    def scatter(self, jacobian, residual):
+        sysd = _elasticity_system(self.space, self.functions)
+        which = "A" if self._mode == 0 else "P"
+        sysd.assemble(which, fp64=True)
+        sysd.assemble(which, fp64=False)
+        if jacobian is not None:
+            jacobian._attach_elasticity(sysd, which)
+        if residual is not None and self._mode == 0:
+            residual += sysd.rhs64.cpu().numpy()
+        return sysd
+
+
+class LERieszMapWeakForm(LEDualWeakForm):
+    """Riesz-map preconditioning operator (``le_riesz_map_weak_form.py:258-360``)."""
+
+    _mode = 1
+
+
+class LEDualWeakFormBCDirichlet(WeakForm):
+    """Stress Dirichlet term: ``functions = {"u": u_D, ...}`` (``le_dual_weak_form.py:514-578``).
+
+    Needs the operator weak form's coefficients: pass ``operator_functions`` (the
+    ``functions`` of the ``LEDualWeakForm`` on the same space)."""
+
+    def __init__(self, space, operator_functions=None):
+        super().__init__(space)
+        self.operator_functions = operator_functions
+
+    def _system(self):
+        sysd = getattr(self.space, "_nfx_elasticity_system", None)
+        if sysd is None:
+            if self.operator_functions is None:
+                raise ValueError("scatter the LEDualWeakForm first or pass operator_functions")
+            sysd = _elasticity_system(self.space, self.operator_functions)
+        return sysd
+
+    def evaluate_all(self):
+        if self._cache is None:
+            import torch
+
+            from . import _native as nat
+            from .device import to_dev
+
+            sysd = self._system()
+            s = self.space.discrete_spaces[self.space.names[0]]
+            fd = s.bc_dest.shape[1]
+            ref = s.bc_ref_dest
+            j_of_a = np.argmax(ref[:, :, None] == s.bc_dest[:, None, :], axis=2)
+            codes = (np.arange(sysd.F)[:, None] * fd + j_of_a).astype(np.int64)
+            codes |= np.where(s.bc_ref_sign < 0, nat.SLOT_NEG, 0)
+            out = torch.zeros(sysd.F * fd, dtype=torch.float64, device="cuda")
+            sysd.assemble_dirichlet(self.functions["u"], fp64=True, rows=to_dev(codes.astype(np.int32)), out=out)
+            self._cache = out.cpu().numpy().reshape(sysd.F, fd)
+        return self._cache
+
+    def evaluate_form(self, element_index, alpha=None):
+        r = self.evaluate_all()[element_index]
+        return r.copy(), np.zeros((len(r), len(r)))
+
+    def scatter(self, jacobian, residual):
+        sysd = self._system()
+        if not sysd.F:
+            return
+        sysd.assemble_dirichlet(self.functions["u"], fp64=True)
+        sysd.assemble_dirichlet(self.functions["u"], fp64=False)
+        if residual is not None:
+            vals = self.evaluate_all()
+            s = self.space.discrete_spaces[self.space.names[0]]
+            shift = self.space.strides[self.space.names[0]] + self.space.dof_shift
+            for j in range(vals.shape[1]):
+                np.add.at(residual, s.bc_dest[:, j] + shift, vals[:, j])
+
+
+def _dense_local_slots(sysd):
+    """Slot codes writing each element matrix densely in the reference's local order."""
+    from .device import to_dev
+    from .elasticity import NL
+
+    if not hasattr(sysd, "_dense_local"):
+        loc = np.argsort(sysd.s_sp.local_src, axis=1)  # reference a -> local i (stress)
+        nb = loc.shape[1]
+        li = np.concatenate([(loc[:, :, None] * 3 + np.arange(3)).reshape(sysd.E, nb * 3),
+                             np.tile(np.arange(36, 42), (sysd.E, 1))], axis=1)
+        from .pattern import local_slot_map
+
+        sysd._dense_local = to_dev(local_slot_map(li, sysd.ref_sign, NL))
+    return sysd._dense_local
