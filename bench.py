@@ -95,6 +95,56 @@ def cpu_baseline(seconds: float, n: int = 16):
     }
 
 
+def bench_elasticity(args, log):
+    """Dual-mixed elasticity, 16^3 x 6 Kuhn tets, BDM1-P0-P0, Lame parameters log-normal per element."""
+    import torch
+
+    from nfempy_amd.elasticity import ElasticitySystem
+    from nfempy_amd.mesh import Mesh, unit_cube
+    from nfempy_amd.spaces import ProductSpace
+
+    n = args.elasticity_n
+    log(f"elasticity {n}^3x6: building")
+    t0 = time.perf_counter()
+    mesh = Mesh.from_blocks(3, *unit_cube(n))
+    ps = ProductSpace({"s": (3, 3, "BDM", 1, mesh), "u": (3, 3, "Lagrange", 0, mesh), "t": (3, 3, "Lagrange", 0, mesh)})
+    ps.make_subspaces_discontinuous({"u": True, "t": True})
+    ps.build_structures({"s": [1], "u": [1], "t": [1]}, {"s": [2, 3, 4, 5, 6, 7]})
+    E = len(ps.discrete_spaces["s"].elements)
+    rng = np.random.default_rng(2024)
+    fun = {"lambda": np.exp(0.5 * rng.standard_normal(E)), "mu": np.exp(0.5 * rng.standard_normal(E)),
+           "rhs": lambda x, y, z: np.array([np.ones_like(x), 0.5 * np.ones_like(x), 0.25 * np.ones_like(x),
+                                            np.zeros_like(x), np.zeros_like(x), np.zeros_like(x)])}
+    sysd = ElasticitySystem(ps, fun)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    uD = lambda x, y, z: np.zeros((3,) + np.shape(x))
+    for _ in range(3):
+        sysd.assemble("A", fp64=False)
+    torch.cuda.synchronize()
+    steps = max(5, args.steps // 5)
+    ts = time.perf_counter()
+    for _ in range(steps):
+        sysd.assemble("A", fp64=False)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - ts) * 1e3 / steps
+    sysd.assemble("A", fp64=True, u_dirichlet=uD)
+    sysd.assemble("A", fp64=False, u_dirichlet=uD)
+    torch.cuda.synchronize()
+    tsolve = time.perf_counter()
+    x, info = sysd.solve(rtol=1e-8, max_inner=100000)
+    torch.cuda.synchronize()
+    tsolve = time.perf_counter() - tsolve
+    out = {"workload": f"elasticity3d-bdm1p0p0-kuhn{n}x6-lognormal-lame(sigma=0.5)", "elements": E, "dofs": sysd.n,
+           "nnz_upper": sysd.pattern_A.nnz, "cold_setup_s": round(t1 - t0, 3),
+           "ms_per_assembly": round(ms, 4), "tets_per_s": round(E / (ms * 1e-3), 1),
+           "time_to_solve_s": round(tsolve, 4), "iterations": info["iterations"],
+           "refinements": info["outer"], "residual": info["residual"], "converged": info["converged"]}
+    log(f"elasticity: {ms:.3f} ms/assembly ({out['tets_per_s']:.3e} tets/s), solve {tsolve:.2f}s "
+        f"{info['iterations']} it, residual {info['residual']:.2e}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,6 +154,8 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-elasticity", action="store_true")
+    ap.add_argument("--elasticity-n", type=int, default=16)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also launch a 256 MiB device copy (for counter calibration runs)")
     args = ap.parse_args()
@@ -212,6 +264,10 @@ def main():
             f"residual {info['residual']:.3e}")
         solver.close()
 
+    elasticity = None
+    if not args.no_elasticity:
+        elasticity = bench_elasticity(args, log)
+
     cpu = None
     if not args.no_cpu_baseline:
         log(f"cpu baseline ({args.cpu_seconds:.0f} s sample, 1 core)")
@@ -240,6 +296,8 @@ def main():
     }
     if solve:
         out["solve"] = solve
+    if elasticity:
+        out["elasticity"] = elasticity
     print(json.dumps(out), flush=True)
 
 

@@ -98,6 +98,21 @@ int nfx_assemble_darcy_rows(int dim, int k, int n_rows, const int32_t* d_inc,
                             const void* d_f, int f_stride_e, int f_stride_q, int fp64,
                             void* d_vals, void* d_rhs, void* stream);
 
+/* Batched persistent variant for many independent 3-D systems (RT1-P0 and RT2-P1
+ * mixed in one launch): d_items is an array of n_items descriptors
+ *   struct { int k, n_rows, ks_e, ks_q, fs_e, fs_q;
+ *            const int32_t* inc; const uint8_t* pos; const int32_t* rowptr;
+ *            const uint32_t* signs; const int32_t* cells; const float* coords;
+ *            const float* tables; const float* kinv; const float* f;
+ *            float* vals; float* rhs; }             (FP32, nfx_darcy_batch_item_size bytes)
+ * and d_tiles (n_tiles, 2) int32 (item, first row) pairs, heaviest first; a fixed
+ * grid of `grid` workgroups of tile_rows (64, 128, 256) threads walks the tiles
+ * in a static order (block b: tiles b, b + grid, ...). Same per-row arithmetic as
+ * nfx_assemble_darcy_rows. */
+int nfx_assemble_darcy_batched(int n_items, const void* d_items, int n_tiles,
+                               const int32_t* d_tiles, int tile_rows, int grid, void* stream);
+int nfx_darcy_batch_item_size(void);
+
 /* Dirichlet pressure facet term, LaplaceDualWeakFormBCDirichlet.evaluate_form
  * (src/weak_forms/laplace_dual_weak_form.py:91-154) for every boundary facet:
  * rg[row] = sum_q w |detJ_f| p_D(x_q) psi(x_q) . n with the flux basis traced from

@@ -45,6 +45,8 @@ SIGNATURES = {
     "nfx_assemble_darcy_rows": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                         c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_int, c_int,
                                         c_void_p, c_void_p, c_void_p]),
+    "nfx_assemble_darcy_batched": (c_int, [c_int, c_void_p, c_int, c_void_p, c_int, c_int, c_void_p]),
+    "nfx_darcy_batch_item_size": (c_int, []),
     "nfx_assemble_darcy_bc": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                       c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p,
                                       c_void_p]),
@@ -114,7 +116,7 @@ def check(rc: int, what: str = "") -> int:
 
 def call(name: str, *args) -> int:
     rc = getattr(lib(), name)(*args)
-    if name in ("nfx_version",):
+    if name in ("nfx_version", "nfx_darcy_batch_item_size"):
         return rc
     return check(rc, name)
 

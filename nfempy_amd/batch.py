@@ -1,0 +1,174 @@
+"""Batched assembly and solve of many independent mixed Darcy systems.
+
+The benchmark's batched configuration: hundreds of unit-cube tetrahedral Darcy
+instances of uneven size (n^3 x 6 tets, n drawn per instance) and degree
+(RT1-P0 or RT2-P1), log-normal permeability per instance, assembled by ONE
+persistent launch (``nfx_assemble_darcy_batched``):
+
+- instances with the same (n, k) share their mesh arrays, pattern and row data
+  (one ``DarcySystem`` per distinct (n, k)); coefficients, values and residuals
+  are per instance;
+- work is cut into tiles of CSR rows, sorted by cost (rows x local DoFs x
+  quadrature points) and walked in a static order by a fixed grid, so the
+  heaviest tiles start first and no atomics are needed;
+- the tile size and grid are chosen by ``autotune`` (timed on the batch itself);
+- ``capture`` records the whole batched assembly in a CUDA/HIP graph (kernels
+  only, no memsets) for replay.
+
+Solves are per instance (FP32 MINRES + FP64 refinement to 1e-8).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import time
+
+import numpy as np
+
+from . import _native as nat
+from .device import CoefficientArg, DarcySystem, check_coefficient, torch
+from .fields import grf_device
+from .mesh import Mesh, unit_cube
+from .spaces import ProductSpace
+
+
+class _Item(ctypes.Structure):
+    _fields_ = [("k", ctypes.c_int), ("n_rows", ctypes.c_int), ("ks_e", ctypes.c_int), ("ks_q", ctypes.c_int),
+                ("fs_e", ctypes.c_int), ("fs_q", ctypes.c_int),
+                ("inc", ctypes.c_void_p), ("pos", ctypes.c_void_p), ("rowptr", ctypes.c_void_p),
+                ("signs", ctypes.c_void_p), ("cells", ctypes.c_void_p), ("coords", ctypes.c_void_p),
+                ("tab", ctypes.c_void_p), ("kinv", ctypes.c_void_p), ("f", ctypes.c_void_p),
+                ("vals", ctypes.c_void_p), ("rhs", ctypes.c_void_p)]
+
+
+def darcy_space(n, k):
+    mesh = Mesh.from_blocks(3, *unit_cube(n))
+    ps = ProductSpace({"u": (3, 1, "RT", k, mesh), "p": (3, 1, "Lagrange", k - 1, mesh)})
+    ps.make_subspaces_discontinuous({"p": True})
+    ps.build_structures({"u": [1], "p": [1]}, {"u": [2, 3, 4, 5, 6, 7]})
+    return ps
+
+
+class DarcyBatch:
+    def __init__(self, specs, seed=0, ell=0.1, sigma=1.0, device="cuda"):
+        """specs: list of (n, k) per instance."""
+        t = torch()
+        if nat.call("nfx_darcy_batch_item_size") != ctypes.sizeof(_Item):
+            raise RuntimeError("batch descriptor layout differs between host and library")
+        self.device = device
+        self.specs = [(int(n), int(k)) for n, k in specs]
+        self.systems = {}
+        for key in sorted(set(self.specs)):
+            self.systems[key] = DarcySystem(darcy_space(*key), device=device)
+        self.instances = []
+        for i, (n, k) in enumerate(self.specs):
+            sysd = self.systems[(n, k)]
+            cent = sysd.coords64[sysd.cells.long()].mean(dim=1)
+            logk = grf_device(cent, seed=seed + i, ell=ell, sigma=sigma)
+            kinv = CoefficientArg(t.exp(-logk).contiguous(), 1, 0)
+            f = CoefficientArg(t.ones(1, dtype=t.float64, device=device), 0, 0)
+            check_coefficient(kinv, sysd.E, sysd.nq, "kappa")
+            check_coefficient(f, sysd.E, sysd.nq, "rhs")
+            self.instances.append({
+                "system": sysd, "kinv": kinv, "f": f, "logk": logk,
+                "vals": t.zeros(sysd.nnz, dtype=t.float32, device=device),
+                "rhs": t.zeros(sysd.n, dtype=t.float32, device=device),
+            })
+        self.n_elements = sum(inst["system"].E for inst in self.instances)
+        self.tile_rows, self.grid = 128, None
+        self._build_items()
+        self._build_tiles(self.tile_rows)
+        self.graph = None
+
+    def _build_items(self):
+        arr = (_Item * len(self.instances))()
+        for i, inst in enumerate(self.instances):
+            s = inst["system"]
+            kinv32, f32 = inst["kinv"].get(False), inst["f"].get(False)
+            arr[i] = _Item(s.k, s.n, inst["kinv"].se, inst["kinv"].sq, inst["f"].se, inst["f"].sq,
+                           nat.ptr(s.row_inc), nat.ptr(s.row_pos), nat.ptr(s.rowptr), nat.ptr(s.row_signs),
+                           nat.ptr(s.cells), nat.ptr(s.coords), nat.ptr(s.tables32), nat.ptr(kinv32),
+                           nat.ptr(f32), nat.ptr(inst["vals"]), nat.ptr(inst["rhs"]))
+        raw = np.frombuffer(bytes(arr), dtype=np.uint8).copy()
+        self._items = torch().from_numpy(raw).to(self.device)
+
+    def _build_tiles(self, tile_rows):
+        tiles, cost = [], []
+        for i, inst in enumerate(self.instances):
+            s = inst["system"]
+            c = s.nl * s.nq
+            for r0 in range(0, s.n, tile_rows):
+                tiles.append((i, r0))
+                cost.append(min(tile_rows, s.n - r0) * c)
+        order = np.argsort(-np.asarray(cost), kind="stable")
+        tl = np.asarray(tiles, dtype=np.int32)[order]
+        self.n_tiles = len(tl)
+        self._tiles = torch().from_numpy(np.ascontiguousarray(tl)).to(self.device)
+        self.tile_rows = tile_rows
+
+    def _launch(self, stream=None):
+        grid = self.grid or min(self.n_tiles, 2048)
+        nat.call("nfx_assemble_darcy_batched", len(self.instances), nat.ptr(self._items), self.n_tiles,
+                 nat.ptr(self._tiles), self.tile_rows, grid, nat.stream_handle(stream))
+
+    def assemble(self, stream=None):
+        """All instances' FP32 CSR values and residuals (homogeneous Dirichlet data)."""
+        if self.graph is not None:
+            self.graph.replay()
+        else:
+            self._launch(stream)
+
+    def autotune(self, reps=5):
+        """Pick tile rows and grid by timing the batch itself; returns the table of trials."""
+        t = torch()
+        cus = t.cuda.get_device_properties(t.cuda.current_device()).multi_processor_count
+        trials = []
+        best = None
+        for tile in (64, 128, 256):
+            self._build_tiles(tile)
+            for mult in (1, 2, 4, 8):
+                self.grid = min(self.n_tiles, cus * mult)
+                self._launch()
+                t.cuda.synchronize()
+                e0, e1 = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+                e0.record()
+                for _ in range(reps):
+                    self._launch()
+                e1.record()
+                t.cuda.synchronize()
+                ms = e0.elapsed_time(e1) / reps
+                trials.append({"tile_rows": tile, "grid": self.grid, "ms": ms})
+                if best is None or ms < best[0]:
+                    best = (ms, tile, self.grid)
+        self._build_tiles(best[1])
+        self.grid = best[2]
+        self.graph = None
+        return trials
+
+    def capture(self):
+        """Record the batched assembly in a graph (replayed by ``assemble``)."""
+        t = torch()
+        s = t.cuda.Stream()
+        s.wait_stream(t.cuda.current_stream())
+        with t.cuda.stream(s):
+            self._launch()
+        t.cuda.current_stream().wait_stream(s)
+        t.cuda.synchronize()
+        g = t.cuda.CUDAGraph()
+        with t.cuda.graph(g):
+            self._launch()
+        self.graph = g
+        return g
+
+    def solve(self, i, rtol=1e-8):
+        """FP64-refined solve of instance i (FP64 values assembled on demand)."""
+        from .solvers import MinresIR, darcy_preconditioner
+
+        inst = self.instances[i]
+        s = inst["system"]
+        s.assemble_elements(inst["kinv"], inst["f"], fp64=True)
+        sign, pinv = darcy_preconditioner(s.rowptr, s.col, s.vals64, s.n, s.n_u)
+        solver = MinresIR(s.rowptr, s.col, inst["vals"], s.vals64, sign, pinv)
+        x, info = solver.solve(-s.rhs64, rtol=rtol, max_inner=100000)
+        solver.close()
+        return x, info

@@ -280,24 +280,24 @@ __global__ __launch_bounds__(256) void k_facet_bc(
 // row a = NB + k: B[k][b] (b < NB), 0 (b >= NB), residual -sum_q w detJ f psi_k.
 constexpr int kRowsPerBlock = 128;
 
-template <int DIM, int NB, int NP, int NQ, typename R>
-__global__ __launch_bounds__(kRowsPerBlock) void k_darcy_rows(
-    int n_rows, const int32_t* __restrict__ inc, const uint8_t* __restrict__ pos,
+// One tile of TILE consecutive rows [r0, r0 + TILE) of one system. st / sv are
+// LDS buffers of at least NT and TILE * 2 * NL values. Called by all threads of
+// the block; contains block barriers.
+template <int DIM, int NB, int NP, int NQ, int TILE, typename R>
+__device__ __forceinline__ void darcy_row_tile(
+    int r0, int n_rows, const int32_t* __restrict__ inc, const uint8_t* __restrict__ pos,
     const int32_t* __restrict__ rowptr, const uint32_t* __restrict__ signs,
     const int32_t* __restrict__ cells, const R* __restrict__ coords, const R* __restrict__ tab,
     const R* __restrict__ kinv, int ks_e, int ks_q, const R* __restrict__ f, int fs_e, int fs_q,
-    R* __restrict__ vals, R* __restrict__ rhs) {
+    R* __restrict__ vals, R* __restrict__ rhs, R* st, R* sv) {
   constexpr int NL = NB + NP;
   constexpr int NT = NQ * (1 + NB * DIM + NB + NP);
-  __shared__ R st[NT];
-  __shared__ R sv[kRowsPerBlock * 2 * NL];
-  const int r0 = blockIdx.x * kRowsPerBlock;
   const int r = r0 + threadIdx.x;
-  const int rl = min(r0 + kRowsPerBlock, n_rows);
-  for (int i = threadIdx.x; i < NT; i += kRowsPerBlock) st[i] = tab[i];
+  const int rl = min(r0 + TILE, n_rows);
+  for (int i = threadIdx.x; i < NT; i += TILE) st[i] = tab[i];
   const int base = rowptr[r0];
   const int span = rowptr[rl] - base;
-  for (int i = threadIdx.x; i < span; i += kRowsPerBlock) sv[i] = R(0);
+  for (int i = threadIdx.x; i < span; i += TILE) sv[i] = R(0);
   __syncthreads();
   const R* tw = st;
   const R* tphi = st + NQ;
@@ -314,13 +314,13 @@ __global__ __launch_bounds__(kRowsPerBlock) void k_darcy_rows(
       const int a = code & 31;
       const uint32_t sg = signs[e] ^ (((signs[e] >> a) & 1u) ? 0xffffffffu : 0u);
       R row[NL];
-      if (a < NB) {
-        Geom<DIM, R> g;
-        R X[DIM + 1][3];
-        load_geometry<DIM, R>(cells, coords, e, g, X);
-        const R invdet = R(1) / g.detJ;
 #pragma unroll
-        for (int b = 0; b < NL; ++b) row[b] = 0;
+      for (int b = 0; b < NL; ++b) row[b] = 0;
+      Geom<DIM, R> g;
+      R X[DIM + 1][3];
+      load_geometry<DIM, R>(cells, coords, e, g, X);
+      if (a < NB) {
+        const R invdet = R(1) / g.detJ;
         for (int q = 0; q < NQ; ++q) {
           const R w = tw[q];
           const R cm = w * kinv[(int64_t)e * ks_e + q * ks_q] * invdet;
@@ -345,20 +345,11 @@ __global__ __launch_bounds__(kRowsPerBlock) void k_darcy_rows(
         }
       } else {
         const int k = a - NB;
-#pragma unroll
-        for (int b = 0; b < NL; ++b) row[b] = 0;
-        R detJ = 0;
-        {
-          Geom<DIM, R> g;
-          R X[DIM + 1][3];
-          load_geometry<DIM, R>(cells, coords, e, g, X);
-          detJ = g.detJ;
-        }
         for (int q = 0; q < NQ; ++q) {
           const R wp = tw[q] * tpsi[q * NP + k];
 #pragma unroll
           for (int b = 0; b < NB; ++b) row[b] += wp * tdiv[q * NB + b];
-          res -= wp * detJ * f[(int64_t)e * fs_e + q * fs_q];
+          res -= wp * g.detJ * f[(int64_t)e * fs_e + q * fs_q];
         }
       }
       const uint8_t* pr = pos + ((int64_t)r * 2 + j) * NL;
@@ -372,7 +363,66 @@ __global__ __launch_bounds__(kRowsPerBlock) void k_darcy_rows(
     rhs[r] = res;
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < span; i += kRowsPerBlock) vals[base + i] = sv[i];
+  for (int i = threadIdx.x; i < span; i += TILE) vals[base + i] = sv[i];
+  __syncthreads();
+}
+
+template <int DIM, int NB, int NP, int NQ, typename R>
+__global__ __launch_bounds__(kRowsPerBlock) void k_darcy_rows(
+    int n_rows, const int32_t* __restrict__ inc, const uint8_t* __restrict__ pos,
+    const int32_t* __restrict__ rowptr, const uint32_t* __restrict__ signs,
+    const int32_t* __restrict__ cells, const R* __restrict__ coords, const R* __restrict__ tab,
+    const R* __restrict__ kinv, int ks_e, int ks_q, const R* __restrict__ f, int fs_e, int fs_q,
+    R* __restrict__ vals, R* __restrict__ rhs) {
+  constexpr int NL = NB + NP;
+  constexpr int NT = NQ * (1 + NB * DIM + NB + NP);
+  __shared__ R st[NT];
+  __shared__ R sv[kRowsPerBlock * 2 * NL];
+  darcy_row_tile<DIM, NB, NP, NQ, kRowsPerBlock, R>(blockIdx.x * kRowsPerBlock, n_rows, inc, pos,
+                                                    rowptr, signs, cells, coords, tab, kinv, ks_e,
+                                                    ks_q, f, fs_e, fs_q, vals, rhs, st, sv);
+}
+
+// ---------------------------------------------------------------------------
+// Batched persistent assembly: many independent 3-D RT_k - P_{k-1} systems.
+struct DarcyBatchItem {
+  int k;
+  int n_rows;
+  int ks_e, ks_q, fs_e, fs_q;
+  const int32_t* inc;
+  const uint8_t* pos;
+  const int32_t* rowptr;
+  const uint32_t* signs;
+  const int32_t* cells;
+  const float* coords;
+  const float* tab;
+  const float* kinv;
+  const float* f;
+  float* vals;
+  float* rhs;
+};
+
+// tiles: (item, first row) pairs, heaviest first; block b takes tiles b, b + G, ...
+template <int TILE>
+__global__ __launch_bounds__(TILE) void k_darcy_rows_batched(const DarcyBatchItem* __restrict__ items,
+                                                             const int2* __restrict__ tiles,
+                                                             int n_tiles) {
+  constexpr int NT2 = 27 * (1 + 15 * 3 + 15 + 4);
+  __shared__ float st[NT2];
+  __shared__ float sv[TILE * 2 * 19];
+  for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    const int2 tl = tiles[t];
+    const DarcyBatchItem& it = items[tl.x];
+    if (it.k == 1)
+      darcy_row_tile<3, 4, 1, 8, TILE, float>(tl.y, it.n_rows, it.inc, it.pos, it.rowptr, it.signs,
+                                              it.cells, it.coords, it.tab, it.kinv, it.ks_e, it.ks_q,
+                                              it.f, it.fs_e, it.fs_q, it.vals, it.rhs, st, sv);
+    else
+      darcy_row_tile<3, 15, 4, 27, TILE, float>(tl.y, it.n_rows, it.inc, it.pos, it.rowptr,
+                                                it.signs, it.cells, it.coords, it.tab, it.kinv,
+                                                it.ks_e, it.ks_q, it.f, it.fs_e, it.fs_q, it.vals,
+                                                it.rhs, st, sv);
+  }
 }
 
 template <int DIM, int NB, int NP, int NQ, typename R>
@@ -615,3 +665,28 @@ extern "C" int nfx_assemble_elasticity_bc(int n_facets, const int32_t* d_facet,
                                                    d_coords, d_trace, d_fw, d_uD, d_rows, d_rhs, s,
                                                    -1.0);
 }
+
+extern "C" int nfx_assemble_darcy_batched(int n_items, const void* d_items, int n_tiles,
+                                          const int32_t* d_tiles, int tile_rows, int grid,
+                                          void* stream) {
+  NFX_REQUIRE(n_items >= 0 && n_tiles >= 0 && grid > 0, "nfx_assemble_darcy_batched: sizes");
+  if (n_tiles == 0) return NFX_OK;
+  NFX_REQUIRE(d_items && d_tiles, "nfx_assemble_darcy_batched: null pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const DarcyBatchItem* items = static_cast<const DarcyBatchItem*>(d_items);
+  const int2* tiles = reinterpret_cast<const int2*>(d_tiles);
+  if (tile_rows == 64)
+    hipLaunchKernelGGL(k_darcy_rows_batched<64>, dim3(grid), dim3(64), 0, s, items, tiles, n_tiles);
+  else if (tile_rows == 128)
+    hipLaunchKernelGGL(k_darcy_rows_batched<128>, dim3(grid), dim3(128), 0, s, items, tiles, n_tiles);
+  else if (tile_rows == 256)
+    hipLaunchKernelGGL(k_darcy_rows_batched<256>, dim3(grid), dim3(256), 0, s, items, tiles, n_tiles);
+  else {
+    set_error("nfx_assemble_darcy_batched: tile_rows must be 64, 128 or 256");
+    return NFX_E_INVALID;
+  }
+  NFX_LAUNCH_CHECK();
+  return NFX_OK;
+}
+
+extern "C" int nfx_darcy_batch_item_size(void) { return (int)sizeof(DarcyBatchItem); }
