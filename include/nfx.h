@@ -248,6 +248,70 @@ int nfx_field_error(int kind, int dim, int nb, int nq, int n_comp, int n_elem,
                     const double* d_alpha, const double* d_exact, double* h_out,
                     void* stream);
 
+/* ---- learned correction of the Darcy block preconditioner (RT1-P0) --------
+ * No reference counterpart: the reference factorises the Darcy system with
+ * LU/MUMPS (examples/md_elliptic_scalar/md_elliptic_scalar.py:351-364). These
+ * entry points build the pressure Schur approximation S = B diag(M)^-1 B^T of the
+ * assembled matrix, run the message-passing network that maps S to a symmetric
+ * sparse smoother W (pattern of S), train it, and apply the two-grid cycle that
+ * replaces diag(S)^-1 in the block preconditioner. Layouts:
+ *   ELL: row p has 4 slots (the cells across its faces in ascending face order),
+ *        d_nbr (E,4) int32 (-1 = boundary face), values (E,4);
+ *   sample blocks: (ns, E) row-major (one sample per row);
+ *   theta: nfx_gnn_num_params() parameters, FP32 master copy + bf16 copy (same
+ *        layout: W0 (8,64) b0 (64) | per layer Wh (64,64) Wm (64,64) b (64) | ud (64)
+ *        cd ue (64) ce);
+ *   activations: h (5,E,64) bf16, m (4,E,64) bf16, z (5,E,64) FP32.
+ * All launches are asynchronous on `stream`. */
+int nfx_gnn_num_params(void);
+/* S in ELL form (FP64) from the assembled FP64 values: pos_b / pos_qb / pos_m are
+ * (E,4) value indices of B_pf, B_qf and M_ff (-1 = none). */
+int nfx_schur_ell(int E, const int32_t* d_pos_b, const int32_t* d_pos_qb,
+                  const int32_t* d_pos_m, const double* d_vals64, double* d_sdiag,
+                  double* d_soff, void* stream);
+/* Couplings a (E,4), node features x (E,8), FP32 copies of S (diag, off). */
+int nfx_gnn_features(int E, const int32_t* d_nbr, const double* d_sdiag,
+                     const double* d_soff, float* d_a, float* d_x, float* d_sd32,
+                     float* d_so32, void* stream);
+/* Forward pass: smoother entries Wd (E), Wo (E,4); activations kept for backward. */
+int nfx_gnn_forward(int E, const int32_t* d_nbr, const float* d_a, const float* d_x,
+                    const float* d_theta32, const uint16_t* d_theta16, uint16_t* d_h,
+                    uint16_t* d_m, float* d_z, const double* d_sdiag, float* d_Wd,
+                    float* d_Wo, void* stream);
+/* Floats needed by the d_partial argument of nfx_gnn_backward. */
+int nfx_gnn_backward_workspace(int E);
+/* Backward pass from gamma_d = dL/dW_pp, gamma_o = dL/dW_pq + dL/dW_qp to the
+ * parameter gradient (FP32, nfx_gnn_num_params() values). d_dh, d_dm: (E,64) FP32. */
+int nfx_gnn_backward(int E, const int32_t* d_nbr, const float* d_a, const float* d_x,
+                     const uint16_t* d_theta16, const uint16_t* d_h, const uint16_t* d_m,
+                     const float* d_z, const double* d_sdiag, const float* d_gamma_d,
+                     const float* d_gamma_o, float* d_dh, float* d_dm, float* d_partial,
+                     float* d_grad, void* stream);
+/* Y = beta Y0 + alpha (diag d + ELL off) X on a sample block (Y0 may be NULL). */
+int nfx_ell_spmm(int E, int ns, const int32_t* d_nbr, const float* d_d, const float* d_off,
+                 float alpha, const float* d_X, float beta, const float* d_Y0, float* d_Y,
+                 void* stream);
+/* Loss gradient w.r.t. the smoother entries:
+ * gamma_d[p] = sum_s w_s Su_sp Se_sp, gamma_o[p,j] = sum_s w_s (Su_sp Se_sq + Su_sq Se_sp). */
+int nfx_smoother_grad(int E, int ns, const int32_t* d_nbr, const float* d_Su,
+                      const float* d_Se, const float* d_w, float* d_gamma_d,
+                      float* d_gamma_o, void* stream);
+/* C[s][a] = sum of X[s][p] over the members of aggregate a (CSR d_agg_ptr / d_agg_mem). */
+int nfx_agg_sum(int n_agg, int ns, int E, const int32_t* d_agg_ptr, const int32_t* d_agg_mem,
+                const float* d_X, float* d_C, void* stream);
+/* Dense coarse operator P^T S P (FP64, row-major n_agg x n_agg); d_Ac must be zero. */
+int nfx_coarse_dense(int n_agg, const int32_t* d_agg_ptr, const int32_t* d_agg_mem,
+                     const int32_t* d_agg_of, const int32_t* d_nbr, const double* d_sdiag,
+                     const double* d_soff, double* d_Ac, void* stream);
+/* Block preconditioner with the two-grid Schur cycle: z_u = dMinv r_u (n_u flux
+ * rows), z_p = psign (x2 + W (r_p - S x2)), x2 = x1 + P Ainv P^T (r_p - S x1),
+ * x1 = W r_p. d_work: 3 E + 2 n_agg floats. */
+int nfx_twogrid_apply(int n_u, int E, int n_agg, const int32_t* d_nbr, const float* d_Wd,
+                      const float* d_Wo, const float* d_sd, const float* d_so,
+                      const int32_t* d_agg_ptr, const int32_t* d_agg_mem,
+                      const int32_t* d_agg_of, const float* d_Ainv, const float* d_dMinv,
+                      float psign, const float* d_r, float* d_z, float* d_work, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -70,7 +70,30 @@ SIGNATURES = {
                                     c_void_p, c_void_p, c_void_p, c_int]),
     "nfx_field_error": (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
                                 c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "nfx_gnn_num_params": (c_int, []),
+    "nfx_schur_ell": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "nfx_gnn_features": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                 c_void_p]),
+    "nfx_gnn_forward": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "nfx_gnn_backward_workspace": (c_int, [c_int]),
+    "nfx_gnn_backward": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                 c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                 c_void_p]),
+    "nfx_ell_spmm": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_float, c_void_p, c_float, c_void_p,
+                             c_void_p, c_void_p]),
+    "nfx_smoother_grad": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                  c_void_p]),
+    "nfx_agg_sum": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "nfx_coarse_dense": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                 c_void_p]),
+    "nfx_twogrid_apply": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                  c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_void_p,
+                                  c_void_p, c_void_p, c_void_p]),
 }
+
+# entry points whose return value is a count, not a status
+_RAW = ("nfx_version", "nfx_darcy_batch_item_size", "nfx_gnn_num_params", "nfx_gnn_backward_workspace")
 
 
 class NfxError(RuntimeError):
@@ -116,7 +139,7 @@ def check(rc: int, what: str = "") -> int:
 
 def call(name: str, *args) -> int:
     rc = getattr(lib(), name)(*args)
-    if name in ("nfx_version", "nfx_darcy_batch_item_size"):
+    if name in _RAW:
         return rc
     return check(rc, name)
 

@@ -1,0 +1,434 @@
+"""Learned correction of the Darcy block preconditioner (RT1-P0), on the device.
+
+The plain block preconditioner of ``solvers.darcy_preconditioner`` uses
+``diag(M)^-1`` on the flux rows and ``diag(S)^-1`` on the pressure rows, with
+``S = B diag(M)^-1 B^T``. The learned preconditioner replaces ``diag(S)^-1`` by one
+symmetric two-grid cycle on ``S``:
+
+    x1 = W r;  x2 = x1 + P Ac^-1 P^T (r - S x1);  z = x2 + W (r - S x2),
+
+with ``P`` the piecewise-constant prolongation from aggregates of cells (boxes of
+side ``agg_h``, two hexes of a Kuhn mesh by default), ``Ac = P^T S P`` inverted
+exactly (dense, FP64 Cholesky, FP32 inverse), and ``W`` a symmetric sparse smoother
+with the pattern of ``S`` whose entries come from a message-passing network:
+4 layers of width 64 on the cell graph of ``S`` (bf16 weights and activations,
+FP32 accumulation), run once per system. With the output heads at zero ``W`` is
+the damped Jacobi smoother ``0.65 diag(S)^-1``: that is the method without the
+network, and the trained heads are what the network adds.
+
+The network is trained on the device with the library's forward and backward
+kernels. The loss for a system is the mean over sample errors ``e`` of
+``||(I - Pi)(I - W S) e||_S^2 / ||e||_S^2`` (``Pi = P Ac^-1 P^T S``), the squared
+two-grid error reduction; samples are Gaussian vectors and the pressure errors of
+plain-preconditioned Krylov iterates of the system (``x* - x_k``).
+
+The outer solver is FGMRES with FP64 iterative refinement (``fgmres.py``). A network
+whose smoother has non-finite entries, or a learned solve that does not converge,
+falls back to the plain block preconditioner with MINRES.
+
+Reference boundary: the KSP solve of the Darcy driver
+(``examples/md_elliptic_scalar/md_elliptic_scalar.py:351-364``); the network has no
+reference counterpart.
+"""
+
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import _native as nat
+from .device import to_dev, torch
+
+F_IN = 8
+WIDTH = 64
+LAYERS = 4
+OMEGA0 = 0.65
+
+
+def param_layout():
+    """Flat parameter layout (name -> (offset, shape)); the library uses the same."""
+    out, off = {}, 0
+    names = [("W0", (F_IN, WIDTH)), ("b0", (WIDTH,))]
+    for l in range(LAYERS):
+        names += [(f"Wh{l}", (WIDTH, WIDTH)), (f"Wm{l}", (WIDTH, WIDTH)), (f"b{l + 1}", (WIDTH,))]
+    names += [("ud", (WIDTH,)), ("cd", (1,)), ("ue", (WIDTH,)), ("ce", (1,))]
+    for name, shape in names:
+        out[name] = (off, shape)
+        off += int(np.prod(shape))
+    return out, off
+
+
+def init_params(seed=0, head_scale=0.0):
+    """He-initialised encoder and layers; output heads at ``head_scale`` (0 = Jacobi)."""
+    lay, n = param_layout()
+    rng = np.random.default_rng(seed)
+    th = np.zeros(n, dtype=np.float32)
+    for name, (o, shape) in lay.items():
+        size = int(np.prod(shape))
+        if name == "W0":
+            th[o:o + size] = rng.standard_normal(size) * np.sqrt(2.0 / F_IN)
+        elif name.startswith("Wh") or name.startswith("Wm"):
+            th[o:o + size] = rng.standard_normal(size) * (0.5 / np.sqrt(2 * WIDTH))
+        elif name in ("ud", "ue"):
+            th[o:o + size] = rng.standard_normal(size) * head_scale / np.sqrt(WIDTH)
+    return th
+
+
+def load_params(path):
+    with np.load(path, allow_pickle=False) as z:
+        th = np.asarray(z["theta"], dtype=np.float32)
+    if th.size != param_layout()[1]:
+        raise ValueError(f"{path}: {th.size} parameters, expected {param_layout()[1]}")
+    return th
+
+
+def default_weights_path():
+    from pathlib import Path
+
+    return Path(__file__).resolve().parent / "data" / "learned_schur_rt1p0.npz"
+
+
+class SchurGraph:
+    """Host-side structure of S and the aggregates, once per Darcy system pattern."""
+
+    def __init__(self, sysd, agg_h=None, device="cuda"):
+        if sysd.k != 1 or sysd.dim != 3:
+            raise NotImplementedError("the learned Schur preconditioner covers 3-D RT1-P0 systems")
+        pat = sysd.pattern
+        rowptr, col = pat.rowptr.astype(np.int64), pat.col.astype(np.int64)
+        n, n_u = sysd.n, sysd.n_u
+        E = n - n_u
+        self.E, self.n, self.n_u = E, n, n_u
+        # diagonal positions of the flux block
+        pos_diag = np.full(n, -1, dtype=np.int64)
+        rows = np.repeat(np.arange(n), np.diff(rowptr))
+        d = np.flatnonzero(rows == col)
+        pos_diag[rows[d]] = d
+        # B rows: pressure row n_u + p; its flux columns (ascending) precede its
+        # pressure columns, so the j-th flux entry sits at rowptr[n_u + p] + j
+        ent = np.arange(rowptr[n_u], rowptr[n])
+        fl = col[ent] < n_u
+        ent = ent[fl]
+        r = rows[ent] - n_u
+        j = ent - rowptr[n_u + r]
+        if len(ent) and j.max() >= 4:
+            raise ValueError("pressure row with more than 4 flux couplings")
+        pos_b = -np.ones((E, 4), dtype=np.int64)
+        face = -np.ones((E, 4), dtype=np.int64)
+        pos_b[r, j] = ent
+        face[r, j] = col[ent]
+        has = face >= 0
+        # faces touched by two cells: pair the two (cell, slot) incidences
+        order = np.argsort(col[ent], kind="stable")
+        fs = col[ent][order]
+        if len(fs) > 2 and (fs[2:] == fs[:-2]).any():
+            raise ValueError("face shared by more than two cells")
+        dup = np.flatnonzero(fs[1:] == fs[:-1])
+        a1, a2 = order[dup], order[dup + 1]
+        nbr = -np.ones((E, 4), dtype=np.int64)
+        pos_qb = -np.ones((E, 4), dtype=np.int64)
+        nbr[r[a1], j[a1]] = r[a2]
+        nbr[r[a2], j[a2]] = r[a1]
+        pos_qb[r[a1], j[a1]] = ent[a2]
+        pos_qb[r[a2], j[a2]] = ent[a1]
+        pos_m = np.where(has, pos_diag[np.where(has, face, 0)], -1)
+        if (pos_m[has] < 0).any():
+            raise ValueError("flux diagonal missing from the pattern")
+        self.nbr_np = nbr
+        # aggregates: boxes of side agg_h over the cell centroids
+        pts = sysd.u_sp.mesh.points
+        X = pts[sysd.cells_np]
+        cent = X.mean(axis=1)
+        if agg_h is None:
+            vol = np.abs(np.einsum("ij,ij->i", X[:, 1] - X[:, 0],
+                                   np.cross(X[:, 2] - X[:, 0], X[:, 3] - X[:, 0]))) / 6.0
+            agg_h = 2.0 * float(np.cbrt(6.0 * vol.mean()))
+        self.agg_h = agg_h
+        agg, n_agg = aggregate_cells(cent, agg_h)
+        # cells of the pressure block follow the element order (P0: one dof per cell)
+        p_rows = sysd.space.reference_destinations()[0][:, sysd.nb:].ravel() - n_u
+        if not np.array_equal(np.sort(p_rows), np.arange(E)):
+            raise ValueError("pressure dofs are not one per cell")
+        agg_of = np.empty(E, dtype=np.int64)
+        agg_of[p_rows] = agg
+        self.agg_of_np, self.n_agg = agg_of, n_agg
+        order = np.argsort(agg_of, kind="stable")
+        ptr = np.zeros(n_agg + 1, dtype=np.int64)
+        np.add.at(ptr, agg_of + 1, 1)
+        ptr = np.cumsum(ptr)
+        i32 = lambda a: to_dev(np.ascontiguousarray(a, dtype=np.int32), device=device)
+        self.nbr, self.pos_b, self.pos_qb, self.pos_m = i32(nbr), i32(pos_b), i32(pos_qb), i32(pos_m)
+        self.agg_ptr, self.agg_mem, self.agg_of = i32(ptr), i32(order), i32(agg_of)
+        self.diag_pos_u = to_dev(pos_diag[:n_u], device=device).long()
+        self.device = device
+
+
+def aggregate_cells(centroids, h):
+    """Aggregate per cell: boxes of side h anchored at the smallest centroid
+    coordinate (shifted by h/16 so that box faces fall between cell centroids of a
+    structured mesh), numbered by ascending (z, y, x) box among the non-empty boxes."""
+    lo = centroids.min(axis=0)
+    key = np.floor((centroids - lo) / h + 1.0 / 16.0).astype(np.int64)
+    nb = key.max(axis=0) + 1
+    lin = key[:, 0] + nb[0] * (key[:, 1] + nb[1] * key[:, 2])
+    u, inv = np.unique(lin, return_inverse=True)
+    return inv.astype(np.int64), len(u)
+
+
+class LearnedSchurPC:
+    """Two-grid Schur block preconditioner with the network smoother (callable z = M^-1 r)."""
+
+    def __init__(self, sysd, theta=None, agg_h=None, psign=-1.0, graph=None):
+        t = torch()
+        self.sys = sysd
+        self.g = graph if graph is not None else SchurGraph(sysd, agg_h, device=sysd.device)
+        g = self.g
+        dev = sysd.device
+        E, n_agg = g.E, g.n_agg
+        self.E = E
+        f32 = lambda *s: t.empty(*s, dtype=t.float32, device=dev)
+        self.sdiag = t.empty(E, dtype=t.float64, device=dev)
+        self.soff = t.empty(E * 4, dtype=t.float64, device=dev)
+        self.a, self.x = f32(E * 4), f32(E * F_IN)
+        self.sd32, self.so32 = f32(E), f32(E * 4)
+        self.Wd, self.Wo = f32(E), f32(E * 4)
+        self.work = f32(3 * E + 2 * n_agg)
+        self.Ainv = None
+        self.dMinv = f32(g.n_u)
+        self.psign = float(psign)
+        self.h = self.m = self.z = None
+        self.n_params = nat.call("nfx_gnn_num_params")
+        if self.n_params != param_layout()[1]:
+            raise RuntimeError("network parameter layout differs between host and library")
+        self.set_params(theta)
+        self.network_used = False
+        self.fallback_reason = None
+
+    def set_params(self, theta):
+        t = torch()
+        if theta is None:
+            self.theta32 = None
+            self.theta16 = None
+            return
+        th = t.as_tensor(np.asarray(theta, dtype=np.float32) if not hasattr(theta, "device") else theta,
+                         dtype=t.float32, device=self.sys.device).contiguous()
+        self.theta32 = th
+        self.theta16 = th.to(t.bfloat16).view(t.int16).contiguous()
+
+    def _activations(self):
+        t = torch()
+        if self.h is None:
+            EW = self.E * WIDTH
+            dev = self.sys.device
+            self.h = t.empty(5 * EW, dtype=t.int16, device=dev)
+            self.m = t.empty(4 * EW, dtype=t.int16, device=dev)
+            self.z = t.empty(5 * EW, dtype=t.float32, device=dev)
+
+    def schur(self, vals64, stream=None):
+        """S (FP64 ELL) and the network inputs from the assembled FP64 values."""
+        g, s = self.g, nat.stream_handle(stream)
+        nat.call("nfx_schur_ell", g.E, nat.ptr(g.pos_b), nat.ptr(g.pos_qb), nat.ptr(g.pos_m), nat.ptr(vals64),
+                 nat.ptr(self.sdiag), nat.ptr(self.soff), s)
+        nat.call("nfx_gnn_features", g.E, nat.ptr(g.nbr), nat.ptr(self.sdiag), nat.ptr(self.soff), nat.ptr(self.a),
+                 nat.ptr(self.x), nat.ptr(self.sd32), nat.ptr(self.so32), s)
+        self.dMinv.copy_(1.0 / vals64[self.g.diag_pos_u].abs())
+
+    def forward(self, stream=None):
+        g = self.g
+        self._activations()
+        nat.call("nfx_gnn_forward", g.E, nat.ptr(g.nbr), nat.ptr(self.a), nat.ptr(self.x), nat.ptr(self.theta32),
+                 nat.ptr(self.theta16), nat.ptr(self.h), nat.ptr(self.m), nat.ptr(self.z), nat.ptr(self.sdiag),
+                 nat.ptr(self.Wd), nat.ptr(self.Wo), nat.stream_handle(stream))
+
+    def coarse(self, stream=None):
+        """Ac = P^T S P (FP64, deterministic kernel), exact inverse via Cholesky."""
+        t = torch()
+        g = self.g
+        Ac = t.zeros((g.n_agg, g.n_agg), dtype=t.float64, device=self.sys.device)
+        nat.call("nfx_coarse_dense", g.n_agg, nat.ptr(g.agg_ptr), nat.ptr(g.agg_mem), nat.ptr(g.agg_of),
+                 nat.ptr(g.nbr), nat.ptr(self.sdiag), nat.ptr(self.soff), nat.ptr(Ac), nat.stream_handle(stream))
+        L = t.linalg.cholesky(Ac)
+        self.L = L
+        self.Ainv = t.cholesky_inverse(L).to(t.float32).contiguous()
+
+    def setup(self, vals64=None, network=True, stream=None):
+        """Per-system set-up after FP64 assembly. Returns True if the network smoother is used."""
+        t = torch()
+        vals64 = self.sys.vals64 if vals64 is None else vals64
+        self.schur(vals64, stream)
+        self.network_used = False
+        self.fallback_reason = None
+        if network and self.theta32 is not None:
+            if not bool(t.isfinite(self.theta32).all()):
+                # a NaN weight can vanish inside relu (max(NaN, 0) = 0): check the weights themselves
+                self.fallback_reason = "non-finite network parameters"
+            else:
+                self.forward(stream)
+                if bool(t.isfinite(self.Wd).all()) and bool(t.isfinite(self.Wo).all()):
+                    self.network_used = True
+                else:
+                    self.fallback_reason = "non-finite smoother entries"
+        if not self.network_used:
+            self.Wd.copy_(OMEGA0 / self.sd32)
+            self.Wo.zero_()
+        self.coarse(stream)
+        return self.network_used
+
+    def __call__(self, r, out=None, stream=None):
+        t = torch()
+        g = self.g
+        z = out if out is not None else t.empty_like(r)
+        nat.call("nfx_twogrid_apply", g.n_u, g.E, g.n_agg, nat.ptr(g.nbr), nat.ptr(self.Wd), nat.ptr(self.Wo),
+                 nat.ptr(self.sd32), nat.ptr(self.so32), nat.ptr(g.agg_ptr), nat.ptr(g.agg_mem), nat.ptr(g.agg_of),
+                 nat.ptr(self.Ainv), nat.ptr(self.dMinv), self.psign, nat.ptr(r), nat.ptr(z), nat.ptr(self.work),
+                 nat.stream_handle(stream))
+        return z
+
+    def smoother_numpy(self):
+        """(Wd, Wo (E,4)) on the host (tests)."""
+        return self.Wd.cpu().numpy(), self.Wo.cpu().numpy().reshape(-1, 4)
+
+
+class Adam:
+    """Adam on the FP32 master parameters (device tensors)."""
+
+    def __init__(self, theta, lr=1e-3, betas=(0.9, 0.999), eps=1e-8, clip=1.0):
+        t = torch()
+        self.theta = theta
+        self.m = t.zeros_like(theta)
+        self.v = t.zeros_like(theta)
+        self.lr, self.b1, self.b2, self.eps, self.clip = lr, betas[0], betas[1], eps, clip
+        self.k = 0
+
+    def step(self, grad):
+        t = torch()
+        gn = t.linalg.vector_norm(grad)
+        if self.clip and float(gn) > self.clip:
+            grad = grad * (self.clip / gn)
+        self.k += 1
+        self.m.mul_(self.b1).add_(grad, alpha=1 - self.b1)
+        self.v.mul_(self.b2).addcmul_(grad, grad, value=1 - self.b2)
+        mh = self.m / (1 - self.b1 ** self.k)
+        vh = self.v / (1 - self.b2 ** self.k)
+        self.theta.sub_(self.lr * mh / (vh.sqrt() + self.eps))
+        return float(gn)
+
+
+class SmootherTrainer:
+    """Loss, gradient and Adam steps of the network smoother on device-resident systems."""
+
+    def __init__(self, pc: LearnedSchurPC, theta0, lr=1e-3, n_random=12, seed=0):
+        t = torch()
+        self.pc = pc
+        pc.set_params(theta0)
+        self.opt = Adam(pc.theta32, lr=lr)
+        self.n_random = n_random
+        self.gen = t.Generator(device=pc.sys.device)
+        self.gen.manual_seed(seed)
+        E = pc.E
+        dev = pc.sys.device
+        self.dh = t.empty(E * WIDTH, dtype=t.float32, device=dev)
+        self.dm = t.empty(E * WIDTH, dtype=t.float32, device=dev)
+        self.partial = t.empty(nat.call("nfx_gnn_backward_workspace", E), dtype=t.float32, device=dev)
+        self.grad = t.zeros(pc.n_params, dtype=t.float32, device=dev)
+        self.gd = t.empty(E, dtype=t.float32, device=dev)
+        self.go = t.empty(E * 4, dtype=t.float32, device=dev)
+
+    def samples(self, extra=None):
+        t = torch()
+        e = t.randn((self.n_random, self.pc.E), generator=self.gen, device=self.pc.sys.device, dtype=t.float32)
+        if extra is not None and len(extra):
+            ex = extra / extra.norm(dim=1, keepdim=True).clamp_min(1e-30) * np.sqrt(self.pc.E)
+            e = t.cat([e, ex.to(t.float32)], dim=0)
+        return e.contiguous()
+
+    def loss_and_gamma(self, e, stream=None):
+        """Two-grid loss of the current smoother on samples e (ns, E); fills gamma."""
+        t = torch()
+        pc, g = self.pc, self.pc.g
+        s = nat.stream_handle(stream)
+        ns, E = e.shape
+        Se = t.empty_like(e)
+        v = t.empty_like(e)
+        Sv = t.empty_like(e)
+        Su = t.empty_like(e)
+        ell = lambda d, o, X, Y, alpha=1.0, beta=0.0, Y0=None: nat.call(
+            "nfx_ell_spmm", E, ns, nat.ptr(g.nbr), nat.ptr(d), nat.ptr(o), alpha, nat.ptr(X), beta, nat.ptr(Y0),
+            nat.ptr(Y), s)
+        ell(pc.sd32, pc.so32, e, Se)
+        en = (e.double() * Se.double()).sum(dim=1)
+        ell(pc.Wd, pc.Wo, Se, v, alpha=-1.0, beta=1.0, Y0=e)
+        ell(pc.sd32, pc.so32, v, Sv)
+        c = t.empty((ns, g.n_agg), dtype=t.float32, device=e.device)
+        nat.call("nfx_agg_sum", g.n_agg, ns, E, nat.ptr(g.agg_ptr), nat.ptr(g.agg_mem), nat.ptr(Sv), nat.ptr(c), s)
+        y = t.cholesky_solve(c.double().T.contiguous(), pc.L).T.to(t.float32)
+        u = (v - y[:, g.agg_of.long()]).contiguous()
+        ell(pc.sd32, pc.so32, u, Su)
+        per = (u.double() * Su.double()).sum(dim=1) / en
+        w = (-2.0 / (ns * en)).to(t.float32).contiguous()
+        nat.call("nfx_smoother_grad", E, ns, nat.ptr(g.nbr), nat.ptr(Su), nat.ptr(Se), nat.ptr(w), nat.ptr(self.gd),
+                 nat.ptr(self.go), s)
+        return per.mean()
+
+    def backward(self, stream=None):
+        pc, g = self.pc, self.pc.g
+        nat.call("nfx_gnn_backward", g.E, nat.ptr(g.nbr), nat.ptr(pc.a), nat.ptr(pc.x), nat.ptr(pc.theta16),
+                 nat.ptr(pc.h), nat.ptr(pc.m), nat.ptr(pc.z), nat.ptr(pc.sdiag), nat.ptr(self.gd), nat.ptr(self.go),
+                 nat.ptr(self.dh), nat.ptr(self.dm), nat.ptr(self.partial), nat.ptr(self.grad),
+                 nat.stream_handle(stream))
+        return self.grad
+
+    def step(self, vals64, extra=None):
+        """One Adam step on one assembled system (FP64 values). Returns the loss before the step."""
+        t = torch()
+        pc = self.pc
+        pc.schur(vals64)
+        pc.forward()
+        pc.coarse()
+        e = self.samples(extra)
+        loss = self.loss_and_gamma(e)
+        self.backward()
+        self.opt.step(self.grad)
+        pc.theta16 = pc.theta32.to(t.bfloat16).view(t.int16).contiguous()
+        return float(loss)
+
+    def evaluate(self, vals64, n=32, network=True):
+        """Loss on fresh Gaussian samples without a step."""
+        pc = self.pc
+        pc.setup(vals64, network=network)
+        e = self.samples()
+        return float(self.loss_and_gamma(e))
+
+
+def solve_learned(sysd, pc: LearnedSchurPC, b64, rtol=1e-8, restart=120, maxit=4000, inner_tol=1e-5,
+                  fallback=True):
+    """FGMRES + FP64 refinement with the learned preconditioner; plain MINRES fallback.
+
+    ``pc.setup`` must have run on the current FP64 values. Returns (x64, info)."""
+    from .fgmres import CsrOperator, fgmres_ir
+    from .solvers import MinresIR, darcy_preconditioner
+
+    t = torch()
+    info = {"fallback": None}
+    x = None
+    if pc.network_used or pc.fallback_reason is None:
+        op = CsrOperator(sysd.rowptr, sysd.col, sysd.vals32, sysd.vals64)
+        x, inf = fgmres_ir(op, pc, b64, rtol=rtol, inner_tol=inner_tol, restart=restart, maxit=maxit)
+        info.update(inf)
+        info["method"] = "fgmres+twogrid" + ("+network" if pc.network_used else "+jacobi")
+        if inf["converged"] and np.isfinite(inf["residual"]):
+            return x, info
+        info["fallback"] = "learned solve did not converge"
+    else:
+        info["fallback"] = pc.fallback_reason
+    if not fallback:
+        return x, info
+    sign, pinv = darcy_preconditioner(sysd.rowptr, sysd.col, sysd.vals64, sysd.n, sysd.n_u)
+    solver = MinresIR(sysd.rowptr, sysd.col, sysd.vals32, sysd.vals64, sign, pinv)
+    x0 = x if (x is not None and bool(t.isfinite(x).all())) else None
+    xs, inf = solver.solve(b64, x=x0, rtol=rtol, max_inner=100000)
+    solver.close()
+    info.update(inf)
+    info["method"] = "minres+block-diagonal (fallback)"
+    return xs, info

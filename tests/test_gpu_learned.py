@@ -1,0 +1,228 @@
+"""GPU parity and behaviour of the learned Schur preconditioner (RT1-P0 Darcy).
+
+Against the FP64 oracle (``oracle/learned.py``) on the same assembled system:
+- S in ELL form (pattern exact, values 1e-12), node features (1e-6);
+- network forward with bf16 weights/activations: stored activations agree with
+  the bf16-emulating oracle except for rare last-bit rounding flips, smoother
+  entries within 2e-2 of the Jacobi scale (max) and 2e-3 (mean);
+- network backward on the device activations: gradient per parameter group
+  within 1e-4 (relative L2) of the FP64 oracle backward;
+- loss and its gradient w.r.t. the smoother entries (1e-4 / 1e-3);
+- one two-grid preconditioner application (1e-4).
+Then: the learned preconditioner in FGMRES reaches an FP64 relative residual of
+1e-8; a short training run lowers the two-grid loss on held-out systems and does
+not need more iterations than the method without the network; non-finite network
+weights fall back to the plain block preconditioner and still converge.
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _system(n, seed, sigma=2.0, ell=0.1):
+    import torch
+    from nfempy_amd.batch import darcy_space
+    from nfempy_amd.device import CoefficientArg, DarcySystem
+    from nfempy_amd.fields import grf_device
+
+    sysd = DarcySystem(darcy_space(n, 1))
+    cent = sysd.coords64[sysd.cells.long()].mean(dim=1)
+    logk = grf_device(cent, seed=seed, ell=ell, sigma=sigma)
+    kinv = CoefficientArg(torch.exp(-logk).contiguous(), 1, 0)
+    f = CoefficientArg(torch.ones(1, dtype=torch.float64, device="cuda"), 0, 0)
+    sysd.assemble_elements(kinv, f, fp64=True)
+    sysd.assemble_elements(kinv, f, fp64=False)
+    torch.cuda.synchronize()
+    return sysd, kinv, f
+
+
+def _theta(seed=11, scale=0.15):
+    from nfempy_amd.learned import param_layout
+
+    rng = np.random.default_rng(seed)
+    return (rng.standard_normal(param_layout()[1]) * scale).astype(np.float32)
+
+
+def _bf(t):
+    import torch
+
+    return t.view(torch.bfloat16).float().cpu().numpy().astype(np.float64)
+
+
+def test_learned_parity_with_oracle():
+    import torch
+    import oracle.learned as ol
+    from nfempy_amd.learned import LearnedSchurPC, SmootherTrainer
+
+    print("[learned] parity n=4", flush=True)
+    sysd, _, _ = _system(4, seed=3)
+    th = _theta()
+    pc = LearnedSchurPC(sysd, th)
+    assert pc.setup()
+    torch.cuda.synchronize()
+    g = pc.g
+    E = g.E
+    rp, ci, v64 = sysd.pattern.rowptr, sysd.pattern.col, sysd.vals64.cpu().numpy()
+    nbr, sd, so = ol.schur_ell(rp, ci, v64, sysd.n_u)
+    assert np.array_equal(g.nbr_np, nbr)
+    assert np.allclose(pc.sdiag.cpu().numpy(), sd, rtol=1e-12, atol=0)
+    assert np.allclose(pc.soff.cpu().numpy().reshape(E, 4), so, rtol=1e-12, atol=1e-300)
+    x_or = ol.features(nbr, sd, so)
+    assert np.allclose(pc.x.cpu().numpy().reshape(E, 8), x_or, rtol=1e-6, atol=1e-7)
+
+    # forward
+    Wd_or, Wo_or, cache = ol.forward(th.astype(np.float64), nbr, sd, so, emulate_bf16=True)
+    EW = E * 64
+    h_dev = _bf(pc.h).reshape(5, E, 64)
+    for l in range(5):
+        ref = cache["h"][l]
+        diff = np.abs(h_dev[l] - ref)
+        flips = np.mean(diff > 0)
+        assert flips < 0.02, (l, flips)
+        assert np.all(diff <= 2 ** -6 * np.abs(ref) + 1e-6), l
+    Wd, Wo = pc.smoother_numpy()
+    errd = np.abs(Wd - Wd_or) * sd
+    q = np.where(nbr >= 0, nbr, 0)
+    erro = np.abs(Wo - Wo_or) * np.sqrt(sd[:, None] * sd[q])
+    print(f"[learned] forward max/mean err diag {errd.max():.2e}/{errd.mean():.2e} off {erro.max():.2e}", flush=True)
+    assert errd.max() < 2e-2 and errd.mean() < 2e-3
+    assert erro.max() < 2e-2 and erro.mean() < 2e-3
+
+    # loss + gamma on device samples, against the oracle with the device smoother
+    tr = SmootherTrainer(pc, th, n_random=6, seed=5)
+    pc.setup()
+    e = tr.samples()
+    loss = float(tr.loss_and_gamma(e))
+    L_or, gd_or, go_or = ol.twogrid_loss(Wd.astype(np.float64), Wo.astype(np.float64), nbr, sd, so,
+                                         g.agg_of_np, g.n_agg, e.double().cpu().numpy().T)
+    assert abs(loss - L_or) <= 1e-4 * abs(L_or)
+    gd, go = tr.gd.cpu().numpy(), tr.go.cpu().numpy().reshape(E, 4)
+    assert np.abs(gd - gd_or).max() <= 1e-3 * np.abs(gd_or).max()
+    assert np.abs(go - go_or).max() <= 1e-3 * np.abs(go_or).max()
+
+    # backward on the device activations
+    grad = tr.backward().cpu().numpy().astype(np.float64)
+    dev_cache = dict(cache)
+    dev_cache["h"] = [h_dev[l] for l in range(5)]
+    dev_cache["m"] = [m for m in _bf(pc.m).reshape(4, E, 64)]
+    zz = pc.z.cpu().numpy().astype(np.float64).reshape(5, E, 64)
+    dev_cache["z0"] = zz[0]
+    dev_cache["z"] = [zz[l + 1] for l in range(4)]
+    dev_cache["a"] = pc.a.cpu().numpy().astype(np.float64).reshape(E, 4)
+    dev_cache["x"] = pc.x.cpu().numpy().astype(np.float64).reshape(E, 8)
+    dev_cache["s"] = dev_cache["h"][4][:, None, :] + dev_cache["h"][4][q]
+    g_or = ol.backward(th.astype(np.float64), dev_cache, tr.gd.cpu().numpy().astype(np.float64),
+                       tr.go.cpu().numpy().astype(np.float64).reshape(E, 4))
+    lay, _ = ol.param_layout()
+    for name, (o, shape) in lay.items():
+        size = int(np.prod(shape))
+        a, b = grad[o:o + size], g_or[o:o + size]
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        assert rel < 1e-4, (name, rel)
+
+    # one preconditioner application
+    r = torch.randn(sysd.n, dtype=torch.float32, device="cuda")
+    z = pc(r)
+    torch.cuda.synchronize()
+    rn = r.double().cpu().numpy()
+    zp_or = ol.twogrid_apply(Wd.astype(np.float64), Wo.astype(np.float64), nbr, sd, so, g.agg_of_np, g.n_agg,
+                             rn[sysd.n_u:])
+    zn = z.double().cpu().numpy()
+    assert np.abs(zn[sysd.n_u:] - pc.psign * zp_or).max() <= 1e-4 * np.abs(zp_or).max()
+    dM = np.array([v64[rp[i]:rp[i + 1]][ci[rp[i]:rp[i + 1]] == i][0] for i in range(sysd.n_u)])
+    assert np.allclose(zn[:sysd.n_u], rn[:sysd.n_u] / np.abs(dM), rtol=1e-5)
+
+
+def test_learned_pc_solves_and_beats_plain_iterations():
+    import torch
+    from nfempy_amd.learned import LearnedSchurPC, solve_learned
+    from nfempy_amd.solvers import MinresIR, darcy_preconditioner
+
+    print("[learned] solve n=8", flush=True)
+    sysd, _, _ = _system(8, seed=10000)
+    b = -sysd.rhs64
+    pc = LearnedSchurPC(sysd, None)
+    pc.setup(network=False)
+    x, info = solve_learned(sysd, pc, b, rtol=1e-8)
+    print(f"[learned] two-grid(jacobi) {info['method']} it={info['iterations']} res={info['residual']:.2e}",
+          flush=True)
+    assert info["fallback"] is None and info["converged"] and info["residual"] <= 1e-8
+    sign, pinv = darcy_preconditioner(sysd.rowptr, sysd.col, sysd.vals64, sysd.n, sysd.n_u)
+    solver = MinresIR(sysd.rowptr, sysd.col, sysd.vals32, sysd.vals64, sign, pinv)
+    x2, info2 = solver.solve(b, rtol=1e-8, max_inner=100000)
+    solver.close()
+    print(f"[learned] plain minres it={info2['iterations']}", flush=True)
+    assert info2["converged"]
+    assert info["iterations"] < info2["iterations"]
+    assert torch.allclose(x, x2, rtol=0, atol=1e-6 * float(x2.abs().max()))
+
+
+def test_training_gain_on_held_out_systems():
+    import torch
+    from nfempy_amd.learned import LearnedSchurPC, SmootherTrainer, init_params, solve_learned
+
+    print("[learned] train n=8", flush=True)
+    sysd, _, _ = _system(8, seed=0)
+    pc = LearnedSchurPC(sysd, None)
+    tr = SmootherTrainer(pc, init_params(0), lr=3e-3, n_random=12, seed=1)
+    from nfempy_amd.device import CoefficientArg
+    from nfempy_amd.fields import grf_device
+
+    cent = sysd.coords64[sysd.cells.long()].mean(dim=1)
+    f = CoefficientArg(torch.ones(1, dtype=torch.float64, device="cuda"), 0, 0)
+
+    def assemble(seed):
+        logk = grf_device(cent, seed=seed, ell=0.1, sigma=2.0)
+        kinv = CoefficientArg(torch.exp(-logk).contiguous(), 1, 0)
+        sysd.assemble_elements(kinv, f, fp64=True)
+        sysd.assemble_elements(kinv, f, fp64=False)
+
+    losses = []
+    for step in range(80):
+        assemble(step % 40)
+        losses.append(tr.step(sysd.vals64))
+    print(f"[learned] train loss first {np.mean(losses[:10]):.4f} last {np.mean(losses[-10:]):.4f}", flush=True)
+    theta = pc.theta32.clone()
+    res = []
+    for seed in (10000, 10001):
+        assemble(seed)
+        pc.set_params(theta)
+        l_net = tr.evaluate(sysd.vals64, network=True)
+        x, inf_net = solve_learned(sysd, pc, -sysd.rhs64, rtol=1e-8)
+        l_jac = tr.evaluate(sysd.vals64, network=False)
+        x, inf_jac = solve_learned(sysd, pc, -sysd.rhs64, rtol=1e-8)
+        print(f"[learned] seed {seed}: loss net {l_net:.4f} jacobi {l_jac:.4f}; it net {inf_net['iterations']} "
+              f"jacobi {inf_jac['iterations']}", flush=True)
+        assert inf_net["fallback"] is None and inf_net["converged"]
+        res.append((l_net, l_jac, inf_net["iterations"], inf_jac["iterations"]))
+    assert all(l_net < 0.8 * l_jac for l_net, l_jac, _, _ in res)
+    assert sum(r[2] for r in res) <= sum(r[3] for r in res)
+
+
+def test_nonfinite_network_falls_back_to_plain():
+    import torch
+    from nfempy_amd.learned import LearnedSchurPC, solve_learned
+
+    from nfempy_amd.learned import param_layout
+
+    sysd, _, _ = _system(4, seed=2)
+    th = _theta()
+    th[100] = np.nan
+    pc = LearnedSchurPC(sysd, th)
+    assert not pc.setup()
+    assert pc.fallback_reason == "non-finite network parameters"
+    x, info = solve_learned(sysd, pc, -sysd.rhs64, rtol=1e-8)
+    assert info["fallback"] == "non-finite network parameters"
+    assert info["converged"] and info["residual"] <= 1e-8
+    assert bool(torch.isfinite(x).all())
+    # finite weights whose smoother overflows: the output check catches it
+    th = _theta()
+    o = param_layout()[0]["ud"][0]
+    th[o:o + 64] = 3e38
+    pc = LearnedSchurPC(sysd, th)
+    assert not pc.setup()
+    assert pc.fallback_reason == "non-finite smoother entries"
+    x, info = solve_learned(sysd, pc, -sysd.rhs64, rtol=1e-8)
+    assert info["fallback"] == "non-finite smoother entries" and info["converged"]
