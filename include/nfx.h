@@ -303,14 +303,56 @@ int nfx_agg_sum(int n_agg, int ns, int E, const int32_t* d_agg_ptr, const int32_
 int nfx_coarse_dense(int n_agg, const int32_t* d_agg_ptr, const int32_t* d_agg_mem,
                      const int32_t* d_agg_of, const int32_t* d_nbr, const double* d_sdiag,
                      const double* d_soff, double* d_Ac, void* stream);
-/* Block preconditioner with the two-grid Schur cycle: z_u = dMinv r_u (n_u flux
- * rows), z_p = psign (x2 + W (r_p - S x2)), x2 = x1 + P Ainv P^T (r_p - S x1),
- * x1 = W r_p. d_work: 3 E + 2 n_agg floats. */
-int nfx_twogrid_apply(int n_u, int E, int n_agg, const int32_t* d_nbr, const float* d_Wd,
-                      const float* d_Wo, const float* d_sd, const float* d_so,
-                      const int32_t* d_agg_ptr, const int32_t* d_agg_mem,
-                      const int32_t* d_agg_of, const float* d_Ainv, const float* d_dMinv,
-                      float psign, const float* d_r, float* d_z, float* d_work, void* stream);
+/* In-place inverse of an SPD block (n <= 128, FP32, row stride lda): the diagonal
+ * blocks of the recursive 2 x 2 block inversion of the coarse operator. */
+int nfx_spd_inverse_small(int n, float* d_A, int lda, void* stream);
+
+/* Two-grid Schur block preconditioner: flux rows z_u = dMinv r_u; pressure rows
+ * z_p = psign (x2 + W (r_p - S x2)), x2 = x1 + P Ainv P^T (r_p - S x1), x1 = W r_p,
+ * with W = (Wd, Wo) and S = (sd, so) in ELL form (nbr), the aggregates as CSR
+ * (agg_ptr, agg_mem) plus agg_of (cell -> aggregate), and the coarse inverse either
+ * FP32 (Ainv) or row-scaled FP16 (Ainv16, Ascale; used when Ainv16 is not NULL).
+ * work: 3 n_cells + 2 n_agg floats. */
+typedef struct nfx_twogrid_pc {
+  int n_u, n_cells, n_agg;
+  const int32_t* nbr;
+  const float* Wd;
+  const float* Wo;
+  const float* sd;
+  const float* so;
+  const int32_t* agg_ptr;
+  const int32_t* agg_mem;
+  const int32_t* agg_of;
+  const float* Ainv;
+  const uint16_t* Ainv16;
+  const float* Ascale;
+  const float* dMinv;
+  float psign;
+  float* work;
+} nfx_twogrid_pc;
+int nfx_twogrid_apply(const nfx_twogrid_pc* pc, const float* d_r, float* d_z, void* stream);
+/* Row-scaled FP16 copy of a dense n x n FP32 matrix (Ascale[r] = max |A[r][:]|). */
+int nfx_half_rows(int n, const float* d_A, uint16_t* d_A16, float* d_scale, void* stream);
+
+/* ---- FGMRES + FP64 iterative refinement -----------------------------------
+ * Replaces KSP preonly + LU/MUMPS of the Darcy driver
+ * (examples/md_elliptic_scalar/md_elliptic_scalar.py:351-364) when the learned
+ * (flexible) preconditioner is used: FGMRES(restart) in FP32 with right
+ * preconditioning by `pc` (two-grid Schur block preconditioner), else by the
+ * diagonal d_pinv (NULL = none), inside FP64 refinement on the as-assembled FP64
+ * matrix until ||b - A x||_2 / ||b||_2 <= rtol. reorth = 1 repeats classical
+ * Gram-Schmidt once; the host reads the residual estimate every check_every
+ * iterations. Synchronous. h_info[0] = refinement steps, h_info[1] = total FGMRES
+ * iterations. Returns NFX_E_NOT_CONVERGED if rtol is not reached. */
+typedef struct nfx_fgmres nfx_fgmres;
+int nfx_fgmres_create(nfx_fgmres** out, int n_rows, int restart, void* stream);
+int nfx_fgmres_destroy(nfx_fgmres* s);
+int nfx_solve_fgmres_ir(nfx_fgmres* s, const int32_t* d_rowptr, const int32_t* d_col,
+                        const float* d_val32, const double* d_val64, const nfx_twogrid_pc* pc,
+                        const float* d_pinv, const double* d_b, double* d_x, double rtol,
+                        double inner_rtol, int max_inner, int max_outer, int reorth,
+                        int check_every, int* h_info, double* h_resid, double* h_hist,
+                        int hist_cap);
 
 #ifdef __cplusplus
 }

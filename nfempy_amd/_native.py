@@ -87,10 +87,25 @@ SIGNATURES = {
     "nfx_agg_sum": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "nfx_coarse_dense": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                  c_void_p]),
-    "nfx_twogrid_apply": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
-                                  c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_void_p,
-                                  c_void_p, c_void_p, c_void_p]),
+    "nfx_twogrid_apply": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "nfx_half_rows": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "nfx_spd_inverse_small": (c_int, [c_int, c_void_p, c_int, c_void_p]),
+    "nfx_fgmres_create": (c_int, [c_void_p, c_int, c_int, c_void_p]),
+    "nfx_fgmres_destroy": (c_int, [c_void_p]),
+    "nfx_solve_fgmres_ir": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                    c_void_p, c_void_p, c_double, c_double, c_int, c_int, c_int, c_int,
+                                    c_void_p, c_void_p, c_void_p, c_int]),
 }
+
+
+class TwoGridPC(ctypes.Structure):
+    """Mirror of ``nfx_twogrid_pc`` (include/nfx.h)."""
+
+    _fields_ = [("n_u", c_int), ("n_cells", c_int), ("n_agg", c_int), ("nbr", c_void_p), ("Wd", c_void_p),
+                ("Wo", c_void_p), ("sd", c_void_p), ("so", c_void_p), ("agg_ptr", c_void_p),
+                ("agg_mem", c_void_p), ("agg_of", c_void_p), ("Ainv", c_void_p), ("Ainv16", c_void_p),
+                ("Ascale", c_void_p), ("dMinv", c_void_p), ("psign", c_float), ("work", c_void_p)]
+
 
 # entry points whose return value is a count, not a status
 _RAW = ("nfx_version", "nfx_darcy_batch_item_size", "nfx_gnn_num_params", "nfx_gnn_backward_workspace")

@@ -74,4 +74,10 @@ __device__ __forceinline__ void put_row(int code, R v, R* __restrict__ rhs) {
 
 inline int grid_for(int64_t n, int block) { return static_cast<int>((n + block - 1) / block); }
 
+// Two-grid block preconditioner z = M^-1 r (nfx_twogrid_apply), with the output
+// divided by *dscale when dscale is not null (FGMRES passes the norm of an
+// unnormalised Arnoldi vector).
+int twogrid_apply_scaled(const nfx_twogrid_pc* pc, const float* r, float* z, const double* dscale,
+                         hipStream_t stream);
+
 }  // namespace nfx

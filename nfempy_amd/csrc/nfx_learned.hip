@@ -556,7 +556,7 @@ __global__ __launch_bounds__(256) void k_tg_restrict(int n_agg, const int32_t* _
   if (lane == 0) c[wave] = acc;
 }
 
-// y = Ainv c  (one wave per row, dense row-major FP32)
+// y = Ainv c  (one wave per row, dense row-major FP32; 16-byte loads, 4 in flight per lane)
 __global__ __launch_bounds__(256) void k_gemv(int n, const float* __restrict__ A,
                                               const float* __restrict__ x, float* __restrict__ y) {
   const int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -564,9 +564,81 @@ __global__ __launch_bounds__(256) void k_gemv(int n, const float* __restrict__ A
   if (row >= n) return;
   const float* a = A + (int64_t)row * n;
   float acc = 0.f;
-  for (int k = lane; k < n; k += 64) acc += a[k] * x[k];
+  if ((n & 3) == 0) {
+    const float4* a4 = reinterpret_cast<const float4*>(a);
+    const float4* x4 = reinterpret_cast<const float4*>(x);
+    const int n4 = n >> 2;
+    int k = lane;
+    for (; k + 192 < n4; k += 256) {
+      const float4 p0 = a4[k], p1 = a4[k + 64], p2 = a4[k + 128], p3 = a4[k + 192];
+      const float4 q0 = x4[k], q1 = x4[k + 64], q2 = x4[k + 128], q3 = x4[k + 192];
+      acc += p0.x * q0.x + p0.y * q0.y + p0.z * q0.z + p0.w * q0.w;
+      acc += p1.x * q1.x + p1.y * q1.y + p1.z * q1.z + p1.w * q1.w;
+      acc += p2.x * q2.x + p2.y * q2.y + p2.z * q2.z + p2.w * q2.w;
+      acc += p3.x * q3.x + p3.y * q3.y + p3.z * q3.z + p3.w * q3.w;
+    }
+    for (; k < n4; k += 64) {
+      const float4 p = a4[k], q = x4[k];
+      acc += p.x * q.x + p.y * q.y + p.z * q.z + p.w * q.w;
+    }
+  } else {
+    for (int k = lane; k < n; k += 64) acc += a[k] * x[k];
+  }
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
   if (lane == 0) y[row] = acc;
+}
+
+// In-place Gauss-Jordan inverse of an SPD block (n <= 128, no pivoting), row stride lda.
+// 512 threads: thread t holds quarter (t & 3) of row t >> 2 (32 columns) of the block
+// padded to 128 with the identity. The pivot loop is unrolled so that every register
+// index is a constant; per pivot the raw pivot row and column go through LDS
+// (double-buffered, one barrier per pivot; the row is read back as float4).
+constexpr int kInvN = 128;
+
+__global__ __launch_bounds__(512) void k_spd_inverse_small(int n, float* __restrict__ A, int lda) {
+  __shared__ __align__(16) float praw[2][kInvN];
+  __shared__ float fcol[2][kInvN];
+  const int i = threadIdx.x >> 2, q = threadIdx.x & 3;
+  float r[32];
+#pragma unroll
+  for (int j = 0; j < 32; ++j) {
+    const int c = 32 * q + j;
+    r[j] = (i < n && c < n) ? A[(int64_t)i * lda + c] : (i == c ? 1.f : 0.f);
+  }
+#pragma clang loop unroll(full)
+  for (int k = 0; k < kInvN; ++k) {
+    const int buf = k & 1;
+    if (i == k) {
+      float4* dst = reinterpret_cast<float4*>(&praw[buf][32 * q]);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) dst[j] = make_float4(r[4 * j], r[4 * j + 1], r[4 * j + 2], r[4 * j + 3]);
+    }
+    if (q == (k >> 5)) fcol[buf][i] = r[k & 31];
+    __syncthreads();
+    const float ip = 1.f / praw[buf][k];
+    const float f = fcol[buf][i];
+    const float4* src = reinterpret_cast<const float4*>(&praw[buf][32 * q]);
+    float pr[32];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const float4 v = src[j];
+      pr[4 * j] = v.x;
+      pr[4 * j + 1] = v.y;
+      pr[4 * j + 2] = v.z;
+      pr[4 * j + 3] = v.w;
+    }
+#pragma unroll
+    for (int j = 0; j < 32; ++j) {
+      const bool ck = (q == (k >> 5)) && (j == (k & 31));
+      const float pj = (ck ? 1.f : pr[j]) * ip;
+      r[j] = (i == k) ? pj : (ck ? 0.f : r[j]) - f * pj;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 32; ++j) {
+    const int c = 32 * q + j;
+    if (i < n && c < n) A[(int64_t)i * lda + c] = r[j];
+  }
 }
 
 // x2 = x1 + y[agg]; t2 = r - S x2 (x2 of neighbours recomputed from x1 and y)
@@ -588,18 +660,75 @@ __global__ void k_tg_post1(int E, const int32_t* __restrict__ nbr, const int32_t
 }
 
 // z_p = psign (x2 + W t2) on pressure rows; z_u = dMinv r_u on flux rows
+// (dscale: optional device scalar; the output is divided by it, 0 gives 0)
 __global__ void k_tg_post2(int n_u, int E, const int32_t* __restrict__ nbr, const float* __restrict__ Wd,
                            const float* __restrict__ Wo, const float* __restrict__ x2,
                            const float* __restrict__ t2, const float* __restrict__ dMinv,
-                           const float* __restrict__ r, float psign, float* __restrict__ z) {
+                           const float* __restrict__ r, float psign, const double* __restrict__ dscale,
+                           float* __restrict__ z) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  float sc = 1.f;
+  if (dscale) {
+    const double d = *dscale;
+    sc = d > 0 ? (float)(1.0 / d) : 0.f;
+  }
   if (i < n_u) {
-    z[i] = dMinv[i] * r[i];
+    z[i] = sc * dMinv[i] * r[i];
     return;
   }
   const int64_t p = i - n_u;
   if (p >= E) return;
-  z[i] = psign * (x2[p] + ell_row(int(p), nbr, Wd, Wo, t2));
+  z[i] = sc * psign * (x2[p] + ell_row(int(p), nbr, Wd, Wo, t2));
+}
+
+// Row-scaled FP16 copy of a dense row-major matrix: scale[r] = max |A[r][:]|,
+// A16[r][k] = half(A[r][k] / scale[r]). One wave per row.
+__global__ __launch_bounds__(256) void k_half_rows(int n, const float* __restrict__ A, uint16_t* __restrict__ A16,
+                                                   float* __restrict__ scale) {
+  const int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  const float* a = A + (int64_t)row * n;
+  float mx = 0.f;
+  for (int k = lane; k < n; k += 64) mx = fmaxf(mx, fabsf(a[k]));
+  for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+  const float inv = mx > 0.f ? 1.f / mx : 0.f;
+  for (int k = lane; k < n; k += 64) {
+    const _Float16 h = (_Float16)(a[k] * inv);
+    A16[(int64_t)row * n + k] = __builtin_bit_cast(uint16_t, h);
+  }
+  if (lane == 0) scale[row] = mx;
+}
+
+// y = diag(scale) A16 c (FP16 storage, FP32 arithmetic); one wave per row, 16-byte loads.
+__global__ __launch_bounds__(256) void k_gemv_h(int n, const uint16_t* __restrict__ A16,
+                                                const float* __restrict__ scale, const float* __restrict__ x,
+                                                float* __restrict__ y) {
+  const int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  const uint16_t* a = A16 + (int64_t)row * n;
+  float acc = 0.f;
+  if ((n & 7) == 0) {
+    const uint4* a8 = reinterpret_cast<const uint4*>(a);
+    const float4* x4 = reinterpret_cast<const float4*>(x);
+    for (int k = lane; k < (n >> 3); k += 64) {
+      const uint4 u = a8[k];
+      const float4 xa = x4[2 * k], xb = x4[2 * k + 1];
+      const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+      const float xs[8] = {xa.x, xa.y, xa.z, xa.w, xb.x, xb.y, xb.z, xb.w};
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const _Float16 lo = __builtin_bit_cast(_Float16, (uint16_t)(w[t] & 0xFFFFu));
+        const _Float16 hi = __builtin_bit_cast(_Float16, (uint16_t)(w[t] >> 16));
+        acc += (float)lo * xs[2 * t] + (float)hi * xs[2 * t + 1];
+      }
+    }
+  } else {
+    for (int k = lane; k < n; k += 64) acc += (float)__builtin_bit_cast(_Float16, a[k]) * x[k];
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if (lane == 0) y[row] = acc * scale[row];
 }
 
 }  // namespace
@@ -743,32 +872,56 @@ int nfx_coarse_dense(int n_agg, const int32_t* agg_ptr, const int32_t* agg_mem, 
   return NFX_OK;
 }
 
-int nfx_twogrid_apply(int n_u, int E, int n_agg, const int32_t* nbr, const float* Wd, const float* Wo,
-                      const float* sd, const float* so, const int32_t* agg_ptr, const int32_t* agg_mem,
-                      const int32_t* agg_of, const float* Ainv, const float* dMinv, float psign, const float* r,
-                      float* z, float* work, void* stream) {
-  NFX_REQUIRE(n_u >= 0 && E > 0 && n_agg > 0, "nfx_twogrid_apply: bad sizes");
-  NFX_REQUIRE(nbr && Wd && Wo && sd && so && agg_ptr && agg_mem && agg_of && Ainv && r && z && work &&
-                  (n_u == 0 || dMinv),
-              "nfx_twogrid_apply: null pointer");
-  hipStream_t st = (hipStream_t)stream;
-  // work: x1[E] | x2[E] | t2[E] | c[n_agg] | y[n_agg]
-  float* x1 = work;
-  float* x2 = work + E;
-  float* t2 = work + 2 * (int64_t)E;
-  float* c = work + 3 * (int64_t)E;
-  float* y = c + n_agg;
-  const float* rp = r + n_u;
-  hipLaunchKernelGGL(k_tg_pre, dim3(grid_for(E, 256)), dim3(256), 0, st, E, nbr, Wd, Wo, rp, x1);
-  hipLaunchKernelGGL(k_tg_restrict, dim3(grid_for((int64_t)n_agg * 64, 256)), dim3(256), 0, st, n_agg, agg_ptr,
-                     agg_mem, nbr, sd, so, rp, x1, c);
-  hipLaunchKernelGGL(k_gemv, dim3(grid_for((int64_t)n_agg * 64, 256)), dim3(256), 0, st, n_agg, Ainv, c, y);
-  hipLaunchKernelGGL(k_tg_post1, dim3(grid_for(E, 256)), dim3(256), 0, st, E, nbr, agg_of, sd, so, rp, x1, y,
-                     x2, t2);
-  hipLaunchKernelGGL(k_tg_post2, dim3(grid_for((int64_t)n_u + E, 256)), dim3(256), 0, st, n_u, E, nbr, Wd, Wo,
-                     x2, t2, dMinv, r, psign, z);
+int nfx_spd_inverse_small(int n, float* A, int lda, void* stream) {
+  NFX_REQUIRE(n >= 1 && n <= kInvN && lda >= n, "nfx_spd_inverse_small: need 1 <= n <= 128 and lda >= n");
+  NFX_REQUIRE(A, "nfx_spd_inverse_small: null pointer");
+  hipLaunchKernelGGL(k_spd_inverse_small, dim3(1), dim3(512), 0, (hipStream_t)stream, n, A, lda);
   NFX_LAUNCH_CHECK();
   return NFX_OK;
 }
 
+int nfx_half_rows(int n, const float* A, uint16_t* A16, float* scale, void* stream) {
+  NFX_REQUIRE(n > 0, "nfx_half_rows: n <= 0");
+  NFX_REQUIRE(A && A16 && scale, "nfx_half_rows: null pointer");
+  hipLaunchKernelGGL(k_half_rows, dim3(grid_for((int64_t)n * 64, 256)), dim3(256), 0, (hipStream_t)stream, n, A,
+                     A16, scale);
+  NFX_LAUNCH_CHECK();
+  return NFX_OK;
+}
+
+int nfx_twogrid_apply(const nfx_twogrid_pc* pc, const float* r, float* z, void* stream) {
+  return nfx::twogrid_apply_scaled(pc, r, z, nullptr, (hipStream_t)stream);
+}
+
 }  // extern "C"
+
+int nfx::twogrid_apply_scaled(const nfx_twogrid_pc* pc, const float* r, float* z, const double* dscale,
+                              hipStream_t st) {
+  NFX_REQUIRE(pc, "nfx_twogrid_apply: null descriptor");
+  const int n_u = pc->n_u, E = pc->n_cells, n_agg = pc->n_agg;
+  NFX_REQUIRE(n_u >= 0 && E > 0 && n_agg > 0, "nfx_twogrid_apply: bad sizes");
+  NFX_REQUIRE(pc->nbr && pc->Wd && pc->Wo && pc->sd && pc->so && pc->agg_ptr && pc->agg_mem && pc->agg_of &&
+                  (pc->Ainv || (pc->Ainv16 && pc->Ascale)) && r && z && pc->work && (n_u == 0 || pc->dMinv),
+              "nfx_twogrid_apply: null pointer");
+  // work: x1[E] | x2[E] | t2[E] | c[n_agg] | y[n_agg]
+  float* x1 = pc->work;
+  float* x2 = x1 + E;
+  float* t2 = x1 + 2 * (int64_t)E;
+  float* c = x1 + 3 * (int64_t)E;
+  float* y = c + n_agg;
+  const float* rp = r + n_u;
+  hipLaunchKernelGGL(k_tg_pre, dim3(grid_for(E, 256)), dim3(256), 0, st, E, pc->nbr, pc->Wd, pc->Wo, rp, x1);
+  hipLaunchKernelGGL(k_tg_restrict, dim3(grid_for((int64_t)n_agg * 64, 256)), dim3(256), 0, st, n_agg, pc->agg_ptr,
+                     pc->agg_mem, pc->nbr, pc->sd, pc->so, rp, x1, c);
+  if (pc->Ainv16)
+    hipLaunchKernelGGL(k_gemv_h, dim3(grid_for((int64_t)n_agg * 64, 256)), dim3(256), 0, st, n_agg, pc->Ainv16,
+                       pc->Ascale, c, y);
+  else
+    hipLaunchKernelGGL(k_gemv, dim3(grid_for((int64_t)n_agg * 64, 256)), dim3(256), 0, st, n_agg, pc->Ainv, c, y);
+  hipLaunchKernelGGL(k_tg_post1, dim3(grid_for(E, 256)), dim3(256), 0, st, E, pc->nbr, pc->agg_of, pc->sd, pc->so,
+                     rp, x1, y, x2, t2);
+  hipLaunchKernelGGL(k_tg_post2, dim3(grid_for((int64_t)n_u + E, 256)), dim3(256), 0, st, n_u, E, pc->nbr, pc->Wd,
+                     pc->Wo, x2, t2, pc->dMinv, r, pc->psign, dscale, z);
+  NFX_LAUNCH_CHECK();
+  return NFX_OK;
+}

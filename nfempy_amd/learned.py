@@ -9,7 +9,8 @@ symmetric two-grid cycle on ``S``:
 
 with ``P`` the piecewise-constant prolongation from aggregates of cells (boxes of
 side ``agg_h``, two hexes of a Kuhn mesh by default), ``Ac = P^T S P`` inverted
-exactly (dense, FP64 Cholesky, FP32 inverse), and ``W`` a symmetric sparse smoother
+exactly (dense FP32 recursive block inverse, replayed from a captured graph), and
+``W`` a symmetric sparse smoother
 with the pattern of ``S`` whose entries come from a message-passing network:
 4 layers of width 64 on the cell graph of ``S`` (bf16 weights and activations,
 FP32 accumulation), run once per system. With the output heads at zero ``W`` is
@@ -22,7 +23,7 @@ kernels. The loss for a system is the mean over sample errors ``e`` of
 two-grid error reduction; samples are Gaussian vectors and the pressure errors of
 plain-preconditioned Krylov iterates of the system (``x* - x_k``).
 
-The outer solver is FGMRES with FP64 iterative refinement (``fgmres.py``). A network
+The outer solver is FGMRES with FP64 iterative refinement (``nfx_solve_fgmres_ir``). A network
 whose smoother has non-finite entries, or a learned solve that does not converge,
 falls back to the plain block preconditioner with MINRES.
 
@@ -176,10 +177,35 @@ def aggregate_cells(centroids, h):
     return inv.astype(np.int64), len(u)
 
 
+def spd_inverse_(A, stream=None):
+    """In-place inverse of an SPD matrix (FP32, row-major view with unit column stride)
+    by recursive 2 x 2 blocking:
+    X11 = A11^-1, T = X11 A12, S = A22 - A21 T, X22 = S^-1, X12 = -T X22,
+    X11 <- X11 - X12 T^T, X21 = X12^T; diagonal blocks of size <= 128 are inverted by
+    ``nfx_spd_inverse_small`` (Gauss-Jordan in LDS), the products are library GEMMs
+    written in place into the strided blocks."""
+    t = torch()
+    n = A.shape[0]
+    if n <= 128:
+        nat.call("nfx_spd_inverse_small", n, A.data_ptr(), A.stride(0), nat.stream_handle(stream))
+        return
+    h = n // 2
+    if h > 128:
+        h = (h // 128) * 128
+    A11, A12, A21, A22 = A[:h, :h], A[:h, h:], A[h:, :h], A[h:, h:]
+    spd_inverse_(A11, stream)
+    T = t.mm(A11, A12)
+    A22.addmm_(A21, T, alpha=-1.0)
+    spd_inverse_(A22, stream)
+    A12.addmm_(T, A22, beta=0.0, alpha=-1.0)
+    A11.addmm_(A12, T.T, alpha=-1.0)
+    A21.copy_(A12.T)
+
+
 class LearnedSchurPC:
     """Two-grid Schur block preconditioner with the network smoother (callable z = M^-1 r)."""
 
-    def __init__(self, sysd, theta=None, agg_h=None, psign=-1.0, graph=None):
+    def __init__(self, sysd, theta=None, agg_h=None, psign=-1.0, graph=None, coarse_fp16=True):
         t = torch()
         self.sys = sysd
         self.g = graph if graph is not None else SchurGraph(sysd, agg_h, device=sysd.device)
@@ -194,7 +220,12 @@ class LearnedSchurPC:
         self.sd32, self.so32 = f32(E), f32(E * 4)
         self.Wd, self.Wo = f32(E), f32(E * 4)
         self.work = f32(3 * E + 2 * n_agg)
-        self.Ainv = None
+        self.Ainv = t.empty((n_agg, n_agg), dtype=t.float32, device=dev)
+        self._Ac64 = t.empty((n_agg, n_agg), dtype=t.float64, device=dev)
+        self.coarse_fp16 = bool(coarse_fp16)
+        self.Ainv16 = t.empty((n_agg, n_agg), dtype=t.int16, device=dev) if self.coarse_fp16 else None
+        self.Ascale = t.empty(n_agg, dtype=t.float32, device=dev) if self.coarse_fp16 else None
+        self._coarse_graph = None
         self.dMinv = f32(g.n_u)
         self.psign = float(psign)
         self.h = self.m = self.z = None
@@ -241,16 +272,37 @@ class LearnedSchurPC:
                  nat.ptr(self.theta16), nat.ptr(self.h), nat.ptr(self.m), nat.ptr(self.z), nat.ptr(self.sdiag),
                  nat.ptr(self.Wd), nat.ptr(self.Wo), nat.stream_handle(stream))
 
-    def coarse(self, stream=None):
-        """Ac = P^T S P (FP64, deterministic kernel), exact inverse via Cholesky."""
-        t = torch()
+    def _coarse_eager(self, stream=None):
         g = self.g
-        Ac = t.zeros((g.n_agg, g.n_agg), dtype=t.float64, device=self.sys.device)
+        self._Ac64.zero_()
         nat.call("nfx_coarse_dense", g.n_agg, nat.ptr(g.agg_ptr), nat.ptr(g.agg_mem), nat.ptr(g.agg_of),
-                 nat.ptr(g.nbr), nat.ptr(self.sdiag), nat.ptr(self.soff), nat.ptr(Ac), nat.stream_handle(stream))
-        L = t.linalg.cholesky(Ac)
-        self.L = L
-        self.Ainv = t.cholesky_inverse(L).to(t.float32).contiguous()
+                 nat.ptr(g.nbr), nat.ptr(self.sdiag), nat.ptr(self.soff), nat.ptr(self._Ac64),
+                 nat.stream_handle(stream))
+        self.Ainv.copy_(self._Ac64)
+        spd_inverse_(self.Ainv, stream)
+        if self.coarse_fp16:
+            nat.call("nfx_half_rows", g.n_agg, nat.ptr(self.Ainv), nat.ptr(self.Ainv16), nat.ptr(self.Ascale),
+                     nat.stream_handle(stream))
+
+    def coarse(self, stream=None, graph=True):
+        """Ainv = (P^T S P)^-1: deterministic coarse kernel (FP64), FP32 recursive block
+        inverse. With graph=True the sequence is captured once and replayed."""
+        t = torch()
+        if not graph or stream is not None:
+            self._coarse_eager(stream)
+            return
+        if self._coarse_graph is None:
+            side = t.cuda.Stream()
+            side.wait_stream(t.cuda.current_stream())
+            with t.cuda.stream(side):
+                self._coarse_eager()
+            t.cuda.current_stream().wait_stream(side)
+            t.cuda.synchronize()
+            gr = t.cuda.CUDAGraph()
+            with t.cuda.graph(gr):
+                self._coarse_eager()
+            self._coarse_graph = gr
+        self._coarse_graph.replay()
 
     def setup(self, vals64=None, network=True, stream=None):
         """Per-system set-up after FP64 assembly. Returns True if the network smoother is used."""
@@ -278,12 +330,20 @@ class LearnedSchurPC:
     def __call__(self, r, out=None, stream=None):
         t = torch()
         g = self.g
+        import ctypes
+
         z = out if out is not None else t.empty_like(r)
-        nat.call("nfx_twogrid_apply", g.n_u, g.E, g.n_agg, nat.ptr(g.nbr), nat.ptr(self.Wd), nat.ptr(self.Wo),
-                 nat.ptr(self.sd32), nat.ptr(self.so32), nat.ptr(g.agg_ptr), nat.ptr(g.agg_mem), nat.ptr(g.agg_of),
-                 nat.ptr(self.Ainv), nat.ptr(self.dMinv), self.psign, nat.ptr(r), nat.ptr(z), nat.ptr(self.work),
-                 nat.stream_handle(stream))
+        self._desc = self.desc()
+        nat.call("nfx_twogrid_apply", ctypes.byref(self._desc), nat.ptr(r), nat.ptr(z), nat.stream_handle(stream))
         return z
+
+    def desc(self):
+        """The ``nfx_twogrid_pc`` descriptor of this preconditioner (for the C++ FGMRES)."""
+        g = self.g
+        return nat.TwoGridPC(g.n_u, g.E, g.n_agg, nat.ptr(g.nbr), nat.ptr(self.Wd), nat.ptr(self.Wo),
+                             nat.ptr(self.sd32), nat.ptr(self.so32), nat.ptr(g.agg_ptr), nat.ptr(g.agg_mem),
+                             nat.ptr(g.agg_of), nat.ptr(self.Ainv), nat.ptr(self.Ainv16), nat.ptr(self.Ascale),
+                             nat.ptr(self.dMinv), self.psign, nat.ptr(self.work))
 
     def smoother_numpy(self):
         """(Wd, Wo (E,4)) on the host (tests)."""
@@ -362,7 +422,7 @@ class SmootherTrainer:
         ell(pc.sd32, pc.so32, v, Sv)
         c = t.empty((ns, g.n_agg), dtype=t.float32, device=e.device)
         nat.call("nfx_agg_sum", g.n_agg, ns, E, nat.ptr(g.agg_ptr), nat.ptr(g.agg_mem), nat.ptr(Sv), nat.ptr(c), s)
-        y = t.cholesky_solve(c.double().T.contiguous(), pc.L).T.to(t.float32)
+        y = t.mm(c, pc.Ainv)
         u = (v - y[:, g.agg_of.long()]).contiguous()
         ell(pc.sd32, pc.so32, u, Su)
         per = (u.double() * Su.double()).sum(dim=1) / en
@@ -401,20 +461,67 @@ class SmootherTrainer:
         return float(self.loss_and_gamma(e))
 
 
-def solve_learned(sysd, pc: LearnedSchurPC, b64, rtol=1e-8, restart=120, maxit=4000, inner_tol=1e-5,
-                  fallback=True):
+class Fgmres:
+    """FGMRES(restart) + FP64 refinement on the device (``nfx_solve_fgmres_ir``)."""
+
+    def __init__(self, n, restart=60, stream=None):
+        import ctypes
+
+        self.n, self.restart = int(n), int(restart)
+        h = ctypes.c_void_p()
+        nat.call("nfx_fgmres_create", ctypes.byref(h), self.n, self.restart, nat.stream_handle(stream))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            nat.lib().nfx_fgmres_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve(self, rowptr, col, vals32, vals64, b, x=None, pc=None, pinv=None, rtol=1e-8, inner_rtol=1e-5,
+              max_inner=4000, max_outer=20, reorth=True, check=4):
+        """pc: a LearnedSchurPC (two-grid block preconditioner) or None; pinv: diagonal."""
+        import ctypes
+
+        t = torch()
+        b = b.to(dtype=t.float64).contiguous()
+        if x is None:
+            x = t.zeros(self.n, dtype=t.float64, device=b.device)
+        info = (ctypes.c_int * 2)()
+        resid = (ctypes.c_double * 1)()
+        hist = (ctypes.c_double * (max_outer + 1))()
+        desc = pc.desc() if pc is not None else None
+        rc = nat.lib().nfx_solve_fgmres_ir(
+            self._h, nat.ptr(rowptr), nat.ptr(col), nat.ptr(vals32), nat.ptr(vals64),
+            ctypes.byref(desc) if desc is not None else None, nat.ptr(pinv), nat.ptr(b), nat.ptr(x), rtol,
+            inner_rtol, max_inner, max_outer, int(bool(reorth)), check, info, resid, hist, max_outer + 1)
+        if rc not in (nat.NFX_OK, nat.NFX_E_NOT_CONVERGED):
+            nat.check(rc, "nfx_solve_fgmres_ir")
+        return x, {"converged": rc == nat.NFX_OK, "outer": info[0], "iterations": info[1],
+                   "residual": resid[0], "history": [hist[i] for i in range(min(info[0] + 1, max_outer + 1))]}
+
+
+def solve_learned(sysd, pc: LearnedSchurPC, b64, rtol=1e-8, restart=60, maxit=4000, inner_tol=1e-5,
+                  fallback=True, reorth=True, check=4, solver=None):
     """FGMRES + FP64 refinement with the learned preconditioner; plain MINRES fallback.
 
     ``pc.setup`` must have run on the current FP64 values. Returns (x64, info)."""
-    from .fgmres import CsrOperator, fgmres_ir
     from .solvers import MinresIR, darcy_preconditioner
 
     t = torch()
     info = {"fallback": None}
     x = None
     if pc.network_used or pc.fallback_reason is None:
-        op = CsrOperator(sysd.rowptr, sysd.col, sysd.vals32, sysd.vals64)
-        x, inf = fgmres_ir(op, pc, b64, rtol=rtol, inner_tol=inner_tol, restart=restart, maxit=maxit)
+        fg = solver if solver is not None else Fgmres(sysd.n, restart)
+        x, inf = fg.solve(sysd.rowptr, sysd.col, sysd.vals32, sysd.vals64, b64, pc=pc, rtol=rtol,
+                          inner_rtol=inner_tol, max_inner=maxit, reorth=reorth, check=check)
+        if solver is None:
+            fg.close()
         info.update(inf)
         info["method"] = "fgmres+twogrid" + ("+network" if pc.network_used else "+jacobi")
         if inf["converged"] and np.isfinite(inf["residual"]):

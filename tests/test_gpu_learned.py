@@ -59,7 +59,7 @@ def test_learned_parity_with_oracle():
     print("[learned] parity n=4", flush=True)
     sysd, _, _ = _system(4, seed=3)
     th = _theta()
-    pc = LearnedSchurPC(sysd, th)
+    pc = LearnedSchurPC(sysd, th, coarse_fp16=False)
     assert pc.setup()
     torch.cuda.synchronize()
     g = pc.g
@@ -133,6 +133,68 @@ def test_learned_parity_with_oracle():
     assert np.abs(zn[sysd.n_u:] - pc.psign * zp_or).max() <= 1e-4 * np.abs(zp_or).max()
     dM = np.array([v64[rp[i]:rp[i + 1]][ci[rp[i]:rp[i + 1]] == i][0] for i in range(sysd.n_u)])
     assert np.allclose(zn[:sysd.n_u], rn[:sysd.n_u] / np.abs(dM), rtol=1e-5)
+    # row-scaled FP16 coarse inverse: same application within 3e-3
+    pc16 = LearnedSchurPC(sysd, th, graph=g, coarse_fp16=True)
+    assert pc16.setup()
+    z16 = pc16(r)
+    torch.cuda.synchronize()
+    d = (z16 - z).abs().max() / z.abs().max()
+    assert float(d) < 3e-3, float(d)
+
+
+def test_coarse_inverse_and_graph_replays():
+    """Recursive block inverse of the 512 x 512 coarse operator against an FP64 inverse;
+    graph replays (other systems, eager work in between) against the eager sequence."""
+    import torch
+    from nfempy_amd.device import CoefficientArg
+    from nfempy_amd.fields import grf_device
+    from nfempy_amd.learned import LearnedSchurPC
+
+    print("[learned] coarse inverse n=16", flush=True)
+    sysd, _, f = _system(16, seed=4)
+    pc = LearnedSchurPC(sysd, None)
+    assert pc.g.n_agg == 512
+    cent = sysd.coords64[sysd.cells.long()].mean(dim=1)
+    for rep, seed in enumerate((4, 5, 4, 6)):
+        logk = grf_device(cent, seed=seed, ell=0.1, sigma=2.0)
+        sysd.assemble_elements(CoefficientArg(torch.exp(-logk).contiguous(), 1, 0), f, fp64=True)
+        pc.schur(sysd.vals64)
+        pc.coarse(graph=False)
+        eager = pc.Ainv.clone()
+        Ac = pc._Ac64.clone()
+        pc.Ainv.fill_(7.0)
+        torch.cuda.synchronize()
+        pc.coarse(graph=True)
+        torch.cuda.synchronize()
+        assert torch.allclose(pc.Ainv, eager, rtol=1e-6, atol=1e-6 * float(eager.abs().max())), rep
+        ref = torch.linalg.inv(Ac)
+        err = float((pc.Ainv.double() - ref).abs().max() / ref.abs().max())
+        print(f"[learned] seed {seed}: inverse rel err {err:.2e}", flush=True)
+        assert err < 1e-4
+        assert torch.allclose(pc.Ainv, pc.Ainv.T, rtol=0, atol=1e-6 * float(eager.abs().max()))
+        _ = (sysd.vals32 * 2).sum()  # eager work between replays
+        torch.cuda.synchronize()
+
+
+def test_fgmres_diagonal_and_none():
+    """The C++ FGMRES with a diagonal preconditioner (no two-grid) on a small system."""
+    import torch
+    from nfempy_amd.learned import Fgmres
+    from nfempy_amd.solvers import darcy_preconditioner
+
+    sysd, _, _ = _system(3, seed=1, sigma=1.0)
+    sign, pinv = darcy_preconditioner(sysd.rowptr, sysd.col, sysd.vals64, sysd.n, sysd.n_u)
+    fg = Fgmres(sysd.n, restart=40)
+    x, info = fg.solve(sysd.rowptr, sysd.col, sysd.vals32, sysd.vals64, -sysd.rhs64, pinv=pinv * sign,
+                       rtol=1e-8, max_inner=5000)
+    fg.close()
+    assert info["converged"] and info["residual"] <= 1e-8
+    import scipy.sparse as sps
+    import scipy.sparse.linalg as spla
+
+    A = sps.csr_matrix((sysd.vals64.cpu().numpy(), sysd.pattern.col, sysd.pattern.rowptr))
+    xr = spla.spsolve(A.tocsc(), -sysd.rhs64.cpu().numpy())
+    assert np.abs(x.cpu().numpy() - xr).max() <= 1e-6 * np.abs(xr).max()
 
 
 def test_learned_pc_solves_and_beats_plain_iterations():
