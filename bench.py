@@ -145,6 +145,86 @@ def bench_elasticity(args, log):
     return out
 
 
+def bench_solve(args, sysd, cent, fsrc, pd, log):
+    """Time-to-solve at 1e-8 FP64 relative residual on a held-out permeability sample:
+    plain block preconditioner (MINRES), the two-grid Schur preconditioner with its
+    Jacobi smoother (the method without the network) and with the trained network
+    smoother (FGMRES). Each timed region starts from the assembled FP64/FP32 system
+    and includes the preconditioner set-up; one untimed warm-up run precedes it."""
+    import torch
+
+    from nfempy_amd.device import CoefficientArg
+    from nfempy_amd.fields import grf_device
+    from nfempy_amd.learned import (Fgmres, LearnedSchurPC, default_weights_path, load_params,
+                                    solve_learned)
+    from nfempy_amd.solvers import MinresIR, darcy_preconditioner
+
+    N = sysd.n
+    logk = grf_device(cent, seed=args.solve_seed, ell=0.05, sigma=2.0)
+    kinv = CoefficientArg(torch.exp(-logk).contiguous(), 1, 0)
+    sysd.assemble(kinv, fsrc, pd, fp64=True)
+    sysd.assemble(kinv, fsrc, pd, fp64=False)
+    torch.cuda.synchronize()
+    b = -sysd.rhs64
+
+    def plain():
+        sign, pinv = darcy_preconditioner(sysd.rowptr, sysd.col, sysd.vals64, N, sysd.n_u)
+        solver = MinresIR(sysd.rowptr, sysd.col, sysd.vals32, sysd.vals64, sign, pinv)
+        x, info = solver.solve(b, rtol=1e-8, inner_rtol=1e-5, max_inner=100000, max_outer=40)
+        solver.close()
+        return x, info
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, out
+
+    log("solve: plain block preconditioner, MINRES + FP64 refinement")
+    tp, (xp, ip) = timed(plain)
+    log(f"plain: {tp * 1e3:.2f} ms, {ip['iterations']} iterations, {ip['outer']} refinements, "
+        f"residual {ip['residual']:.2e}")
+    out = {"system_seed": args.solve_seed, "time_to_solve_plain_s": round(tp, 5),
+           "plain_iterations": ip["iterations"], "plain_refinements": ip["outer"], "plain_residual": ip["residual"],
+           "plain_converged": ip["converged"], "time_to_solve_learned_s": None}
+    if args.no_learned:
+        return out
+    theta = load_params(default_weights_path())
+    pc = LearnedSchurPC(sysd, theta)
+    fg = Fgmres(N, 60)
+
+    def learned(network):
+        t0 = time.perf_counter()
+        pc.setup(network=network)
+        torch.cuda.synchronize()
+        t_setup = time.perf_counter() - t0
+        x, info = solve_learned(sysd, pc, b, rtol=1e-8, restart=60, reorth=False, check=4, solver=fg)
+        info["setup_s"] = t_setup
+        return x, info
+
+    for network, tag in ((False, "twogrid_jacobi"), (True, "learned")):
+        tl, (xl, il) = timed(lambda: learned(network))
+        rel = float((xl - xp).norm() / xp.norm())
+        log(f"{tag}: {tl * 1e3:.2f} ms (set-up {il['setup_s'] * 1e3:.2f} ms), {il['iterations']} FGMRES "
+            f"iterations, {il['outer']} refinements, residual {il['residual']:.2e}, fallback {il['fallback']}, "
+            f"|x - x_plain|/|x_plain| {rel:.1e}")
+        key = "learned" if network else "twogrid_jacobi"
+        out[f"time_to_solve_{key}_s"] = round(tl, 5)
+        out[f"{key}_setup_s"] = round(il["setup_s"], 5)
+        out[f"{key}_iterations"] = il["iterations"]
+        out[f"{key}_refinements"] = il["outer"]
+        out[f"{key}_residual"] = il["residual"]
+        out[f"{key}_fallback"] = il["fallback"]
+        out[f"{key}_method"] = il["method"]
+    out["speedup_learned_vs_plain"] = round(out["time_to_solve_plain_s"] / out["time_to_solve_learned_s"], 2)
+    out["speedup_learned_vs_twogrid_jacobi"] = round(
+        out["time_to_solve_twogrid_jacobi_s"] / out["time_to_solve_learned_s"], 3)
+    fg.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -152,6 +232,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=32, help="cube divisions per axis")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner solve")
+    ap.add_argument("--solve-seed", type=int, default=10000,
+                    help="permeability seed of the solved system (held out from the training seeds 0-1999)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-elasticity", action="store_true")
@@ -191,7 +274,7 @@ def main():
     log(f"E={E} N={N} nnz={nnz} setup numbering {t1 - t0:.2f}s pattern+upload {t2 - t1:.2f}s")
 
     cent = sysd.coords64[sysd.cells.long()].mean(dim=1)
-    logk = grf_device(cent, seed=7, ell=0.05, sigma=2.0)
+    logk = grf_device(cent, seed=args.solve_seed, ell=0.05, sigma=2.0)
     kinv = CoefficientArg(torch.exp(-logk).contiguous(), 1, 0)
     fsrc = CoefficientArg(torch.ones(1, dtype=torch.float64, device=dev), 0, 0)
     pd = CoefficientArg(torch.zeros(sysd.F * sysd.nqf, dtype=torch.float64, device=dev), sysd.nqf, 1)
@@ -247,23 +330,7 @@ def main():
 
     solve = None
     if not args.no_solve:
-        log("solving (FP64 assembly, block preconditioner, MINRES + FP64 refinement)")
-        step(fp64=True)
-        step(fp64=False)
-        torch.cuda.synchronize()
-        tsolve0 = time.perf_counter()
-        sign, pinv = darcy_preconditioner(sysd.rowptr, sysd.col, sysd.vals64, N, sysd.n_u)
-        solver = MinresIR(sysd.rowptr, sysd.col, sysd.vals32, sysd.vals64, sign, pinv)
-        x, info = solver.solve(-sysd.rhs64, rtol=1e-8, inner_rtol=1e-5, max_inner=50000, max_outer=40)
-        torch.cuda.synchronize()
-        tsolve = time.perf_counter() - tsolve0
-        solve = {"time_to_solve_plain_s": round(tsolve, 4), "plain_iterations": info["iterations"],
-                 "plain_refinements": info["outer"], "plain_residual": info["residual"],
-                 "plain_converged": info["converged"], "time_to_solve_learned_s": None}
-        log(f"solve: {tsolve:.3f}s, {info['iterations']} MINRES iterations, {info['outer']} refinements, "
-            f"residual {info['residual']:.3e}")
-        solver.close()
-
+        solve = bench_solve(args, sysd, cent, fsrc, pd, log)
     elasticity = None
     if not args.no_elasticity:
         elasticity = bench_elasticity(args, log)

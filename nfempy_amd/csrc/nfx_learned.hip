@@ -592,7 +592,8 @@ __global__ __launch_bounds__(256) void k_gemv(int n, const float* __restrict__ A
 // 512 threads: thread t holds quarter (t & 3) of row t >> 2 (32 columns) of the block
 // padded to 128 with the identity. The pivot loop is unrolled so that every register
 // index is a constant; per pivot the raw pivot row and column go through LDS
-// (double-buffered, one barrier per pivot; the row is read back as float4).
+// (double-buffered, one barrier per pivot; the row is read back as float4). The
+// kernel is bound by the per-pivot rank-1 update on one CU.
 constexpr int kInvN = 128;
 
 __global__ __launch_bounds__(512) void k_spd_inverse_small(int n, float* __restrict__ A, int lda) {

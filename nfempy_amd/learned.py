@@ -154,6 +154,11 @@ class SchurGraph:
         agg_of = np.empty(E, dtype=np.int64)
         agg_of[p_rows] = agg
         self.agg_of_np, self.n_agg = agg_of, n_agg
+        hasn = nbr >= 0
+        ca = np.repeat(agg_of, 4)[hasn.ravel()]
+        cb = agg_of[nbr[hasn]]
+        off = ca != cb
+        self.coarse_plan = coupling_plan(n_agg, ca[off], cb[off])
         order = np.argsort(agg_of, kind="stable")
         ptr = np.zeros(n_agg + 1, dtype=np.int64)
         np.add.at(ptr, agg_of + 1, 1)
@@ -177,28 +182,71 @@ def aggregate_cells(centroids, h):
     return inv.astype(np.int64), len(u)
 
 
-def spd_inverse_(A, stream=None):
+def _split(n):
+    h = n // 2
+    return (h // 128) * 128 if h > 128 else h
+
+
+def coupling_plan(n, rows, cols):
+    """Coupling envelopes of the recursive 2 x 2 inversion (``spd_inverse_``) of a
+    symmetric matrix with off-diagonal nonzeros at (rows, cols): for the node
+    (offset o, size m) split at h, (lo, hi) with every nonzero of the block
+    A[o:o+h, o+h:o+m] inside rows [lo, h) and columns [h, hi) (node-relative;
+    lo = hi = h when the halves do not couple). The fill of each Schur complement
+    (a dense block on F x F of the second half) is propagated symbolically."""
+    P = np.zeros((n, n), dtype=bool)
+    P[np.asarray(rows, dtype=np.int64), np.asarray(cols, dtype=np.int64)] = True
+    P |= P.T
+    plan = {}
+
+    def rec(o, m):
+        if m <= 128:
+            return
+        h = _split(m)
+        blk = P[o:o + h, o + h:o + m]
+        rr = np.flatnonzero(blk.any(axis=1))
+        cc = np.flatnonzero(blk.any(axis=0))
+        rec(o, h)
+        if len(rr):
+            lo, hi = int(rr[0]), int(h + cc[-1] + 1)
+            plan[(o, m)] = (lo, hi)
+            P[o + h:o + hi, o + h:o + hi] = True
+        else:
+            plan[(o, m)] = (h, h)
+        rec(o + h, m - h)
+
+    rec(0, n)
+    return plan
+
+
+def spd_inverse_(A, plan=None, stream=None, o=0):
     """In-place inverse of an SPD matrix (FP32, row-major view with unit column stride)
-    by recursive 2 x 2 blocking:
-    X11 = A11^-1, T = X11 A12, S = A22 - A21 T, X22 = S^-1, X12 = -T X22,
-    X11 <- X11 - X12 T^T, X21 = X12^T; diagonal blocks of size <= 128 are inverted by
-    ``nfx_spd_inverse_small`` (Gauss-Jordan in LDS), the products are library GEMMs
+    by recursive 2 x 2 blocking. With B = A12[L, F] the only nonzero part of the
+    coupling block (rows L = [lo, h), columns F = [h, hi), from ``coupling_plan``):
+    X11 = A11^-1, T = X11[:, L] B, S = A22 - B^T T[L, :] (on F x F only), X22 = S^-1,
+    X12 = -T X22[F, :], X11 <- X11 - X12[:, F] T^T, X21 = X12^T. Without a plan the
+    envelope is the whole block. Diagonal blocks of size <= 128 are inverted by
+    ``nfx_spd_inverse_small`` (Gauss-Jordan in LDS); the products are library GEMMs
     written in place into the strided blocks."""
     t = torch()
     n = A.shape[0]
     if n <= 128:
         nat.call("nfx_spd_inverse_small", n, A.data_ptr(), A.stride(0), nat.stream_handle(stream))
         return
-    h = n // 2
-    if h > 128:
-        h = (h // 128) * 128
+    h = _split(n)
+    lo, hi = plan[(o, n)] if plan is not None else (0, n)
     A11, A12, A21, A22 = A[:h, :h], A[:h, h:], A[h:, :h], A[h:, h:]
-    spd_inverse_(A11, stream)
-    T = t.mm(A11, A12)
-    A22.addmm_(A21, T, alpha=-1.0)
-    spd_inverse_(A22, stream)
-    A12.addmm_(T, A22, beta=0.0, alpha=-1.0)
-    A11.addmm_(A12, T.T, alpha=-1.0)
+    spd_inverse_(A11, plan, stream, o)
+    if hi <= h:
+        spd_inverse_(A22, plan, stream, o + h)
+        return
+    f = hi - h
+    B = A12[lo:, :f]
+    T = t.mm(A11[:, lo:], B)
+    A22[:f, :f].addmm_(B.T, T[lo:, :], alpha=-1.0)
+    spd_inverse_(A22, plan, stream, o + h)
+    A12.addmm_(T, A22[:f, :], beta=0.0, alpha=-1.0)
+    A11.addmm_(A12[:, :f], T.T, alpha=-1.0)
     A21.copy_(A12.T)
 
 
@@ -279,7 +327,7 @@ class LearnedSchurPC:
                  nat.ptr(g.nbr), nat.ptr(self.sdiag), nat.ptr(self.soff), nat.ptr(self._Ac64),
                  nat.stream_handle(stream))
         self.Ainv.copy_(self._Ac64)
-        spd_inverse_(self.Ainv, stream)
+        spd_inverse_(self.Ainv, self.g.coarse_plan, stream)
         if self.coarse_fp16:
             nat.call("nfx_half_rows", g.n_agg, nat.ptr(self.Ainv), nat.ptr(self.Ainv16), nat.ptr(self.Ascale),
                      nat.stream_handle(stream))

@@ -174,6 +174,19 @@ def test_coarse_inverse_and_graph_replays():
         assert torch.allclose(pc.Ainv, pc.Ainv.T, rtol=0, atol=1e-6 * float(eager.abs().max()))
         _ = (sysd.vals32 * 2).sum()  # eager work between replays
         torch.cuda.synchronize()
+    # full C4 size: 4096 aggregates, banded recursion with fill (eager and graph)
+    print("[learned] coarse inverse n=32", flush=True)
+    sysd, _, f = _system(32, seed=10000, ell=0.05)
+    pc = LearnedSchurPC(sysd, None)
+    assert pc.g.n_agg == 4096
+    pc.schur(sysd.vals64)
+    for graph in (False, True):
+        pc.coarse(graph=graph)
+        torch.cuda.synchronize()
+        ref = torch.linalg.inv(pc._Ac64)
+        err = float((pc.Ainv.double() - ref).abs().max() / ref.abs().max())
+        print(f"[learned] n_agg=4096 graph={graph}: inverse rel err {err:.2e}", flush=True)
+        assert err < 1e-4
 
 
 def test_fgmres_diagonal_and_none():
