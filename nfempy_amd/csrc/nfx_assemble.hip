@@ -24,6 +24,35 @@ struct Geom {
   R detJ;
 };
 
+// Geometry from already loaded vertex coordinates (same arithmetic as load_geometry).
+template <int DIM, typename R>
+__device__ __forceinline__ void geometry_from(const R (&X)[DIM + 1][3], Geom<DIM, R>& g) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) g.J[c][d] = X[d + 1][c] - X[0][c];
+#pragma unroll
+  for (int i = 0; i < DIM; ++i)
+#pragma unroll
+    for (int j = 0; j < DIM; ++j) {
+      R s = 0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) s += g.J[c][i] * g.J[c][j];
+      g.G[i][j] = s;
+    }
+  if constexpr (DIM == 3) {
+    const R d = g.J[0][0] * (g.J[1][1] * g.J[2][2] - g.J[1][2] * g.J[2][1]) -
+                g.J[0][1] * (g.J[1][0] * g.J[2][2] - g.J[1][2] * g.J[2][0]) +
+                g.J[0][2] * (g.J[1][0] * g.J[2][1] - g.J[1][1] * g.J[2][0]);
+    g.detJ = d < R(0) ? -d : d;
+  } else {
+    const R cx = g.J[1][0] * g.J[2][1] - g.J[2][0] * g.J[1][1];
+    const R cy = g.J[2][0] * g.J[0][1] - g.J[0][0] * g.J[2][1];
+    const R cz = g.J[0][0] * g.J[1][1] - g.J[1][0] * g.J[0][1];
+    g.detJ = sqrt(cx * cx + cy * cy + cz * cz);
+  }
+}
+
 template <int DIM, typename R>
 __device__ __forceinline__ void load_geometry(const int32_t* __restrict__ cells,
                                               const R* __restrict__ coords,
@@ -283,6 +312,23 @@ constexpr int kRowsPerBlock = 128;
 // One tile of TILE consecutive rows [r0, r0 + TILE) of one system. st / sv are
 // LDS buffers of at least NT and TILE * 2 * NL values. Called by all threads of
 // the block; contains block barriers.
+// Table layout: w[NQ] | phi[NQ][NB][DIM] | div[NQ][NB] | psi[NQ][NP] | Rs[NB][NB][NS] |
+// Bc[NB][NP] | ck[NP]  (NS = DIM (DIM + 1) / 2 symmetric pairs of G = J^T J).
+template <int DIM, int NB, int NP, int NQ>
+struct DarcyTab {
+  static constexpr int NS = DIM * (DIM + 1) / 2;
+  static constexpr int NT = NQ * (1 + NB * DIM + NB + NP) + NB * NB * NS + NB * NP + NP;
+};
+
+// One tile of TILE consecutive rows [r0, r0 + TILE) of one system. st / sv are
+// LDS buffers of at least DarcyTab::NT and TILE * 2 * NL values. Called by all
+// threads of the block; contains block barriers.
+// With per-element coefficients (ks_q == 0 and fs_q == 0) the quadrature sums are
+// the precomputed reference tensors: flux row a of element e is
+//   M_ab = kinv_e / detJ sum_s G_s Rs[a][b][s],  B part -Bc[a][k];
+// pressure row k: Bc[b][k] and residual -detJ f_e ck[k]. Otherwise the sums run
+// over the quadrature points. The loads of both incidences are issued before
+// either row is computed.
 template <int DIM, int NB, int NP, int NQ, int TILE, typename R>
 __device__ __forceinline__ void darcy_row_tile(
     int r0, int n_rows, const int32_t* __restrict__ inc, const uint8_t* __restrict__ pos,
@@ -291,7 +337,8 @@ __device__ __forceinline__ void darcy_row_tile(
     const R* __restrict__ kinv, int ks_e, int ks_q, const R* __restrict__ f, int fs_e, int fs_q,
     R* __restrict__ vals, R* __restrict__ rhs, R* st, R* sv) {
   constexpr int NL = NB + NP;
-  constexpr int NT = NQ * (1 + NB * DIM + NB + NP);
+  constexpr int NS = DarcyTab<DIM, NB, NP, NQ>::NS;
+  constexpr int NT = DarcyTab<DIM, NB, NP, NQ>::NT;
   const int r = r0 + threadIdx.x;
   const int rl = min(r0 + TILE, n_rows);
   for (int i = threadIdx.x; i < NT; i += TILE) st[i] = tab[i];
@@ -303,33 +350,87 @@ __device__ __forceinline__ void darcy_row_tile(
   const R* tphi = st + NQ;
   const R* tdiv = tphi + NQ * NB * DIM;
   const R* tpsi = tdiv + NQ * NB;
+  const R* tRs = tpsi + NQ * NP;
+  const R* tBc = tRs + NB * NB * NS;
+  const R* tck = tBc + NB * NP;
+  const bool per_element = (ks_q == 0 && fs_q == 0);
   if (r < n_rows) {
     const int off = rowptr[r] - base;
     R res = 0;
+    int code[2], e[2], a[2];
+    uint32_t sg[2];
+    R X[2][DIM + 1][3];
+    R kv[2], fv[2];
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-      const int code = inc[(int64_t)j * n_rows + r];
-      if (code < 0) continue;
-      const int e = code >> 5;
-      const int a = code & 31;
-      const uint32_t sg = signs[e] ^ (((signs[e] >> a) & 1u) ? 0xffffffffu : 0u);
+      code[j] = inc[(int64_t)j * n_rows + r];
+      e[j] = code[j] < 0 ? 0 : code[j] >> 5;
+      a[j] = code[j] & 31;
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      sg[j] = 0;
+      kv[j] = fv[j] = R(0);
+      if (code[j] >= 0) {
+        const uint32_t s0 = signs[e[j]];
+        sg[j] = s0 ^ (((s0 >> a[j]) & 1u) ? 0xffffffffu : 0u);
+        int v[DIM + 1];
+#pragma unroll
+        for (int c = 0; c <= DIM; ++c) v[c] = cells[e[j] * (DIM + 1) + c];
+#pragma unroll
+        for (int c = 0; c <= DIM; ++c)
+#pragma unroll
+          for (int d = 0; d < 3; ++d) X[j][c][d] = coords[3 * v[c] + d];
+        if (per_element) {
+          kv[j] = kinv[(int64_t)e[j] * ks_e];
+          fv[j] = f[(int64_t)e[j] * fs_e];
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      if (code[j] < 0) continue;
+      Geom<DIM, R> g;
+      geometry_from<DIM, R>(X[j], g);
       R row[NL];
 #pragma unroll
       for (int b = 0; b < NL; ++b) row[b] = 0;
-      Geom<DIM, R> g;
-      R X[DIM + 1][3];
-      load_geometry<DIM, R>(cells, coords, e, g, X);
-      if (a < NB) {
+      if (per_element) {
+        if (a[j] < NB) {
+          const R sc = kv[j] / g.detJ;
+          R gs[NS];
+          int s = 0;
+#pragma unroll
+          for (int d = 0; d < DIM; ++d)
+#pragma unroll
+            for (int d2 = d; d2 < DIM; ++d2) gs[s++] = g.G[d][d2] * sc;
+          const R* rs = tRs + a[j] * NB * NS;
+#pragma unroll
+          for (int b = 0; b < NB; ++b) {
+            R m = 0;
+#pragma unroll
+            for (int t = 0; t < NS; ++t) m += gs[t] * rs[b * NS + t];
+            row[b] = m;
+          }
+#pragma unroll
+          for (int k = 0; k < NP; ++k) row[NB + k] = -tBc[a[j] * NP + k];
+        } else {
+          const int k = a[j] - NB;
+#pragma unroll
+          for (int b = 0; b < NB; ++b) row[b] = tBc[b * NP + k];
+          res -= g.detJ * fv[j] * tck[k];
+        }
+      } else if (a[j] < NB) {
         const R invdet = R(1) / g.detJ;
         for (int q = 0; q < NQ; ++q) {
           const R w = tw[q];
-          const R cm = w * kinv[(int64_t)e * ks_e + q * ks_q] * invdet;
+          const R cm = w * kinv[(int64_t)e[j] * ks_e + q * ks_q] * invdet;
           R ga[DIM];
 #pragma unroll
           for (int d = 0; d < DIM; ++d) {
             R s = 0;
 #pragma unroll
-            for (int d2 = 0; d2 < DIM; ++d2) s += g.G[d][d2] * tphi[(q * NB + a) * DIM + d2];
+            for (int d2 = 0; d2 < DIM; ++d2) s += g.G[d][d2] * tphi[(q * NB + a[j]) * DIM + d2];
             ga[d] = s * cm;
           }
 #pragma unroll
@@ -339,17 +440,17 @@ __device__ __forceinline__ void darcy_row_tile(
             for (int d = 0; d < DIM; ++d) s += ga[d] * tphi[(q * NB + b) * DIM + d];
             row[b] += s;
           }
-          const R wd = w * tdiv[q * NB + a];
+          const R wd = w * tdiv[q * NB + a[j]];
 #pragma unroll
           for (int k = 0; k < NP; ++k) row[NB + k] -= wd * tpsi[q * NP + k];
         }
       } else {
-        const int k = a - NB;
+        const int k = a[j] - NB;
         for (int q = 0; q < NQ; ++q) {
           const R wp = tw[q] * tpsi[q * NP + k];
 #pragma unroll
           for (int b = 0; b < NB; ++b) row[b] += wp * tdiv[q * NB + b];
-          res -= wp * g.detJ * f[(int64_t)e * fs_e + q * fs_q];
+          res -= wp * g.detJ * f[(int64_t)e[j] * fs_e + q * fs_q];
         }
       }
       const uint8_t* pr = pos + ((int64_t)r * 2 + j) * NL;
@@ -357,7 +458,7 @@ __device__ __forceinline__ void darcy_row_tile(
       for (int b = 0; b < NL; ++b) {
         const int p = pr[b];
         if (p == 255) continue;
-        sv[off + p] += ((sg >> b) & 1u) ? -row[b] : row[b];
+        sv[off + p] += ((sg[j] >> b) & 1u) ? -row[b] : row[b];
       }
     }
     rhs[r] = res;
@@ -375,7 +476,7 @@ __global__ __launch_bounds__(kRowsPerBlock) void k_darcy_rows(
     const R* __restrict__ kinv, int ks_e, int ks_q, const R* __restrict__ f, int fs_e, int fs_q,
     R* __restrict__ vals, R* __restrict__ rhs) {
   constexpr int NL = NB + NP;
-  constexpr int NT = NQ * (1 + NB * DIM + NB + NP);
+  constexpr int NT = DarcyTab<DIM, NB, NP, NQ>::NT;
   __shared__ R st[NT];
   __shared__ R sv[kRowsPerBlock * 2 * NL];
   darcy_row_tile<DIM, NB, NP, NQ, kRowsPerBlock, R>(blockIdx.x * kRowsPerBlock, n_rows, inc, pos,
@@ -407,7 +508,7 @@ template <int TILE>
 __global__ __launch_bounds__(TILE) void k_darcy_rows_batched(const DarcyBatchItem* __restrict__ items,
                                                              const int2* __restrict__ tiles,
                                                              int n_tiles) {
-  constexpr int NT2 = 27 * (1 + 15 * 3 + 15 + 4);
+  constexpr int NT2 = DarcyTab<3, 15, 4, 27>::NT;
   __shared__ float st[NT2];
   __shared__ float sv[TILE * 2 * 19];
   for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {

@@ -53,13 +53,27 @@ def p1_bary(points):
 
 
 def darcy_tables(u_el, p_el, points, weights):
-    """w | phi (nq, nb, dim) | div (nq, nb) | psi (nq, np) as one FP64 vector."""
+    """w | phi (nq, nb, dim) | div (nq, nb) | psi (nq, np) | Rs (nb, nb, ns) | Bc (nb, np) |
+    ck (np) as one FP64 vector.
+
+    The last three are reference tensors of the same quadrature sums, used when the
+    coefficients are constant per element:
+    Rs[a, b, s] = sum_q w phi_a . (E_s phi_b) over the ns = dim (dim + 1) / 2 symmetric
+    pairs s = (d <= d') (E_s the symmetric unit pair), so that the mass row is
+    kinv / detJ sum_s G_s Rs[a, b, s] with G = J^T J; Bc[a, k] = sum_q w div_a psi_k;
+    ck[k] = sum_q w psi_k."""
     tu = u_el.tabulate(points)
     tp = p_el.tabulate(points)
     phi = tu[0]
     div = np.einsum("kqjk->qj", tu[1:])
     psi = tp[0, :, :, 0]
-    return np.concatenate([weights, phi.ravel(), div.ravel(), psi.ravel()])
+    dim = phi.shape[2]
+    R = np.einsum("q,qax,qby->abxy", weights, phi, phi)
+    pairs = [(d, d2) for d in range(dim) for d2 in range(d, dim)]
+    Rs = np.stack([R[:, :, d2, d] + (R[:, :, d, d2] if d != d2 else 0.0) for d, d2 in pairs], axis=-1)
+    Bc = np.einsum("q,qa,qk->ak", weights, div, psi)
+    ck = np.einsum("q,qk->k", weights, psi)
+    return np.concatenate([weights, phi.ravel(), div.ravel(), psi.ravel(), Rs.ravel(), Bc.ravel(), ck])
 
 
 def facet_combo_tables(el, D, fpoints):
@@ -284,7 +298,8 @@ class DarcySystem:
         V = len(mesh.points)
         if self.cells_np.shape != (self.E, self.dim + 1) or (self.E and (self.cells_np.min() < 0 or self.cells_np.max() >= V)):
             raise ValueError("cells out of range")
-        if self.tables64.numel() != NQ * (1 + NB * self.dim + NB + NP):
+        NS = self.dim * (self.dim + 1) // 2
+        if self.tables64.numel() != NQ * (1 + NB * self.dim + NB + NP) + NB * NB * NS + NB * NP + NP:
             raise ValueError("table size mismatch")
         pat = self.pattern
         if pat.slot.shape != (self.nl * self.nl, self.E):

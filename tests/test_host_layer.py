@@ -102,3 +102,38 @@ def test_launch_validation(case):
     sysd.pattern.slot = bad
     with pytest.raises(ValueError):
         sysd._validate(sysd.u_sp.mesh)
+
+
+@pytest.mark.parametrize("dim,k", [(2, 1), (2, 2), (3, 1), (3, 2)])
+def test_reference_tensors_match_quadrature_sums(dim, k):
+    """The per-element fast path of the row assembly kernel (reference tensors Rs, Bc,
+    ck appended to the Darcy tables) equals the quadrature sums of the general path
+    for a random affine map."""
+    from nfempy_amd.device import darcy_tables
+    from tests._cases import product_darcy_space
+
+    ps = product_darcy_space(dim, n=2, k=k)
+    u_el, p_el = ps.discrete_spaces["u"].element, ps.discrete_spaces["p"].element
+    pts, wts = ps.quadrature[dim]
+    tab = darcy_tables(u_el, p_el, pts, wts)
+    nq, nb, npp = len(wts), u_el.ndofs, p_el.ndofs
+    ns = dim * (dim + 1) // 2
+    o = 0
+    w = tab[o:o + nq]; o += nq
+    phi = tab[o:o + nq * nb * dim].reshape(nq, nb, dim); o += nq * nb * dim
+    div = tab[o:o + nq * nb].reshape(nq, nb); o += nq * nb
+    psi = tab[o:o + nq * npp].reshape(nq, npp); o += nq * npp
+    Rs = tab[o:o + nb * nb * ns].reshape(nb, nb, ns); o += nb * nb * ns
+    Bc = tab[o:o + nb * npp].reshape(nb, npp); o += nb * npp
+    ck = tab[o:o + npp]; o += npp
+    assert o == len(tab)
+    rng = np.random.default_rng(dim * 10 + k)
+    J = rng.standard_normal((3, dim))
+    G = J.T @ J
+    kinv = 0.37
+    M_q = kinv * np.einsum("q,qax,xy,qby->ab", w, phi, G, phi)
+    gs = np.array([G[d, d2] for d in range(dim) for d2 in range(d, dim)])
+    M_t = kinv * np.einsum("s,abs->ab", gs, Rs)
+    assert np.allclose(M_t, M_q, rtol=1e-12, atol=1e-12 * np.abs(M_q).max())
+    assert np.allclose(Bc, np.einsum("q,qa,qk->ak", w, div, psi), rtol=1e-12, atol=1e-14)
+    assert np.allclose(ck, np.einsum("q,qk->k", w, psi), rtol=1e-12, atol=1e-14)
