@@ -331,6 +331,13 @@ typedef struct nfx_twogrid_pc {
   float* work;
 } nfx_twogrid_pc;
 int nfx_twogrid_apply(const nfx_twogrid_pc* pc, const float* d_r, float* d_z, void* stream);
+/* C = beta C + alpha op(A) op(B) (FP32, row-major, leading dimensions lda/ldb/ldc;
+ * op(A) = A (M x K) or A^T when transA (A stored K x M); op(B) = B (K x N) or B^T when
+ * transB). The products of the coarse-operator inversion (thin K, 64 x 64 output
+ * tiles on MFMA). beta = 0 does not read C. */
+int nfx_sgemm(int M, int N, int K, float alpha, const float* d_A, int lda, int transA,
+              const float* d_B, int ldb, int transB, float beta, float* d_C, int ldc,
+              void* stream);
 /* Row-scaled FP16 copy of a dense n x n FP32 matrix (Ascale[r] = max |A[r][:]|). */
 int nfx_half_rows(int n, const float* d_A, uint16_t* d_A16, float* d_scale, void* stream);
 

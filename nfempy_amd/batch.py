@@ -50,16 +50,20 @@ def darcy_space(n, k):
 
 
 class DarcyBatch:
-    def __init__(self, specs, seed=0, ell=0.1, sigma=1.0, device="cuda"):
-        """specs: list of (n, k) per instance."""
+    def __init__(self, specs, seed=0, ell=0.1, sigma=1.0, device="cuda", log=None):
+        """specs: list of (n, k) per instance; log: optional progress callable."""
         t = torch()
         if nat.call("nfx_darcy_batch_item_size") != ctypes.sizeof(_Item):
             raise RuntimeError("batch descriptor layout differs between host and library")
         self.device = device
         self.specs = [(int(n), int(k)) for n, k in specs]
         self.systems = {}
-        for key in sorted(set(self.specs)):
+        keys = sorted(set(self.specs))
+        for j, key in enumerate(keys):
             self.systems[key] = DarcySystem(darcy_space(*key), device=device)
+            if log is not None:
+                log(f"batch system {j + 1}/{len(keys)}: n={key[0]} k={key[1]} E={self.systems[key].E} "
+                    f"nnz={self.systems[key].nnz}")
         self.instances = []
         for i, (n, k) in enumerate(self.specs):
             sysd = self.systems[(n, k)]

@@ -314,6 +314,8 @@ constexpr int kRowsPerBlock = 128;
 // the block; contains block barriers.
 // Table layout: w[NQ] | phi[NQ][NB][DIM] | div[NQ][NB] | psi[NQ][NP] | Rs[NB][NB][NS] |
 // Bc[NB][NP] | ck[NP]  (NS = DIM (DIM + 1) / 2 symmetric pairs of G = J^T J).
+constexpr int kXcds = 8;  // MI355X: 8 XCDs, each with its own L2
+
 template <int DIM, int NB, int NP, int NQ>
 struct DarcyTab {
   static constexpr int NS = DIM * (DIM + 1) / 2;
@@ -479,7 +481,12 @@ __global__ __launch_bounds__(kRowsPerBlock) void k_darcy_rows(
   constexpr int NT = DarcyTab<DIM, NB, NP, NQ>::NT;
   __shared__ R st[NT];
   __shared__ R sv[kRowsPerBlock * 2 * NL];
-  darcy_row_tile<DIM, NB, NP, NQ, kRowsPerBlock, R>(blockIdx.x * kRowsPerBlock, n_rows, inc, pos,
+  // XCD-aware tile order: workgroups are dealt to the 8 XCDs round-robin, so block b
+  // takes tile (b % 8) * (nb / 8) + b / 8: each XCD walks a contiguous range of row
+  // tiles, and the elements shared by neighbouring tiles stay in that XCD's L2.
+  const int nb = gridDim.x, nb8 = nb / kXcds, b = blockIdx.x;
+  const int tile = b < nb8 * kXcds ? (b % kXcds) * nb8 + b / kXcds : b;
+  darcy_row_tile<DIM, NB, NP, NQ, kRowsPerBlock, R>(tile * kRowsPerBlock, n_rows, inc, pos,
                                                     rowptr, signs, cells, coords, tab, kinv, ks_e,
                                                     ks_q, f, fs_e, fs_q, vals, rhs, st, sv);
 }

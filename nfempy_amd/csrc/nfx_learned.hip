@@ -642,6 +642,91 @@ __global__ __launch_bounds__(512) void k_spd_inverse_small(int n, float* __restr
   }
 }
 
+// C = beta C + alpha op(A) op(B), FP32, row-major with leading dimensions
+// (op(A) is M x K: A[i*lda + k], or A[k*lda + i] when transA; op(B) is K x N:
+// B[k*ldb + j], or B[j*ldb + k] when transB). The products of the coarse-operator
+// inversion have K <= 256 and outputs up to 2048 x 2048: 64 x 64 output tiles keep
+// every CU busy where a library kernel with 256 x 256 tiles leaves most idle.
+// 4 waves per tile, each 32 x 32 = 2 x 2 MFMA tiles of v_mfma_f32_16x16x4_f32;
+// K in chunks of 16 staged through LDS. Beta = 0 does not read C.
+constexpr int kGT = 64;
+constexpr int kGK = 16;
+
+__global__ __launch_bounds__(256) void k_sgemm(int M, int N, int K, float alpha, const float* __restrict__ A,
+                                               int lda, int transA, const float* __restrict__ B, int ldb,
+                                               int transB, float beta, float* __restrict__ C, int ldc) {
+  __shared__ float sA[kGT][kGK + 1];
+  __shared__ float sB[kGK][kGT + 1];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int i0 = blockIdx.y * kGT, j0 = blockIdx.x * kGT;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  float4_t acc[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) acc[a][b] = float4_t{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < K; k0 += kGK) {
+    // stage A tile (64 x 16) and B tile (16 x 64): 4 elements each per thread
+    for (int e = tid; e < kGT * kGK; e += 256) {
+      int r, c;
+      if (transA) {  // coalesce along the row index of the stored K x M matrix
+        r = e % kGT;
+        c = e / kGT;
+      } else {
+        r = e / kGK;
+        c = e % kGK;
+      }
+      const int gi = i0 + r, gk = k0 + c;
+      float v = 0.f;
+      if (gi < M && gk < K) v = transA ? A[(int64_t)gk * lda + gi] : A[(int64_t)gi * lda + gk];
+      sA[r][c] = v;
+    }
+    for (int e = tid; e < kGT * kGK; e += 256) {
+      int r, c;
+      if (transB) {
+        r = e % kGK;
+        c = e / kGK;
+      } else {
+        r = e / kGT;
+        c = e % kGT;
+      }
+      const int gk = k0 + r, gj = j0 + c;
+      float v = 0.f;
+      if (gk < K && gj < N) v = transB ? B[(int64_t)gj * ldb + gk] : B[(int64_t)gk * ldb + gj];
+      sB[r][c] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < kGK; kk += 4) {
+      const int kl = kk + (lane >> 4);
+      float av[2], bv[2];
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        av[t] = sA[wr + 16 * t + (lane & 15)][kl];
+        bv[t] = sB[kl][wc + 16 * t + (lane & 15)];
+      }
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[a], bv[b], acc[a][b], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // lane holds D[4 (lane >> 4) + i][lane & 15] of each 16 x 16 tile
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int gi = i0 + wr + 16 * a + 4 * (lane >> 4) + i;
+        const int gj = j0 + wc + 16 * b + (lane & 15);
+        if (gi < M && gj < N) {
+          float* cp = C + (int64_t)gi * ldc + gj;
+          const float v = alpha * acc[a][b][i];
+          *cp = beta == 0.f ? v : beta * *cp + v;
+        }
+      }
+}
+
 // x2 = x1 + y[agg]; t2 = r - S x2 (x2 of neighbours recomputed from x1 and y)
 __global__ void k_tg_post1(int E, const int32_t* __restrict__ nbr, const int32_t* __restrict__ agg_of,
                            const float* __restrict__ sd, const float* __restrict__ so,
@@ -877,6 +962,18 @@ int nfx_spd_inverse_small(int n, float* A, int lda, void* stream) {
   NFX_REQUIRE(n >= 1 && n <= kInvN && lda >= n, "nfx_spd_inverse_small: need 1 <= n <= 128 and lda >= n");
   NFX_REQUIRE(A, "nfx_spd_inverse_small: null pointer");
   hipLaunchKernelGGL(k_spd_inverse_small, dim3(1), dim3(512), 0, (hipStream_t)stream, n, A, lda);
+  NFX_LAUNCH_CHECK();
+  return NFX_OK;
+}
+
+int nfx_sgemm(int M, int N, int K, float alpha, const float* A, int lda, int transA, const float* B, int ldb,
+              int transB, float beta, float* C, int ldc, void* stream) {
+  NFX_REQUIRE(M >= 0 && N >= 0 && K >= 0, "nfx_sgemm: negative size");
+  if (M == 0 || N == 0) return NFX_OK;
+  NFX_REQUIRE(A && B && C, "nfx_sgemm: null pointer");
+  NFX_REQUIRE(lda >= (transA ? M : K) && ldb >= (transB ? K : N) && ldc >= N, "nfx_sgemm: leading dimension");
+  hipLaunchKernelGGL(k_sgemm, dim3(grid_for(N, kGT), grid_for(M, kGT)), dim3(256), 0, (hipStream_t)stream, M, N,
+                     K, alpha, A, lda, transA, B, ldb, transB, beta, C, ldc);
   NFX_LAUNCH_CHECK();
   return NFX_OK;
 }

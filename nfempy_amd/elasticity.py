@@ -295,43 +295,79 @@ class ElasticitySystem:
         return sps.csr_matrix((vals, pat.col, pat.rowptr), shape=(self.n, self.n))
 
     # -- preconditioner -----------------------------------------------------------
-    def preconditioner(self, P=None):
-        """(sign, pc) for MinresIR: block-diagonal inverse of the Riesz operator P.
+    def preconditioner(self, P=None, kind="mass_schur"):
+        """(sign, pc) for MinresIR: an SPD block-diagonal preconditioner as a CSR matrix.
 
-        sigma: one 9 x 9 block per face (its 3 BDM1 DoFs x 3 stress rows are
-        consecutive in the DoF numbering); u, t: inverse diagonal."""
-        t = torch()
-        self.assemble("P", fp64=True)
-        rp, col, _, v64 = self.full("P")
-        rp_h, col_h, v_h = rp.cpu().numpy(), col.cpu().numpy(), v64.cpu().numpy()
+        kind "mass_schur" (default): sigma rows get the inverse of the 9 x 9 face blocks
+        of the compliance block A_ss (each face's 3 BDM1 DoFs x 3 stress rows are
+        consecutive in the DoF numbering); the (u, t) rows get the inverse of the 6 x 6
+        cell blocks of the Schur complement S = B D^-1 B^T, D the 9 x 9 face blocks,
+        B the (u, t) x sigma block of A.
+        kind "riesz_block": 9 x 9 face blocks of the Riesz operator P and its diagonal
+        on u, t (the reference factorises P's sigma block exactly,
+        ``mfe_elasticity_example_1.py:246-262``; a block-Jacobi of that H(div) operator
+        is a much weaker stand-in)."""
         import scipy.sparse as sps
 
-        Pm = sps.csr_matrix((v_h, col_h, rp_h), shape=(self.n, self.n))
+        which = "A" if kind == "mass_schur" else "P"
+        if which == "P":
+            self.assemble("P", fp64=True)
+        # A: its FP64 values from the caller's assembly (re-assembling here would replace
+        # the residual, which carries the caller's Dirichlet data)
+        rp, col, _, v64 = self.full(which)
+        Mh = sps.csr_matrix((v64.cpu().numpy(), col.cpu().numpy(), rp.cpu().numpy()), shape=(self.n, self.n))
         ns = self.n_s
         nb = ns // 9
         blocks = np.zeros((nb, 9, 9))
-        Ps = Pm[:ns, :ns].tocoo()
-        same = (Ps.row // 9) == (Ps.col // 9)
-        blocks[Ps.row[same] // 9, Ps.row[same] % 9, Ps.col[same] % 9] = Ps.data[same]
-        inv = np.linalg.inv(blocks)
-        inv = 0.5 * (inv + inv.transpose(0, 2, 1))
-        dg = Pm.diagonal()[ns:]
+        Ms = Mh[:ns, :ns].tocoo()
+        same = (Ms.row // 9) == (Ms.col // 9)
+        blocks[Ms.row[same] // 9, Ms.row[same] % 9, Ms.col[same] % 9] = Ms.data[same]
+        Dinv = np.linalg.inv(blocks)
+        Dinv = 0.5 * (Dinv + Dinv.transpose(0, 2, 1))
         rows_b = (np.arange(nb)[:, None, None] * 9 + np.arange(9)[None, :, None]).repeat(9, axis=2)
         cols_b = (np.arange(nb)[:, None, None] * 9 + np.arange(9)[None, None, :]).repeat(9, axis=1)
-        r = np.concatenate([rows_b.ravel(), np.arange(ns, self.n)])
-        c = np.concatenate([cols_b.ravel(), np.arange(ns, self.n)])
-        v = np.concatenate([inv.ravel(), 1.0 / dg])
-        M = sps.csr_matrix((v, (r, c)), shape=(self.n, self.n))
+        r_list, c_list, v_list = [rows_b.ravel()], [cols_b.ravel()], [Dinv.ravel()]
+        if kind == "mass_schur":
+            nu = self.u_sp.n_dof
+            E = nu // 3
+            B = Mh[ns:, :ns].tocoo()
+            loc = B.row
+            cell = (loc % nu) // 3
+            comp = (loc // nu) * 3 + loc % 3
+            face = B.col // 9
+            key = cell.astype(np.int64) * nb + face
+            uk, inv = np.unique(key, return_inverse=True)
+            Bblk = np.zeros((len(uk), 6, 9))
+            np.add.at(Bblk, (inv, comp, B.col % 9), B.data)
+            pc_cell = uk // nb
+            pc_face = uk % nb
+            contrib = np.einsum("pai,pij,pbj->pab", Bblk, Dinv[pc_face], Bblk)
+            S = np.zeros((E, 6, 6))
+            np.add.at(S, pc_cell, contrib)
+            Sinv = np.linalg.inv(S)
+            Sinv = 0.5 * (Sinv + Sinv.transpose(0, 2, 1))
+            dof = ns + np.concatenate([np.arange(E)[:, None] * 3 + np.arange(3),
+                                       nu + np.arange(E)[:, None] * 3 + np.arange(3)], axis=1)  # (E, 6)
+            r_list.append(np.repeat(dof, 6, axis=1).ravel())
+            c_list.append(np.tile(dof, (1, 6)).ravel())
+            v_list.append(Sinv.ravel())
+        else:
+            dg = Mh.diagonal()[ns:]
+            r_list.append(np.arange(ns, self.n))
+            c_list.append(np.arange(ns, self.n))
+            v_list.append(1.0 / dg)
+        M = sps.csr_matrix((np.concatenate(v_list), (np.concatenate(r_list), np.concatenate(c_list))),
+                           shape=(self.n, self.n))
         M.sort_indices()
         pc = (to_dev(M.indptr.astype(np.int32), device=self.device), to_dev(M.indices.astype(np.int32), device=self.device),
               to_dev(M.data.astype(np.float32), device=self.device))
         return None, pc
 
-    def solve(self, rtol=1e-8, inner_rtol=1e-5, max_inner=50000, max_outer=30):
+    def solve(self, rtol=1e-8, inner_rtol=1e-5, max_inner=50000, max_outer=30, pc_kind="mass_schur"):
         from .solvers import MinresIR
 
+        sign, pc = self.preconditioner(kind=pc_kind)
         rp, col, v32, v64 = self.full("A")
-        sign, pc = self.preconditioner()
         solver = MinresIR(rp, col, v32, v64, sign=None, pc=pc)
         x, info = solver.solve(-self.rhs64, rtol=rtol, inner_rtol=inner_rtol, max_inner=max_inner,
                                max_outer=max_outer)

@@ -34,8 +34,6 @@ reference counterpart.
 
 from __future__ import annotations
 
-import time
-
 import numpy as np
 
 from . import _native as nat
@@ -219,15 +217,22 @@ def coupling_plan(n, rows, cols):
     return plan
 
 
-def spd_inverse_(A, plan=None, stream=None, o=0):
+def _sgemm(M, N, K, alpha, A, transA, B, transB, beta, C, stream=None):
+    """C = beta C + alpha op(A) op(B) with ``nfx_sgemm`` on row-major views (A and B
+    are the stored matrices; op transposes them when the flag is set)."""
+    nat.call("nfx_sgemm", M, N, K, alpha, A.data_ptr(), A.stride(0), int(transA), B.data_ptr(), B.stride(0),
+             int(transB), beta, C.data_ptr(), C.stride(0), nat.stream_handle(stream))
+
+
+def spd_inverse_(A, plan=None, stream=None, o=0, gemm="nfx"):
     """In-place inverse of an SPD matrix (FP32, row-major view with unit column stride)
     by recursive 2 x 2 blocking. With B = A12[L, F] the only nonzero part of the
     coupling block (rows L = [lo, h), columns F = [h, hi), from ``coupling_plan``):
     X11 = A11^-1, T = X11[:, L] B, S = A22 - B^T T[L, :] (on F x F only), X22 = S^-1,
     X12 = -T X22[F, :], X11 <- X11 - X12[:, F] T^T, X21 = X12^T. Without a plan the
     envelope is the whole block. Diagonal blocks of size <= 128 are inverted by
-    ``nfx_spd_inverse_small`` (Gauss-Jordan in LDS); the products are library GEMMs
-    written in place into the strided blocks."""
+    ``nfx_spd_inverse_small`` (Gauss-Jordan in LDS); the products run on ``nfx_sgemm``
+    (gemm="nfx") or the library GEMM (gemm="torch"), in place in the strided blocks."""
     t = torch()
     n = A.shape[0]
     if n <= 128:
@@ -236,17 +241,26 @@ def spd_inverse_(A, plan=None, stream=None, o=0):
     h = _split(n)
     lo, hi = plan[(o, n)] if plan is not None else (0, n)
     A11, A12, A21, A22 = A[:h, :h], A[:h, h:], A[h:, :h], A[h:, h:]
-    spd_inverse_(A11, plan, stream, o)
+    spd_inverse_(A11, plan, stream, o, gemm)
     if hi <= h:
-        spd_inverse_(A22, plan, stream, o + h)
+        spd_inverse_(A22, plan, stream, o + h, gemm)
         return
     f = hi - h
+    l = h - lo
     B = A12[lo:, :f]
-    T = t.mm(A11[:, lo:], B)
-    A22[:f, :f].addmm_(B.T, T[lo:, :], alpha=-1.0)
-    spd_inverse_(A22, plan, stream, o + h)
-    A12.addmm_(T, A22[:f, :], beta=0.0, alpha=-1.0)
-    A11.addmm_(A12[:, :f], T.T, alpha=-1.0)
+    if gemm == "torch":
+        T = t.mm(A11[:, lo:], B)
+        A22[:f, :f].addmm_(B.T, T[lo:, :], alpha=-1.0)
+        spd_inverse_(A22, plan, stream, o + h, gemm)
+        A12.addmm_(T, A22[:f, :], beta=0.0, alpha=-1.0)
+        A11.addmm_(A12[:, :f], T.T, alpha=-1.0)
+    else:
+        T = t.empty((h, f), dtype=A.dtype, device=A.device)
+        _sgemm(h, f, l, 1.0, A11[:, lo:], False, B, False, 0.0, T, stream)
+        _sgemm(f, f, l, -1.0, B, True, T[lo:, :], False, 1.0, A22[:f, :f], stream)
+        spd_inverse_(A22, plan, stream, o + h, gemm)
+        _sgemm(h, n - h, f, -1.0, T, False, A22[:f, :], False, 0.0, A12, stream)
+        _sgemm(h, h, f, -1.0, A12[:, :f], False, T, True, 1.0, A11, stream)
     A21.copy_(A12.T)
 
 

@@ -189,6 +189,37 @@ def test_coarse_inverse_and_graph_replays():
         assert err < 1e-4
 
 
+@pytest.mark.parametrize("M,N,K,ta,tb", [(64, 64, 16, 0, 0), (200, 130, 37, 0, 0), (130, 200, 256, 1, 0),
+                                         (256, 96, 70, 0, 1), (77, 301, 5, 1, 1), (2048, 2048, 256, 0, 1)])
+def test_sgemm_against_library(M, N, K, ta, tb):
+    """nfx_sgemm (MFMA FP32, 64 x 64 tiles) on strided row-major views, both transposes,
+    sizes off the tile grid, beta 0 and 1."""
+    import torch
+    from nfempy_amd.learned import _sgemm
+
+    g = torch.Generator(device="cuda").manual_seed(M * 7 + N)
+    Abuf = torch.randn((K + 3, M + 5) if ta else (M + 3, K + 5), generator=g, device="cuda")
+    Bbuf = torch.randn((N + 2, K + 4) if tb else (K + 2, N + 4), generator=g, device="cuda")
+    A = Abuf[1:K + 1, 2:M + 2] if ta else Abuf[1:M + 1, 2:K + 2]
+    B = Bbuf[1:N + 1, 2:K + 2] if tb else Bbuf[1:K + 1, 2:N + 2]
+    opA = A.T if ta else A
+    opB = B.T if tb else B
+    ref = opA.double() @ opB.double()
+    Cbuf = torch.randn((M + 1, N + 6), generator=g, device="cuda")
+    C = Cbuf[1:, 3:N + 3]
+    C0 = C.clone()
+    _sgemm(M, N, K, -0.5, A, ta, B, tb, 1.0, C)
+    torch.cuda.synchronize()
+    exp = C0.double() - 0.5 * ref
+    tol = 1e-5 * (ref.abs().max().item() + 1) * max(K, 1) ** 0.5
+    assert (C.double() - exp).abs().max().item() <= tol
+    C.fill_(float("nan"))
+    _sgemm(M, N, K, 2.0, A, ta, B, tb, 0.0, C)
+    torch.cuda.synchronize()
+    assert (C.double() - 2.0 * ref).abs().max().item() <= 2 * tol
+    assert torch.isfinite(Cbuf[0]).all() and torch.isfinite(Cbuf[:, :3]).all()
+
+
 def test_fgmres_diagonal_and_none():
     """The C++ FGMRES with a diagonal preconditioner (no two-grid) on a small system."""
     import torch
