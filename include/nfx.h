@@ -333,11 +333,12 @@ typedef struct nfx_twogrid_pc {
 int nfx_twogrid_apply(const nfx_twogrid_pc* pc, const float* d_r, float* d_z, void* stream);
 /* C = beta C + alpha op(A) op(B) (FP32, row-major, leading dimensions lda/ldb/ldc;
  * op(A) = A (M x K) or A^T when transA (A stored K x M); op(B) = B (K x N) or B^T when
- * transB). The products of the coarse-operator inversion (thin K, 64 x 64 output
- * tiles on MFMA). beta = 0 does not read C. */
+ * transB); when d_C2 is not null the result is also stored transposed into it
+ * (N x M, leading dimension ldc2). The products of the coarse-operator inversion
+ * (K <= 256, MFMA, 64 x 64 or 32 x 32 output tiles). beta = 0 does not read C. */
 int nfx_sgemm(int M, int N, int K, float alpha, const float* d_A, int lda, int transA,
               const float* d_B, int ldb, int transB, float beta, float* d_C, int ldc,
-              void* stream);
+              float* d_C2, int ldc2, void* stream);
 /* Row-scaled FP16 copy of a dense n x n FP32 matrix (Ascale[r] = max |A[r][:]|). */
 int nfx_half_rows(int n, const float* d_A, uint16_t* d_A16, float* d_scale, void* stream);
 

@@ -90,7 +90,7 @@ SIGNATURES = {
     "nfx_twogrid_apply": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "nfx_half_rows": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "nfx_sgemm": (c_int, [c_int, c_int, c_int, c_float, c_void_p, c_int, c_int, c_void_p, c_int, c_int, c_float,
-                          c_void_p, c_int, c_void_p]),
+                          c_void_p, c_int, c_void_p, c_int, c_void_p]),
     "nfx_spd_inverse_small": (c_int, [c_int, c_void_p, c_int, c_void_p]),
     "nfx_fgmres_create": (c_int, [c_void_p, c_int, c_int, c_void_p]),
     "nfx_fgmres_destroy": (c_int, [c_void_p]),

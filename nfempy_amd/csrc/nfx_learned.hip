@@ -588,143 +588,231 @@ __global__ __launch_bounds__(256) void k_gemv(int n, const float* __restrict__ A
   if (lane == 0) y[row] = acc;
 }
 
-// In-place Gauss-Jordan inverse of an SPD block (n <= 128, no pivoting), row stride lda.
-// 512 threads: thread t holds quarter (t & 3) of row t >> 2 (32 columns) of the block
-// padded to 128 with the identity. The pivot loop is unrolled so that every register
-// index is a constant; per pivot the raw pivot row and column go through LDS
-// (double-buffered, one barrier per pivot; the row is read back as float4). The
-// kernel is bound by the per-pivot rank-1 update on one CU.
+// In-place inverse of an SPD block (n <= 128, no pivoting), row stride lda, by block
+// Gauss-Jordan over 16 x 16 pivot blocks. 512 threads = 8 waves; wave w holds block
+// row w (rows 16w .. 16w + 15 of the block padded to 128 with the identity) as eight
+// 16 x 16 accumulator tiles of v_mfma_f32_16x16x4_f32 (lane l: rows 4 (l >> 4) + i,
+// column l & 15). Per pivot block k the pivot block row and column go through LDS
+// (double-buffered, one barrier per step); every wave inverts the 16 x 16 pivot
+// block P itself (scalar Gauss-Jordan across the wave, lane l holding row l >> 2,
+// columns 4 (l & 3) .. + 3); the pivot wave forms A[k, j] <- P^-1 A[k, j] and
+// A[k, k] <- P^-1, every other wave X = A[I, k] P^-1, A[I, j] -= X A[k, j] and
+// A[I, k] <- -X. Steps past the last pivot block holding a row < n are skipped: those
+// blocks are identity with no coupling.
 constexpr int kInvN = 128;
+constexpr int kInvRS = 144;  // pivot-row LDS stride (16 mod 32 banks)
+constexpr int kInvCS = 18;   // pivot-column and per-wave scratch stride
 
-__global__ __launch_bounds__(512) void k_spd_inverse_small(int n, float* __restrict__ A, int lda) {
-  __shared__ __align__(16) float praw[2][kInvN];
-  __shared__ float fcol[2][kInvN];
-  const int i = threadIdx.x >> 2, q = threadIdx.x & 3;
-  float r[32];
-#pragma unroll
-  for (int j = 0; j < 32; ++j) {
-    const int c = 32 * q + j;
-    r[j] = (i < n && c < n) ? A[(int64_t)i * lda + c] : (i == c ? 1.f : 0.f);
-  }
-#pragma clang loop unroll(full)
-  for (int k = 0; k < kInvN; ++k) {
-    const int buf = k & 1;
-    if (i == k) {
-      float4* dst = reinterpret_cast<float4*>(&praw[buf][32 * q]);
-#pragma unroll
-      for (int j = 0; j < 8; ++j) dst[j] = make_float4(r[4 * j], r[4 * j + 1], r[4 * j + 2], r[4 * j + 3]);
-    }
-    if (q == (k >> 5)) fcol[buf][i] = r[k & 31];
-    __syncthreads();
-    const float ip = 1.f / praw[buf][k];
-    const float f = fcol[buf][i];
-    const float4* src = reinterpret_cast<const float4*>(&praw[buf][32 * q]);
-    float pr[32];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const float4 v = src[j];
-      pr[4 * j] = v.x;
-      pr[4 * j + 1] = v.y;
-      pr[4 * j + 2] = v.z;
-      pr[4 * j + 3] = v.w;
-    }
-#pragma unroll
-    for (int j = 0; j < 32; ++j) {
-      const bool ck = (q == (k >> 5)) && (j == (k & 31));
-      const float pj = (ck ? 1.f : pr[j]) * ip;
-      r[j] = (i == k) ? pj : (ck ? 0.f : r[j]) - f * pj;
-    }
-  }
-#pragma unroll
-  for (int j = 0; j < 32; ++j) {
-    const int c = 32 * q + j;
-    if (i < n && c < n) A[(int64_t)i * lda + c] = r[j];
-  }
+__device__ __forceinline__ float4_t mfma_f32_4(float a, float b, float4_t c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-// C = beta C + alpha op(A) op(B), FP32, row-major with leading dimensions
-// (op(A) is M x K: A[i*lda + k], or A[k*lda + i] when transA; op(B) is K x N:
-// B[k*ldb + j], or B[j*ldb + k] when transB). The products of the coarse-operator
-// inversion have K <= 256 and outputs up to 2048 x 2048: 64 x 64 output tiles keep
-// every CU busy where a library kernel with 256 x 256 tiles leaves most idle.
-// 4 waves per tile, each 32 x 32 = 2 x 2 MFMA tiles of v_mfma_f32_16x16x4_f32;
-// K in chunks of 16 staged through LDS. Beta = 0 does not read C.
-constexpr int kGT = 64;
-constexpr int kGK = 16;
-
-__global__ __launch_bounds__(256) void k_sgemm(int M, int N, int K, float alpha, const float* __restrict__ A,
-                                               int lda, int transA, const float* __restrict__ B, int ldb,
-                                               int transB, float beta, float* __restrict__ C, int ldc) {
-  __shared__ float sA[kGT][kGK + 1];
-  __shared__ float sB[kGK][kGT + 1];
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int i0 = blockIdx.y * kGT, j0 = blockIdx.x * kGT;
-  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-  float4_t acc[2][2];
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b) acc[a][b] = float4_t{0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < K; k0 += kGK) {
-    // stage A tile (64 x 16) and B tile (16 x 64): 4 elements each per thread
-    for (int e = tid; e < kGT * kGK; e += 256) {
-      int r, c;
-      if (transA) {  // coalesce along the row index of the stored K x M matrix
-        r = e % kGT;
-        c = e / kGT;
-      } else {
-        r = e / kGK;
-        c = e % kGK;
-      }
-      const int gi = i0 + r, gk = k0 + c;
-      float v = 0.f;
-      if (gi < M && gk < K) v = transA ? A[(int64_t)gk * lda + gi] : A[(int64_t)gi * lda + gk];
-      sA[r][c] = v;
+__global__ __launch_bounds__(512) void k_spd_inverse_small(int n, float* __restrict__ A, int lda) {
+  __shared__ float rowp[2][16][kInvRS];
+  __shared__ float colp[2][kInvN][kInvCS];
+  __shared__ float pinv[8][16][kInvCS];
+  __shared__ float xs[8][16][kInvCS];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, lr = l & 15, lq = l >> 4;
+  float4_t acc[8];
+#pragma unroll
+  for (int J = 0; J < 8; ++J)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int r = 16 * w + 4 * lq + i, c = 16 * J + lr;
+      acc[J][i] = (r < n && c < n) ? A[(int64_t)r * lda + c] : (r == c ? 1.f : 0.f);
     }
-    for (int e = tid; e < kGT * kGK; e += 256) {
-      int r, c;
-      if (transB) {
-        r = e % kGK;
-        c = e / kGK;
-      } else {
-        r = e / kGT;
-        c = e % kGT;
-      }
-      const int gk = k0 + r, gj = j0 + c;
-      float v = 0.f;
-      if (gk < K && gj < N) v = transB ? B[(int64_t)gj * ldb + gk] : B[(int64_t)gk * ldb + gj];
-      sB[r][c] = v;
+  const int nb = (n + 15) >> 4;
+  float(*pv)[kInvCS] = pinv[w];
+  float(*xv)[kInvCS] = xs[w];
+  const int pr = l >> 2, pc = 4 * (l & 3);
+#pragma unroll
+  for (int kb = 0; kb < 8; ++kb) {
+    if (kb >= nb) break;
+    const int buf = kb & 1;
+    if (w == kb) {
+#pragma unroll
+      for (int J = 0; J < 8; ++J)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) rowp[buf][4 * lq + i][16 * J + lr] = acc[J][i];
     }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) colp[buf][16 * w + 4 * lq + i][lr] = acc[kb][i];
     __syncthreads();
+    float p[4];
 #pragma unroll
-    for (int kk = 0; kk < kGK; kk += 4) {
-      const int kl = kk + (lane >> 4);
-      float av[2], bv[2];
+    for (int t = 0; t < 4; ++t) p[t] = rowp[buf][pr][16 * kb + pc + t];
 #pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        av[t] = sA[wr + 16 * t + (lane & 15)][kl];
-        bv[t] = sB[kl][wc + 16 * t + (lane & 15)];
+    for (int k = 0; k < 16; ++k) {
+      float prow[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) prow[t] = __shfl(p[t], 4 * k + (l & 3), 64);
+      const float f = __shfl(p[k & 3], 4 * pr + (k >> 2), 64);
+      const float ip = 1.f / __shfl(p[k & 3], 4 * k + (k >> 2), 64);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const bool ck = (pc + t) == k;
+        const float pj = (ck ? 1.f : prow[t]) * ip;
+        p[t] = (pr == k) ? pj : (ck ? 0.f : p[t]) - f * pj;
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) pv[pr][pc + t] = p[t];
+    __builtin_amdgcn_wave_barrier();
+    if (w == kb) {
+      float4_t nacc[8];
+#pragma unroll
+      for (int J = 0; J < 8; ++J) nacc[J] = float4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const float a = pv[lr][4 * s + lq];
+#pragma unroll
+        for (int J = 0; J < 8; ++J)
+          if (J != kb) nacc[J] = mfma_f32_4(a, rowp[buf][4 * s + lq][16 * J + lr], nacc[J]);
       }
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
+      for (int J = 0; J < 8; ++J)
+        if (J != kb) acc[J] = nacc[J];
 #pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[a], bv[b], acc[a][b], 0, 0, 0);
+      for (int i = 0; i < 4; ++i) acc[kb][i] = pv[4 * lq + i][lr];
+    } else {
+      float4_t x = float4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int s = 0; s < 4; ++s) x = mfma_f32_4(colp[buf][16 * w + lr][4 * s + lq], pv[4 * s + lq][lr], x);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) xv[4 * lq + i][lr] = x[i];
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const float a = -xv[lr][4 * s + lq];
+#pragma unroll
+        for (int J = 0; J < 8; ++J)
+          if (J != kb) acc[J] = mfma_f32_4(a, rowp[buf][4 * s + lq][16 * J + lr], acc[J]);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) acc[kb][i] = -x[i];
+    }
+  }
+#pragma unroll
+  for (int J = 0; J < 8; ++J)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int r = 16 * w + 4 * lq + i, c = 16 * J + lr;
+      if (r < n && c < n) A[(int64_t)r * lda + c] = acc[J][i];
+    }
+}
+
+// C = beta C + alpha op(A) op(B), and C2 = C^T when C2 is not null (FP32, row-major
+// with leading dimensions; op(A) is M x K: A[i*lda + k], or A[k*lda + i] when transA;
+// op(B) is K x N: B[k*ldb + j], or B[j*ldb + k] when transB). TILE x TILE output
+// tiles (64, or 32 when 64 x 64 tiles would leave CUs idle), 4 waves each holding a
+// (TILE/2)^2 block of 16 x 16 v_mfma_f32_16x16x4_f32 tiles. K is taken in panels of
+// NP chunks of 32: every load of a panel is issued before its first product, so one
+// memory latency covers the panel (the products of the coarse inversion have K <= 256:
+// one or two panels); the panel moves through LDS chunk by chunk (double-buffered, one
+// barrier per chunk). 64-wide tiles use 4-chunk panels so that two workgroups fit on a
+// CU and overlap one's loads with the other's products. Beta = 0 does not read C.
+constexpr int kGK = 32;
+
+template <int TILE, int kGP, bool transA, bool transB>
+__global__ __launch_bounds__(256) void k_sgemm(int M, int N, int K, float alpha, const float* __restrict__ A,
+                                               int lda, const float* __restrict__ B, int ldb, float beta,
+                                               float* __restrict__ C, int ldc, float* __restrict__ C2, int ldc2) {
+  constexpr int EPT = TILE * kGK / 256;  // elements of each operand per thread and chunk
+  constexpr int WT = TILE / 32;          // 16 x 16 tiles per wave side
+  __shared__ float sA[2][TILE][kGK + 2];
+  __shared__ float sB[2][kGK][TILE + 16];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, lr = lane & 15, lq = lane >> 4;
+  const int i0 = blockIdx.y * TILE, j0 = blockIdx.x * TILE;
+  const int wr = (wave >> 1) * (TILE / 2), wc = (wave & 1) * (TILE / 2);
+  float4_t acc[WT][WT];
+#pragma unroll
+  for (int a = 0; a < WT; ++a)
+#pragma unroll
+    for (int b = 0; b < WT; ++b) acc[a][b] = float4_t{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < K; k0 += kGK * kGP) {
+    const int nch = min(kGP, (K - k0 + kGK - 1) / kGK);
+    float ra[kGP][EPT], rb[kGP][EPT];
+#pragma unroll
+    for (int c = 0; c < kGP; ++c) {
+      if (c < nch) {
+#pragma unroll
+        for (int u = 0; u < EPT; ++u) {
+          const int e = tid + 256 * u;
+          const int ar = transA ? e % TILE : e / kGK, ak = transA ? e / TILE : e % kGK;
+          const int gi = i0 + ar, gka = k0 + c * kGK + ak;
+          ra[c][u] = (gi < M && gka < K) ? (transA ? A[(int64_t)gka * lda + gi] : A[(int64_t)gi * lda + gka]) : 0.f;
+          const int bk = transB ? e % kGK : e / TILE, bj = transB ? e / kGK : e % TILE;
+          const int gj = j0 + bj, gkb = k0 + c * kGK + bk;
+          rb[c][u] = (gj < N && gkb < K) ? (transB ? B[(int64_t)gj * ldb + gkb] : B[(int64_t)gkb * ldb + gj]) : 0.f;
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < kGP; ++c) {
+      if (c < nch) {
+        const int buf = c & 1;
+#pragma unroll
+        for (int u = 0; u < EPT; ++u) {
+          const int e = tid + 256 * u;
+          const int ar = transA ? e % TILE : e / kGK, ak = transA ? e / TILE : e % kGK;
+          sA[buf][ar][ak] = ra[c][u];
+          const int bk = transB ? e % kGK : e / TILE, bj = transB ? e / kGK : e % TILE;
+          sB[buf][bk][bj] = rb[c][u];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < kGK; ks += 4) {
+          const int kl = ks + lq;
+          float av[WT], bv[WT];
+#pragma unroll
+          for (int t = 0; t < WT; ++t) {
+            av[t] = sA[buf][wr + 16 * t + lr][kl];
+            bv[t] = sB[buf][kl][wc + 16 * t + lr];
+          }
+#pragma unroll
+          for (int a = 0; a < WT; ++a)
+#pragma unroll
+            for (int b = 0; b < WT; ++b) acc[a][b] = mfma_f32_4(av[a], bv[b], acc[a][b]);
+        }
+      }
     }
     __syncthreads();
   }
   // lane holds D[4 (lane >> 4) + i][lane & 15] of each 16 x 16 tile
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < WT; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int b = 0; b < WT; ++b) {
+      const int gj = j0 + wc + 16 * b + lr;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int gi = i0 + wr + 16 * a + 4 * (lane >> 4) + i;
-        const int gj = j0 + wc + 16 * b + (lane & 15);
+        const int gi = i0 + wr + 16 * a + 4 * lq + i;
         if (gi < M && gj < N) {
           float* cp = C + (int64_t)gi * ldc + gj;
-          const float v = alpha * acc[a][b][i];
-          *cp = beta == 0.f ? v : beta * *cp + v;
+          const float v = alpha * acc[a][b][i] + (beta == 0.f ? 0.f : beta * *cp);
+          *cp = v;
+          if (C2) C2[(int64_t)gj * ldc2 + gi] = v;
         }
       }
+    }
+}
+
+template <int TILE, int NP>
+void launch_sgemm(int ta, int tb, int M, int N, int K, float alpha, const float* A, int lda, const float* B, int ldb,
+                  float beta, float* C, int ldc, float* C2, int ldc2, hipStream_t st) {
+  const dim3 grid(grid_for(N, TILE), grid_for(M, TILE)), block(256);
+  if (ta && tb)
+    hipLaunchKernelGGL((k_sgemm<TILE, NP, true, true>), grid, block, 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C,
+                       ldc, C2, ldc2);
+  else if (ta)
+    hipLaunchKernelGGL((k_sgemm<TILE, NP, true, false>), grid, block, 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C,
+                       ldc, C2, ldc2);
+  else if (tb)
+    hipLaunchKernelGGL((k_sgemm<TILE, NP, false, true>), grid, block, 0, st, M, N, K, alpha, A, lda, B, ldb, beta, C,
+                       ldc, C2, ldc2);
+  else
+    hipLaunchKernelGGL((k_sgemm<TILE, NP, false, false>), grid, block, 0, st, M, N, K, alpha, A, lda, B, ldb, beta,
+                       C, ldc, C2, ldc2);
 }
 
 // x2 = x1 + y[agg]; t2 = r - S x2 (x2 of neighbours recomputed from x1 and y)
@@ -967,13 +1055,20 @@ int nfx_spd_inverse_small(int n, float* A, int lda, void* stream) {
 }
 
 int nfx_sgemm(int M, int N, int K, float alpha, const float* A, int lda, int transA, const float* B, int ldb,
-              int transB, float beta, float* C, int ldc, void* stream) {
+              int transB, float beta, float* C, int ldc, float* C2, int ldc2, void* stream) {
   NFX_REQUIRE(M >= 0 && N >= 0 && K >= 0, "nfx_sgemm: negative size");
   if (M == 0 || N == 0) return NFX_OK;
   NFX_REQUIRE(A && B && C, "nfx_sgemm: null pointer");
   NFX_REQUIRE(lda >= (transA ? M : K) && ldb >= (transB ? K : N) && ldc >= N, "nfx_sgemm: leading dimension");
-  hipLaunchKernelGGL(k_sgemm, dim3(grid_for(N, kGT), grid_for(M, kGT)), dim3(256), 0, (hipStream_t)stream, M, N,
-                     K, alpha, A, lda, transA, B, ldb, transB, beta, C, ldc);
+  NFX_REQUIRE(!C2 || ldc2 >= M, "nfx_sgemm: leading dimension of the transposed output");
+  // 64 x 64 tiles unless fewer than two per CU (256 CUs) would result
+  const bool big = (int64_t)grid_for(M, 64) * grid_for(N, 64) >= 512;
+  const int ta = transA ? 1 : 0, tb = transB ? 1 : 0;
+  if (big) {
+    launch_sgemm<64, 4>(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, C2, ldc2, (hipStream_t)stream);
+  } else {
+    launch_sgemm<32, 8>(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, C2, ldc2, (hipStream_t)stream);
+  }
   NFX_LAUNCH_CHECK();
   return NFX_OK;
 }

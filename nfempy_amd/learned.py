@@ -217,11 +217,13 @@ def coupling_plan(n, rows, cols):
     return plan
 
 
-def _sgemm(M, N, K, alpha, A, transA, B, transB, beta, C, stream=None):
+def _sgemm(M, N, K, alpha, A, transA, B, transB, beta, C, stream=None, C2=None):
     """C = beta C + alpha op(A) op(B) with ``nfx_sgemm`` on row-major views (A and B
-    are the stored matrices; op transposes them when the flag is set)."""
+    are the stored matrices; op transposes them when the flag is set); C2, when given,
+    receives the result transposed."""
     nat.call("nfx_sgemm", M, N, K, alpha, A.data_ptr(), A.stride(0), int(transA), B.data_ptr(), B.stride(0),
-             int(transB), beta, C.data_ptr(), C.stride(0), nat.stream_handle(stream))
+             int(transB), beta, C.data_ptr(), C.stride(0), None if C2 is None else C2.data_ptr(),
+             0 if C2 is None else C2.stride(0), nat.stream_handle(stream))
 
 
 def spd_inverse_(A, plan=None, stream=None, o=0, gemm="nfx"):
@@ -231,8 +233,9 @@ def spd_inverse_(A, plan=None, stream=None, o=0, gemm="nfx"):
     X11 = A11^-1, T = X11[:, L] B, S = A22 - B^T T[L, :] (on F x F only), X22 = S^-1,
     X12 = -T X22[F, :], X11 <- X11 - X12[:, F] T^T, X21 = X12^T. Without a plan the
     envelope is the whole block. Diagonal blocks of size <= 128 are inverted by
-    ``nfx_spd_inverse_small`` (Gauss-Jordan in LDS); the products run on ``nfx_sgemm``
-    (gemm="nfx") or the library GEMM (gemm="torch"), in place in the strided blocks."""
+    ``nfx_spd_inverse_small`` (block Gauss-Jordan on one CU); the products run on
+    ``nfx_sgemm`` (gemm="nfx", which also writes T^T and X21 = X12^T as transposed
+    outputs) or the library GEMM (gemm="torch"), in place in the strided blocks."""
     t = torch()
     n = A.shape[0]
     if n <= 128:
@@ -254,14 +257,15 @@ def spd_inverse_(A, plan=None, stream=None, o=0, gemm="nfx"):
         spd_inverse_(A22, plan, stream, o + h, gemm)
         A12.addmm_(T, A22[:f, :], beta=0.0, alpha=-1.0)
         A11.addmm_(A12[:, :f], T.T, alpha=-1.0)
+        A21.copy_(A12.T)
     else:
         T = t.empty((h, f), dtype=A.dtype, device=A.device)
-        _sgemm(h, f, l, 1.0, A11[:, lo:], False, B, False, 0.0, T, stream)
+        Tt = t.empty((f, h), dtype=A.dtype, device=A.device)
+        _sgemm(h, f, l, 1.0, A11[:, lo:], False, B, False, 0.0, T, stream, C2=Tt)
         _sgemm(f, f, l, -1.0, B, True, T[lo:, :], False, 1.0, A22[:f, :f], stream)
         spd_inverse_(A22, plan, stream, o + h, gemm)
-        _sgemm(h, n - h, f, -1.0, T, False, A22[:f, :], False, 0.0, A12, stream)
-        _sgemm(h, h, f, -1.0, A12[:, :f], False, T, True, 1.0, A11, stream)
-    A21.copy_(A12.T)
+        _sgemm(h, n - h, f, -1.0, T, False, A22[:f, :], False, 0.0, A12, stream, C2=A21)
+        _sgemm(h, h, f, -1.0, A12[:, :f], False, Tt, False, 1.0, A11, stream)
 
 
 class LearnedSchurPC:

@@ -189,11 +189,37 @@ def test_coarse_inverse_and_graph_replays():
         assert err < 1e-4
 
 
+@pytest.mark.parametrize("n", [1, 5, 16, 17, 100, 127, 128])
+def test_spd_inverse_small_block_gauss_jordan(n):
+    """nfx_spd_inverse_small (block Gauss-Jordan over 16 x 16 pivots, MFMA updates) on a
+    strided SPD block of every size class: under one pivot block, exact multiples,
+    ragged tails; entries outside the n x n block are left untouched."""
+    import torch
+    from nfempy_amd import _native as nat
+
+    g = torch.Generator(device="cuda").manual_seed(n)
+    X = torch.randn((n, n), generator=g, device="cuda", dtype=torch.float64)
+    A64 = X @ X.T / n + torch.diag(torch.rand(n, generator=g, device="cuda", dtype=torch.float64) + 0.5)
+    buf = torch.full((n + 2, n + 9), 3.0, device="cuda")
+    A = buf[1:n + 1, 4:n + 4]
+    A.copy_(A64.float())
+    nat.call("nfx_spd_inverse_small", n, A.data_ptr(), A.stride(0), nat.stream_handle())
+    torch.cuda.synchronize()
+    ref = torch.linalg.inv(A64)
+    err = float((A.double() - ref).abs().max() / ref.abs().max())
+    cond = float(torch.linalg.cond(A64))
+    print(f"[learned] inverse n={n}: rel err {err:.2e} (cond {cond:.1e})", flush=True)
+    assert err < 2e-6 * max(cond, 10.0)
+    assert (buf[0] == 3.0).all() and (buf[:, :4] == 3.0).all() and (buf[:, n + 4:] == 3.0).all()
+
+
 @pytest.mark.parametrize("M,N,K,ta,tb", [(64, 64, 16, 0, 0), (200, 130, 37, 0, 0), (130, 200, 256, 1, 0),
-                                         (256, 96, 70, 0, 1), (77, 301, 5, 1, 1), (2048, 2048, 256, 0, 1)])
+                                         (256, 96, 70, 0, 1), (77, 301, 5, 1, 1), (2048, 2048, 256, 0, 1),
+                                         (1500, 1400, 300, 1, 0), (100, 90, 600, 0, 0)])
 def test_sgemm_against_library(M, N, K, ta, tb):
-    """nfx_sgemm (MFMA FP32, 64 x 64 tiles) on strided row-major views, both transposes,
-    sizes off the tile grid, beta 0 and 1."""
+    """nfx_sgemm (MFMA FP32, 64 x 64 and 32 x 32 tiles, K in panels of 256) on strided
+    row-major views, both transposes, sizes off the tile grid, K past one panel, beta 0
+    and 1, and the transposed second output."""
     import torch
     from nfempy_amd.learned import _sgemm
 
@@ -214,10 +240,14 @@ def test_sgemm_against_library(M, N, K, ta, tb):
     tol = 1e-5 * (ref.abs().max().item() + 1) * max(K, 1) ** 0.5
     assert (C.double() - exp).abs().max().item() <= tol
     C.fill_(float("nan"))
-    _sgemm(M, N, K, 2.0, A, ta, B, tb, 0.0, C)
+    C2buf = torch.full((N + 2, M + 3), 7.0, device="cuda")
+    C2 = C2buf[1:N + 1, 1:M + 1]
+    _sgemm(M, N, K, 2.0, A, ta, B, tb, 0.0, C, C2=C2)
     torch.cuda.synchronize()
     assert (C.double() - 2.0 * ref).abs().max().item() <= 2 * tol
     assert torch.isfinite(Cbuf[0]).all() and torch.isfinite(Cbuf[:, :3]).all()
+    assert torch.equal(C2, C.T)
+    assert (C2buf[0] == 7.0).all() and (C2buf[:, 0] == 7.0).all() and (C2buf[:, M + 1:] == 7.0).all()
 
 
 def test_fgmres_diagonal_and_none():
