@@ -85,7 +85,7 @@ int nfx_assemble_darcy(int dim, int k, int n_elem, const int32_t* d_cells,
  * incidences, element order) and the block stores its contiguous slice of CSR
  * values; the residual of every row is written (pressure rows: -sum w detJ f psi,
  * flux rows: 0, before the Dirichlet facet term). No slot map, no staging.
- *  d_inc    (2, n_rows) int32 incidences e*32 + a (reference DoF a), -1 = none
+ *  d_inc    (2, n_rows) int32 incidences e*32 + a (reference DoF a), negative = none
  *  d_pos    (n_rows, 2, nl) uint8 position of reference DoF b's column in the row
  *           (255 = not present)
  *  d_rowptr CSR row pointer; d_signs (E) uint32, bit a = reference DoF a negated
@@ -131,6 +131,25 @@ int nfx_assemble_darcy_bc(int dim, int k, int n_facets, const int32_t* d_facet,
                           const void* d_trace, const void* d_fw, int nqf,
                           const void* d_pD, const int32_t* d_rows, int fp64,
                           void* d_rhs, void* stream);
+
+/* The whole warm assembly in one launch: nfx_assemble_darcy_rows followed by
+ * nfx_assemble_darcy_bc (same reference interfaces: the element loop of
+ * LaplaceDualWeakForm plus the boundary loop of LaplaceDualWeakFormBCDirichlet in
+ * the Darcy driver), with the facet term evaluated by the thread of the row it
+ * belongs to. A row owned by Dirichlet facet f (facet DoF a) carries second
+ * incidence -2 - (f * fd + a) in d_inc (such a row has one element incidence, and
+ * it is the facet's neighbour); d_rows (F, fd) are its row codes. Results equal
+ * the two-launch sequence. n_facets = 0: same as nfx_assemble_darcy_rows. */
+int nfx_assemble_darcy_rows_bc(int dim, int k, int n_rows, const int32_t* d_inc,
+                               const uint8_t* d_pos, const int32_t* d_rowptr,
+                               const uint32_t* d_signs, const int32_t* d_cells,
+                               const void* d_coords, const void* d_tables, const void* d_kinv,
+                               int kinv_stride_e, int kinv_stride_q, const void* d_f,
+                               int f_stride_e, int f_stride_q, int n_facets,
+                               const int32_t* d_facet, const int32_t* d_combo,
+                               const void* d_trace, const void* d_fw, int nqf, const void* d_pD,
+                               const int32_t* d_rows, int fp64, void* d_vals, void* d_rhs,
+                               void* stream);
 
 /* ---- assembly: dual-mixed elasticity (BDM1 - P0 - P0, tetrahedra) --------
  * mode 0 replaces LEDualWeakForm.evaluate_form_vectorized

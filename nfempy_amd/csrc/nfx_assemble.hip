@@ -194,18 +194,13 @@ __global__ __launch_bounds__(256) void k_darcy(
 // trace: (ncombo, NQF, FD, DIM) reference flux basis at the facet points mapped
 // into the neighbour's reference cell, per (local facet, facet vertex order);
 // g: (F, NQF, NC); rows: (F, FD * NC) row codes, entry a * NC + c.
+// val[a][c] of facet f from the neighbour's vertices X and geometry g.
 template <int DIM, int FD, int NQF, int NC, typename R>
-__global__ __launch_bounds__(256) void k_facet_bc(
-    int n_facets, const int32_t* __restrict__ facet_nodes, const int32_t* __restrict__ neigh,
-    const int32_t* __restrict__ combo, const int32_t* __restrict__ cells,
-    const R* __restrict__ coords, const R* __restrict__ trace, const R* __restrict__ fw,
-    const R* __restrict__ g_data, const int32_t* __restrict__ rows, R scale, R* __restrict__ rhs) {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= n_facets) return;
-  const int e = neigh[f];
-  Geom<DIM, R> g;
-  R X[DIM + 1][3];
-  load_geometry<DIM, R>(cells, coords, e, g, X);
+__device__ __forceinline__ void facet_term(int f, const R (&X)[DIM + 1][3], const Geom<DIM, R>& g,
+                                           const int32_t* __restrict__ facet_nodes,
+                                           const int32_t* __restrict__ combo, const R* __restrict__ coords,
+                                           const R* __restrict__ trace, const R* __restrict__ fw,
+                                           const R* __restrict__ g_data, R (&val)[FD][NC]) {
   // facet geometry in stored node order
   R Y[DIM][3];
 #pragma unroll
@@ -268,7 +263,6 @@ __global__ __launch_bounds__(256) void k_facet_bc(
     jtn[d] = s / g.detJ;
   }
   const R* __restrict__ tr = trace + (int64_t)combo[f] * NQF * FD * DIM;
-  R val[FD][NC];
 #pragma unroll
   for (int a = 0; a < FD; ++a)
 #pragma unroll
@@ -285,6 +279,22 @@ __global__ __launch_bounds__(256) void k_facet_bc(
       for (int c = 0; c < NC; ++c) val[a][c] += wq * g_data[((int64_t)f * NQF + q) * NC + c] * s;
     }
   }
+}
+
+template <int DIM, int FD, int NQF, int NC, typename R>
+__global__ __launch_bounds__(256) void k_facet_bc(
+    int n_facets, const int32_t* __restrict__ facet_nodes, const int32_t* __restrict__ neigh,
+    const int32_t* __restrict__ combo, const int32_t* __restrict__ cells,
+    const R* __restrict__ coords, const R* __restrict__ trace, const R* __restrict__ fw,
+    const R* __restrict__ g_data, const int32_t* __restrict__ rows, R scale, R* __restrict__ rhs) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n_facets) return;
+  const int e = neigh[f];
+  Geom<DIM, R> g;
+  R X[DIM + 1][3];
+  load_geometry<DIM, R>(cells, coords, e, g, X);
+  R val[FD][NC];
+  facet_term<DIM, FD, NQF, NC, R>(f, X, g, facet_nodes, combo, coords, trace, fw, g_data, val);
 #pragma unroll
   for (int a = 0; a < FD; ++a)
 #pragma unroll
@@ -309,35 +319,69 @@ __global__ __launch_bounds__(256) void k_facet_bc(
 // row a = NB + k: B[k][b] (b < NB), 0 (b >= NB), residual -sum_q w detJ f psi_k.
 constexpr int kRowsPerBlock = 128;
 
-// One tile of TILE consecutive rows [r0, r0 + TILE) of one system. st / sv are
-// LDS buffers of at least NT and TILE * 2 * NL values. Called by all threads of
-// the block; contains block barriers.
 // Table layout: w[NQ] | phi[NQ][NB][DIM] | div[NQ][NB] | psi[NQ][NP] | Rs[NB][NB][NS] |
 // Bc[NB][NP] | ck[NP]  (NS = DIM (DIM + 1) / 2 symmetric pairs of G = J^T J).
-constexpr int kXcds = 8;  // MI355X: 8 XCDs, each with its own L2
-
 template <int DIM, int NB, int NP, int NQ>
 struct DarcyTab {
   static constexpr int NS = DIM * (DIM + 1) / 2;
   static constexpr int NT = NQ * (1 + NB * DIM + NB + NP) + NB * NB * NS + NB * NP + NP;
 };
 
+// Facet DoFs and facet quadrature points of the pressure Dirichlet term, by element.
+template <int DIM, int NB>
+struct DarcyFacetShape;
+template <>
+struct DarcyFacetShape<2, 3> {
+  static constexpr int FD = 1, NQF = 2;
+};
+template <>
+struct DarcyFacetShape<2, 8> {
+  static constexpr int FD = 2, NQF = 3;
+};
+template <>
+struct DarcyFacetShape<3, 4> {
+  static constexpr int FD = 1, NQF = 4;
+};
+template <>
+struct DarcyFacetShape<3, 15> {
+  static constexpr int FD = 3, NQF = 9;
+};
+
+// Pressure Dirichlet term folded into the row kernel. A row owned by a Dirichlet
+// facet carries second incidence -2 - i, i = f * FD + a indexing the facet row codes
+// (facet f, facet DoF a); its thread evaluates the facet term (facet_term, the same
+// arithmetic as k_facet_bc) and stores it as the row's residual, which the volume
+// term leaves at zero for flux rows. pD == nullptr: no Dirichlet data, every row
+// stores its volume residual.
+template <typename R>
+struct DarcyRowBc {
+  int n_codes;  // F * FD
+  const int32_t* facet_nodes;
+  const int32_t* combo;
+  const R* trace;
+  const R* fw;
+  const R* pD;
+  const int32_t* rows;
+};
+
 // One tile of TILE consecutive rows [r0, r0 + TILE) of one system. st / sv are
 // LDS buffers of at least DarcyTab::NT and TILE * 2 * NL values. Called by all
-// threads of the block; contains block barriers.
+// threads of the block; contains block barriers. (Reading Rs / Bc / ck from global
+// memory instead of LDS, with one barrier per tile, measured 6 % slower.)
 // With per-element coefficients (ks_q == 0 and fs_q == 0) the quadrature sums are
 // the precomputed reference tensors: flux row a of element e is
 //   M_ab = kinv_e / detJ sum_s G_s Rs[a][b][s],  B part -Bc[a][k];
 // pressure row k: Bc[b][k] and residual -detJ f_e ck[k]. Otherwise the sums run
 // over the quadrature points. The loads of both incidences are issued before
-// either row is computed.
+// either row is computed. Rows owned by a Dirichlet facet take the facet term
+// (DarcyRowBc) after the volume rows are written to LDS.
 template <int DIM, int NB, int NP, int NQ, int TILE, typename R>
 __device__ __forceinline__ void darcy_row_tile(
     int r0, int n_rows, const int32_t* __restrict__ inc, const uint8_t* __restrict__ pos,
     const int32_t* __restrict__ rowptr, const uint32_t* __restrict__ signs,
     const int32_t* __restrict__ cells, const R* __restrict__ coords, const R* __restrict__ tab,
     const R* __restrict__ kinv, int ks_e, int ks_q, const R* __restrict__ f, int fs_e, int fs_q,
-    R* __restrict__ vals, R* __restrict__ rhs, R* st, R* sv) {
+    R* __restrict__ vals, R* __restrict__ rhs, R* st, R* sv, const DarcyRowBc<R>& bc) {
   constexpr int NL = NB + NP;
   constexpr int NS = DarcyTab<DIM, NB, NP, NQ>::NS;
   constexpr int NT = DarcyTab<DIM, NB, NP, NQ>::NT;
@@ -463,6 +507,20 @@ __device__ __forceinline__ void darcy_row_tile(
         sv[off + p] += ((sg[j] >> b) & 1u) ? -row[b] : row[b];
       }
     }
+    if (bc.pD != nullptr && code[1] <= -2 && -code[1] - 2 < bc.n_codes) {
+      constexpr int FD = DarcyFacetShape<DIM, NB>::FD, NQF = DarcyFacetShape<DIM, NB>::NQF;
+      const int i = -code[1] - 2, fct = i / FD, a = i - fct * FD;
+      Geom<DIM, R> g;
+      R X0[DIM + 1][3];
+      load_geometry<DIM, R>(cells, coords, e[0], g, X0);
+      R val[FD][1];
+      facet_term<DIM, FD, NQF, 1, R>(fct, X0, g, bc.facet_nodes, bc.combo, coords, bc.trace, bc.fw, bc.pD,
+                                     val);
+      R v = val[0][0];
+#pragma unroll
+      for (int t = 1; t < FD; ++t) v = a == t ? val[t][0] : v;
+      res = (bc.rows[i] & kSlotNeg) ? -v : v;
+    }
     rhs[r] = res;
   }
   __syncthreads();
@@ -476,19 +534,18 @@ __global__ __launch_bounds__(kRowsPerBlock) void k_darcy_rows(
     const int32_t* __restrict__ rowptr, const uint32_t* __restrict__ signs,
     const int32_t* __restrict__ cells, const R* __restrict__ coords, const R* __restrict__ tab,
     const R* __restrict__ kinv, int ks_e, int ks_q, const R* __restrict__ f, int fs_e, int fs_q,
-    R* __restrict__ vals, R* __restrict__ rhs) {
+    R* __restrict__ vals, R* __restrict__ rhs, DarcyRowBc<R> bc) {
   constexpr int NL = NB + NP;
   constexpr int NT = DarcyTab<DIM, NB, NP, NQ>::NT;
   __shared__ R st[NT];
   __shared__ R sv[kRowsPerBlock * 2 * NL];
-  // XCD-aware tile order: workgroups are dealt to the 8 XCDs round-robin, so block b
-  // takes tile (b % 8) * (nb / 8) + b / 8: each XCD walks a contiguous range of row
-  // tiles, and the elements shared by neighbouring tiles stay in that XCD's L2.
-  const int nb = gridDim.x, nb8 = nb / kXcds, b = blockIdx.x;
-  const int tile = b < nb8 * kXcds ? (b % kXcds) * nb8 + b / kXcds : b;
-  darcy_row_tile<DIM, NB, NP, NQ, kRowsPerBlock, R>(tile * kRowsPerBlock, n_rows, inc, pos,
+  // Tiles in launch order. An XCD-aware order (block b taking tile (b % 8) * (nb / 8)
+  // + b / 8, so that each of the 8 XCDs walks a contiguous range and keeps the shared
+  // elements in its own L2) measured 7 % slower at 32^3 x 6 (16.9-17.6 us against
+  // 15.6-16.1 us, alternating runs of both builds).
+  darcy_row_tile<DIM, NB, NP, NQ, kRowsPerBlock, R>(blockIdx.x * kRowsPerBlock, n_rows, inc, pos,
                                                     rowptr, signs, cells, coords, tab, kinv, ks_e,
-                                                    ks_q, f, fs_e, fs_q, vals, rhs, st, sv);
+                                                    ks_q, f, fs_e, fs_q, vals, rhs, st, sv, bc);
 }
 
 // ---------------------------------------------------------------------------
@@ -518,18 +575,19 @@ __global__ __launch_bounds__(TILE) void k_darcy_rows_batched(const DarcyBatchIte
   constexpr int NT2 = DarcyTab<3, 15, 4, 27>::NT;
   __shared__ float st[NT2];
   __shared__ float sv[TILE * 2 * 19];
+  const DarcyRowBc<float> no_bc{0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {
     const int2 tl = tiles[t];
     const DarcyBatchItem& it = items[tl.x];
     if (it.k == 1)
       darcy_row_tile<3, 4, 1, 8, TILE, float>(tl.y, it.n_rows, it.inc, it.pos, it.rowptr, it.signs,
                                               it.cells, it.coords, it.tab, it.kinv, it.ks_e, it.ks_q,
-                                              it.f, it.fs_e, it.fs_q, it.vals, it.rhs, st, sv);
+                                              it.f, it.fs_e, it.fs_q, it.vals, it.rhs, st, sv, no_bc);
     else
       darcy_row_tile<3, 15, 4, 27, TILE, float>(tl.y, it.n_rows, it.inc, it.pos, it.rowptr,
                                                 it.signs, it.cells, it.coords, it.tab, it.kinv,
                                                 it.ks_e, it.ks_q, it.f, it.fs_e, it.fs_q, it.vals,
-                                                it.rhs, st, sv);
+                                                it.rhs, st, sv, no_bc);
   }
 }
 
@@ -538,12 +596,17 @@ static int launch_darcy_rows(int n_rows, const int32_t* inc, const uint8_t* pos,
                              const int32_t* rowptr, const uint32_t* signs, const int32_t* cells,
                              const void* coords, const void* tab, const void* kinv, int kse,
                              int ksq, const void* f, int fse, int fsq, void* vals, void* rhs,
-                             hipStream_t s) {
+                             hipStream_t s, int n_bc = 0, const int32_t* facet = nullptr,
+                             const int32_t* combo = nullptr, const void* trace = nullptr,
+                             const void* fw = nullptr, const void* pD = nullptr,
+                             const int32_t* bc_rows = nullptr) {
+  const DarcyRowBc<R> bc{n_bc, facet, combo, static_cast<const R*>(trace), static_cast<const R*>(fw),
+                         static_cast<const R*>(pD), bc_rows};
   hipLaunchKernelGGL((k_darcy_rows<DIM, NB, NP, NQ, R>), dim3(grid_for(n_rows, kRowsPerBlock)),
                      dim3(kRowsPerBlock), 0, s, n_rows, inc, pos, rowptr, signs, cells,
                      static_cast<const R*>(coords), static_cast<const R*>(tab),
                      static_cast<const R*>(kinv), kse, ksq, static_cast<const R*>(f), fse, fsq,
-                     static_cast<R*>(vals), static_cast<R*>(rhs));
+                     static_cast<R*>(vals), static_cast<R*>(rhs), bc);
   NFX_LAUNCH_CHECK();
   return NFX_OK;
 }
@@ -750,6 +813,46 @@ extern "C" int nfx_assemble_darcy_rows(int dim, int k, int n_rows, const int32_t
   if (dim == 3 && k == 2) NFX_DROWS(3, 15, 4, 27);
 #undef NFX_DROWS
   set_error("nfx_assemble_darcy_rows: unsupported (dim, k)");
+  return NFX_E_UNSUPPORTED;
+}
+
+extern "C" int nfx_assemble_darcy_rows_bc(
+    int dim, int k, int n_rows, const int32_t* d_inc, const uint8_t* d_pos, const int32_t* d_rowptr,
+    const uint32_t* d_signs, const int32_t* d_cells, const void* d_coords, const void* d_tables,
+    const void* d_kinv, int kinv_stride_e, int kinv_stride_q, const void* d_f, int f_stride_e,
+    int f_stride_q, int n_facets, const int32_t* d_facet, const int32_t* d_combo, const void* d_trace,
+    const void* d_fw, int nqf, const void* d_pD, const int32_t* d_rows, int fp64, void* d_vals,
+    void* d_rhs, void* stream) {
+  NFX_REQUIRE(n_rows >= 0 && n_facets >= 0, "nfx_assemble_darcy_rows_bc: negative size");
+  if (n_rows == 0) return NFX_OK;
+  NFX_REQUIRE(d_inc && d_pos && d_rowptr && d_signs && d_cells && d_coords && d_tables && d_kinv &&
+                  d_f && d_vals && d_rhs,
+              "nfx_assemble_darcy_rows_bc: null pointer");
+  NFX_REQUIRE(n_facets == 0 || (d_facet && d_combo && d_trace && d_fw && d_pD && d_rows),
+              "nfx_assemble_darcy_rows_bc: null facet pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const void* pD = n_facets ? d_pD : nullptr;
+#define NFX_DROWSBC(DIM, NB, NP, NQ)                                                                  \
+  do {                                                                                                \
+    constexpr int FD = DarcyFacetShape<DIM, NB>::FD;                                                  \
+    constexpr int NQF = DarcyFacetShape<DIM, NB>::NQF;                                                \
+    NFX_REQUIRE(n_facets == 0 || nqf == NQF,                                                          \
+                "nfx_assemble_darcy_rows_bc: facet quadrature does not match the element");           \
+    return fp64 ? launch_darcy_rows<DIM, NB, NP, NQ, double>(                                         \
+                      n_rows, d_inc, d_pos, d_rowptr, d_signs, d_cells, d_coords, d_tables, d_kinv,   \
+                      kinv_stride_e, kinv_stride_q, d_f, f_stride_e, f_stride_q, d_vals, d_rhs, s,     \
+                      n_facets * FD, d_facet, d_combo, d_trace, d_fw, pD, d_rows)                      \
+                : launch_darcy_rows<DIM, NB, NP, NQ, float>(                                          \
+                      n_rows, d_inc, d_pos, d_rowptr, d_signs, d_cells, d_coords, d_tables, d_kinv,   \
+                      kinv_stride_e, kinv_stride_q, d_f, f_stride_e, f_stride_q, d_vals, d_rhs, s,     \
+                      n_facets * FD, d_facet, d_combo, d_trace, d_fw, pD, d_rows);                     \
+  } while (0)
+  if (dim == 2 && k == 1) NFX_DROWSBC(2, 3, 1, 4);
+  if (dim == 2 && k == 2) NFX_DROWSBC(2, 8, 3, 9);
+  if (dim == 3 && k == 1) NFX_DROWSBC(3, 4, 1, 8);
+  if (dim == 3 && k == 2) NFX_DROWSBC(3, 15, 4, 27);
+#undef NFX_DROWSBC
+  set_error("nfx_assemble_darcy_rows_bc: unsupported (dim, k)");
   return NFX_E_UNSUPPORTED;
 }
 

@@ -266,6 +266,14 @@ class DarcySystem:
         e_of = np.where(inc >= 0, inc // nl, 0)
         a_of = np.where(inc >= 0, inc % nl, 0)
         packed = np.where(inc >= 0, e_of * 32 + a_of, -1)
+        if self.F:
+            # rows owned by Dirichlet facet f (facet DoF a): second incidence -2 - (f * fd + a)
+            bc_r = (self.bc_rows_np & nat.SLOT_INDEX_MASK).reshape(-1)
+            if len(np.unique(bc_r)) != bc_r.size or np.any(packed[1, bc_r] != -1):
+                raise ValueError("a Dirichlet facet row must have one element incidence and one facet")
+            if np.any(e_of[0, bc_r] != np.repeat(self.bc_neigh_np, self.bc_rows_np.shape[1])):
+                raise ValueError("a Dirichlet facet row's element must be the facet's neighbour")
+            packed[1, bc_r] = -2 - np.arange(bc_r.size)
         # column positions of every reference DoF b of each incident element, inside row r
         rowptr = self.pattern.rowptr.astype(np.int64)
         keys = np.repeat(np.arange(n, dtype=np.int64), np.diff(rowptr)) * n + self.pattern.col
@@ -312,6 +320,8 @@ class DarcySystem:
         live = inc >= 0
         if np.any((inc[live] >> 5) >= self.E) or np.any((inc[live] & 31) >= self.nl) or not np.all(live[0]):
             raise ValueError("row incidences out of range")
+        if np.any(-inc[1][inc[1] <= -2] - 2 >= self.F * FD):
+            raise ValueError("row facet codes out of range")
         lens = np.diff(pat.rowptr)
         if lens.max(initial=0) > 2 * self.nl:
             raise ValueError("row longer than the row kernel's LDS slice (2 * n_loc)")
@@ -354,17 +364,19 @@ class DarcySystem:
         self.bc_fw64 = to_dev(fw, device=device)
         self.bc_fw32 = to_dev(fw.astype(np.float32), device=device)
         shift = self.space.strides[self.space.names[0]] + self.space.dof_shift
-        self.bc_rows = to_dev(row_codes(u_sp.bc_ref_dest + shift, u_sp.bc_ref_sign), device=device)
+        self.bc_rows_np = row_codes(u_sp.bc_ref_dest + shift, u_sp.bc_ref_sign)
+        self.bc_neigh_np = u_sp.bc_neighbour.astype(np.int32)
+        self.bc_rows = to_dev(self.bc_rows_np, device=device)
         Xf = mesh.points[fnodes]  # (F, D, 3)
         self.bc_xq = np.einsum("qa,fac->fqc", p1_bary(fpts), Xf)
 
     # -- warm assembly ------------------------------------------------------------
     def assemble(self, kappa, rhs, p_dirichlet=None, fp64=False, stream=None):
-        """Values + residual rg (the reference's ``rg``, so b = -rg)."""
-        vals, rg = self.assemble_elements(kappa, rhs, fp64=fp64, stream=stream)
-        if p_dirichlet is not None:
-            self.assemble_dirichlet(p_dirichlet, fp64=fp64, stream=stream)
-        return vals, rg
+        """Values + residual rg (the reference's ``rg``, so b = -rg): one launch of the
+        row kernel with the Dirichlet facet term folded in."""
+        if p_dirichlet is not None and self.F:
+            return self._assemble_rows(kappa, rhs, fp64, stream, p_dirichlet=p_dirichlet)
+        return self.assemble_elements(kappa, rhs, fp64=fp64, stream=stream)
 
     def assemble_elements(self, kappa, rhs, fp64=False, stream=None, method="rows"):
         """Matrix values + element residual (rg of every row rewritten).
@@ -393,7 +405,7 @@ class DarcySystem:
                  int(fp64), nat.ptr(stage), nat.ptr(vals), s)
         return vals, rg
 
-    def _assemble_rows(self, kappa, rhs, fp64, stream):
+    def _assemble_rows(self, kappa, rhs, fp64, stream, p_dirichlet=None):
         s = nat.stream_handle(stream)
         vals = self.vals64 if fp64 else self.vals32
         rg = self.rhs64 if fp64 else self.rhs32
@@ -401,11 +413,22 @@ class DarcySystem:
         fa = rhs if isinstance(rhs, CoefficientArg) else coefficient(rhs, self.xq, self.nq, device=self.device)
         check_coefficient(ka, self.E, self.nq, "kappa")
         check_coefficient(fa, self.E, self.nq, "rhs")
-        nat.call("nfx_assemble_darcy_rows", self.dim, self.k, self.n, nat.ptr(self.row_inc), nat.ptr(self.row_pos),
-                 nat.ptr(self.rowptr), nat.ptr(self.row_signs), nat.ptr(self.cells),
-                 nat.ptr(self.coords64 if fp64 else self.coords), nat.ptr(self.tables64 if fp64 else self.tables32),
-                 nat.ptr(ka.get(fp64)), ka.se, ka.sq, nat.ptr(fa.get(fp64)), fa.se, fa.sq, int(fp64),
-                 nat.ptr(vals), nat.ptr(rg), s)
+        coords = nat.ptr(self.coords64 if fp64 else self.coords)
+        tables = nat.ptr(self.tables64 if fp64 else self.tables32)
+        if p_dirichlet is None:
+            nat.call("nfx_assemble_darcy_rows", self.dim, self.k, self.n, nat.ptr(self.row_inc),
+                     nat.ptr(self.row_pos), nat.ptr(self.rowptr), nat.ptr(self.row_signs), nat.ptr(self.cells),
+                     coords, tables, nat.ptr(ka.get(fp64)), ka.se, ka.sq, nat.ptr(fa.get(fp64)), fa.se, fa.sq,
+                     int(fp64), nat.ptr(vals), nat.ptr(rg), s)
+            return vals, rg
+        pd = self.dirichlet_values(p_dirichlet)
+        nat.call("nfx_assemble_darcy_rows_bc", self.dim, self.k, self.n, nat.ptr(self.row_inc),
+                 nat.ptr(self.row_pos), nat.ptr(self.rowptr), nat.ptr(self.row_signs), nat.ptr(self.cells),
+                 coords, tables, nat.ptr(ka.get(fp64)), ka.se, ka.sq, nat.ptr(fa.get(fp64)), fa.se, fa.sq,
+                 self.F, nat.ptr(self.bc_facet), nat.ptr(self.bc_combo),
+                 nat.ptr(self.bc_trace64 if fp64 else self.bc_trace32),
+                 nat.ptr(self.bc_fw64 if fp64 else self.bc_fw32), self.nqf, nat.ptr(pd.get(fp64)),
+                 nat.ptr(self.bc_rows), int(fp64), nat.ptr(vals), nat.ptr(rg), s)
         return vals, rg
 
     def dirichlet_values(self, p_dirichlet):

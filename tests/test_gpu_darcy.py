@@ -65,6 +65,21 @@ def test_darcy_assembly_parity(case):
             assert _row_rel_err(A, G) <= tol, (method, fp64, _row_rel_err(A, G))
             rr = r.double().cpu().numpy()
             assert np.abs(rr - rg).max() <= tol * max(np.abs(rg).max(), 1e-300), (method, fp64)
+    # one launch (row kernel with the Dirichlet facet term folded in) against the two
+    # launches above and against the oracle
+    for fp64, tol in ((True, 1e-12), (False, 1e-5)):
+        v_two, r_two = (a.clone() for a in sysd.assemble_elements(fun["kappa"], fun["rhs"], fp64=fp64))
+        sysd.assemble_dirichlet(ex["p"], fp64=fp64)
+        r_two = (sysd.rhs64 if fp64 else sysd.rhs32).clone()
+        (sysd.rhs64 if fp64 else sysd.rhs32).fill_(float("nan"))
+        vals, r = sysd.assemble(fun["kappa"], fun["rhs"], ex["p"], fp64=fp64)
+        torch.cuda.synchronize()
+        assert torch.equal(vals, v_two)
+        dr = float((r - r_two).abs().max() / r_two.abs().max())
+        print(f"[darcy parity] fused vs two launches fp64={fp64}: rhs max rel diff {dr:.1e}", flush=True)
+        assert dr <= (1e-15 if fp64 else 1e-7)
+        rr = r.double().cpu().numpy()
+        assert np.abs(rr - rg).max() <= tol * max(np.abs(rg).max(), 1e-300), ("fused", fp64)
     # run-to-run determinism: a second assembly is bitwise identical
     v1 = sysd.assemble_elements(fun["kappa"], fun["rhs"], fp64=False)[0].clone()
     v2 = sysd.assemble_elements(fun["kappa"], fun["rhs"], fp64=False)[0].clone()
