@@ -108,9 +108,14 @@ int nfx_assemble_darcy_rows(int dim, int k, int n_rows, const int32_t* d_inc,
  * and d_tiles (n_tiles, 2) int32 (item, first row) pairs, heaviest first; a fixed
  * grid of `grid` workgroups of tile_rows (64, 128, 256) threads walks the tiles
  * in a static order (block b: tiles b, b + grid, ...). Same per-row arithmetic as
- * nfx_assemble_darcy_rows. */
+ * nfx_assemble_darcy_rows. A workgroup keeps the reference table in LDS while
+ * consecutive tiles carry the same tables pointer (give all instances of one degree
+ * one table to benefit; any pointers are correct). degree = 0: tiles of both
+ * degrees; degree = 1 or 2: every tile is of that degree (tiles of the other degree
+ * are skipped) and the kernel is specialised for it (RT1: 8 waves per SIMD). */
 int nfx_assemble_darcy_batched(int n_items, const void* d_items, int n_tiles,
-                               const int32_t* d_tiles, int tile_rows, int grid, void* stream);
+                               const int32_t* d_tiles, int tile_rows, int grid, int degree,
+                               void* stream);
 int nfx_darcy_batch_item_size(void);
 
 /* Dirichlet pressure facet term, LaplaceDualWeakFormBCDirichlet.evaluate_form

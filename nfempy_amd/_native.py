@@ -49,7 +49,7 @@ SIGNATURES = {
                                            c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_int,
                                            c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p,
                                            c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
-    "nfx_assemble_darcy_batched": (c_int, [c_int, c_void_p, c_int, c_void_p, c_int, c_int, c_void_p]),
+    "nfx_assemble_darcy_batched": (c_int, [c_int, c_void_p, c_int, c_void_p, c_int, c_int, c_int, c_void_p]),
     "nfx_darcy_batch_item_size": (c_int, []),
     "nfx_assemble_darcy_bc": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                       c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p,

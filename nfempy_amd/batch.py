@@ -50,8 +50,9 @@ def darcy_space(n, k):
 
 
 class DarcyBatch:
-    def __init__(self, specs, seed=0, ell=0.1, sigma=1.0, device="cuda", log=None):
-        """specs: list of (n, k) per instance; log: optional progress callable."""
+    def __init__(self, specs, seed=0, ell=0.1, sigma=1.0, device="cuda", log=None, split=True):
+        """specs: list of (n, k) per instance; log: optional progress callable; split: one
+        launch per degree with a degree-specialised kernel (else one launch over both)."""
         t = torch()
         if nat.call("nfx_darcy_batch_item_size") != ctypes.sizeof(_Item):
             raise RuntimeError("batch descriptor layout differs between host and library")
@@ -79,41 +80,63 @@ class DarcyBatch:
                 "rhs": t.zeros(sysd.n, dtype=t.float32, device=device),
             })
         self.n_elements = sum(inst["system"].E for inst in self.instances)
-        self.tile_rows, self.grid = 128, None
+        # launch groups: degree -> tiles, tile rows, grid (degree 0: both degrees in one launch)
+        self.split = bool(split)
+        degrees = sorted({k for _, k in self.specs}, reverse=True) if self.split else [0]
+        self.groups = {d: {"tile_rows": 128, "grid": None} for d in degrees}
         self._build_items()
-        self._build_tiles(self.tile_rows)
+        for d in degrees:
+            self._build_tiles(d, 128)
         self.graph = None
 
     def _build_items(self):
+        # one table per degree (the kernel keeps a table in LDS while the pointer repeats)
+        tables = {}
+        for s in self.systems.values():
+            ref = tables.setdefault(s.k, s.tables32)
+            if not torch().equal(ref, s.tables32):
+                raise RuntimeError("instances of one degree must share their reference tables")
         arr = (_Item * len(self.instances))()
         for i, inst in enumerate(self.instances):
             s = inst["system"]
             kinv32, f32 = inst["kinv"].get(False), inst["f"].get(False)
             arr[i] = _Item(s.k, s.n, inst["kinv"].se, inst["kinv"].sq, inst["f"].se, inst["f"].sq,
                            nat.ptr(s.row_inc), nat.ptr(s.row_pos), nat.ptr(s.rowptr), nat.ptr(s.row_signs),
-                           nat.ptr(s.cells), nat.ptr(s.coords), nat.ptr(s.tables32), nat.ptr(kinv32),
+                           nat.ptr(s.cells), nat.ptr(s.coords), nat.ptr(tables[s.k]), nat.ptr(kinv32),
                            nat.ptr(f32), nat.ptr(inst["vals"]), nat.ptr(inst["rhs"]))
+        self._tables = tables
         raw = np.frombuffer(bytes(arr), dtype=np.uint8).copy()
         self._items = torch().from_numpy(raw).to(self.device)
 
-    def _build_tiles(self, tile_rows):
+    def _build_tiles(self, degree, tile_rows):
+        """Tiles of the instances of `degree` (0: all), heaviest first."""
         tiles, cost = [], []
         for i, inst in enumerate(self.instances):
             s = inst["system"]
+            if degree and s.k != degree:
+                continue
             c = s.nl * s.nq
             for r0 in range(0, s.n, tile_rows):
                 tiles.append((i, r0))
                 cost.append(min(tile_rows, s.n - r0) * c)
         order = np.argsort(-np.asarray(cost), kind="stable")
-        tl = np.asarray(tiles, dtype=np.int32)[order]
-        self.n_tiles = len(tl)
-        self._tiles = torch().from_numpy(np.ascontiguousarray(tl)).to(self.device)
-        self.tile_rows = tile_rows
+        tl = np.asarray(tiles, dtype=np.int32).reshape(-1, 2)[order]
+        g = self.groups[degree]
+        g["n_tiles"] = len(tl)
+        g["tiles"] = torch().from_numpy(np.ascontiguousarray(tl)).to(self.device)
+        g["tile_rows"] = tile_rows
 
-    def _launch(self, stream=None):
-        grid = self.grid or min(self.n_tiles, 2048)
-        nat.call("nfx_assemble_darcy_batched", len(self.instances), nat.ptr(self._items), self.n_tiles,
-                 nat.ptr(self._tiles), self.tile_rows, grid, nat.stream_handle(stream))
+    @property
+    def n_tiles(self):
+        return sum(g["n_tiles"] for g in self.groups.values())
+
+    def _launch(self, stream=None, only=None):
+        for d, g in self.groups.items():
+            if only is not None and d != only:
+                continue
+            grid = g["grid"] or min(g["n_tiles"], 2048)
+            nat.call("nfx_assemble_darcy_batched", len(self.instances), nat.ptr(self._items), g["n_tiles"],
+                     nat.ptr(g["tiles"]), g["tile_rows"], max(grid, 1), d, nat.stream_handle(stream))
 
     def assemble(self, stream=None):
         """All instances' FP32 CSR values and residuals (homogeneous Dirichlet data)."""
@@ -123,29 +146,32 @@ class DarcyBatch:
             self._launch(stream)
 
     def autotune(self, reps=5):
-        """Pick tile rows and grid by timing the batch itself; returns the table of trials."""
+        """Pick tile rows and grid of every launch group by timing the batch itself;
+        returns the table of trials."""
         t = torch()
         cus = t.cuda.get_device_properties(t.cuda.current_device()).multi_processor_count
         trials = []
-        best = None
-        for tile in (64, 128, 256):
-            self._build_tiles(tile)
-            for mult in (1, 2, 4, 8):
-                self.grid = min(self.n_tiles, cus * mult)
-                self._launch()
-                t.cuda.synchronize()
-                e0, e1 = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
-                e0.record()
-                for _ in range(reps):
-                    self._launch()
-                e1.record()
-                t.cuda.synchronize()
-                ms = e0.elapsed_time(e1) / reps
-                trials.append({"tile_rows": tile, "grid": self.grid, "ms": ms})
-                if best is None or ms < best[0]:
-                    best = (ms, tile, self.grid)
-        self._build_tiles(best[1])
-        self.grid = best[2]
+        for d, g in self.groups.items():
+            best = None
+            for tile in (64, 128, 256):
+                self._build_tiles(d, tile)
+                for mult in (1, 2, 4, 8):
+                    g["grid"] = max(1, min(g["n_tiles"], cus * mult))
+                    self._launch(only=d)
+                    t.cuda.synchronize()
+                    e0, e1 = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+                    e0.record()
+                    for _ in range(reps):
+                        self._launch(only=d)
+                    e1.record()
+                    t.cuda.synchronize()
+                    ms = e0.elapsed_time(e1) / reps
+                    trials.append({"degree": d, "tile_rows": tile, "grid": g["grid"], "ms": ms})
+                    if best is None or ms < best[0]:
+                        best = (ms, tile, g["grid"])
+            self._build_tiles(d, best[1])
+            g["grid"] = best[2]
+            g["ms"] = best[0]
         self.graph = None
         return trials
 

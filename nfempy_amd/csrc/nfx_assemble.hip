@@ -381,13 +381,19 @@ __device__ __forceinline__ void darcy_row_tile(
     const int32_t* __restrict__ rowptr, const uint32_t* __restrict__ signs,
     const int32_t* __restrict__ cells, const R* __restrict__ coords, const R* __restrict__ tab,
     const R* __restrict__ kinv, int ks_e, int ks_q, const R* __restrict__ f, int fs_e, int fs_q,
-    R* __restrict__ vals, R* __restrict__ rhs, R* st, R* sv, const DarcyRowBc<R>& bc) {
+    R* __restrict__ vals, R* __restrict__ rhs, R* st, R* sv, const DarcyRowBc<R>& bc,
+    bool load_table = true) {
   constexpr int NL = NB + NP;
   constexpr int NS = DarcyTab<DIM, NB, NP, NQ>::NS;
   constexpr int NT = DarcyTab<DIM, NB, NP, NQ>::NT;
+  constexpr int NT_RS = NQ * (1 + NB * DIM + NB + NP);  // start of Rs | Bc | ck
   const int r = r0 + threadIdx.x;
   const int rl = min(r0 + TILE, n_rows);
-  for (int i = threadIdx.x; i < NT; i += TILE) st[i] = tab[i];
+  const bool per_element = (ks_q == 0 && fs_q == 0);
+  // the reference-tensor path reads only Rs | Bc | ck; a caller that already holds the
+  // table in st (same element, same path) passes load_table = false
+  if (load_table)
+    for (int i = (per_element ? NT_RS : 0) + threadIdx.x; i < NT; i += TILE) st[i] = tab[i];
   const int base = rowptr[r0];
   const int span = rowptr[rl] - base;
   for (int i = threadIdx.x; i < span; i += TILE) sv[i] = R(0);
@@ -399,7 +405,6 @@ __device__ __forceinline__ void darcy_row_tile(
   const R* tRs = tpsi + NQ * NP;
   const R* tBc = tRs + NB * NB * NS;
   const R* tck = tBc + NB * NP;
-  const bool per_element = (ks_q == 0 && fs_q == 0);
   if (r < n_rows) {
     const int off = rowptr[r] - base;
     R res = 0;
@@ -549,13 +554,23 @@ __attribute__((amdgpu_waves_per_eu(row_kernel_min_waves<NB, R>(), 8))) void k_da
   constexpr int NT = DarcyTab<DIM, NB, NP, NQ>::NT;
   __shared__ R st[NT];
   __shared__ R sv[kRowsPerBlock * 2 * NL];
-  // Tiles in launch order. An XCD-aware order (block b taking tile (b % 8) * (nb / 8)
-  // + b / 8, so that each of the 8 XCDs walks a contiguous range and keeps the shared
-  // elements in its own L2) measured 7 % slower at 32^3 x 6 (16.9-17.6 us against
-  // 15.6-16.1 us, alternating runs of both builds).
-  darcy_row_tile<DIM, NB, NP, NQ, kRowsPerBlock, R>(blockIdx.x * kRowsPerBlock, n_rows, inc, pos,
-                                                    rowptr, signs, cells, coords, tab, kinv, ks_e,
-                                                    ks_q, f, fs_e, fs_q, vals, rhs, st, sv, bc);
+  // Tiles in launch order, block b taking tiles b, b + grid, ... (one tile per block for
+  // the lowest-order elements; the larger elements run a persistent grid that loads the
+  // reference table into LDS once). An XCD-aware order (block b taking tile
+  // (b % 8) * (nb / 8) + b / 8, so that each of the 8 XCDs walks a contiguous range and
+  // keeps the shared elements in its own L2) measured 7 % slower at 32^3 x 6 (16.9-17.6
+  // us against 15.6-16.1 us, alternating runs of both builds).
+  if constexpr (NB <= 4) {
+    darcy_row_tile<DIM, NB, NP, NQ, kRowsPerBlock, R>(blockIdx.x * kRowsPerBlock, n_rows, inc, pos, rowptr,
+                                                      signs, cells, coords, tab, kinv, ks_e, ks_q, f, fs_e,
+                                                      fs_q, vals, rhs, st, sv, bc);
+  } else {
+    const int n_tiles = (n_rows + kRowsPerBlock - 1) / kRowsPerBlock;
+    for (int t = blockIdx.x; t < n_tiles; t += gridDim.x)
+      darcy_row_tile<DIM, NB, NP, NQ, kRowsPerBlock, R>(t * kRowsPerBlock, n_rows, inc, pos, rowptr, signs,
+                                                        cells, coords, tab, kinv, ks_e, ks_q, f, fs_e, fs_q,
+                                                        vals, rhs, st, sv, bc, t == (int)blockIdx.x);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -578,26 +593,48 @@ struct DarcyBatchItem {
 };
 
 // tiles: (item, first row) pairs, heaviest first; block b takes tiles b, b + G, ...
-template <int TILE>
-__global__ __launch_bounds__(TILE) void k_darcy_rows_batched(const DarcyBatchItem* __restrict__ items,
-                                                             const int2* __restrict__ tiles,
-                                                             int n_tiles) {
-  constexpr int NT2 = DarcyTab<3, 15, 4, 27>::NT;
-  __shared__ float st[NT2];
-  __shared__ float sv[TILE * 2 * 19];
+// KD = 0: tiles of both degrees; KD = 1 or 2: every tile is of that degree (a tile of
+// the other degree is skipped), so the kernel carries only that element's registers
+// and LDS (RT1: 8 waves per SIMD, as the single-system kernel).
+template <int KD>
+constexpr int batched_min_waves() {
+  return KD == 1 ? 8 : 1;
+}
+
+template <int TILE, int KD>
+__global__ __launch_bounds__(TILE)
+__attribute__((amdgpu_waves_per_eu(batched_min_waves<KD>(), 8))) void k_darcy_rows_batched(
+    const DarcyBatchItem* __restrict__ items, const int2* __restrict__ tiles, int n_tiles) {
+  constexpr int NTS = KD == 1 ? DarcyTab<3, 4, 1, 8>::NT : DarcyTab<3, 15, 4, 27>::NT;
+  constexpr int NLS = KD == 1 ? 5 : 19;
+  __shared__ float st[NTS];
+  __shared__ float sv[TILE * 2 * NLS];
   const DarcyRowBc<float> no_bc{0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  // The table stays in LDS across consecutive tiles with the same table pointer and
+  // path (the host gives all instances of one degree one table; tiles come grouped by
+  // cost, hence by degree).
+  const float* held_tab = nullptr;
+  int held_path = -1;
   for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {
     const int2 tl = tiles[t];
     const DarcyBatchItem& it = items[tl.x];
-    if (it.k == 1)
-      darcy_row_tile<3, 4, 1, 8, TILE, float>(tl.y, it.n_rows, it.inc, it.pos, it.rowptr, it.signs,
-                                              it.cells, it.coords, it.tab, it.kinv, it.ks_e, it.ks_q,
-                                              it.f, it.fs_e, it.fs_q, it.vals, it.rhs, st, sv, no_bc);
-    else
-      darcy_row_tile<3, 15, 4, 27, TILE, float>(tl.y, it.n_rows, it.inc, it.pos, it.rowptr,
-                                                it.signs, it.cells, it.coords, it.tab, it.kinv,
-                                                it.ks_e, it.ks_q, it.f, it.fs_e, it.fs_q, it.vals,
-                                                it.rhs, st, sv, no_bc);
+    const int path = (it.ks_q == 0 && it.fs_q == 0) ? 1 : 0;
+    const bool load = it.tab != held_tab || path != held_path;
+    held_tab = it.tab;
+    held_path = path;
+    if constexpr (KD != 2) {
+      if (it.k == 1)
+        darcy_row_tile<3, 4, 1, 8, TILE, float>(tl.y, it.n_rows, it.inc, it.pos, it.rowptr, it.signs,
+                                                it.cells, it.coords, it.tab, it.kinv, it.ks_e, it.ks_q,
+                                                it.f, it.fs_e, it.fs_q, it.vals, it.rhs, st, sv, no_bc, load);
+    }
+    if constexpr (KD != 1) {
+      if (it.k == 2)
+        darcy_row_tile<3, 15, 4, 27, TILE, float>(tl.y, it.n_rows, it.inc, it.pos, it.rowptr,
+                                                  it.signs, it.cells, it.coords, it.tab, it.kinv,
+                                                  it.ks_e, it.ks_q, it.f, it.fs_e, it.fs_q, it.vals,
+                                                  it.rhs, st, sv, no_bc, load);
+    }
   }
 }
 
@@ -612,7 +649,11 @@ static int launch_darcy_rows(int n_rows, const int32_t* inc, const uint8_t* pos,
                              const int32_t* bc_rows = nullptr) {
   const DarcyRowBc<R> bc{n_bc, facet, combo, static_cast<const R*>(trace), static_cast<const R*>(fw),
                          static_cast<const R*>(pD), bc_rows};
-  hipLaunchKernelGGL((k_darcy_rows<DIM, NB, NP, NQ, R>), dim3(grid_for(n_rows, kRowsPerBlock)),
+  // one tile per block for the lowest-order elements; a persistent grid (8 blocks per
+  // CU of 256) for the larger tables
+  const int n_tiles = grid_for(n_rows, kRowsPerBlock);
+  const int grid = NB <= 4 ? n_tiles : std::min(n_tiles, 2048);
+  hipLaunchKernelGGL((k_darcy_rows<DIM, NB, NP, NQ, R>), dim3(grid),
                      dim3(kRowsPerBlock), 0, s, n_rows, inc, pos, rowptr, signs, cells,
                      static_cast<const R*>(coords), static_cast<const R*>(tab),
                      static_cast<const R*>(kinv), kse, ksq, static_cast<const R*>(f), fse, fsq,
@@ -887,27 +928,36 @@ extern "C" int nfx_assemble_elasticity_bc(int n_facets, const int32_t* d_facet,
                                                    -1.0);
 }
 
-extern "C" int nfx_assemble_darcy_batched(int n_items, const void* d_items, int n_tiles,
-                                          const int32_t* d_tiles, int tile_rows, int grid,
-                                          void* stream) {
-  NFX_REQUIRE(n_items >= 0 && n_tiles >= 0 && grid > 0, "nfx_assemble_darcy_batched: sizes");
-  if (n_tiles == 0) return NFX_OK;
-  NFX_REQUIRE(d_items && d_tiles, "nfx_assemble_darcy_batched: null pointer");
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  const DarcyBatchItem* items = static_cast<const DarcyBatchItem*>(d_items);
-  const int2* tiles = reinterpret_cast<const int2*>(d_tiles);
+template <int KD>
+static int launch_batched(const DarcyBatchItem* items, const int2* tiles, int n_tiles, int tile_rows, int grid,
+                          hipStream_t s) {
   if (tile_rows == 64)
-    hipLaunchKernelGGL(k_darcy_rows_batched<64>, dim3(grid), dim3(64), 0, s, items, tiles, n_tiles);
+    hipLaunchKernelGGL((k_darcy_rows_batched<64, KD>), dim3(grid), dim3(64), 0, s, items, tiles, n_tiles);
   else if (tile_rows == 128)
-    hipLaunchKernelGGL(k_darcy_rows_batched<128>, dim3(grid), dim3(128), 0, s, items, tiles, n_tiles);
+    hipLaunchKernelGGL((k_darcy_rows_batched<128, KD>), dim3(grid), dim3(128), 0, s, items, tiles, n_tiles);
   else if (tile_rows == 256)
-    hipLaunchKernelGGL(k_darcy_rows_batched<256>, dim3(grid), dim3(256), 0, s, items, tiles, n_tiles);
+    hipLaunchKernelGGL((k_darcy_rows_batched<256, KD>), dim3(grid), dim3(256), 0, s, items, tiles, n_tiles);
   else {
     set_error("nfx_assemble_darcy_batched: tile_rows must be 64, 128 or 256");
     return NFX_E_INVALID;
   }
   NFX_LAUNCH_CHECK();
   return NFX_OK;
+}
+
+extern "C" int nfx_assemble_darcy_batched(int n_items, const void* d_items, int n_tiles,
+                                          const int32_t* d_tiles, int tile_rows, int grid, int degree,
+                                          void* stream) {
+  NFX_REQUIRE(n_items >= 0 && n_tiles >= 0 && grid > 0, "nfx_assemble_darcy_batched: sizes");
+  NFX_REQUIRE(degree >= 0 && degree <= 2, "nfx_assemble_darcy_batched: degree must be 0, 1 or 2");
+  if (n_tiles == 0) return NFX_OK;
+  NFX_REQUIRE(d_items && d_tiles, "nfx_assemble_darcy_batched: null pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const DarcyBatchItem* items = static_cast<const DarcyBatchItem*>(d_items);
+  const int2* tiles = reinterpret_cast<const int2*>(d_tiles);
+  if (degree == 1) return launch_batched<1>(items, tiles, n_tiles, tile_rows, grid, s);
+  if (degree == 2) return launch_batched<2>(items, tiles, n_tiles, tile_rows, grid, s);
+  return launch_batched<0>(items, tiles, n_tiles, tile_rows, grid, s);
 }
 
 extern "C" int nfx_darcy_batch_item_size(void) { return (int)sizeof(DarcyBatchItem); }

@@ -33,7 +33,7 @@ def test_batched_matches_single_and_graph():
     for inst, (v, r) in zip(b.instances, ref):
         assert torch.equal(inst["vals"], v) and torch.equal(inst["rhs"], r)
     trials = b.autotune(reps=2)
-    assert len(trials) == 12
+    assert len(trials) == 12 * len(b.groups) and sorted(b.groups) == [1, 2]
     for inst in b.instances:
         inst["vals"].zero_()
     b.assemble()
@@ -53,6 +53,22 @@ def test_batched_matches_single_and_graph():
             assert torch.equal(inst["vals"], v) and torch.equal(inst["rhs"], r), rep
         _single(b.instances[0])  # eager work between replays
         torch.cuda.synchronize()
+
+
+def test_batched_one_launch_over_both_degrees():
+    """degree = 0 (one launch, generic kernel over RT1 and RT2 tiles) gives the same bits
+    as the degree-specialised launches."""
+    import torch
+    from nfempy_amd.batch import DarcyBatch
+
+    b1 = DarcyBatch(SPECS, seed=4)
+    b0 = DarcyBatch(SPECS, seed=4, split=False)
+    assert sorted(b0.groups) == [0]
+    b1.assemble()
+    b0.assemble()
+    torch.cuda.synchronize()
+    for i1, i0 in zip(b1.instances, b0.instances):
+        assert torch.equal(i1["vals"], i0["vals"]) and torch.equal(i1["rhs"], i0["rhs"])
 
 
 def test_batched_instance_vs_oracle_and_solve():
