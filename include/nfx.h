@@ -86,8 +86,9 @@ int nfx_assemble_darcy(int dim, int k, int n_elem, const int32_t* d_cells,
  * values; the residual of every row is written (pressure rows: -sum w detJ f psi,
  * flux rows: 0, before the Dirichlet facet term). No slot map, no staging.
  *  d_inc    (2, n_rows) int32 incidences e*32 + a (reference DoF a), negative = none
- *  d_pos    (n_rows, 2, nl) uint8 position of reference DoF b's column in the row
- *           (255 = not present)
+ *  d_pos    (n_rows, 2, nlp) uint8 position of reference DoF b's column in the row
+ *           (255 = not present); nlp = nl, or nl rounded up to a multiple of 4 when
+ *           nl > 8 (RT2 elements: word loads)
  *  d_rowptr CSR row pointer; d_signs (E) uint32, bit a = reference DoF a negated
  * Other arguments as nfx_assemble_darcy. */
 int nfx_assemble_darcy_rows(int dim, int k, int n_rows, const int32_t* d_inc,

@@ -312,12 +312,17 @@ __global__ __launch_bounds__(256) void k_facet_bc(
 // every column (first incidence then second: the reference's addv order). The
 // block then stores its contiguous range of CSR values with coalesced writes.
 //   inc[2][n_rows]  packed incidence e << 5 | a, -1 = none
-//   pos[n_rows][2][NL] column position of reference DoF b inside the row (255 = none)
+//   pos[n_rows][2][NLP] column position of reference DoF b inside the row (255 = none);
+//                   NLP = row_pos_stride(NL): NL, or NL rounded up to whole words when NL > 8
 //   signs[E]        bit b set: reference DoF b of element e carries a minus sign;
 //                   entry (a, b) enters with the sign (-1)^(bit a xor bit b)
 // Element row a < NB: M[a][b] (b < NB) and -B[k][a] (b = NB + k);
 // row a = NB + k: B[k][b] (b < NB), 0 (b >= NB), residual -sum_q w detJ f psi_k.
 constexpr int kRowsPerBlock = 128;
+
+// Bytes per incidence in the column-position array: the larger elements pad to whole
+// 32-bit words so that a row reads its positions with word loads.
+constexpr int row_pos_stride(int nl) { return nl > 8 ? (nl + 3) / 4 * 4 : nl; }
 
 // Table layout: w[NQ] | phi[NQ][NB][DIM] | div[NQ][NB] | psi[NQ][NP] | Rs[NB][NB][NS] |
 // Bc[NB][NP] | ck[NP]  (NS = DIM (DIM + 1) / 2 symmetric pairs of G = J^T J).
@@ -504,12 +509,27 @@ __device__ __forceinline__ void darcy_row_tile(
           res -= wp * g.detJ * f[(int64_t)e[j] * fs_e + q * fs_q];
         }
       }
-      const uint8_t* pr = pos + ((int64_t)r * 2 + j) * NL;
+      if constexpr (row_pos_stride(NL) % 4 == 0) {
+        // padded to whole words: NLP / 4 word loads instead of NL byte loads
+        constexpr int NW = row_pos_stride(NL) / 4;
+        const uint32_t* pw = reinterpret_cast<const uint32_t*>(pos) + ((int64_t)r * 2 + j) * NW;
+        uint32_t wd[NW];
 #pragma unroll
-      for (int b = 0; b < NL; ++b) {
-        const int p = pr[b];
-        if (p == 255) continue;
-        sv[off + p] += ((sg[j] >> b) & 1u) ? -row[b] : row[b];
+        for (int w = 0; w < NW; ++w) wd[w] = pw[w];
+#pragma unroll
+        for (int b = 0; b < NL; ++b) {
+          const int p = (wd[b >> 2] >> (8 * (b & 3))) & 0xff;
+          if (p == 255) continue;
+          sv[off + p] += ((sg[j] >> b) & 1u) ? -row[b] : row[b];
+        }
+      } else {
+        const uint8_t* pr = pos + ((int64_t)r * 2 + j) * NL;
+#pragma unroll
+        for (int b = 0; b < NL; ++b) {
+          const int p = pr[b];
+          if (p == 255) continue;
+          sv[off + p] += ((sg[j] >> b) & 1u) ? -row[b] : row[b];
+        }
       }
     }
     if (bc.pD != nullptr && code[1] <= -2 && -code[1] - 2 < bc.n_codes) {
@@ -575,22 +595,27 @@ __attribute__((amdgpu_waves_per_eu(row_kernel_min_waves<NB, R>(), 8))) void k_da
 
 // ---------------------------------------------------------------------------
 // Batched persistent assembly: many independent 3-D RT_k - P_{k-1} systems.
+// The arrays are device memory: the pointers carry the global address space, so that
+// the kernel reading them from the descriptor uses global loads and stores (through a
+// generic pointer the compiler emits flat ones, which also occupy the LDS counter).
+#define NFX_GLOBAL __attribute__((address_space(1)))
 struct DarcyBatchItem {
   int k;
   int n_rows;
   int ks_e, ks_q, fs_e, fs_q;
-  const int32_t* inc;
-  const uint8_t* pos;
-  const int32_t* rowptr;
-  const uint32_t* signs;
-  const int32_t* cells;
-  const float* coords;
-  const float* tab;
-  const float* kinv;
-  const float* f;
-  float* vals;
-  float* rhs;
+  const NFX_GLOBAL int32_t* inc;
+  const NFX_GLOBAL uint8_t* pos;
+  const NFX_GLOBAL int32_t* rowptr;
+  const NFX_GLOBAL uint32_t* signs;
+  const NFX_GLOBAL int32_t* cells;
+  const NFX_GLOBAL float* coords;
+  const NFX_GLOBAL float* tab;
+  const NFX_GLOBAL float* kinv;
+  const NFX_GLOBAL float* f;
+  NFX_GLOBAL float* vals;
+  NFX_GLOBAL float* rhs;
 };
+#undef NFX_GLOBAL
 
 // tiles: (item, first row) pairs, heaviest first; block b takes tiles b, b + G, ...
 // KD = 0: tiles of both degrees; KD = 1 or 2: every tile is of that degree (a tile of
@@ -618,22 +643,34 @@ __attribute__((amdgpu_waves_per_eu(batched_min_waves<KD>(), 8))) void k_darcy_ro
   for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {
     const int2 tl = tiles[t];
     const DarcyBatchItem& it = items[tl.x];
+    // explicit casts from the global-typed fields: the compiler then knows every access
+    // below is a global one
+    const int32_t* inc = (const int32_t*)it.inc;
+    const uint8_t* pos = (const uint8_t*)it.pos;
+    const int32_t* rowptr = (const int32_t*)it.rowptr;
+    const uint32_t* signs = (const uint32_t*)it.signs;
+    const int32_t* cells = (const int32_t*)it.cells;
+    const float* coords = (const float*)it.coords;
+    const float* tab = (const float*)it.tab;
+    const float* kinv = (const float*)it.kinv;
+    const float* f = (const float*)it.f;
+    float* vals = (float*)it.vals;
+    float* rhs = (float*)it.rhs;
     const int path = (it.ks_q == 0 && it.fs_q == 0) ? 1 : 0;
-    const bool load = it.tab != held_tab || path != held_path;
-    held_tab = it.tab;
+    const bool load = tab != held_tab || path != held_path;
+    held_tab = tab;
     held_path = path;
     if constexpr (KD != 2) {
       if (it.k == 1)
-        darcy_row_tile<3, 4, 1, 8, TILE, float>(tl.y, it.n_rows, it.inc, it.pos, it.rowptr, it.signs,
-                                                it.cells, it.coords, it.tab, it.kinv, it.ks_e, it.ks_q,
-                                                it.f, it.fs_e, it.fs_q, it.vals, it.rhs, st, sv, no_bc, load);
+        darcy_row_tile<3, 4, 1, 8, TILE, float>(tl.y, it.n_rows, inc, pos, rowptr, signs, cells, coords, tab,
+                                                kinv, it.ks_e, it.ks_q, f, it.fs_e, it.fs_q, vals, rhs, st, sv,
+                                                no_bc, load);
     }
     if constexpr (KD != 1) {
       if (it.k == 2)
-        darcy_row_tile<3, 15, 4, 27, TILE, float>(tl.y, it.n_rows, it.inc, it.pos, it.rowptr,
-                                                  it.signs, it.cells, it.coords, it.tab, it.kinv,
-                                                  it.ks_e, it.ks_q, it.f, it.fs_e, it.fs_q, it.vals,
-                                                  it.rhs, st, sv, no_bc, load);
+        darcy_row_tile<3, 15, 4, 27, TILE, float>(tl.y, it.n_rows, inc, pos, rowptr, signs, cells, coords,
+                                                  tab, kinv, it.ks_e, it.ks_q, f, it.fs_e, it.fs_q, vals, rhs,
+                                                  st, sv, no_bc, load);
     }
   }
 }

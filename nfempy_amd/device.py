@@ -277,7 +277,10 @@ class DarcySystem:
         # column positions of every reference DoF b of each incident element, inside row r
         rowptr = self.pattern.rowptr.astype(np.int64)
         keys = np.repeat(np.arange(n, dtype=np.int64), np.diff(rowptr)) * n + self.pattern.col
-        pos = np.full((n, 2, nl), 255, np.uint8)
+        # bytes per incidence: nl, padded to whole words for the larger elements (the
+        # kernel's row_pos_stride)
+        nlp = nl if nl <= 8 else (nl + 3) // 4 * 4
+        pos = np.full((n, 2, nlp), 255, np.uint8)
         if (np.diff(rowptr).max(initial=0)) >= 255:
             raise ValueError("row too long for uint8 column positions")
         for j in range(2):
@@ -288,7 +291,7 @@ class DarcySystem:
             idx = np.searchsorted(keys, q)
             if not np.array_equal(keys[np.minimum(idx, len(keys) - 1)], q):
                 raise ValueError("incident column missing from the pattern")
-            pos[r, j, :] = (idx - rowptr[r][:, None]).astype(np.uint8)
+            pos[r, j, :nl] = (idx - rowptr[r][:, None]).astype(np.uint8)
         signs = ((self.ref_sign < 0).astype(np.uint64) << np.arange(nl, dtype=np.uint64)[None, :]).sum(axis=1)
         self.row_inc_np, self.row_pos_np = packed.astype(np.int32), pos
         self.row_inc = to_dev(np.ascontiguousarray(packed.astype(np.int32)), device=device)
@@ -326,6 +329,8 @@ class DarcySystem:
         if lens.max(initial=0) > 2 * self.nl:
             raise ValueError("row longer than the row kernel's LDS slice (2 * n_loc)")
         pp = self.row_pos_np
+        if pp.shape != (self.n, 2, self.nl if self.nl <= 8 else (self.nl + 3) // 4 * 4):
+            raise ValueError("row column positions shape")
         if np.any((pp != 255) & (pp >= lens[:, None, None])):
             raise ValueError("row column positions out of range")
         if self.F:
