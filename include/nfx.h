@@ -367,6 +367,45 @@ int nfx_sgemm(int M, int N, int K, float alpha, const float* d_A, int lda, int t
 /* Row-scaled FP16 copy of a dense n x n FP32 matrix (Ascale[r] = max |A[r][:]|). */
 int nfx_half_rows(int n, const float* d_A, uint16_t* d_A16, float* d_scale, void* stream);
 
+/* ---- Mixed-dimensional transient Darcy -----------------------------------
+ * Interface coupling of CouplingWeakForm.evaluate_form
+ * (examples/md_elliptic_scalar/CouplingWeakForm.py:11-151), RT1 matrix facet DoFs and P0
+ * fracture pressure, for m fracture triangles: triangle i lies on a matrix face with
+ * vertex ids d_facet_nodes[3i..3i+2]; side s = 0 (positive) / 1 (negative) has element
+ * d_side_elem[2i+s], combo index d_side_combo[2i+s] into d_trace (the facet trace
+ * tables of nfx_assemble_darcy_bc) and row code d_side_code[2i+s] (its facet DoF, bit
+ * 30 = negated). With s_q the normal trace out of that element and
+ * 1/alpha = d_inv_alpha[i] = delta / (2 kappa_n):
+ *   vals[pos[6i+3s]] += (1/alpha) sum_q w |det J_F| s_q^2,
+ *   vals[pos[6i+3s+1]] and vals[pos[6i+3s+2]] += sum_q w |det J_F| s_q (signed),
+ * the (u_s, u_s), (u_s, p) and (p, u_s) entries; all positions distinct. exchange_sign
+ * multiplies the (p, u_s) entries: +1 the reference form (its fracture mass row adds the
+ * matrix outflow u_s . n_s with the sign of the fracture divergence), -1 the conservative
+ * exchange (the outflow is a source of the fracture). */
+int nfx_md_coupling(int m, const int32_t* d_side_elem, const int32_t* d_side_combo,
+                    const int32_t* d_side_code, const int32_t* d_facet_nodes, const int32_t* d_cells,
+                    const void* d_coords, const void* d_trace, const void* d_fw,
+                    const void* d_inv_alpha, const int32_t* d_pos, int exchange_sign, int fp64,
+                    void* d_vals, void* stream);
+/* dst[map[i]] = src[i] (float or double by fp64): places a block system's CSR values
+ * or residual into the coupled system's (the reference scatters every block into one
+ * PETSc matrix, md_elliptic_scalar.py:250-343). */
+int nfx_md_scatter(int n, const void* d_src, const int32_t* d_map, int fp64, void* d_dst,
+                   void* stream);
+/* vals[pos[i]] = v[i] (mode 0) or vals[pos[i]] += v[i] (mode 1), distinct positions:
+ * storage terms on the pressure diagonal, elimination of the fracture-tip flux DoFs. */
+int nfx_md_update(int n, const int32_t* d_pos, const double* d_v, int mode, int fp64,
+                  void* d_vals, void* stream);
+/* Backward-Euler right-hand side (SundusWeakForm.py:148-150, mixed_transport_model.py:
+ * 254-350): b[i] = b0[i] for i < n_u, b0[i] + mass[i - n_u] x[i] for the pressure rows. */
+int nfx_md_rhs(int n, int n_u, const double* d_b0, const double* d_mass, const double* d_x,
+               double* d_b, void* stream);
+/* Diagonal block preconditioner of the coupled system (no reference counterpart: the
+ * reference factorises with MUMPS): flux rows 1/|a_kk|, pressure rows
+ * 1/(|a_kk| + sum_j |a_kj a_jk| / |a_jj|) over flux columns j. */
+int nfx_md_block_diag(int n, int n_u, const int32_t* d_rowptr, const int32_t* d_col,
+                      const double* d_vals, float* d_pinv, void* stream);
+
 /* ---- FGMRES + FP64 iterative refinement -----------------------------------
  * Replaces KSP preonly + LU/MUMPS of the Darcy driver
  * (examples/md_elliptic_scalar/md_elliptic_scalar.py:351-364) when the learned

@@ -49,6 +49,12 @@ SIGNATURES = {
                                            c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_int,
                                            c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p,
                                            c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "nfx_md_coupling": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p]),
+    "nfx_md_scatter": (c_int, [c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+    "nfx_md_update": (c_int, [c_int, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p]),
+    "nfx_md_rhs": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "nfx_md_block_diag": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "nfx_assemble_darcy_batched": (c_int, [c_int, c_void_p, c_int, c_void_p, c_int, c_int, c_int, c_void_p]),
     "nfx_darcy_batch_item_size": (c_int, []),
     "nfx_assemble_darcy_bc": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,

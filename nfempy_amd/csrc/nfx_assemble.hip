@@ -194,18 +194,20 @@ __global__ __launch_bounds__(256) void k_darcy(
 // trace: (ncombo, NQF, FD, DIM) reference flux basis at the facet points mapped
 // into the neighbour's reference cell, per (local facet, facet vertex order);
 // g: (F, NQF, NC); rows: (F, FD * NC) row codes, entry a * NC + c.
-// val[a][c] of facet f from the neighbour's vertices X and geometry g.
-template <int DIM, int FD, int NQF, int NC, typename R>
-__device__ __forceinline__ void facet_term(int f, const R (&X)[DIM + 1][3], const Geom<DIM, R>& g,
-                                           const int32_t* __restrict__ facet_nodes,
-                                           const int32_t* __restrict__ combo, const R* __restrict__ coords,
-                                           const R* __restrict__ trace, const R* __restrict__ fw,
-                                           const R* __restrict__ g_data, R (&val)[FD][NC]) {
+// Weights and normal traces of a facet: wq[q] = w_q |det J_f|, s[q][a] = psi_a(x_q) . n
+// (reference flux basis of facet DoF a traced from the neighbour with vertices X and
+// geometry g, pushed forward, n the unit normal pointing out of the neighbour).
+// fn: the facet's DIM vertex ids in stored order; cmb: its combo index into trace.
+template <int DIM, int FD, int NQF, typename R>
+__device__ __forceinline__ void facet_trace(const int32_t* __restrict__ fn, int cmb, const R (&X)[DIM + 1][3],
+                                            const Geom<DIM, R>& g, const R* __restrict__ coords,
+                                            const R* __restrict__ trace, const R* __restrict__ fw,
+                                            R (&wq)[NQF], R (&s)[NQF][FD]) {
   // facet geometry in stored node order
   R Y[DIM][3];
 #pragma unroll
   for (int a = 0; a < DIM; ++a) {
-    const int v = facet_nodes[f * DIM + a];
+    const int v = fn[a];
 #pragma unroll
     for (int c = 0; c < 3; ++c) Y[a][c] = coords[3 * v + c];
   }
@@ -262,23 +264,39 @@ __device__ __forceinline__ void facet_term(int f, const R (&X)[DIM + 1][3], cons
     for (int c = 0; c < 3; ++c) s += g.J[c][d] * n[c];
     jtn[d] = s / g.detJ;
   }
-  const R* __restrict__ tr = trace + (int64_t)combo[f] * NQF * FD * DIM;
+  const R* __restrict__ tr = trace + (int64_t)cmb * NQF * FD * DIM;
+#pragma unroll
+  for (int q = 0; q < NQF; ++q) {
+    wq[q] = fw[q] * detf;
+#pragma unroll
+    for (int a = 0; a < FD; ++a) {
+      R sv = 0;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) sv += tr[(q * FD + a) * DIM + d] * jtn[d];
+      s[q][a] = sv;
+    }
+  }
+}
+
+// val[a][c] of facet f from the neighbour's vertices X and geometry g.
+template <int DIM, int FD, int NQF, int NC, typename R>
+__device__ __forceinline__ void facet_term(int f, const R (&X)[DIM + 1][3], const Geom<DIM, R>& g,
+                                           const int32_t* __restrict__ facet_nodes,
+                                           const int32_t* __restrict__ combo, const R* __restrict__ coords,
+                                           const R* __restrict__ trace, const R* __restrict__ fw,
+                                           const R* __restrict__ g_data, R (&val)[FD][NC]) {
+  R wq[NQF], s[NQF][FD];
+  facet_trace<DIM, FD, NQF, R>(facet_nodes + (int64_t)f * DIM, combo[f], X, g, coords, trace, fw, wq, s);
 #pragma unroll
   for (int a = 0; a < FD; ++a)
 #pragma unroll
     for (int c = 0; c < NC; ++c) val[a][c] = 0;
 #pragma unroll
-  for (int q = 0; q < NQF; ++q) {
-    const R wq = fw[q] * detf;
+  for (int q = 0; q < NQF; ++q)
 #pragma unroll
-    for (int a = 0; a < FD; ++a) {
-      R s = 0;
+    for (int a = 0; a < FD; ++a)
 #pragma unroll
-      for (int d = 0; d < DIM; ++d) s += tr[(q * FD + a) * DIM + d] * jtn[d];
-#pragma unroll
-      for (int c = 0; c < NC; ++c) val[a][c] += wq * g_data[((int64_t)f * NQF + q) * NC + c] * s;
-    }
-  }
+      for (int c = 0; c < NC; ++c) val[a][c] += wq[q] * g_data[((int64_t)f * NQF + q) * NC + c] * s[q][a];
 }
 
 template <int DIM, int FD, int NQF, int NC, typename R>
@@ -764,6 +782,109 @@ static int launch_darcy_bc(int n, const int32_t* facet, const int32_t* neigh, co
   return NFX_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Mixed-dimensional coupling (CouplingWeakForm.evaluate_form,
+// examples/md_elliptic_scalar/CouplingWeakForm.py:11-151) for RT1 facet DoFs: fracture
+// triangle i lies on matrix face F with the element of side s (0 = positive, 1 = negative
+// side of the fracture plane) and that element's facet DoF; with the normal trace
+// s_q = phi . n_s (n_s out of that element) and 1/alpha = delta / (2 kappa_n):
+//   (u_s, u_s) += (1/alpha) sum_q w |det J_F| s_q^2
+//   (u_s, p) and (p, u_s) += sum_q w |det J_F| s_q          (P0 fracture pressure)
+// each signed by the DoF's orientation (row code bit 30). pos (m, 6): CSR positions of
+// (u0,u0), (u0,p), (p,u0), (u1,u1), (u1,p), (p,u1); every position belongs to one
+// triangle (the split gives each side its own DoF), so += has no race. The (p, u_s)
+// entries are multiplied by psign: +1 is the reference form (the fracture mass row adds
+// the matrix outflow with the sign of the fracture divergence), -1 the conservative
+// exchange (the outflow enters the fracture as a source).
+template <typename R>
+__global__ __launch_bounds__(256) void k_md_coupling(
+    int m, const int32_t* __restrict__ side_elem, const int32_t* __restrict__ side_combo,
+    const int32_t* __restrict__ side_code, const int32_t* __restrict__ facet_nodes,
+    const int32_t* __restrict__ cells, const R* __restrict__ coords, const R* __restrict__ trace,
+    const R* __restrict__ fw, const R* __restrict__ inv_alpha, const int32_t* __restrict__ pos, R psign,
+    R* __restrict__ vals) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const R ia = inv_alpha[i];
+#pragma unroll
+  for (int sd = 0; sd < 2; ++sd) {
+    Geom<3, R> g;
+    R X[4][3];
+    load_geometry<3, R>(cells, coords, side_elem[2 * i + sd], g, X);
+    R wq[4], sq[4][1];
+    facet_trace<3, 1, 4, R>(facet_nodes + 3 * (int64_t)i, side_combo[2 * i + sd], X, g, coords, trace, fw, wq,
+                            sq);
+    R s1 = 0, s2 = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      s1 += wq[q] * sq[q][0];
+      s2 += wq[q] * sq[q][0] * sq[q][0];
+    }
+    const R sg = (side_code[2 * i + sd] & kSlotNeg) ? R(-1) : R(1);
+    const int32_t* p = pos + 6 * (int64_t)i + 3 * sd;
+    vals[p[0]] += ia * s2;
+    vals[p[1]] += sg * s1;
+    vals[p[2]] += psign * sg * s1;
+  }
+}
+
+// dst[map[i]] = src[i]
+template <typename R>
+__global__ __launch_bounds__(256) void k_md_scatter(int n, const R* __restrict__ src, const int32_t* __restrict__ map,
+                                                    R* __restrict__ dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[map[i]] = src[i];
+}
+
+// vals[pos[i]] = v[i] (mode 0) or += v[i] (mode 1); positions distinct
+template <typename R>
+__global__ __launch_bounds__(256) void k_md_update(int n, const int32_t* __restrict__ pos, const double* __restrict__ v,
+                                                   int mode, R* __restrict__ vals) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const R x = (R)v[i];
+  vals[pos[i]] = mode ? vals[pos[i]] + x : x;
+}
+
+// Backward-Euler right-hand side: b[i] = b0[i] on the flux rows (i < n_u) and
+// b0[i] + mass[i - n_u] x[i] on the pressure rows (one thread per row).
+__global__ __launch_bounds__(256) void k_md_rhs(int n, int n_u, const double* __restrict__ b0,
+                                                const double* __restrict__ mass, const double* __restrict__ x,
+                                                double* __restrict__ b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  b[i] = i < n_u ? b0[i] : b0[i] + mass[i - n_u] * x[i];
+}
+
+// Block preconditioner diagonal of a coupled saddle point whose coupling blocks are not
+// antisymmetric: flux rows 1/|a_kk|, pressure rows 1/(|a_kk| + sum_j |a_kj a_jk| / |a_jj|)
+// over flux columns j (the absolute Schur diagonal).
+__global__ __launch_bounds__(256) void k_md_block_diag(int n, int nu, const int32_t* __restrict__ rp,
+                                                       const int32_t* __restrict__ ci,
+                                                       const double* __restrict__ va, float* __restrict__ pinv) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double akk = 0;
+  for (int jj = rp[k]; jj < rp[k + 1]; ++jj)
+    if (ci[jj] == k) akk = va[jj];
+  if (k < nu) {
+    pinv[k] = akk != 0 ? (float)(1.0 / fabs(akk)) : 1.f;
+    return;
+  }
+  double s = fabs(akk);
+  for (int jj = rp[k]; jj < rp[k + 1]; ++jj) {
+    const int j = ci[jj];
+    if (j >= nu) continue;
+    double ajk = 0, ajj = 0;
+    for (int t = rp[j]; t < rp[j + 1]; ++t) {
+      if (ci[t] == k) ajk = va[t];
+      if (ci[t] == j) ajj = va[t];
+    }
+    if (ajj != 0) s += fabs(va[jj] * ajk) / fabs(ajj);
+  }
+  pinv[k] = s > 0 ? (float)(1.0 / s) : 1.f;
+}
+
 }  // namespace nfx
 
 using namespace nfx;
@@ -998,3 +1119,83 @@ extern "C" int nfx_assemble_darcy_batched(int n_items, const void* d_items, int 
 }
 
 extern "C" int nfx_darcy_batch_item_size(void) { return (int)sizeof(DarcyBatchItem); }
+
+extern "C" int nfx_md_coupling(int m, const int32_t* d_side_elem, const int32_t* d_side_combo,
+                               const int32_t* d_side_code, const int32_t* d_facet_nodes, const int32_t* d_cells,
+                               const void* d_coords, const void* d_trace, const void* d_fw,
+                               const void* d_inv_alpha, const int32_t* d_pos, int exchange_sign, int fp64,
+                               void* d_vals, void* stream) {
+  NFX_REQUIRE(m >= 0 && (exchange_sign == 1 || exchange_sign == -1), "nfx_md_coupling: bad arguments");
+  if (m == 0) return NFX_OK;
+  NFX_REQUIRE(d_side_elem && d_side_combo && d_side_code && d_facet_nodes && d_cells && d_coords && d_trace &&
+                  d_fw && d_inv_alpha && d_pos && d_vals,
+              "nfx_md_coupling: null pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (fp64)
+    hipLaunchKernelGGL(k_md_coupling<double>, dim3(grid_for(m, 256)), dim3(256), 0, s, m, d_side_elem, d_side_combo,
+                       d_side_code, d_facet_nodes, d_cells, static_cast<const double*>(d_coords),
+                       static_cast<const double*>(d_trace), static_cast<const double*>(d_fw),
+                       static_cast<const double*>(d_inv_alpha), d_pos, (double)exchange_sign,
+                       static_cast<double*>(d_vals));
+  else
+    hipLaunchKernelGGL(k_md_coupling<float>, dim3(grid_for(m, 256)), dim3(256), 0, s, m, d_side_elem, d_side_combo,
+                       d_side_code, d_facet_nodes, d_cells, static_cast<const float*>(d_coords),
+                       static_cast<const float*>(d_trace), static_cast<const float*>(d_fw),
+                       static_cast<const float*>(d_inv_alpha), d_pos, (float)exchange_sign,
+                       static_cast<float*>(d_vals));
+  NFX_LAUNCH_CHECK();
+  return NFX_OK;
+}
+
+extern "C" int nfx_md_scatter(int n, const void* d_src, const int32_t* d_map, int fp64, void* d_dst, void* stream) {
+  NFX_REQUIRE(n >= 0, "nfx_md_scatter: n < 0");
+  if (n == 0) return NFX_OK;
+  NFX_REQUIRE(d_src && d_map && d_dst, "nfx_md_scatter: null pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (fp64)
+    hipLaunchKernelGGL(k_md_scatter<double>, dim3(grid_for(n, 256)), dim3(256), 0, s, n,
+                       static_cast<const double*>(d_src), d_map, static_cast<double*>(d_dst));
+  else
+    hipLaunchKernelGGL(k_md_scatter<float>, dim3(grid_for(n, 256)), dim3(256), 0, s, n,
+                       static_cast<const float*>(d_src), d_map, static_cast<float*>(d_dst));
+  NFX_LAUNCH_CHECK();
+  return NFX_OK;
+}
+
+extern "C" int nfx_md_update(int n, const int32_t* d_pos, const double* d_v, int mode, int fp64, void* d_vals,
+                             void* stream) {
+  NFX_REQUIRE(n >= 0 && (mode == 0 || mode == 1), "nfx_md_update: bad arguments");
+  if (n == 0) return NFX_OK;
+  NFX_REQUIRE(d_pos && d_v && d_vals, "nfx_md_update: null pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (fp64)
+    hipLaunchKernelGGL(k_md_update<double>, dim3(grid_for(n, 256)), dim3(256), 0, s, n, d_pos, d_v, mode,
+                       static_cast<double*>(d_vals));
+  else
+    hipLaunchKernelGGL(k_md_update<float>, dim3(grid_for(n, 256)), dim3(256), 0, s, n, d_pos, d_v, mode,
+                       static_cast<float*>(d_vals));
+  NFX_LAUNCH_CHECK();
+  return NFX_OK;
+}
+
+extern "C" int nfx_md_rhs(int n, int n_u, const double* d_b0, const double* d_mass, const double* d_x,
+                          double* d_b, void* stream) {
+  NFX_REQUIRE(n >= 0 && n_u >= 0 && n_u <= n, "nfx_md_rhs: sizes");
+  if (n == 0) return NFX_OK;
+  NFX_REQUIRE(d_b0 && d_x && d_b && (n_u == n || d_mass), "nfx_md_rhs: null pointer");
+  hipLaunchKernelGGL(k_md_rhs, dim3(grid_for(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), n, n_u, d_b0,
+                     d_mass, d_x, d_b);
+  NFX_LAUNCH_CHECK();
+  return NFX_OK;
+}
+
+extern "C" int nfx_md_block_diag(int n, int n_u, const int32_t* d_rowptr, const int32_t* d_col, const double* d_vals,
+                                 float* d_pinv, void* stream) {
+  NFX_REQUIRE(n >= 0 && n_u >= 0 && n_u <= n, "nfx_md_block_diag: sizes");
+  if (n == 0) return NFX_OK;
+  NFX_REQUIRE(d_rowptr && d_col && d_vals && d_pinv, "nfx_md_block_diag: null pointer");
+  hipLaunchKernelGGL(k_md_block_diag, dim3(grid_for(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), n, n_u,
+                     d_rowptr, d_col, d_vals, d_pinv);
+  NFX_LAUNCH_CHECK();
+  return NFX_OK;
+}

@@ -194,7 +194,11 @@ def check_coefficient(arg, E, nq, what):
 class DarcySystem:
     """Mixed Darcy RT_k - P_{k-1} on one product space, resident on the GPU."""
 
-    def __init__(self, space, flux="u", potential="p", device="cuda"):
+    def __init__(self, space, flux="u", potential="p", device="cuda", split=None):
+        """split: optional (m, 2) (element index, local flux DoF) pairs whose flux DoF is
+        replaced by a new one of their own (appended after the flux DoFs, pressure DoFs
+        shifted): the facet DoF of a face cut by a fracture, on one side of it, so that
+        the normal flux may jump across the fracture."""
         t = torch()
         self.space = space
         self.device = device
@@ -229,6 +233,22 @@ class DarcySystem:
         X = mesh.points[cells]  # (E, D+1, 3)
         self.xq = np.einsum("qa,eac->eqc", p1_bary(points), X)
         ref_dest, ref_sign = space.reference_destinations()
+        self.n_split = 0
+        if split is not None and len(split):
+            split = np.asarray(split, dtype=np.int64).reshape(-1, 2)
+            if np.any(split[:, 0] < 0) or np.any(split[:, 0] >= self.E) or np.any(split[:, 1] < 0) \
+                    or np.any(split[:, 1] >= self.nb):
+                raise ValueError("split: (element, flux DoF) out of range")
+            if len(np.unique(split[:, 0] * self.nb + split[:, 1])) != len(split):
+                raise ValueError("split: repeated (element, flux DoF)")
+            m, n_u0 = len(split), self.n_u
+            ref_dest = ref_dest.copy()
+            ref_dest[ref_dest >= n_u0] += m
+            ref_dest[split[:, 0], split[:, 1]] = n_u0 + np.arange(m)
+            self.n_u += m
+            self.n += m
+            self.n_split = m
+        self.split = split
         self.ref_dest, self.ref_sign = ref_dest, ref_sign
         self.pattern = build_pattern(self.n, ref_dest, ref_sign)
         pat = self.pattern
