@@ -212,9 +212,13 @@ def bench_md(args, log):
     coupling, backward Euler with dt = 1e-3 from a zero state under boundary pressure 1 - x.
     The coupled system is assembled once on the device (both precisions); a step forms the
     right-hand side from the previous state and solves it with FGMRES + FP64 refinement to
-    1e-8, warm-started, with the diagonal block preconditioner set up once. The exchange is
-    the conservative one: with the reference coupling's sign the loop is unstable (the
-    pressure grew to 1e73 in 200 steps although every solve converged)."""
+    1e-8, warm-started, with the preconditioner set up once (--md-pc: "learned", the two-grid
+    Schur preconditioner on the matrix pressures with the network smoother and a diagonal
+    fallback; "twogrid", the same with Jacobi; "diag", the diagonal block preconditioner).
+    Before the timed loop, --md-compare-steps steps from the zero state with each of the three
+    report ms/step and iterations. The exchange is the conservative one: with the reference
+    coupling's sign the loop is unstable (the pressure grew to 1e73 in 200 steps although
+    every solve converged)."""
     import torch
 
     from nfempy_amd.mixed_dim import FractureNetwork, MixedDimDarcy
@@ -224,7 +228,33 @@ def bench_md(args, log):
     md = MixedDimDarcy(net, dt=1e-3, conservative=not args.md_reference_exchange)
     t_host = time.perf_counter() - t0
     log(f"md: N={md.N} nnz={md.nnz} fracture triangles {len(net.faces)} host set-up {t_host:.2f} s")
-    md.setup()
+    compare = {}
+    for pc_kind in ("diag", "twogrid", "learned"):
+        if args.md_compare_steps <= 0:
+            break
+        tsu = time.perf_counter()
+        md.setup(preconditioner=pc_kind)
+        torch.cuda.synchronize()
+        t_su = time.perf_counter() - tsu
+        x = torch.zeros(md.N, dtype=torch.float64, device="cuda")
+        x, _ = md.step(x)  # warm-up
+        x = torch.zeros(md.N, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        tc = time.perf_counter()
+        its = []
+        fb = 0
+        for _ in range(args.md_compare_steps):
+            x, info = md.step(x)
+            its.append(info["iterations"])
+            fb += int(info.get("fallback", False))
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - tc) * 1e3 / args.md_compare_steps
+        net_used = bool(md.pc.network_used) if md.pc is not None else None
+        compare[pc_kind] = {"ms_per_step": round(ms, 3), "mean_iterations": round(float(np.mean(its)), 1),
+                            "setup_ms": round(t_su * 1e3, 2), "fallbacks": fb, "network_used": net_used}
+        log(f"md compare {pc_kind}: {ms:.3f} ms/step, {np.mean(its):.1f} iterations/step, set-up {t_su * 1e3:.1f} ms, "
+            f"fallbacks {fb}, network used {net_used}")
+    md.setup(preconditioner=args.md_pc)
     torch.cuda.synchronize()
     for _ in range(3):
         md.assemble(False)
@@ -242,11 +272,13 @@ def bench_md(args, log):
     x = torch.zeros(md.N, dtype=torch.float64, device="cuda")
     torch.cuda.synchronize()
     its, res = [], []
+    fbs = 0
     ts = time.perf_counter()
     for k in range(args.md_steps):
         x, info = md.step(x)
         if not info["converged"]:
             raise RuntimeError(f"md step {k} did not converge: {info}")
+        fbs += int(info.get("fallback", False))
         its.append(info["iterations"])
         res.append(info["residual"])
         if (k + 1) % 25 == 0:
@@ -263,6 +295,7 @@ def bench_md(args, log):
                    "matrix_cells": md.m.E, "fracture_triangles": md.f.E, "dofs": md.N, "nnz": md.nnz,
                    "split_dofs": md.m.n_split, "exchange": "reference" if args.md_reference_exchange else "conservative",
                    "host_setup_s": round(t_host, 2)},
+        "preconditioner": args.md_pc, "pc_comparison": compare, "fallbacks": fbs,
         "assembly_ms": round(ms_asm, 4), "mean_iterations": round(float(np.mean(its)), 1),
         "max_iterations": int(np.max(its)), "max_residual": float(np.max(res)),
         "final_pressure_range": [float(p.min()), float(p.max())],
@@ -370,6 +403,8 @@ def main():
     ap.add_argument("--md", action="store_true",
                     help="run the mixed-dimensional transient workload instead (24^3 x 6 matrix, 8 fractures)")
     ap.add_argument("--md-steps", type=int, default=200)
+    ap.add_argument("--md-pc", choices=("learned", "twogrid", "diag"), default="learned")
+    ap.add_argument("--md-compare-steps", type=int, default=10)
     ap.add_argument("--md-reference-exchange", action="store_true",
                     help="the reference coupling's exchange sign (unstable in time: the state grows without bound)")
     ap.add_argument("--calibrate-copy", action="store_true",

@@ -337,7 +337,15 @@ int nfx_spd_inverse_small(int n, float* d_A, int lda, void* stream);
  * with W = (Wd, Wo) and S = (sd, so) in ELL form (nbr), the aggregates as CSR
  * (agg_ptr, agg_mem) plus agg_of (cell -> aggregate), and the coarse inverse either
  * FP32 (Ainv) or row-scaled FP16 (Ainv16, Ascale; used when Ainv16 is not NULL).
- * work: 3 n_cells + 2 n_agg floats. */
+ * Rows after the cell block, n_tail of them (the fracture pressures of the
+ * mixed-dimensional system; n_tail = 0 for none), are scaled by tail_dinv, or, when
+ * slot_tail is not NULL, eliminated by block LU with the cells: cell p's ELL slot j
+ * couples to tail row slot_tail[4p+j] (-1 none) with coefficient slot_coef[4p+j] =
+ * S_pf / S_ff, tail row f to cells tail_cells[2f+k] (-1 none) with tail_coef[2f+k] = S_fp,
+ * and tail_dinv = 1 / S_ff; then r_p <- r_p - sum slot_coef r_f before the cycle (S in
+ * ELL form then holds the cells' Schur complement S - S_pf S_ff^-1 S_fq) and
+ * z_f = tail_dinv (r_f - sum tail_coef z_p) after it.
+ * work: 3 n_cells + 2 n_agg floats (4 n_cells + 2 n_agg with slot_tail). */
 typedef struct nfx_twogrid_pc {
   int n_u, n_cells, n_agg;
   const int32_t* nbr;
@@ -354,6 +362,12 @@ typedef struct nfx_twogrid_pc {
   const float* dMinv;
   float psign;
   float* work;
+  int n_tail;
+  const float* tail_dinv;
+  const int32_t* slot_tail;
+  const float* slot_coef;
+  const int32_t* tail_cells;
+  const float* tail_coef;
 } nfx_twogrid_pc;
 int nfx_twogrid_apply(const nfx_twogrid_pc* pc, const float* d_r, float* d_z, void* stream);
 /* C = beta C + alpha op(A) op(B) (FP32, row-major, leading dimensions lda/ldb/ldc;

@@ -116,7 +116,10 @@ class TwoGridPC(ctypes.Structure):
     _fields_ = [("n_u", c_int), ("n_cells", c_int), ("n_agg", c_int), ("nbr", c_void_p), ("Wd", c_void_p),
                 ("Wo", c_void_p), ("sd", c_void_p), ("so", c_void_p), ("agg_ptr", c_void_p),
                 ("agg_mem", c_void_p), ("agg_of", c_void_p), ("Ainv", c_void_p), ("Ainv16", c_void_p),
-                ("Ascale", c_void_p), ("dMinv", c_void_p), ("psign", c_float), ("work", c_void_p)]
+                ("Ascale", c_void_p), ("dMinv", c_void_p), ("psign", c_float), ("work", c_void_p),
+                ("n_tail", c_int), ("tail_dinv", c_void_p),
+                ("slot_tail", c_void_p), ("slot_coef", c_void_p), ("tail_cells", c_void_p),
+                ("tail_coef", c_void_p)]
 
 
 # entry points whose return value is a count, not a status

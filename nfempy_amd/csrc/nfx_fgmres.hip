@@ -396,7 +396,8 @@ extern "C" int nfx_solve_fgmres_ir(nfx_fgmres* s, const int32_t* d_rowptr, const
   NFX_REQUIRE(s && d_rowptr && d_col && d_val32 && d_val64 && d_b && d_x, "nfx_solve_fgmres_ir: null pointer");
   NFX_REQUIRE(max_inner > 0 && max_outer >= 0 && check_every > 0, "nfx_solve_fgmres_ir: iteration limits");
   if (pc) {
-    NFX_REQUIRE(pc->n_u + pc->n_cells == s->n, "nfx_solve_fgmres_ir: preconditioner size differs from the matrix");
+    NFX_REQUIRE(pc->n_u + pc->n_cells + pc->n_tail == s->n,
+                "nfx_solve_fgmres_ir: preconditioner size differs from the matrix");
     NFX_REQUIRE(pc->work && pc->Ainv && pc->nbr, "nfx_solve_fgmres_ir: incomplete two-grid preconditioner");
   }
   const int n = s->n;

@@ -839,7 +839,7 @@ __global__ void k_tg_post2(int n_u, int E, const int32_t* __restrict__ nbr, cons
                            const float* __restrict__ Wo, const float* __restrict__ x2,
                            const float* __restrict__ t2, const float* __restrict__ dMinv,
                            const float* __restrict__ r, float psign, const double* __restrict__ dscale,
-                           float* __restrict__ z) {
+                           int n_tail, const float* __restrict__ tail_dinv, float* __restrict__ z) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   float sc = 1.f;
   if (dscale) {
@@ -851,8 +851,49 @@ __global__ void k_tg_post2(int n_u, int E, const int32_t* __restrict__ nbr, cons
     return;
   }
   const int64_t p = i - n_u;
-  if (p >= E) return;
+  if (p >= E) {
+    if (p - E < n_tail) z[i] = sc * tail_dinv[p - E] * r[i];
+    return;
+  }
   z[i] = sc * psign * (x2[p] + ell_row(int(p), nbr, Wd, Wo, t2));
+}
+
+// Block LU with a diagonal tail block, before the cycle: rq_p = r_p - sum_j c_pj r_f(p,j)
+// over the slots of cell p that couple to a tail row.
+__global__ void k_tg_tail_pre(int E, int n_u, const int32_t* __restrict__ slot_tail,
+                              const float* __restrict__ slot_coef, const float* __restrict__ r,
+                              float* __restrict__ rq) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= E) return;
+  const float* rf = r + (int64_t)n_u + E;
+  float v = r[(int64_t)n_u + p];
+  for (int j = 0; j < 4; ++j) {
+    const int f = slot_tail[4 * p + j];
+    if (f >= 0) v -= slot_coef[4 * p + j] * rf[f];
+  }
+  rq[p] = v;
+}
+
+// ... and after it: z_f = dinv_f (sc r_f - sum_k S_fp(k) z_p(k)) (z of the cells already
+// scaled by sc = 1 / dscale)
+__global__ void k_tg_tail_post(int n_tail, int n_u, int E, const int32_t* __restrict__ tail_cells,
+                               const float* __restrict__ tail_coef, const float* __restrict__ tail_dinv,
+                               const float* __restrict__ r, const double* __restrict__ dscale,
+                               float* __restrict__ z) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n_tail) return;
+  float sc = 1.f;
+  if (dscale) {
+    const double d = *dscale;
+    sc = d > 0 ? (float)(1.0 / d) : 0.f;
+  }
+  const int64_t i = (int64_t)n_u + E + f;
+  float v = sc * r[i];
+  for (int k = 0; k < 2; ++k) {
+    const int c = tail_cells[2 * f + k];
+    if (c >= 0) v -= tail_coef[2 * f + k] * z[(int64_t)n_u + c];
+  }
+  z[i] = tail_dinv[f] * v;
 }
 
 // Row-scaled FP16 copy of a dense row-major matrix: scale[r] = max |A[r][:]|,
@@ -1092,17 +1133,27 @@ int nfx::twogrid_apply_scaled(const nfx_twogrid_pc* pc, const float* r, float* z
                               hipStream_t st) {
   NFX_REQUIRE(pc, "nfx_twogrid_apply: null descriptor");
   const int n_u = pc->n_u, E = pc->n_cells, n_agg = pc->n_agg;
-  NFX_REQUIRE(n_u >= 0 && E > 0 && n_agg > 0, "nfx_twogrid_apply: bad sizes");
+  NFX_REQUIRE(n_u >= 0 && E > 0 && n_agg > 0 && pc->n_tail >= 0, "nfx_twogrid_apply: bad sizes");
+  NFX_REQUIRE(pc->n_tail == 0 || pc->tail_dinv, "nfx_twogrid_apply: null tail diagonal");
   NFX_REQUIRE(pc->nbr && pc->Wd && pc->Wo && pc->sd && pc->so && pc->agg_ptr && pc->agg_mem && pc->agg_of &&
                   (pc->Ainv || (pc->Ainv16 && pc->Ascale)) && r && z && pc->work && (n_u == 0 || pc->dMinv),
               "nfx_twogrid_apply: null pointer");
-  // work: x1[E] | x2[E] | t2[E] | c[n_agg] | y[n_agg]
+  const bool lu = pc->slot_tail != nullptr;
+  NFX_REQUIRE(!lu || (pc->n_tail > 0 && pc->slot_coef && pc->tail_cells && pc->tail_coef),
+              "nfx_twogrid_apply: incomplete tail coupling");
+  // work: x1[E] | x2[E] | t2[E] | c[n_agg] | y[n_agg] (| rq[E] with the tail coupling)
   float* x1 = pc->work;
   float* x2 = x1 + E;
   float* t2 = x1 + 2 * (int64_t)E;
   float* c = x1 + 3 * (int64_t)E;
   float* y = c + n_agg;
   const float* rp = r + n_u;
+  if (lu) {
+    float* rq = y + n_agg;
+    hipLaunchKernelGGL(k_tg_tail_pre, dim3(grid_for(E, 256)), dim3(256), 0, st, E, n_u, pc->slot_tail,
+                       pc->slot_coef, r, rq);
+    rp = rq;
+  }
   hipLaunchKernelGGL(k_tg_pre, dim3(grid_for(E, 256)), dim3(256), 0, st, E, pc->nbr, pc->Wd, pc->Wo, rp, x1);
   hipLaunchKernelGGL(k_tg_restrict, dim3(grid_for((int64_t)n_agg * 64, 256)), dim3(256), 0, st, n_agg, pc->agg_ptr,
                      pc->agg_mem, pc->nbr, pc->sd, pc->so, rp, x1, c);
@@ -1113,8 +1164,13 @@ int nfx::twogrid_apply_scaled(const nfx_twogrid_pc* pc, const float* r, float* z
     hipLaunchKernelGGL(k_gemv, dim3(grid_for((int64_t)n_agg * 64, 256)), dim3(256), 0, st, n_agg, pc->Ainv, c, y);
   hipLaunchKernelGGL(k_tg_post1, dim3(grid_for(E, 256)), dim3(256), 0, st, E, pc->nbr, pc->agg_of, pc->sd, pc->so,
                      rp, x1, y, x2, t2);
-  hipLaunchKernelGGL(k_tg_post2, dim3(grid_for((int64_t)n_u + E, 256)), dim3(256), 0, st, n_u, E, pc->nbr, pc->Wd,
-                     pc->Wo, x2, t2, pc->dMinv, r, pc->psign, dscale, z);
+  const int n_diag_tail = lu ? 0 : pc->n_tail;
+  hipLaunchKernelGGL(k_tg_post2, dim3(grid_for((int64_t)n_u + E + n_diag_tail, 256)), dim3(256), 0, st, n_u, E,
+                     pc->nbr, pc->Wd, pc->Wo, x2, t2, pc->dMinv, r, pc->psign, dscale, n_diag_tail, pc->tail_dinv,
+                     z);
+  if (lu)
+    hipLaunchKernelGGL(k_tg_tail_post, dim3(grid_for(pc->n_tail, 256)), dim3(256), 0, st, pc->n_tail, n_u, E,
+                       pc->tail_cells, pc->tail_coef, pc->tail_dinv, r, dscale, z);
   NFX_LAUNCH_CHECK();
   return NFX_OK;
 }

@@ -91,7 +91,13 @@ def default_weights_path():
 class SchurGraph:
     """Host-side structure of S and the aggregates, once per Darcy system pattern."""
 
-    def __init__(self, sysd, agg_h=None, device="cuda"):
+    def __init__(self, sysd, agg_h=None, device="cuda", links=None, pos_map=None):
+        """links: optional (dofs_a, dofs_b), pairs of flux DoFs of sysd that each belong to
+        one cell (split faces); the two cells become ELL neighbours in the slots of those
+        DoFs (their S entries are the caller's: the Schur ELL gives 0 there), recorded in
+        link_slots = (cell_a, slot_a, cell_b, slot_b). pos_map: maps positions in sysd's
+        pattern to positions in a larger value array (a coupled system) that the Schur ELL
+        then reads."""
         if sysd.k != 1 or sysd.dim != 3:
             raise NotImplementedError("the learned Schur preconditioner covers 3-D RT1-P0 systems")
         pat = sysd.pattern
@@ -134,6 +140,26 @@ class SchurGraph:
         pos_m = np.where(has, pos_diag[np.where(has, face, 0)], -1)
         if (pos_m[has] < 0).any():
             raise ValueError("flux diagonal missing from the pattern")
+        self.link_slots = None
+        if links is not None:
+            da, db = (np.asarray(d, dtype=np.int64) for d in links)
+            cnt = np.bincount(face[has], minlength=n_u)
+            owner = -np.ones((n_u, 2), dtype=np.int64)
+            fr, fj = np.nonzero(has)
+            owner[face[fr, fj]] = np.stack([fr, fj], 1)
+            if len(da) != len(db) or (cnt[da] != 1).any() or (cnt[db] != 1).any():
+                raise ValueError("linked flux DoFs must each belong to exactly one cell")
+            ca, ja = owner[da, 0], owner[da, 1]
+            cb, jb = owner[db, 0], owner[db, 1]
+            if (nbr[ca, ja] >= 0).any() or (nbr[cb, jb] >= 0).any() or (ca == cb).any():
+                raise ValueError("linked slot already has a neighbour")
+            nbr[ca, ja] = cb
+            nbr[cb, jb] = ca
+            self.link_slots = (ca, ja, cb, jb)
+        if pos_map is not None:
+            pos_map = np.asarray(pos_map, dtype=np.int64)
+            remap = lambda a: np.where(a >= 0, pos_map[np.maximum(a, 0)], -1)
+            pos_b, pos_qb, pos_m, pos_diag = remap(pos_b), remap(pos_qb), remap(pos_m), remap(pos_diag)
         self.nbr_np = nbr
         # aggregates: boxes of side agg_h over the cell centroids
         pts = sysd.u_sp.mesh.points
@@ -146,7 +172,7 @@ class SchurGraph:
         self.agg_h = agg_h
         agg, n_agg = aggregate_cells(cent, agg_h)
         # cells of the pressure block follow the element order (P0: one dof per cell)
-        p_rows = sysd.space.reference_destinations()[0][:, sysd.nb:].ravel() - n_u
+        p_rows = sysd.ref_dest[:, sysd.nb:].ravel() - n_u
         if not np.array_equal(np.sort(p_rows), np.arange(E)):
             raise ValueError("pressure dofs are not one per cell")
         agg_of = np.empty(E, dtype=np.int64)
@@ -271,7 +297,11 @@ def spd_inverse_(A, plan=None, stream=None, o=0, gemm="nfx"):
 class LearnedSchurPC:
     """Two-grid Schur block preconditioner with the network smoother (callable z = M^-1 r)."""
 
-    def __init__(self, sysd, theta=None, agg_h=None, psign=-1.0, graph=None, coarse_fp16=True):
+    def __init__(self, sysd, theta=None, agg_h=None, psign=-1.0, graph=None, coarse_fp16=True, extra_diag=None,
+                 extra_off=None):
+        """extra_diag / extra_off: optional FP64 (n_cells) / (n_cells * 4) added to S (diagonal,
+        ELL off-diagonals) before the network inputs and the coarse operator are formed (a
+        storage term; the Schur complement of an eliminated block)."""
         t = torch()
         self.sys = sysd
         self.g = graph if graph is not None else SchurGraph(sysd, agg_h, device=sysd.device)
@@ -285,7 +315,7 @@ class LearnedSchurPC:
         self.a, self.x = f32(E * 4), f32(E * F_IN)
         self.sd32, self.so32 = f32(E), f32(E * 4)
         self.Wd, self.Wo = f32(E), f32(E * 4)
-        self.work = f32(3 * E + 2 * n_agg)
+        self.work = f32(4 * E + 2 * n_agg)
         self.Ainv = t.empty((n_agg, n_agg), dtype=t.float32, device=dev)
         self._Ac64 = t.empty((n_agg, n_agg), dtype=t.float64, device=dev)
         self.coarse_fp16 = bool(coarse_fp16)
@@ -294,6 +324,14 @@ class LearnedSchurPC:
         self._coarse_graph = None
         self.dMinv = f32(g.n_u)
         self.psign = float(psign)
+        self.extra_diag = extra_diag
+        self.extra_off = extra_off
+        # descriptor overrides for a larger system: flux diagonal of all flux rows, a
+        # diagonal block of rows after the cells, and its block-LU coupling to the cells
+        # (slot_tail, slot_coef, tail_cells, tail_coef; see nfx_twogrid_pc)
+        self.flux_dinv = None
+        self.tail_dinv = None
+        self.tail_lu = None
         self.h = self.m = self.z = None
         self.n_params = nat.call("nfx_gnn_num_params")
         if self.n_params != param_layout()[1]:
@@ -327,6 +365,10 @@ class LearnedSchurPC:
         g, s = self.g, nat.stream_handle(stream)
         nat.call("nfx_schur_ell", g.E, nat.ptr(g.pos_b), nat.ptr(g.pos_qb), nat.ptr(g.pos_m), nat.ptr(vals64),
                  nat.ptr(self.sdiag), nat.ptr(self.soff), s)
+        if self.extra_diag is not None:
+            self.sdiag.add_(self.extra_diag)
+        if self.extra_off is not None:
+            self.soff.add_(self.extra_off)
         nat.call("nfx_gnn_features", g.E, nat.ptr(g.nbr), nat.ptr(self.sdiag), nat.ptr(self.soff), nat.ptr(self.a),
                  nat.ptr(self.x), nat.ptr(self.sd32), nat.ptr(self.so32), s)
         self.dMinv.copy_(1.0 / vals64[self.g.diag_pos_u].abs())
@@ -406,10 +448,21 @@ class LearnedSchurPC:
     def desc(self):
         """The ``nfx_twogrid_pc`` descriptor of this preconditioner (for the C++ FGMRES)."""
         g = self.g
-        return nat.TwoGridPC(g.n_u, g.E, g.n_agg, nat.ptr(g.nbr), nat.ptr(self.Wd), nat.ptr(self.Wo),
+        dminv = self.dMinv if self.flux_dinv is None else self.flux_dinv
+        tail = self.tail_dinv
+        lu = self.tail_lu if tail is not None else None
+        if lu is not None:
+            st, sc, tc, tk = lu
+            if st.numel() != 4 * g.E or sc.numel() != 4 * g.E or tc.numel() != 2 * tail.numel() \
+                    or tk.numel() != 2 * tail.numel():
+                raise ValueError("tail coupling arrays do not match the cell and tail counts")
+        return nat.TwoGridPC(int(dminv.numel()) if self.flux_dinv is not None else g.n_u, g.E, g.n_agg,
+                             nat.ptr(g.nbr), nat.ptr(self.Wd), nat.ptr(self.Wo),
                              nat.ptr(self.sd32), nat.ptr(self.so32), nat.ptr(g.agg_ptr), nat.ptr(g.agg_mem),
                              nat.ptr(g.agg_of), nat.ptr(self.Ainv), nat.ptr(self.Ainv16), nat.ptr(self.Ascale),
-                             nat.ptr(self.dMinv), self.psign, nat.ptr(self.work))
+                             nat.ptr(dminv), self.psign, nat.ptr(self.work), 0 if tail is None else int(tail.numel()),
+                             None if tail is None else nat.ptr(tail),
+                             *(nat.ptr(a) for a in lu) if lu is not None else (None,) * 4)
 
     def smoother_numpy(self):
         """(Wd, Wo (E,4)) on the host (tests)."""

@@ -271,6 +271,13 @@ class MixedDimDarcy:
         self.mass_np = mass
         pr = np.arange(self.n_u, N)
         pos_d = np.searchsorted(keys, pr * N + pr)
+        # signed Schur diagonal of the pressure rows, a_kk - sum_j a_kj a_jk / a_jj over flux
+        # columns j: positions of a_kj, a_jk and a_jj
+        kj = np.flatnonzero((rows >= self.n_u) & (cols < self.n_u))
+        jk = np.searchsorted(keys, cols[kj] * N + rows[kj])
+        jj = np.searchsorted(keys, cols[kj] * N + cols[kj])
+        if np.any(jk >= len(keys)) or np.any(keys[np.minimum(jk, len(keys) - 1)] != cols[kj] * N + rows[kj]):
+            raise RuntimeError("coupled pattern is not structurally symmetric")
         # fracture-tip flux DoFs: u = 0 (row -> identity, column -> 0)
         tips = np.unique(gf(f.bc_rows.cpu().numpy().reshape(-1) & nat.SLOT_INDEX_MASK)) if f.F else np.zeros(0, np.int64)
         self.tips = tips
@@ -285,6 +292,26 @@ class MixedDimDarcy:
         self.vals64 = t.zeros(self.nnz, dtype=t.float64, device=device)
         self._pos_m, self._pos_f, self._pos_c = dv(pos_m), dv(pos_f), dv(pos_c)
         self._pos_d, self._mass_d = dv(pos_d), dv(mass, np.float64)
+        # block-LU elimination of the fracture pressures in the preconditioner: for face i,
+        # the matrix cells on its two sides (pressure rows a, b), their facet DoFs u0, u1
+        # and the fracture pressure pf; positions of a_{a,u0}, a_{u0,pf}, a_{u0,u0},
+        # a_{pf,u0}, a_{u0,a} (and the same on the negative side)
+        def at(r, c):
+            k = np.asarray(r, np.int64) * N + np.asarray(c, np.int64)
+            i = np.searchsorted(keys, k)
+            if np.any(i >= len(keys)) or np.any(keys[np.minimum(i, len(keys) - 1)] != k):
+                raise RuntimeError("coupling entry missing from the coupled pattern")
+            return i
+
+        ra, rb = gm(m.ref_dest[pos_e, m.nb]), gm(m.ref_dest[neg_e, m.nb])
+        lk = [at(ra, u0), at(u0, pf), at(u0, u0), at(pf, u0), at(u0, ra),
+              at(rb, u1), at(u1, pf), at(u1, u1), at(pf, u1), at(u1, rb)]
+        self._lk = [dv(a, np.int64) for a in lk]
+        self._link_dofs = (m.ref_dest[pos_e, dof_p], m.ref_dest[neg_e, dof_n])
+        self._link_rows = (ra - self.n_u, rb - self.n_u, pf - self.n_u)
+        self._pos_m_np = pos_m
+        self._sd_row = to_dev(rows[kj] - self.n_u, device=device)
+        self._sd_kj, self._sd_jk, self._sd_jj = (to_dev(a, device=device) for a in (kj, jk, jj))
         self._pos_tip, self._tip_v = dv(hit), dv(tip_v, np.float64)
         self._map_m, self._map_f = dv(gm(np.arange(m.n))), dv(gf(np.arange(f.n)))
         self._side_elem = dv(np.stack([pos_e, neg_e], 1))
@@ -345,17 +372,129 @@ class MixedDimDarcy:
                 self.b0[to_dev(self.tips, device=self.device)] = 0.0
         return vals
 
-    def setup(self, restart=60):
-        """Assemble both precisions and set up the preconditioner and the solver."""
-        from .learned import Fgmres
+    def setup(self, restart=60, preconditioner="learned"):
+        """Assemble both precisions and set up the preconditioner and the solver, once.
+
+        preconditioner: "diag" (the diagonal block preconditioner), "twogrid" (the two-grid
+        Schur preconditioner on the matrix pressures, storage term included in S, Jacobi
+        smoother) or "learned" (the same with the trained network smoother; it falls back to
+        Jacobi when the network output is not finite); see _coupled_pc."""
+        from .learned import Fgmres, default_weights_path, load_params
 
         self.assemble(True)
         self.assemble(False)
         nat.call("nfx_md_block_diag", self.N, self.n_u, nat.ptr(self.rowptr), nat.ptr(self.col),
                  nat.ptr(self.vals64), nat.ptr(self.pinv), nat.stream_handle())
+        self.pc = None
+        if preconditioner in ("twogrid", "learned"):
+            theta = load_params(default_weights_path()) if preconditioner == "learned" else None
+            self.pc = self._coupled_pc(theta)
+        elif preconditioner != "diag":
+            raise ValueError(f"unknown preconditioner {preconditioner!r}")
         if self._solver is None:
             self._solver = Fgmres(self.N, restart)
         return self
+
+    def schur_diag(self):
+        """a_kk - sum_j a_kj a_jk / a_jj over the pressure rows (FP64, global pressure order),
+        from the FP64 values: the pressure Schur diagonal with the flux block replaced by its
+        diagonal."""
+        v = self.vals64
+        s = v[self._pos_d].clone()
+        s.index_add_(0, self._sd_row, -(v[self._sd_kj] * v[self._sd_jk] / v[self._sd_jj]))
+        return s
+
+    def schur_diag_inverse(self):
+        """1 / schur_diag() in FP32 (1 where it vanishes)."""
+        t = torch()
+        s = self.schur_diag()
+        return t.where(s != 0, 1.0 / s, t.ones_like(s)).float()
+
+    def _coupled_pc(self, theta):
+        """Two-grid preconditioner of the coupled system. Flux rows: 1/|a_kk|. Pressures: with
+        the flux block replaced by its diagonal D, the pressure Schur complement
+        [[S_mm, S_mf], [S_fm, S_ff]] over matrix cells m and fracture cells f. S_ff is taken
+        as its diagonal (the tangential fracture conduction, ~delta^3, is dropped: a
+        fracture cell then couples only to its two side cells), and the fracture pressures
+        are eliminated by one block-LU step: the cells see S_mm - S_mf S_ff^-1 S_fm, an ELL
+        matrix in which
+        the two sides of a fracture face are neighbours through the face's slot, solved by
+        the two-grid cycle; the fracture pressures follow by block back-substitution.
+        S_mm is the Schur ELL of the matrix block read from the coupled values (so D
+        includes the interface terms on the facet DoFs) plus the storage. The flux blocks
+        of the matrix system are antisymmetric (a_up = -a_pu^T), so S_mm = C + B D^-1 B^T is
+        positive and the two-grid sign psign is +1. With the reference exchange sign the
+        coupling is not antisymmetric (S_mf != S_fm^T, S_ff of either sign): there the
+        fracture pressures keep the diagonal 1 / S_ff and the cells S_mm + C (no
+        elimination, no links)."""
+        from .learned import LearnedSchurPC
+
+        t = self.coupled_pc_terms()
+        pc = LearnedSchurPC(self.m, theta, psign=1.0, graph=self._schur_graph, extra_diag=t["extra_diag"],
+                            extra_off=t["extra_off"])
+        pc.setup(vals64=self.vals64, network=theta is not None)
+        pc.flux_dinv = self.pinv[: self.n_u].contiguous()
+        pc.tail_dinv = t["tail_dinv"]
+        if "slot_tail" in t:
+            pc.tail_lu = (t["slot_tail"], t["slot_coef"], t["tail_cells"], t["tail_coef"])
+        return pc
+
+    def coupled_pc_terms(self):
+        """The host-built parts of the coupled preconditioner from the FP64 values (no
+        library call): extra_diag / extra_off (added to the Schur ELL of the cells),
+        tail_dinv, and with the conservative exchange the block-LU coupling slot_tail,
+        slot_coef (n_cells * 4), tail_cells, tail_coef (n_tail * 2)."""
+        from .learned import SchurGraph
+
+        t = torch()
+        m = self.m
+        E = m.n - m.n_u
+        lu = self.exchange_sign < 0
+        if getattr(self, "_schur_graph", None) is None:
+            self._schur_graph = SchurGraph(m, device=self.device, links=self._link_dofs if lu else None,
+                                           pos_map=self._pos_m_np)
+        g = self._schur_graph
+        v = self.vals64
+        sd = self.schur_diag()
+        out = {"tail_dinv": t.where(sd[E:] != 0, 1.0 / sd[E:], t.ones_like(sd[E:])).float().contiguous(),
+               "extra_diag": to_dev(self.mass_np[:E], device=self.device), "extra_off": None}
+        if not lu:
+            return out
+        lk = [v[i] for i in self._lk]
+        s_af = -lk[0] * lk[1] / lk[2]
+        s_fa = -lk[3] * lk[4] / lk[2]
+        s_bf = -lk[5] * lk[6] / lk[7]
+        s_fb = -lk[8] * lk[9] / lk[7]
+        ca, cb, fi = (to_dev(a, device=self.device) for a in self._link_rows)
+        fi = fi - E
+        s_ff = sd[fi + E]
+        extra_diag = out["extra_diag"]
+        extra_diag.index_add_(0, ca, -s_af * s_fa / s_ff)
+        extra_diag.index_add_(0, cb, -s_bf * s_fb / s_ff)
+        la, ja, lb, jb = (to_dev(a, device=self.device) for a in g.link_slots)
+        sa, sb = la * 4 + ja, lb * 4 + jb
+        extra_off = t.zeros(4 * E, dtype=t.float64, device=self.device)
+        extra_off.index_add_(0, sa, -s_af * s_fb / s_ff)
+        extra_off.index_add_(0, sb, -s_bf * s_fa / s_ff)
+        n_tail = self.N - self.n_u - E
+        slot_tail = t.full((4 * E,), -1, dtype=t.int32, device=self.device)
+        slot_coef = t.zeros(4 * E, dtype=t.float32, device=self.device)
+        slot_tail[sa] = fi.int()
+        slot_tail[sb] = fi.int()
+        slot_coef[sa] = (s_af / s_ff).float()
+        slot_coef[sb] = (s_bf / s_ff).float()
+        tail_cells = t.full((n_tail, 2), -1, dtype=t.int32, device=self.device)
+        tail_coef = t.zeros((n_tail, 2), dtype=t.float32, device=self.device)
+        tail_cells[fi, 0] = ca.int()
+        tail_cells[fi, 1] = cb.int()
+        tail_coef[fi, 0] = s_fa.float()
+        tail_coef[fi, 1] = s_fb.float()
+        # indices the kernels follow: cells in [0, E), tail rows in [0, n_tail)
+        if int(fi.min()) < 0 or int(fi.max()) >= n_tail or int(t.maximum(ca, cb).max()) >= E:
+            raise RuntimeError("fracture link indices out of range")
+        out.update(extra_off=extra_off, slot_tail=slot_tail, slot_coef=slot_coef,
+                   tail_cells=tail_cells.reshape(-1), tail_coef=tail_coef.reshape(-1))
+        return out
 
     def rhs(self, x_prev, out=None):
         t = torch()
@@ -364,12 +503,31 @@ class MixedDimDarcy:
                  nat.stream_handle())
         return b
 
-    def step(self, x_prev, rtol=1e-8, inner_rtol=1e-5, max_inner=20000):
-        """One backward-Euler step from state x_prev (FP64, global order); returns (x, info)."""
+    def step(self, x_prev, rtol=1e-8, inner_rtol=1e-5, max_inner=20000, pc_max_inner=2000):
+        """One backward-Euler step from state x_prev (FP64, global order); returns (x, info).
+
+        With a two-grid preconditioner set up, a solve that has not converged after
+        pc_max_inner iterations continues from its iterate with the diagonal block
+        preconditioner (info["fallback"] = True)."""
         b = self.rhs(x_prev)
         x = x_prev.clone()
+        pc = getattr(self, "pc", None)
+        if pc is not None:
+            x, info = self._solver.solve(self.rowptr, self.col, self.vals32, self.vals64, b, x=x, pc=pc,
+                                         rtol=rtol, inner_rtol=inner_rtol, max_inner=pc_max_inner, reorth=True,
+                                         check=4)
+            info["fallback"] = False
+            if info["converged"]:
+                return x, info
+            its = info["iterations"]
+            if not bool(torch().isfinite(x).all()):
+                x = x_prev.clone()
+        else:
+            its = 0
         x, info = self._solver.solve(self.rowptr, self.col, self.vals32, self.vals64, b, x=x, pinv=self.pinv,
                                      rtol=rtol, inner_rtol=inner_rtol, max_inner=max_inner, reorth=True, check=4)
+        info["fallback"] = pc is not None
+        info["iterations"] += its
         return x, info
 
     def run(self, steps, x0=None, rtol=1e-8, callback=None):

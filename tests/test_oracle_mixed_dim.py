@@ -147,3 +147,110 @@ def test_fracture_element_is_intrinsic():
     # the mass block is symmetric positive definite
     M = J_flat[:3, :3]
     assert np.allclose(M, M.T) and np.linalg.eigvalsh(M).min() > 0
+
+
+def test_signed_schur_diagonal_positions(case):
+    """The pressure rows' signed Schur diagonal a_kk - sum_j a_kj a_jk / a_jj (the fracture
+    pressure block of the coupled preconditioner) from the host-side positions, on the
+    oracle's values in the coupled pattern, against the same quantity from scipy's blocks;
+    conservative exchange: positive everywhere (storage plus B M^-1 B^T)."""
+    import torch
+    import oracle.mixed_dim as omd
+    from nfempy_amd.mixed_dim import MixedDimDarcy
+
+    net, fo, normals = case[:3]
+    for sign, conservative in ((1, False), (-1, True)):
+        A, b0, mass, n_u, sz = omd.assemble_md(6, net.faces + fo, normals, net.aperture, kappa_normal=2.0,
+                                               exchange_sign=sign)
+        A = A.tocsr()
+        md = MixedDimDarcy(net, device="cpu", kappa_normal=2.0, conservative=conservative)
+        rows = np.repeat(np.arange(md.N), np.diff(md.rowptr_np))
+        md.vals64 = torch.as_tensor(np.asarray(A[rows, md.col_np]).ravel())
+        got = md.schur_diag_inverse().double().numpy()
+        Auu, Aup, Apu, App = A[:n_u, :n_u], A[:n_u, n_u:], A[n_u:, :n_u], A[n_u:, n_u:]
+        S = App.diagonal() - np.asarray((Apu.multiply((Aup / Auu.diagonal()[:, None]).T)).sum(axis=1)).ravel()
+        assert np.allclose(got, 1.0 / S, rtol=1e-6, atol=0)
+        if conservative:
+            assert S.min() > 0
+
+
+def test_schur_graph_of_split_matrix(case):
+    """The two-grid graph of the matrix system with split fracture faces: one pressure row
+    per cell, symmetric neighbours, and no neighbour link across a fracture face (the two
+    sides' flux DoFs are separate, flow passes through the fracture)."""
+    from nfempy_amd.learned import SchurGraph
+    from nfempy_amd.mixed_dim import MixedDimDarcy
+
+    net = case[0]
+    md = MixedDimDarcy(net, device="cpu", kappa_normal=2.0)
+    g = SchurGraph(md.m, device="cpu")
+    nbr = g.nbr_np
+    E = md.m.n - md.m.n_u
+    assert g.E == E and nbr.shape == (E, 4)
+    has = nbr >= 0
+    p, j = np.nonzero(has)
+    assert all(p_ in nbr[q] for p_, q in zip(p, nbr[p, j]))
+    cell_row = md.m.ref_dest[:, md.m.nb] - md.m.n_u
+    e_of_tet = {c: e for e, c in enumerate(md.m.u_sp.topology.cell_index)}
+    for a, b in zip(net.pos_side[:, 0], net.neg_side[:, 0]):
+        ra, rb = cell_row[e_of_tet[a]], cell_row[e_of_tet[b]]
+        assert rb not in nbr[ra] and ra not in nbr[rb]
+    # an interior face away from the fractures still links its two cells
+    assert has.sum() > 0.9 * 4 * E - 2 * 6 * 6 * 6 * 2
+
+
+def test_coupled_preconditioner_elimination(case):
+    """The cells' operator of the coupled preconditioner (Schur ELL read from the coupled
+    values through the remapped positions, plus storage and the eliminated fracture
+    pressures) against the same reduction done densely with scipy on the oracle's matrix:
+    P = A_pp - A_pu diag(A_uu)^-1 A_up split into cells m and fracture cells f,
+    S' = P_mm - P_mf diag(P_ff)^-1 P_fm; and the block-LU coefficients P_mf / P_ff, P_fm."""
+    import scipy.sparse as sps
+    import torch
+    import oracle.mixed_dim as omd
+    from nfempy_amd.mixed_dim import MixedDimDarcy
+
+    net, fo, normals = case[:3]
+    A, b0, mass, n_u, sz = omd.assemble_md(6, net.faces + fo, normals, net.aperture, kappa_normal=2.0,
+                                           exchange_sign=-1)
+    A = A.tocsr()
+    md = MixedDimDarcy(net, device="cpu", kappa_normal=2.0, conservative=True)
+    rows = np.repeat(np.arange(md.N), np.diff(md.rowptr_np))
+    vals = np.asarray(A[rows, md.col_np]).ravel()
+    md.vals64 = torch.as_tensor(vals)
+    terms = md.coupled_pc_terms()
+    g = md._schur_graph
+    E = md.m.n - md.m.n_u
+    # the Schur ELL (the library's nfx_schur_ell) restated on the host
+    pb, pq, pm = (a.numpy().reshape(E, 4) for a in (g.pos_b, g.pos_qb, g.pos_m))
+    has = (pb >= 0) & (pm >= 0)
+    bv = np.where(has, vals[np.maximum(pb, 0)], 0.0)
+    mv = np.where(has, vals[np.maximum(pm, 0)], 1.0)
+    qv = np.where(pq >= 0, vals[np.maximum(pq, 0)], 0.0)
+    sdiag = (bv * bv / mv).sum(1) + terms["extra_diag"].numpy()
+    soff = bv * qv / mv + terms["extra_off"].numpy().reshape(E, 4)
+    nbr = g.nbr_np
+    r_ = np.repeat(np.arange(E), 4)[nbr.ravel() >= 0]
+    S_ell = sps.csr_matrix((soff.ravel()[nbr.ravel() >= 0], (r_, nbr.ravel()[nbr.ravel() >= 0])), shape=(E, E))
+    S_ell = S_ell + sps.diags(sdiag)
+    # dense reduction
+    Dinv = sps.diags(1.0 / A[:n_u, :n_u].diagonal())
+    P = (A[n_u:, n_u:] - A[n_u:, :n_u] @ Dinv @ A[:n_u, n_u:]).tocsr()
+    Pmm, Pmf, Pfm, Pff = P[:E, :E], P[:E, E:], P[E:, :E], P[E:, E:].diagonal()
+    S_red = (Pmm - Pmf @ sps.diags(1.0 / Pff) @ Pfm).toarray()
+    err = np.abs(S_ell.toarray() - S_red).max() / np.abs(S_red).max()
+    assert err < 1e-12, err
+    # symmetric, positive diagonal, links across every fracture face
+    assert np.abs(S_red - S_red.T).max() < 1e-12 * np.abs(S_red).max() and sdiag.min() > 0
+    assert np.allclose(terms["tail_dinv"].numpy(), 1.0 / Pff, rtol=1e-6)
+    st = terms["slot_tail"].numpy().reshape(E, 4)
+    sc = terms["slot_coef"].numpy().reshape(E, 4)
+    ref = (Pmf @ sps.diags(1.0 / Pff)).tocsr()
+    pi, pj = np.nonzero(st >= 0)
+    assert len(pi) == 2 * len(net.faces)
+    assert np.allclose(sc[pi, pj], np.asarray(ref[pi, st[pi, pj]]).ravel(), rtol=1e-6)
+    tc = terms["tail_cells"].numpy().reshape(-1, 2)
+    tk = terms["tail_coef"].numpy().reshape(-1, 2)
+    fi = np.repeat(np.arange(len(tc)), 2)
+    assert np.allclose(tk.ravel(), np.asarray(Pfm[fi, tc.ravel()]).ravel(), rtol=1e-6)
+    assert Pfm.nnz == 2 * len(net.faces)
