@@ -260,6 +260,26 @@ int nfx_solve_minres_ir(nfx_solver* s, const int32_t* d_rowptr, const int32_t* d
                         double rtol, double inner_rtol, int max_inner, int max_outer,
                         int* h_info, double* h_resid, double* h_hist, int hist_cap);
 
+/* ---- L2 projection ---------------------------------------------------------
+ * Element mass matrix and right-hand side of one field of an L2 projection, as
+ * L2ProjectorWeakForm.evaluate_form (src/weak_forms/l2_projector_weak_form.py:8-93)
+ * scattered by l2_projector (src/postprocess/projectors.py:8-91). kind 0: vector
+ * field (RT / BDM rows, contravariant Piola, the first dim components as the
+ * reference), 1: scalar (Lagrange). The field occupies element-local DoFs off +
+ * a * n_comp + c of nl; d_slot (nl*nl, E) and d_rhs_slot (nl, E) are slot codes
+ * (CSR / stage index, bit 30 = orientation sign; code < 0 skips); multi-contributor
+ * slots go to d_stage / d_rhs_stage, summed afterwards by nfx_stage_reduce. d_f: the
+ * function at the element quadrature points, (E, nq, n_comp, dim) for kind 0 or
+ * (E, nq, n_comp). d_tables64: w[nq] | phi[nq][nb][dim] (kind 0) or w[nq] | psi[nq][nb].
+ * Matrix contributions within 1e-8 of 1e-16 are dropped, as the reference's scatter
+ * filter (numpy.isclose) drops them. FP64 arithmetic; values stored as double when
+ * fp64, else float. The right-hand side is b = -rg of the reference's driver. */
+int nfx_assemble_projector(int kind, int dim, int nb, int nq, int n_comp, int nl, int off, int n_elem,
+                           const int32_t* d_cells, const double* d_coords64,
+                           const double* d_tables64, const double* d_f, const int32_t* d_slot,
+                           const int32_t* d_rhs_slot, int fp64, void* d_vals, void* d_stage,
+                           void* d_rhs, void* d_rhs_stage, void* stream);
+
 /* ---- error norms (FP64) ---------------------------------------------------
  * Squared error of one field summed over elements, as l2_error / div_error
  * (src/postprocess/l2_error_post_processor.py:7-63,185-237). kind 0: vector

@@ -80,6 +80,7 @@ SIGNATURES = {
                                     c_void_p, c_void_p, c_void_p, c_int]),
     "nfx_field_error": (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
                                 c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "nfx_assemble_projector": (c_int, [c_int] * 8 + [c_void_p] * 6 + [c_int] + [c_void_p] * 5),
     "nfx_gnn_num_params": (c_int, []),
     "nfx_schur_ell": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "nfx_gnn_features": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,

@@ -115,6 +115,43 @@ def build_pattern(n, ref_dest, ref_sign, keep=None, zero_diag_dest=None, upper=F
     )
 
 
+def vector_slots(n, ref_dest, ref_sign, keep=None):
+    """Slot codes of a right-hand side assembled by ``rg[dest] += r_el`` over elements:
+    (nl, E) codes (row index when one element contributes to the row, else a stage
+    index (bit 29), orientation sign as bit 30; -1 where keep is False), and the
+    segments (segptr, target) that ``nfx_stage_reduce`` sums in element order."""
+    E, nl = ref_dest.shape
+    sel = np.ones((E, nl), dtype=bool) if keep is None else np.asarray(keep, bool)
+    r = ref_dest.reshape(-1)[sel.reshape(-1)].astype(np.int64)
+    if len(r) and (r.min() < 0 or r.max() >= n):
+        raise ValueError("destination outside the vector")
+    order = np.argsort(r, kind="stable")
+    rs = r[order]
+    start = np.r_[True, rs[1:] != rs[:-1]] if len(rs) else np.zeros(0, bool)
+    uid_sorted = np.cumsum(start) - 1
+    counts = np.bincount(uid_sorted, minlength=int(start.sum()))
+    seg_first = np.flatnonzero(start)
+    rank_sorted = np.arange(len(rs)) - seg_first[uid_sorted]
+    uid = np.empty(len(rs), np.int64)
+    uid[order] = uid_sorted
+    rank = np.empty(len(rs), np.int64)
+    rank[order] = rank_sorted
+    multi = counts > 1
+    seg_ids = np.flatnonzero(multi)
+    segptr = np.concatenate([[0], np.cumsum(counts[seg_ids])]).astype(np.int64)
+    seg_of_uid = np.full(len(counts), -1, np.int64)
+    seg_of_uid[seg_ids] = np.arange(len(seg_ids))
+    urow = rs[start]
+    code = np.where(multi[uid], SLOT_STAGE | (segptr[np.maximum(seg_of_uid[uid], 0)] + rank), urow[uid])
+    code = code | np.where(ref_sign.reshape(-1)[sel.reshape(-1)] < 0, SLOT_NEG, 0)
+    if max(n, int(segptr[-1])) > SLOT_INDEX_MASK:
+        raise ValueError("vector too large for 28-bit slot codes")
+    full = np.full(E * nl, -1, np.int64)
+    full[sel.reshape(-1)] = code
+    return (np.ascontiguousarray(full.reshape(E, nl).T).astype(np.int32), segptr.astype(np.int32),
+            urow[seg_ids].astype(np.int32))
+
+
 def row_codes(dest, sign=None):
     """int32 row codes (bit 30 = negate) from destinations and signs."""
     code = np.asarray(dest, dtype=np.int64).copy()
