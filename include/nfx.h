@@ -293,6 +293,16 @@ int nfx_field_error(int kind, int dim, int nb, int nq, int n_comp, int n_elem,
                     const double* d_alpha, const double* d_exact, double* h_out,
                     void* stream);
 
+/* Field values at n_points reference points of every element, FP64, as
+ * node_average_quatity / cell_centered_quatity evaluate them for the VTK output
+ * (src/postprocess/solution_post_processor.py:11-103): kind 0 (vector, RT / BDM rows)
+ * d_out (E, n_points, n_comp, dim) = the first dim components of the Piola-mapped field;
+ * kind 1 (scalar) d_out (E, n_points, n_comp). d_tables64: phi[n_points][nb][dim] or
+ * psi[n_points][nb] (reference basis at the points); d_rows as nfx_field_error. */
+int nfx_field_values(int kind, int dim, int nb, int n_points, int n_comp, int n_elem,
+                     const int32_t* d_cells, const double* d_coords64, const double* d_tables64,
+                     const int32_t* d_rows, const double* d_alpha, double* d_out, void* stream);
+
 /* ---- learned correction of the Darcy block preconditioner (RT1-P0) --------
  * No reference counterpart: the reference factorises the Darcy system with
  * LU/MUMPS (examples/md_elliptic_scalar/md_elliptic_scalar.py:351-364). These

@@ -113,9 +113,81 @@ __global__ __launch_bounds__(kNB) void k_field_error(
   if (threadIdx.x == 0) partial[blockIdx.x] = bs;
 }
 
+// Field values at np reference points of every element (the vertex and centroid values
+// of the VTK output): kind 0 vector, out[e][p][c][d] = (J phi_h)_d / detJ for d < dim;
+// kind 1 scalar, out[e][p][c] = sum_k alpha psi_k(p). tab: phi[np][nb][dim] or psi[np][nb].
+__global__ __launch_bounds__(kNB) void k_field_values(int kind, int dim, int nb, int np, int nc, int n_elem,
+                                                      const int32_t* __restrict__ cells,
+                                                      const double* __restrict__ coords,
+                                                      const double* __restrict__ tab,
+                                                      const int32_t* __restrict__ rows,
+                                                      const double* __restrict__ alpha, double* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * kNB + threadIdx.x;
+  if (t >= (int64_t)n_elem * np) return;
+  const int e = (int)(t / np), p = (int)(t % np);
+  if (kind == 1) {
+    for (int c = 0; c < nc; ++c) {
+      double fh = 0;
+      for (int k = 0; k < nb; ++k) {
+        const int code = rows[((int64_t)e * nb + k) * nc + c];
+        fh += alpha[code & kSlotIndexMask] * tab[p * nb + k];
+      }
+      out[((int64_t)e * np + p) * nc + c] = fh;
+    }
+    return;
+  }
+  double X[4][3];
+  for (int a = 0; a <= dim; ++a) {
+    const int v = cells[(int64_t)e * (dim + 1) + a];
+    for (int c = 0; c < 3; ++c) X[a][c] = coords[3 * (int64_t)v + c];
+  }
+  double J[3][3];
+  for (int c = 0; c < 3; ++c)
+    for (int d = 0; d < dim; ++d) J[c][d] = X[d + 1][c] - X[0][c];
+  double detJ;
+  if (dim == 3) {
+    detJ = fabs(J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]));
+  } else {
+    const double cx = J[1][0] * J[2][1] - J[2][0] * J[1][1];
+    const double cy = J[2][0] * J[0][1] - J[0][0] * J[2][1];
+    const double cz = J[0][0] * J[1][1] - J[1][0] * J[0][1];
+    detJ = sqrt(cx * cx + cy * cy + cz * cz);
+  }
+  for (int c = 0; c < nc; ++c) {
+    double ref[3] = {0, 0, 0};
+    for (int a = 0; a < nb; ++a) {
+      const int code = rows[((int64_t)e * nb + a) * nc + c];
+      double al = alpha[code & kSlotIndexMask];
+      if (code & kSlotNeg) al = -al;
+      for (int d = 0; d < dim; ++d) ref[d] += al * tab[(p * nb + a) * dim + d];
+    }
+    for (int d = 0; d < dim; ++d) {
+      double fh = 0;
+      for (int d2 = 0; d2 < dim; ++d2) fh += J[d][d2] * ref[d2];
+      out[(((int64_t)e * np + p) * nc + c) * dim + d] = fh / detJ;
+    }
+  }
+}
+
 }  // namespace nfx
 
 using namespace nfx;
+
+extern "C" int nfx_field_values(int kind, int dim, int nb, int n_points, int n_comp, int n_elem,
+                                const int32_t* d_cells, const double* d_coords64, const double* d_tables64,
+                                const int32_t* d_rows, const double* d_alpha, double* d_out, void* stream) {
+  NFX_REQUIRE(kind == 0 || kind == 1, "nfx_field_values: kind");
+  NFX_REQUIRE((dim == 2 || dim == 3) && nb > 0 && n_points > 0 && n_comp > 0 && n_elem >= 0,
+              "nfx_field_values: sizes");
+  if (n_elem == 0) return NFX_OK;
+  NFX_REQUIRE(d_cells && d_coords64 && d_tables64 && d_rows && d_alpha && d_out, "nfx_field_values: null pointer");
+  hipLaunchKernelGGL(k_field_values, dim3(grid_for((int64_t)n_elem * n_points, kNB)), dim3(kNB), 0,
+                     static_cast<hipStream_t>(stream), kind, dim, nb, n_points, n_comp, n_elem, d_cells, d_coords64,
+                     d_tables64, d_rows, d_alpha, d_out);
+  NFX_LAUNCH_CHECK();
+  return NFX_OK;
+}
 
 extern "C" int nfx_field_error(int kind, int dim, int nb, int nq, int n_comp, int n_elem,
                                const int32_t* d_cells, const double* d_coords64,

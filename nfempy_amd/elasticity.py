@@ -266,11 +266,11 @@ class ElasticitySystem:
         if uD.shape != (self.F, self.nqf, 3):
             raise ValueError("u_D must give 3 components per facet point")
         rg = out if out is not None else (self.rhs64 if fp64 else self.rhs32)
-        g = uD if fp64 else uD.to(torch().float32)
+        g = (uD if fp64 else uD.to(torch().float32)).contiguous()
         nat.call("nfx_assemble_elasticity_bc", self.F, nat.ptr(self.bc_facet), nat.ptr(self.bc_neigh),
                  nat.ptr(self.bc_combo), nat.ptr(self.cells), nat.ptr(self.coords64 if fp64 else self.coords),
                  nat.ptr(self.bc_trace64 if fp64 else self.bc_trace32), nat.ptr(self.bc_fw64 if fp64 else self.bc_fw32),
-                 self.nqf, nat.ptr(g.contiguous()), nat.ptr(self.bc_rows if rows is None else rows), int(fp64),
+                 self.nqf, nat.ptr(g), nat.ptr(self.bc_rows if rows is None else rows), int(fp64),
                  nat.ptr(rg), nat.stream_handle(stream))
 
     def full(self, which="A"):

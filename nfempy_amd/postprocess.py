@@ -71,7 +71,7 @@ def _field_error(kind, dim, fe_space, name, exact, alpha):
     dev = "cuda"
     a = alpha if isinstance(alpha, t.Tensor) else t.from_numpy(np.asarray(alpha, dtype=np.float64))
     a = a.to(device=dev, dtype=t.float64).contiguous()
-    if int(codes.max(initial=0) & nat.SLOT_INDEX_MASK) >= a.numel():
+    if int((codes & nat.SLOT_INDEX_MASK).max(initial=0)) >= a.numel():
         raise ValueError("alpha is shorter than the space's DoF range")
     out = (nat.ctypes.c_double * 1)()
     d_cells = to_dev(cells.astype(np.int32), device=dev)

@@ -160,7 +160,7 @@ def l2_error_projected(dim, fe_space, alpha, skip_fields=(), dof_shift=0):
         shift = fe_space.strides[name] + fe_space.dof_shift + int(dof_shift)
         codes = row_codes(sp.ref_dest + shift, sp.ref_sign).reshape(len(cells), nb, sp.n_comp)
         codes = np.ascontiguousarray(codes[:, :, :nc])
-        if int(codes.max(initial=0) & nat.SLOT_INDEX_MASK) >= a.numel():
+        if int((codes & nat.SLOT_INDEX_MASK).max(initial=0)) >= a.numel():
             raise ValueError("alpha is shorter than the space's DoF range")
         zero = t.zeros(len(cells) * len(weights) * nc * (dim if vector else 1), dtype=t.float64, device="cuda")
         res = (ctypes.c_double * 1)()
