@@ -90,14 +90,25 @@ int nfx_assemble_darcy(int dim, int k, int n_elem, const int32_t* d_cells,
  *           (255 = not present); nlp = nl, or nl rounded up to a multiple of 4 when
  *           nl > 8 (RT2 elements: word loads)
  *  d_rowptr CSR row pointer; d_signs (E) uint32, bit a = reference DoF a negated
+ *  d_geo    optional per-element metric from nfx_element_geometry (same precision as
+ *           the values): the kernel then reads it instead of deriving it from d_cells /
+ *           d_coords (same values); NULL derives it
  * Other arguments as nfx_assemble_darcy. */
 int nfx_assemble_darcy_rows(int dim, int k, int n_rows, const int32_t* d_inc,
                             const uint8_t* d_pos, const int32_t* d_rowptr,
                             const uint32_t* d_signs, const int32_t* d_cells,
                             const void* d_coords, const void* d_tables,
                             const void* d_kinv, int kinv_stride_e, int kinv_stride_q,
-                            const void* d_f, int f_stride_e, int f_stride_q, int fp64,
-                            void* d_vals, void* d_rhs, void* stream);
+                            const void* d_f, int f_stride_e, int f_stride_q, const void* d_geo,
+                            int fp64, void* d_vals, void* d_rhs, void* stream);
+
+/* Per-element metric of the row kernels, computed once per mesh (the mapping of
+ * src/geometry/mapping.py evaluated on the vertices): for element e, geo[e * gs + s] =
+ * (J^T J)[d][d2] / |det J| over d <= d2 (row-major upper order), then |det J| (for a
+ * 2-D element in 3-D, sqrt(det J^T J)), zero padding; gs = 8 (dim 3) or 4 (dim 2).
+ * float or double by fp64. */
+int nfx_element_geometry(int dim, int n_elem, const int32_t* d_cells, const void* d_coords, int fp64,
+                         void* d_geo, void* stream);
 
 /* Batched persistent variant for many independent 3-D systems (RT1-P0 and RT2-P1
  * mixed in one launch): d_items is an array of n_items descriptors
@@ -105,7 +116,9 @@ int nfx_assemble_darcy_rows(int dim, int k, int n_rows, const int32_t* d_inc,
  *            const int32_t* inc; const uint8_t* pos; const int32_t* rowptr;
  *            const uint32_t* signs; const int32_t* cells; const float* coords;
  *            const float* tables; const float* kinv; const float* f;
- *            float* vals; float* rhs; }             (FP32, nfx_darcy_batch_item_size bytes)
+ *            float* vals; float* rhs;
+ *            const float* geo; }      (FP32, nfx_darcy_batch_item_size bytes; geo, the
+ *                                      nfx_element_geometry metric, is required)
  * and d_tiles (n_tiles, 2) int32 (item, first row) pairs, heaviest first; a fixed
  * grid of `grid` workgroups of tile_rows (64, 128, 256) threads walks the tiles
  * in a static order (block b: tiles b, b + grid, ...). Same per-row arithmetic as
@@ -154,8 +167,8 @@ int nfx_assemble_darcy_rows_bc(int dim, int k, int n_rows, const int32_t* d_inc,
                                int f_stride_e, int f_stride_q, int n_facets,
                                const int32_t* d_facet, const int32_t* d_combo,
                                const void* d_trace, const void* d_fw, int nqf, const void* d_pD,
-                               const int32_t* d_rows, int fp64, void* d_vals, void* d_rhs,
-                               void* stream);
+                               const int32_t* d_rows, const void* d_geo, int fp64, void* d_vals,
+                               void* d_rhs, void* stream);
 
 /* ---- assembly: dual-mixed elasticity (BDM1 - P0 - P0, tetrahedra) --------
  * mode 0 replaces LEDualWeakForm.evaluate_form_vectorized

@@ -43,12 +43,13 @@ SIGNATURES = {
                                    c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
                                    c_void_p, c_void_p]),
     "nfx_assemble_darcy_rows": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
-                                        c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_int, c_int,
-                                        c_void_p, c_void_p, c_void_p]),
+                                        c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_int,
+                                        c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "nfx_assemble_darcy_rows_bc": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                            c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_int,
                                            c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p,
-                                           c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+                                           c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "nfx_element_geometry": (c_int, [c_int, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     "nfx_md_coupling": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                 c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "nfx_md_scatter": (c_int, [c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),

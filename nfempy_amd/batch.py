@@ -38,7 +38,7 @@ class _Item(ctypes.Structure):
                 ("inc", ctypes.c_void_p), ("pos", ctypes.c_void_p), ("rowptr", ctypes.c_void_p),
                 ("signs", ctypes.c_void_p), ("cells", ctypes.c_void_p), ("coords", ctypes.c_void_p),
                 ("tab", ctypes.c_void_p), ("kinv", ctypes.c_void_p), ("f", ctypes.c_void_p),
-                ("vals", ctypes.c_void_p), ("rhs", ctypes.c_void_p)]
+                ("vals", ctypes.c_void_p), ("rhs", ctypes.c_void_p), ("geo", ctypes.c_void_p)]
 
 
 def darcy_space(n, k):
@@ -96,6 +96,8 @@ class DarcyBatch:
             ref = tables.setdefault(s.k, s.tables32)
             if not torch().equal(ref, s.tables32):
                 raise RuntimeError("instances of one degree must share their reference tables")
+        if any(inst["system"].geo32 is None for inst in self.instances):
+            raise RuntimeError("batched assembly needs the per-element metric of every instance (device systems)")
         arr = (_Item * len(self.instances))()
         for i, inst in enumerate(self.instances):
             s = inst["system"]
@@ -103,7 +105,7 @@ class DarcyBatch:
             arr[i] = _Item(s.k, s.n, inst["kinv"].se, inst["kinv"].sq, inst["f"].se, inst["f"].sq,
                            nat.ptr(s.row_inc), nat.ptr(s.row_pos), nat.ptr(s.rowptr), nat.ptr(s.row_signs),
                            nat.ptr(s.cells), nat.ptr(s.coords), nat.ptr(tables[s.k]), nat.ptr(kinv32),
-                           nat.ptr(f32), nat.ptr(inst["vals"]), nat.ptr(inst["rhs"]))
+                           nat.ptr(f32), nat.ptr(inst["vals"]), nat.ptr(inst["rhs"]), nat.ptr(s.geo32))
         self._tables = tables
         raw = np.frombuffer(bytes(arr), dtype=np.uint8).copy()
         self._items = torch().from_numpy(raw).to(self.device)

@@ -53,6 +53,26 @@ __device__ __forceinline__ void geometry_from(const R (&X)[DIM + 1][3], Geom<DIM
   }
 }
 
+// Per-element metric of the row kernels: gd[s] = G[d][d2] / detJ over the symmetric
+// pairs d <= d2 (s in row-major upper order) and detJ. The geometry kernel stores the
+// same values (geo_stride floats per element: gd, detJ, zero padding), so a row
+// kernel reading them computes what one deriving them from the vertices computes.
+template <int DIM>
+constexpr int geo_stride() {
+  return DIM == 3 ? 8 : 4;
+}
+
+template <int DIM, typename R>
+__device__ __forceinline__ void scaled_metric(const Geom<DIM, R>& g, R (&gd)[DIM * (DIM + 1) / 2], R& dj) {
+  const R inv = R(1) / g.detJ;
+  int s = 0;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d)
+#pragma unroll
+    for (int d2 = d; d2 < DIM; ++d2) gd[s++] = g.G[d][d2] * inv;
+  dj = g.detJ;
+}
+
 template <int DIM, typename R>
 __device__ __forceinline__ void load_geometry(const int32_t* __restrict__ cells,
                                               const R* __restrict__ coords,
@@ -387,31 +407,44 @@ struct DarcyRowBc {
   const int32_t* rows;
 };
 
-// One tile of TILE consecutive rows [r0, r0 + TILE) of one system. st / sv are
-// LDS buffers of at least DarcyTab::NT and TILE * 2 * NL values. Called by all
-// threads of the block; contains block barriers. (Reading Rs / Bc / ck from global
-// memory instead of LDS, with one barrier per tile, measured 6 % slower.)
+// One tile of TILE * RPT consecutive rows [r0, r0 + TILE * RPT) of one system: thread t
+// owns rows r0 + h * TILE + t, h < RPT. st / sv are LDS buffers of at least
+// DarcyTab::NT and TILE * RPT * 2 * NL values. Called by all threads of the block;
+// contains block barriers. (Reading Rs / Bc / ck from global memory instead of LDS,
+// with one barrier per tile, measured 6 % slower.)
 // With per-element coefficients (ks_q == 0 and fs_q == 0) the quadrature sums are
 // the precomputed reference tensors: flux row a of element e is
 //   M_ab = kinv_e / detJ sum_s G_s Rs[a][b][s],  B part -Bc[a][k];
 // pressure row k: Bc[b][k] and residual -detJ f_e ck[k]. Otherwise the sums run
-// over the quadrature points. The loads of both incidences are issued before
-// either row is computed. Rows owned by a Dirichlet facet take the facet term
-// (DarcyRowBc) after the volume rows are written to LDS.
-template <int DIM, int NB, int NP, int NQ, int TILE, typename R>
+// over the quadrature points. The loads of every incidence of all RPT rows are
+// issued before any row is computed (RPT > 1: more loads in flight per wave). Rows
+// owned by a Dirichlet facet take the facet term (DarcyRowBc) after the volume rows
+// are written to LDS.
+template <int DIM, typename R, bool GEO>
+struct RowLoad {
+  static constexpr int NS = DIM * (DIM + 1) / 2;
+  int code[2], e[2], a[2];
+  uint32_t sg[2];
+  // GEO: the element metric comes precomputed (one load level less than cells ->
+  // coordinates); otherwise the vertex coordinates
+  R X[GEO ? 1 : 2][DIM + 1][3];
+  R gd[2][NS], dj[2];
+  R kv[2], fv[2];
+};
+
+template <int DIM, int NB, int NP, int NQ, int TILE, typename R, bool GEO = false, int RPT = 1>
 __device__ __forceinline__ void darcy_row_tile(
     int r0, int n_rows, const int32_t* __restrict__ inc, const uint8_t* __restrict__ pos,
     const int32_t* __restrict__ rowptr, const uint32_t* __restrict__ signs,
     const int32_t* __restrict__ cells, const R* __restrict__ coords, const R* __restrict__ tab,
     const R* __restrict__ kinv, int ks_e, int ks_q, const R* __restrict__ f, int fs_e, int fs_q,
     R* __restrict__ vals, R* __restrict__ rhs, R* st, R* sv, const DarcyRowBc<R>& bc,
-    bool load_table = true) {
+    bool load_table = true, const R* __restrict__ geo = nullptr) {
   constexpr int NL = NB + NP;
   constexpr int NS = DarcyTab<DIM, NB, NP, NQ>::NS;
   constexpr int NT = DarcyTab<DIM, NB, NP, NQ>::NT;
   constexpr int NT_RS = NQ * (1 + NB * DIM + NB + NP);  // start of Rs | Bc | ck
-  const int r = r0 + threadIdx.x;
-  const int rl = min(r0 + TILE, n_rows);
+  const int rl = min(r0 + TILE * RPT, n_rows);
   const bool per_element = (ks_q == 0 && fs_q == 0);
   // the reference-tensor path reads only Rs | Bc | ck; a caller that already holds the
   // table in st (same element, same path) passes load_table = false
@@ -428,57 +461,76 @@ __device__ __forceinline__ void darcy_row_tile(
   const R* tRs = tpsi + NQ * NP;
   const R* tBc = tRs + NB * NB * NS;
   const R* tck = tBc + NB * NP;
-  if (r < n_rows) {
-    const int off = rowptr[r] - base;
-    R res = 0;
-    int code[2], e[2], a[2];
-    uint32_t sg[2];
-    R X[2][DIM + 1][3];
-    R kv[2], fv[2];
+  RowLoad<DIM, R, GEO> in[RPT];
+  // load phase: incidences, then signs / metric (or vertices) / coefficients
+#pragma unroll
+  for (int h = 0; h < RPT; ++h) {
+    const int r = r0 + h * TILE + threadIdx.x;
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-      code[j] = inc[(int64_t)j * n_rows + r];
-      e[j] = code[j] < 0 ? 0 : code[j] >> 5;
-      a[j] = code[j] & 31;
+      const int c = r < n_rows ? inc[(int64_t)j * n_rows + r] : -1;
+      in[h].code[j] = c;
+      in[h].e[j] = c < 0 ? 0 : c >> 5;
+      in[h].a[j] = c & 31;
     }
+  }
+#pragma unroll
+  for (int h = 0; h < RPT; ++h) {
+    RowLoad<DIM, R, GEO>& L = in[h];
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-      sg[j] = 0;
-      kv[j] = fv[j] = R(0);
-      if (code[j] >= 0) {
-        const uint32_t s0 = signs[e[j]];
-        sg[j] = s0 ^ (((s0 >> a[j]) & 1u) ? 0xffffffffu : 0u);
-        int v[DIM + 1];
+      L.sg[j] = 0;
+      L.kv[j] = L.fv[j] = R(0);
+      if (L.code[j] >= 0) {
+        const uint32_t s0 = signs[L.e[j]];
+        L.sg[j] = s0 ^ (((s0 >> L.a[j]) & 1u) ? 0xffffffffu : 0u);
+        if constexpr (GEO) {
+          const R* gp = geo + (int64_t)L.e[j] * geo_stride<DIM>();
 #pragma unroll
-        for (int c = 0; c <= DIM; ++c) v[c] = cells[e[j] * (DIM + 1) + c];
+          for (int t = 0; t < NS; ++t) L.gd[j][t] = gp[t];
+          L.dj[j] = gp[NS];
+        } else {
+          int v[DIM + 1];
 #pragma unroll
-        for (int c = 0; c <= DIM; ++c)
+          for (int c = 0; c <= DIM; ++c) v[c] = cells[L.e[j] * (DIM + 1) + c];
 #pragma unroll
-          for (int d = 0; d < 3; ++d) X[j][c][d] = coords[3 * v[c] + d];
+          for (int c = 0; c <= DIM; ++c)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) L.X[j][c][d] = coords[3 * v[c] + d];
+        }
         if (per_element) {
-          kv[j] = kinv[(int64_t)e[j] * ks_e];
-          fv[j] = f[(int64_t)e[j] * fs_e];
+          L.kv[j] = kinv[(int64_t)L.e[j] * ks_e];
+          L.fv[j] = f[(int64_t)L.e[j] * fs_e];
         }
       }
     }
+  }
+  // compute phase
+#pragma unroll
+  for (int h = 0; h < RPT; ++h) {
+    const int r = r0 + h * TILE + threadIdx.x;
+    if (r >= n_rows) continue;
+    RowLoad<DIM, R, GEO>& L = in[h];
+    const int off = rowptr[r] - base;
+    R res = 0;
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-      if (code[j] < 0) continue;
-      Geom<DIM, R> g;
-      geometry_from<DIM, R>(X[j], g);
+      if (L.code[j] < 0) continue;
+      if constexpr (!GEO) {
+        Geom<DIM, R> g;
+        geometry_from<DIM, R>(L.X[j], g);
+        scaled_metric<DIM, R>(g, L.gd[j], L.dj[j]);
+      }
+      const int aj = L.a[j], ej = L.e[j];
       R row[NL];
 #pragma unroll
       for (int b = 0; b < NL; ++b) row[b] = 0;
       if (per_element) {
-        if (a[j] < NB) {
-          const R sc = kv[j] / g.detJ;
+        if (aj < NB) {
           R gs[NS];
-          int s = 0;
 #pragma unroll
-          for (int d = 0; d < DIM; ++d)
-#pragma unroll
-            for (int d2 = d; d2 < DIM; ++d2) gs[s++] = g.G[d][d2] * sc;
-          const R* rs = tRs + a[j] * NB * NS;
+          for (int t = 0; t < NS; ++t) gs[t] = L.gd[j][t] * L.kv[j];
+          const R* rs = tRs + aj * NB * NS;
 #pragma unroll
           for (int b = 0; b < NB; ++b) {
             R m = 0;
@@ -487,24 +539,36 @@ __device__ __forceinline__ void darcy_row_tile(
             row[b] = m;
           }
 #pragma unroll
-          for (int k = 0; k < NP; ++k) row[NB + k] = -tBc[a[j] * NP + k];
+          for (int k = 0; k < NP; ++k) row[NB + k] = -tBc[aj * NP + k];
         } else {
-          const int k = a[j] - NB;
+          const int k = aj - NB;
 #pragma unroll
           for (int b = 0; b < NB; ++b) row[b] = tBc[b * NP + k];
-          res -= g.detJ * fv[j] * tck[k];
+          res -= L.dj[j] * L.fv[j] * tck[k];
         }
-      } else if (a[j] < NB) {
-        const R invdet = R(1) / g.detJ;
+      } else if (aj < NB) {
+        // symmetric G / detJ from the packed upper triangle
+        R Gd[DIM][DIM];
+        {
+          int t = 0;
+#pragma unroll
+          for (int d = 0; d < DIM; ++d)
+#pragma unroll
+            for (int d2 = d; d2 < DIM; ++d2) {
+              Gd[d][d2] = L.gd[j][t];
+              Gd[d2][d] = L.gd[j][t];
+              ++t;
+            }
+        }
         for (int q = 0; q < NQ; ++q) {
           const R w = tw[q];
-          const R cm = w * kinv[(int64_t)e[j] * ks_e + q * ks_q] * invdet;
+          const R cm = w * kinv[(int64_t)ej * ks_e + q * ks_q];
           R ga[DIM];
 #pragma unroll
           for (int d = 0; d < DIM; ++d) {
             R s = 0;
 #pragma unroll
-            for (int d2 = 0; d2 < DIM; ++d2) s += g.G[d][d2] * tphi[(q * NB + a[j]) * DIM + d2];
+            for (int d2 = 0; d2 < DIM; ++d2) s += Gd[d][d2] * tphi[(q * NB + aj) * DIM + d2];
             ga[d] = s * cm;
           }
 #pragma unroll
@@ -514,17 +578,17 @@ __device__ __forceinline__ void darcy_row_tile(
             for (int d = 0; d < DIM; ++d) s += ga[d] * tphi[(q * NB + b) * DIM + d];
             row[b] += s;
           }
-          const R wd = w * tdiv[q * NB + a[j]];
+          const R wd = w * tdiv[q * NB + aj];
 #pragma unroll
           for (int k = 0; k < NP; ++k) row[NB + k] -= wd * tpsi[q * NP + k];
         }
       } else {
-        const int k = a[j] - NB;
+        const int k = aj - NB;
         for (int q = 0; q < NQ; ++q) {
           const R wp = tw[q] * tpsi[q * NP + k];
 #pragma unroll
           for (int b = 0; b < NB; ++b) row[b] += wp * tdiv[q * NB + b];
-          res -= wp * g.detJ * f[(int64_t)e[j] * fs_e + q * fs_q];
+          res -= wp * L.dj[j] * f[(int64_t)ej * fs_e + q * fs_q];
         }
       }
       if constexpr (row_pos_stride(NL) % 4 == 0) {
@@ -538,7 +602,7 @@ __device__ __forceinline__ void darcy_row_tile(
         for (int b = 0; b < NL; ++b) {
           const int p = (wd[b >> 2] >> (8 * (b & 3))) & 0xff;
           if (p == 255) continue;
-          sv[off + p] += ((sg[j] >> b) & 1u) ? -row[b] : row[b];
+          sv[off + p] += ((L.sg[j] >> b) & 1u) ? -row[b] : row[b];
         }
       } else {
         const uint8_t* pr = pos + ((int64_t)r * 2 + j) * NL;
@@ -546,16 +610,16 @@ __device__ __forceinline__ void darcy_row_tile(
         for (int b = 0; b < NL; ++b) {
           const int p = pr[b];
           if (p == 255) continue;
-          sv[off + p] += ((sg[j] >> b) & 1u) ? -row[b] : row[b];
+          sv[off + p] += ((L.sg[j] >> b) & 1u) ? -row[b] : row[b];
         }
       }
     }
-    if (bc.pD != nullptr && code[1] <= -2 && -code[1] - 2 < bc.n_codes) {
+    if (bc.pD != nullptr && L.code[1] <= -2 && -L.code[1] - 2 < bc.n_codes) {
       constexpr int FD = DarcyFacetShape<DIM, NB>::FD, NQF = DarcyFacetShape<DIM, NB>::NQF;
-      const int i = -code[1] - 2, fct = i / FD, a = i - fct * FD;
+      const int i = -L.code[1] - 2, fct = i / FD, a = i - fct * FD;
       Geom<DIM, R> g;
       R X0[DIM + 1][3];
-      load_geometry<DIM, R>(cells, coords, e[0], g, X0);
+      load_geometry<DIM, R>(cells, coords, L.e[0], g, X0);
       R val[FD][1];
       facet_term<DIM, FD, NQF, 1, R>(fct, X0, g, bc.facet_nodes, bc.combo, coords, bc.trace, bc.fw, bc.pD,
                                      val);
@@ -571,27 +635,34 @@ __device__ __forceinline__ void darcy_row_tile(
   __syncthreads();
 }
 
-// Waves per SIMD the row kernel is compiled for: the FP32 lowest-order elements fit
-// the hardware maximum of 8 in 64 VGPRs (16 bytes of spill), which measured 10 %
-// faster than the default allocation (93 VGPRs, 5 waves) at 32^3 x 6; the larger
-// and FP64 elements keep the compiler's choice.
+// Rows per thread and waves per SIMD of the row kernel. The FP32 lowest-order kernel
+// runs one row per thread at the hardware maximum of 8 waves (64 VGPRs). At 32^3 x 6
+// (595,968 rows = 9,312 waves, more than the 8,192 wave slots) two rows per thread would
+// fit one round at 5 waves per SIMD (87 VGPRs), but measured 18.7 us against 14.9 us:
+// the occupancy hides the load latency better than the second row's loads do. The
+// larger and FP64 elements keep the compiler's register choice.
+template <int NB, typename R>
+constexpr int row_kernel_rpt() {
+  return 1;
+}
 template <int NB, typename R>
 constexpr int row_kernel_min_waves() {
   return (NB <= 4 && sizeof(R) == 4) ? 8 : 1;
 }
 
-template <int DIM, int NB, int NP, int NQ, typename R>
+template <int DIM, int NB, int NP, int NQ, typename R, bool GEO>
 __global__ __launch_bounds__(kRowsPerBlock)
 __attribute__((amdgpu_waves_per_eu(row_kernel_min_waves<NB, R>(), 8))) void k_darcy_rows(
     int n_rows, const int32_t* __restrict__ inc, const uint8_t* __restrict__ pos,
     const int32_t* __restrict__ rowptr, const uint32_t* __restrict__ signs,
     const int32_t* __restrict__ cells, const R* __restrict__ coords, const R* __restrict__ tab,
     const R* __restrict__ kinv, int ks_e, int ks_q, const R* __restrict__ f, int fs_e, int fs_q,
-    R* __restrict__ vals, R* __restrict__ rhs, DarcyRowBc<R> bc) {
+    R* __restrict__ vals, R* __restrict__ rhs, DarcyRowBc<R> bc, const R* __restrict__ geo) {
   constexpr int NL = NB + NP;
   constexpr int NT = DarcyTab<DIM, NB, NP, NQ>::NT;
+  constexpr int RPT = row_kernel_rpt<NB, R>();
   __shared__ R st[NT];
-  __shared__ R sv[kRowsPerBlock * 2 * NL];
+  __shared__ R sv[kRowsPerBlock * RPT * 2 * NL];
   // Tiles in launch order, block b taking tiles b, b + grid, ... (one tile per block for
   // the lowest-order elements; the larger elements run a persistent grid that loads the
   // reference table into LDS once). An XCD-aware order (block b taking tile
@@ -599,15 +670,16 @@ __attribute__((amdgpu_waves_per_eu(row_kernel_min_waves<NB, R>(), 8))) void k_da
   // keeps the shared elements in its own L2) measured 7 % slower at 32^3 x 6 (16.9-17.6
   // us against 15.6-16.1 us, alternating runs of both builds).
   if constexpr (NB <= 4) {
-    darcy_row_tile<DIM, NB, NP, NQ, kRowsPerBlock, R>(blockIdx.x * kRowsPerBlock, n_rows, inc, pos, rowptr,
-                                                      signs, cells, coords, tab, kinv, ks_e, ks_q, f, fs_e,
-                                                      fs_q, vals, rhs, st, sv, bc);
+    darcy_row_tile<DIM, NB, NP, NQ, kRowsPerBlock, R, GEO, RPT>(blockIdx.x * kRowsPerBlock * RPT, n_rows, inc,
+                                                                pos, rowptr, signs, cells, coords, tab, kinv, ks_e,
+                                                                ks_q, f, fs_e, fs_q, vals, rhs, st, sv, bc, true, geo);
   } else {
-    const int n_tiles = (n_rows + kRowsPerBlock - 1) / kRowsPerBlock;
+    const int n_tiles = (n_rows + kRowsPerBlock * RPT - 1) / (kRowsPerBlock * RPT);
     for (int t = blockIdx.x; t < n_tiles; t += gridDim.x)
-      darcy_row_tile<DIM, NB, NP, NQ, kRowsPerBlock, R>(t * kRowsPerBlock, n_rows, inc, pos, rowptr, signs,
-                                                        cells, coords, tab, kinv, ks_e, ks_q, f, fs_e, fs_q,
-                                                        vals, rhs, st, sv, bc, t == (int)blockIdx.x);
+      darcy_row_tile<DIM, NB, NP, NQ, kRowsPerBlock, R, GEO, RPT>(t * kRowsPerBlock * RPT, n_rows, inc, pos,
+                                                                  rowptr, signs, cells, coords, tab, kinv, ks_e,
+                                                                  ks_q, f, fs_e, fs_q, vals, rhs, st, sv, bc,
+                                                                  t == (int)blockIdx.x, geo);
   }
 }
 
@@ -632,6 +704,7 @@ struct DarcyBatchItem {
   const NFX_GLOBAL float* f;
   NFX_GLOBAL float* vals;
   NFX_GLOBAL float* rhs;
+  const NFX_GLOBAL float* geo;  // per-element metric (nfx_element_geometry), required
 };
 #undef NFX_GLOBAL
 
@@ -674,21 +747,22 @@ __attribute__((amdgpu_waves_per_eu(batched_min_waves<KD>(), 8))) void k_darcy_ro
     const float* f = (const float*)it.f;
     float* vals = (float*)it.vals;
     float* rhs = (float*)it.rhs;
+    const float* geo = (const float*)it.geo;
     const int path = (it.ks_q == 0 && it.fs_q == 0) ? 1 : 0;
     const bool load = tab != held_tab || path != held_path;
     held_tab = tab;
     held_path = path;
     if constexpr (KD != 2) {
       if (it.k == 1)
-        darcy_row_tile<3, 4, 1, 8, TILE, float>(tl.y, it.n_rows, inc, pos, rowptr, signs, cells, coords, tab,
-                                                kinv, it.ks_e, it.ks_q, f, it.fs_e, it.fs_q, vals, rhs, st, sv,
-                                                no_bc, load);
+        darcy_row_tile<3, 4, 1, 8, TILE, float, true>(tl.y, it.n_rows, inc, pos, rowptr, signs, cells, coords,
+                                                      tab, kinv, it.ks_e, it.ks_q, f, it.fs_e, it.fs_q, vals, rhs,
+                                                      st, sv, no_bc, load, geo);
     }
     if constexpr (KD != 1) {
       if (it.k == 2)
-        darcy_row_tile<3, 15, 4, 27, TILE, float>(tl.y, it.n_rows, inc, pos, rowptr, signs, cells, coords,
-                                                  tab, kinv, it.ks_e, it.ks_q, f, it.fs_e, it.fs_q, vals, rhs,
-                                                  st, sv, no_bc, load);
+        darcy_row_tile<3, 15, 4, 27, TILE, float, true>(tl.y, it.n_rows, inc, pos, rowptr, signs, cells,
+                                                        coords, tab, kinv, it.ks_e, it.ks_q, f, it.fs_e, it.fs_q,
+                                                        vals, rhs, st, sv, no_bc, load, geo);
     }
   }
 }
@@ -701,18 +775,23 @@ static int launch_darcy_rows(int n_rows, const int32_t* inc, const uint8_t* pos,
                              hipStream_t s, int n_bc = 0, const int32_t* facet = nullptr,
                              const int32_t* combo = nullptr, const void* trace = nullptr,
                              const void* fw = nullptr, const void* pD = nullptr,
-                             const int32_t* bc_rows = nullptr) {
+                             const int32_t* bc_rows = nullptr, const void* geo = nullptr) {
   const DarcyRowBc<R> bc{n_bc, facet, combo, static_cast<const R*>(trace), static_cast<const R*>(fw),
                          static_cast<const R*>(pD), bc_rows};
   // one tile per block for the lowest-order elements; a persistent grid (8 blocks per
   // CU of 256) for the larger tables
-  const int n_tiles = grid_for(n_rows, kRowsPerBlock);
+  const int n_tiles = grid_for(n_rows, kRowsPerBlock * row_kernel_rpt<NB, R>());
   const int grid = NB <= 4 ? n_tiles : std::min(n_tiles, 2048);
-  hipLaunchKernelGGL((k_darcy_rows<DIM, NB, NP, NQ, R>), dim3(grid),
-                     dim3(kRowsPerBlock), 0, s, n_rows, inc, pos, rowptr, signs, cells,
-                     static_cast<const R*>(coords), static_cast<const R*>(tab),
-                     static_cast<const R*>(kinv), kse, ksq, static_cast<const R*>(f), fse, fsq,
-                     static_cast<R*>(vals), static_cast<R*>(rhs), bc);
+  if (geo)
+    hipLaunchKernelGGL((k_darcy_rows<DIM, NB, NP, NQ, R, true>), dim3(grid), dim3(kRowsPerBlock), 0, s, n_rows, inc,
+                       pos, rowptr, signs, cells, static_cast<const R*>(coords), static_cast<const R*>(tab),
+                       static_cast<const R*>(kinv), kse, ksq, static_cast<const R*>(f), fse, fsq,
+                       static_cast<R*>(vals), static_cast<R*>(rhs), bc, static_cast<const R*>(geo));
+  else
+    hipLaunchKernelGGL((k_darcy_rows<DIM, NB, NP, NQ, R, false>), dim3(grid), dim3(kRowsPerBlock), 0, s, n_rows,
+                       inc, pos, rowptr, signs, cells, static_cast<const R*>(coords), static_cast<const R*>(tab),
+                       static_cast<const R*>(kinv), kse, ksq, static_cast<const R*>(f), fse, fsq,
+                       static_cast<R*>(vals), static_cast<R*>(rhs), bc, static_cast<const R*>(nullptr));
   NFX_LAUNCH_CHECK();
   return NFX_OK;
 }
@@ -753,6 +832,25 @@ __global__ __launch_bounds__(256) void k_qpoints(int dim, int n_elem, int nq,
     for (int c = 0; c < 3; ++c) x[c] += s * coords[3 * v + c];
   }
   for (int c = 0; c < 3; ++c) out[t * 3 + c] = x[c];
+}
+
+template <int DIM, typename R>
+__global__ __launch_bounds__(256) void k_element_geometry(int n_elem, const int32_t* __restrict__ cells,
+                                                          const R* __restrict__ coords, R* __restrict__ geo) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_elem) return;
+  Geom<DIM, R> g;
+  R X[DIM + 1][3];
+  load_geometry<DIM, R>(cells, coords, e, g, X);
+  constexpr int NS = DIM * (DIM + 1) / 2;
+  R gd[NS], dj;
+  scaled_metric<DIM, R>(g, gd, dj);
+  R* o = geo + (int64_t)e * geo_stride<DIM>();
+#pragma unroll
+  for (int t = 0; t < NS; ++t) o[t] = gd[t];
+  o[NS] = dj;
+#pragma unroll
+  for (int t = NS + 1; t < geo_stride<DIM>(); ++t) o[t] = R(0);
 }
 
 template <int DIM, int NB, int NP, int NQ, typename R>
@@ -1001,8 +1099,8 @@ extern "C" int nfx_assemble_darcy_rows(int dim, int k, int n_rows, const int32_t
                                        const uint32_t* d_signs, const int32_t* d_cells,
                                        const void* d_coords, const void* d_tables,
                                        const void* d_kinv, int kinv_stride_e, int kinv_stride_q,
-                                       const void* d_f, int f_stride_e, int f_stride_q, int fp64,
-                                       void* d_vals, void* d_rhs, void* stream) {
+                                       const void* d_f, int f_stride_e, int f_stride_q, const void* d_geo,
+                                       int fp64, void* d_vals, void* d_rhs, void* stream) {
   NFX_REQUIRE(n_rows >= 0, "nfx_assemble_darcy_rows: n_rows < 0");
   if (n_rows == 0) return NFX_OK;
   NFX_REQUIRE(d_inc && d_pos && d_rowptr && d_signs && d_cells && d_coords && d_tables && d_kinv &&
@@ -1012,10 +1110,12 @@ extern "C" int nfx_assemble_darcy_rows(int dim, int k, int n_rows, const int32_t
 #define NFX_DROWS(DIM, NB, NP, NQ)                                                                 \
   return fp64 ? launch_darcy_rows<DIM, NB, NP, NQ, double>(                                        \
                     n_rows, d_inc, d_pos, d_rowptr, d_signs, d_cells, d_coords, d_tables, d_kinv,  \
-                    kinv_stride_e, kinv_stride_q, d_f, f_stride_e, f_stride_q, d_vals, d_rhs, s)   \
+                    kinv_stride_e, kinv_stride_q, d_f, f_stride_e, f_stride_q, d_vals, d_rhs, s,   \
+                    0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, d_geo)                \
               : launch_darcy_rows<DIM, NB, NP, NQ, float>(                                         \
                     n_rows, d_inc, d_pos, d_rowptr, d_signs, d_cells, d_coords, d_tables, d_kinv,  \
-                    kinv_stride_e, kinv_stride_q, d_f, f_stride_e, f_stride_q, d_vals, d_rhs, s)
+                    kinv_stride_e, kinv_stride_q, d_f, f_stride_e, f_stride_q, d_vals, d_rhs, s,   \
+                    0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, d_geo)
   if (dim == 2 && k == 1) NFX_DROWS(2, 3, 1, 4);
   if (dim == 2 && k == 2) NFX_DROWS(2, 8, 3, 9);
   if (dim == 3 && k == 1) NFX_DROWS(3, 4, 1, 8);
@@ -1030,8 +1130,8 @@ extern "C" int nfx_assemble_darcy_rows_bc(
     const uint32_t* d_signs, const int32_t* d_cells, const void* d_coords, const void* d_tables,
     const void* d_kinv, int kinv_stride_e, int kinv_stride_q, const void* d_f, int f_stride_e,
     int f_stride_q, int n_facets, const int32_t* d_facet, const int32_t* d_combo, const void* d_trace,
-    const void* d_fw, int nqf, const void* d_pD, const int32_t* d_rows, int fp64, void* d_vals,
-    void* d_rhs, void* stream) {
+    const void* d_fw, int nqf, const void* d_pD, const int32_t* d_rows, const void* d_geo, int fp64,
+    void* d_vals, void* d_rhs, void* stream) {
   NFX_REQUIRE(n_rows >= 0 && n_facets >= 0, "nfx_assemble_darcy_rows_bc: negative size");
   if (n_rows == 0) return NFX_OK;
   NFX_REQUIRE(d_inc && d_pos && d_rowptr && d_signs && d_cells && d_coords && d_tables && d_kinv &&
@@ -1050,11 +1150,11 @@ extern "C" int nfx_assemble_darcy_rows_bc(
     return fp64 ? launch_darcy_rows<DIM, NB, NP, NQ, double>(                                         \
                       n_rows, d_inc, d_pos, d_rowptr, d_signs, d_cells, d_coords, d_tables, d_kinv,   \
                       kinv_stride_e, kinv_stride_q, d_f, f_stride_e, f_stride_q, d_vals, d_rhs, s,     \
-                      n_facets * FD, d_facet, d_combo, d_trace, d_fw, pD, d_rows)                      \
+                      n_facets * FD, d_facet, d_combo, d_trace, d_fw, pD, d_rows, d_geo)               \
                 : launch_darcy_rows<DIM, NB, NP, NQ, float>(                                          \
                       n_rows, d_inc, d_pos, d_rowptr, d_signs, d_cells, d_coords, d_tables, d_kinv,   \
                       kinv_stride_e, kinv_stride_q, d_f, f_stride_e, f_stride_q, d_vals, d_rhs, s,     \
-                      n_facets * FD, d_facet, d_combo, d_trace, d_fw, pD, d_rows);                     \
+                      n_facets * FD, d_facet, d_combo, d_trace, d_fw, pD, d_rows, d_geo);              \
   } while (0)
   if (dim == 2 && k == 1) NFX_DROWSBC(2, 3, 1, 4);
   if (dim == 2 && k == 2) NFX_DROWSBC(2, 8, 3, 9);
@@ -1063,6 +1163,32 @@ extern "C" int nfx_assemble_darcy_rows_bc(
 #undef NFX_DROWSBC
   set_error("nfx_assemble_darcy_rows_bc: unsupported (dim, k)");
   return NFX_E_UNSUPPORTED;
+}
+
+extern "C" int nfx_element_geometry(int dim, int n_elem, const int32_t* d_cells, const void* d_coords, int fp64,
+                                    void* d_geo, void* stream) {
+  NFX_REQUIRE((dim == 2 || dim == 3) && n_elem >= 0, "nfx_element_geometry: sizes");
+  if (n_elem == 0) return NFX_OK;
+  NFX_REQUIRE(d_cells && d_coords && d_geo, "nfx_element_geometry: null pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const dim3 grid(grid_for(n_elem, 256));
+  if (dim == 3) {
+    if (fp64)
+      hipLaunchKernelGGL((k_element_geometry<3, double>), grid, dim3(256), 0, s, n_elem, d_cells,
+                         static_cast<const double*>(d_coords), static_cast<double*>(d_geo));
+    else
+      hipLaunchKernelGGL((k_element_geometry<3, float>), grid, dim3(256), 0, s, n_elem, d_cells,
+                         static_cast<const float*>(d_coords), static_cast<float*>(d_geo));
+  } else {
+    if (fp64)
+      hipLaunchKernelGGL((k_element_geometry<2, double>), grid, dim3(256), 0, s, n_elem, d_cells,
+                         static_cast<const double*>(d_coords), static_cast<double*>(d_geo));
+    else
+      hipLaunchKernelGGL((k_element_geometry<2, float>), grid, dim3(256), 0, s, n_elem, d_cells,
+                         static_cast<const float*>(d_coords), static_cast<float*>(d_geo));
+  }
+  NFX_LAUNCH_CHECK();
+  return NFX_OK;
 }
 
 extern "C" int nfx_assemble_elasticity_bc(int n_facets, const int32_t* d_facet,

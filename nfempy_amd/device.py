@@ -226,6 +226,19 @@ class DarcySystem:
         self.cells = to_dev(cells, device=device)
         self.coords = to_dev(mesh.points.astype(np.float32), device=device)
         self.coords64 = to_dev(mesh.points, device=device)
+        # per-element metric of the row kernels (G / detJ, detJ), once per mesh: the warm
+        # assembly then reads it instead of the vertex coordinates (use_geometry = False
+        # derives it in the kernel; same values)
+        self.geo32 = self.geo64 = None
+        self.use_geometry = True
+        if str(device).startswith("cuda"):
+            t = torch()
+            gs = 8 if D == 3 else 4
+            self.geo32 = t.empty(self.E * gs, dtype=t.float32, device=device)
+            self.geo64 = t.empty(self.E * gs, dtype=t.float64, device=device)
+            for fl, crd, geo in ((0, self.coords, self.geo32), (1, self.coords64, self.geo64)):
+                nat.call("nfx_element_geometry", D, self.E, nat.ptr(self.cells), nat.ptr(crd), fl, nat.ptr(geo),
+                         nat.stream_handle())
         tab = darcy_tables(self.u_el, self.p_el, points, weights)
         self.tables64 = to_dev(tab, device=device)
         self.tables32 = to_dev(tab.astype(np.float32), device=device)
@@ -440,11 +453,12 @@ class DarcySystem:
         check_coefficient(fa, self.E, self.nq, "rhs")
         coords = nat.ptr(self.coords64 if fp64 else self.coords)
         tables = nat.ptr(self.tables64 if fp64 else self.tables32)
+        geo = nat.ptr(self.geo64 if fp64 else self.geo32) if self.use_geometry else None
         if p_dirichlet is None:
             nat.call("nfx_assemble_darcy_rows", self.dim, self.k, self.n, nat.ptr(self.row_inc),
                      nat.ptr(self.row_pos), nat.ptr(self.rowptr), nat.ptr(self.row_signs), nat.ptr(self.cells),
                      coords, tables, nat.ptr(ka.get(fp64)), ka.se, ka.sq, nat.ptr(fa.get(fp64)), fa.se, fa.sq,
-                     int(fp64), nat.ptr(vals), nat.ptr(rg), s)
+                     geo, int(fp64), nat.ptr(vals), nat.ptr(rg), s)
             return vals, rg
         pd = self.dirichlet_values(p_dirichlet)
         nat.call("nfx_assemble_darcy_rows_bc", self.dim, self.k, self.n, nat.ptr(self.row_inc),
@@ -453,7 +467,7 @@ class DarcySystem:
                  self.F, nat.ptr(self.bc_facet), nat.ptr(self.bc_combo),
                  nat.ptr(self.bc_trace64 if fp64 else self.bc_trace32),
                  nat.ptr(self.bc_fw64 if fp64 else self.bc_fw32), self.nqf, nat.ptr(pd.get(fp64)),
-                 nat.ptr(self.bc_rows), int(fp64), nat.ptr(vals), nat.ptr(rg), s)
+                 nat.ptr(self.bc_rows), geo, int(fp64), nat.ptr(vals), nat.ptr(rg), s)
         return vals, rg
 
     def dirichlet_values(self, p_dirichlet):
