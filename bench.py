@@ -505,7 +505,7 @@ def main():
     # + FP32 values 4 nnz / E + residual 4 N / E out (the facet data, 0.2 MB, not counted)
     alg_bytes_per_tet = 16 + 48 + 4 + 16 + 1 + 4.0 * nnz / E + 4.0 * N / E
     achieved = alg_bytes_per_tet * E / (k_ms * 1e-3) / 1e9
-    kname = "k_darcy_rows<3,4,1,8,float>"
+    kname = "k_darcy_rows<3,4,1,8,float,true>" if geo is not None else "k_darcy_rows<3,4,1,8,float,false>"
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": 8000.0, "unit": "GB/s",
                 "frac": round(achieved / 8000.0, 4), "traffic": profiled_traffic(kname, n),
                 "kernel": kname, "kernel_ms": round(k_ms, 5),

@@ -92,7 +92,7 @@ int nfx_assemble_darcy(int dim, int k, int n_elem, const int32_t* d_cells,
  *  d_rowptr CSR row pointer; d_signs (E) uint32, bit a = reference DoF a negated
  *  d_geo    optional per-element metric from nfx_element_geometry (same precision as
  *           the values): the kernel then reads it instead of deriving it from d_cells /
- *           d_coords (same values); NULL derives it
+ *           d_coords (same values); NULL derives it; 16-byte aligned
  * Other arguments as nfx_assemble_darcy. */
 int nfx_assemble_darcy_rows(int dim, int k, int n_rows, const int32_t* d_inc,
                             const uint8_t* d_pos, const int32_t* d_rowptr,

@@ -484,7 +484,18 @@ __device__ __forceinline__ void darcy_row_tile(
       if (L.code[j] >= 0) {
         const uint32_t s0 = signs[L.e[j]];
         L.sg[j] = s0 ^ (((s0 >> L.a[j]) & 1u) ? 0xffffffffu : 0u);
-        if constexpr (GEO) {
+        if constexpr (GEO && DIM == 3 && sizeof(R) == 4) {
+          // two 16-byte loads (the per-element stride is 32 bytes, the array 256-byte aligned)
+          const float4* gp = reinterpret_cast<const float4*>(geo + (int64_t)L.e[j] * geo_stride<DIM>());
+          const float4 g0 = gp[0], g1 = gp[1];
+          L.gd[j][0] = g0.x;
+          L.gd[j][1] = g0.y;
+          L.gd[j][2] = g0.z;
+          L.gd[j][3] = g0.w;
+          L.gd[j][4] = g1.x;
+          L.gd[j][5] = g1.y;
+          L.dj[j] = g1.z;
+        } else if constexpr (GEO) {
           const R* gp = geo + (int64_t)L.e[j] * geo_stride<DIM>();
 #pragma unroll
           for (int t = 0; t < NS; ++t) L.gd[j][t] = gp[t];
@@ -1102,6 +1113,7 @@ extern "C" int nfx_assemble_darcy_rows(int dim, int k, int n_rows, const int32_t
                                        const void* d_f, int f_stride_e, int f_stride_q, const void* d_geo,
                                        int fp64, void* d_vals, void* d_rhs, void* stream) {
   NFX_REQUIRE(n_rows >= 0, "nfx_assemble_darcy_rows: n_rows < 0");
+  NFX_REQUIRE(reinterpret_cast<uintptr_t>(d_geo) % 16 == 0, "nfx_assemble_darcy_rows: d_geo not 16-byte aligned");
   if (n_rows == 0) return NFX_OK;
   NFX_REQUIRE(d_inc && d_pos && d_rowptr && d_signs && d_cells && d_coords && d_tables && d_kinv &&
                   d_f && d_vals && d_rhs,
@@ -1139,6 +1151,7 @@ extern "C" int nfx_assemble_darcy_rows_bc(
               "nfx_assemble_darcy_rows_bc: null pointer");
   NFX_REQUIRE(n_facets == 0 || (d_facet && d_combo && d_trace && d_fw && d_pD && d_rows),
               "nfx_assemble_darcy_rows_bc: null facet pointer");
+  NFX_REQUIRE(reinterpret_cast<uintptr_t>(d_geo) % 16 == 0, "nfx_assemble_darcy_rows_bc: d_geo not 16-byte aligned");
   hipStream_t s = static_cast<hipStream_t>(stream);
   const void* pD = n_facets ? d_pD : nullptr;
 #define NFX_DROWSBC(DIM, NB, NP, NQ)                                                                  \
