@@ -179,7 +179,8 @@ int nfx_assemble_darcy_rows_bc(int dim, int k, int n_rows, const int32_t* d_inc,
  *  d_tables  w[8] | phi[8][12][3] | div[8][12]  (BDM1 reference basis)
  *  d_lam, d_mu  Lame parameters at quadrature points (stride_e, stride_q)
  *  d_f       body force + rotation source, 6 components: f[e*fs_e + q*fs_q + c*fs_c]
- *  d_slot    (42*42, E) slot codes; d_rhs_row (6, E) rows of r(u_c), r(t_c)
+ *  d_slot    (E, 42*42) slot codes, element-major (entry (I, J) of element e at
+ *            e*1764 + 42 I + J); d_rhs_row (6, E) rows of r(u_c), r(t_c)
  * One wavefront per element. */
 int nfx_assemble_elasticity(int mode, int n_elem, const int32_t* d_cells,
                             const void* d_coords, const void* d_tables,

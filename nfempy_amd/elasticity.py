@@ -117,8 +117,8 @@ class ElasticitySystem:
         self.functions = functions
         self._coef = self._coefficients(functions)
         # cold: FP64 element matrices -> value filter -> SBAIJ patterns
-        loc_slot = np.ascontiguousarray(
-            (np.arange(self.E, dtype=np.int64)[:, None] * NL * NL + np.arange(NL * NL)[None, :]).T).astype(np.int32)
+        # element-major slot map (the kernel's layout): code e * NL * NL + i * NL + j
+        loc_slot = np.arange(self.E * NL * NL, dtype=np.int32)
         d_loc = to_dev(loc_slot, device=device)
         KA = self._local(0, d_loc)
         KP = self._local(1, d_loc)
@@ -189,7 +189,8 @@ class ElasticitySystem:
             self.ops[key] = {
                 "pattern": pat,
                 "rowptr": to_dev(pat.rowptr, device=dev), "col": to_dev(pat.col, device=dev),
-                "slot": to_dev(pat.slot, device=dev), "segptr": to_dev(pat.segptr, device=dev),
+                # the host pattern is entry-major (NL * NL, E); the kernel reads element-major
+                "slot": to_dev(np.ascontiguousarray(pat.slot.T), device=dev), "segptr": to_dev(pat.segptr, device=dev),
                 "target": to_dev(pat.target, device=dev),
                 "vals32": t.zeros(pat.nnz, dtype=t.float32, device=dev),
                 "vals64": t.zeros(pat.nnz, dtype=t.float64, device=dev),

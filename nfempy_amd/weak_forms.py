@@ -272,5 +272,6 @@ def _dense_local_slots(sysd):
                              np.tile(np.arange(36, 42), (sysd.E, 1))], axis=1)
         from .pattern import local_slot_map
 
-        sysd._dense_local = to_dev(local_slot_map(li, sysd.ref_sign, NL))
+        # the elasticity kernel reads its slot map element-major
+        sysd._dense_local = to_dev(np.ascontiguousarray(local_slot_map(li, sysd.ref_sign, NL).T))
     return sysd._dense_local
