@@ -179,14 +179,17 @@ int nfx_assemble_darcy_rows_bc(int dim, int k, int n_rows, const int32_t* d_inc,
  *  d_tables  w[8] | phi[8][12][3] | div[8][12]  (BDM1 reference basis)
  *  d_lam, d_mu  Lame parameters at quadrature points (stride_e, stride_q)
  *  d_f       body force + rotation source, 6 components: f[e*fs_e + q*fs_q + c*fs_c]
- *  d_slot    (E, 42*42) slot codes, element-major (entry (I, J) of element e at
- *            e*1764 + 42 I + J); d_rhs_row (6, E) rows of r(u_c), r(t_c)
+ *  d_slot    slot codes, element-major: slot_pairs = 0: (E, 42*42), entry (I, J) of
+ *            element e at e*1764 + 42 I + J; slot_pairs = 1: (E, 903), one code per
+ *            upper-triangle entry I <= J (row-major) receiving the symmetric value
+ *            (an upper SBAIJ operator keeps at most one of (I, J), (J, I));
+ *            d_rhs_row (6, E) rows of r(u_c), r(t_c)
  * One wavefront per element. */
 int nfx_assemble_elasticity(int mode, int n_elem, const int32_t* d_cells,
                             const void* d_coords, const void* d_tables,
                             const void* d_lam, const void* d_mu, int coef_stride_e,
                             int coef_stride_q, const void* d_f, int f_stride_e,
-                            int f_stride_q, int f_stride_c, const int32_t* d_slot,
+                            int f_stride_q, int f_stride_c, const int32_t* d_slot, int slot_pairs,
                             const int32_t* d_rhs_row, int fp64, void* d_vals,
                             void* d_stage, void* d_rhs, void* stream);
 

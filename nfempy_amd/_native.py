@@ -62,7 +62,7 @@ SIGNATURES = {
                                       c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p,
                                       c_void_p]),
     "nfx_assemble_elasticity": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
-                                        c_int, c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_int,
+                                        c_int, c_void_p, c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_int,
                                         c_void_p, c_void_p, c_void_p, c_void_p]),
     "nfx_assemble_elasticity_bc": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                            c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),

@@ -6,9 +6,11 @@
 // pushed-forward stress basis psi_a(x_q) = J phi_a / detJ (12 functions x 8
 // points x 3 components), its divergence and the per-point weights are staged
 // in LDS; lanes then evaluate the 903 entries (I <= J) of the symmetric 42 x 42
-// matrix and store both (I, J) and (J, I) through the slot map, which is
-// element-major (the 1,764 codes of one element are contiguous: a wave reads them
-// with coalesced loads; an entry-major map measured 379 us per assembly at 16^3 x 6).
+// matrix and store them through the slot map, which is element-major (the codes of
+// one element are contiguous: a wave reads them with coalesced loads; an entry-major
+// map measured 379 us per assembly at 16^3 x 6). Dense maps hold 1,764 codes per
+// element, (I, J) and (J, I); pair maps 903, one per upper-triangle entry (an upper
+// SBAIJ operator keeps at most one of the two: half the code traffic).
 //
 // Reference local order: stress (a, c) -> 3a + c (a < 12), displacement u_c ->
 // 36 + c, rotation t_c -> 39 + c. Entries:
@@ -33,7 +35,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_elasticity(
     int n_elem, const int32_t* __restrict__ cells, const R* __restrict__ coords,
     const R* __restrict__ tab, const R* __restrict__ lam, const R* __restrict__ mu, int cs_e,
     int cs_q, const R* __restrict__ f, int fs_e, int fs_q, int fs_c,
-    const int32_t* __restrict__ slot, const int32_t* __restrict__ rhs_row, R* __restrict__ vals,
+    const int32_t* __restrict__ slot, int pairs, const int32_t* __restrict__ rhs_row, R* __restrict__ vals,
     R* __restrict__ stage, R* __restrict__ rhs) {
   __shared__ R s_psi[kWavesPerBlock][kEB * kEQ * 3];
   __shared__ R s_div[kWavesPerBlock][kEB * kEQ];
@@ -105,7 +107,8 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_elasticity(
   __syncthreads();
   if (!live) return;
 
-  const int32_t* es = slot + (int64_t)e * kEL * kEL;
+  constexpr int kPairs = kEL * (kEL + 1) / 2;
+  const int32_t* es = slot + (int64_t)e * (pairs ? kPairs : kEL * kEL);
   for (int p = lane; p < kEL * (kEL + 1) / 2; p += 64) {
     // p-th entry of the upper triangle in row-major order
     int I = 0, rem = p;
@@ -154,8 +157,12 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_elasticity(
         v = s;
       }
     }
-    put_slot<R>(es[I * kEL + Jj], v, vals, stage);
-    if (I != Jj) put_slot<R>(es[Jj * kEL + I], v, vals, stage);
+    if (pairs) {
+      put_slot<R>(es[p], v, vals, stage);
+    } else {
+      put_slot<R>(es[I * kEL + Jj], v, vals, stage);
+      if (I != Jj) put_slot<R>(es[Jj * kEL + I], v, vals, stage);
+    }
   }
   const int64_t E = n_elem;
   if (MODE == 0 && lane < 6) {
@@ -170,13 +177,13 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_elasticity(
 template <int MODE, typename R>
 static int launch_elasticity(int n, const int32_t* cells, const void* coords, const void* tab,
                              const void* lam, const void* mu, int cse, int csq, const void* f,
-                             int fse, int fsq, int fsc, const int32_t* slot, const int32_t* rhs_row,
+                             int fse, int fsq, int fsc, const int32_t* slot, int pairs, const int32_t* rhs_row,
                              void* vals, void* stage, void* rhs, hipStream_t s) {
   hipLaunchKernelGGL((k_elasticity<MODE, R>), dim3(grid_for(n, kWavesPerBlock)),
                      dim3(64 * kWavesPerBlock), 0, s, n, cells, static_cast<const R*>(coords),
                      static_cast<const R*>(tab), static_cast<const R*>(lam),
                      static_cast<const R*>(mu), cse, csq, static_cast<const R*>(f), fse, fsq, fsc,
-                     slot, rhs_row, static_cast<R*>(vals), static_cast<R*>(stage),
+                     slot, pairs, rhs_row, static_cast<R*>(vals), static_cast<R*>(stage),
                      static_cast<R*>(rhs));
   NFX_LAUNCH_CHECK();
   return NFX_OK;
@@ -198,7 +205,7 @@ extern "C" int nfx_assemble_elasticity(int mode, int n_elem, const int32_t* d_ce
                                        const void* d_coords, const void* d_tables,
                                        const void* d_lam, const void* d_mu, int coef_stride_e,
                                        int coef_stride_q, const void* d_f, int f_stride_e,
-                                       int f_stride_q, int f_stride_c, const int32_t* d_slot,
+                                       int f_stride_q, int f_stride_c, const int32_t* d_slot, int slot_pairs,
                                        const int32_t* d_rhs_row, int fp64, void* d_vals,
                                        void* d_stage, void* d_rhs, void* stream) {
   NFX_REQUIRE(mode == 0 || mode == 1, "nfx_assemble_elasticity: mode");
@@ -211,18 +218,18 @@ extern "C" int nfx_assemble_elasticity(int mode, int n_elem, const int32_t* d_ce
   if (mode == 0)
     return fp64 ? launch_elasticity<0, double>(n_elem, d_cells, d_coords, d_tables, d_lam, d_mu,
                                                coef_stride_e, coef_stride_q, d_f, f_stride_e,
-                                               f_stride_q, f_stride_c, d_slot, d_rhs_row, d_vals,
+                                               f_stride_q, f_stride_c, d_slot, slot_pairs, d_rhs_row, d_vals,
                                                d_stage, d_rhs, s)
                 : launch_elasticity<0, float>(n_elem, d_cells, d_coords, d_tables, d_lam, d_mu,
                                               coef_stride_e, coef_stride_q, d_f, f_stride_e,
-                                              f_stride_q, f_stride_c, d_slot, d_rhs_row, d_vals,
+                                              f_stride_q, f_stride_c, d_slot, slot_pairs, d_rhs_row, d_vals,
                                               d_stage, d_rhs, s);
   return fp64 ? launch_elasticity<1, double>(n_elem, d_cells, d_coords, d_tables, d_lam, d_mu,
                                              coef_stride_e, coef_stride_q, nullptr, 0, 0, 0,
-                                             d_slot, nullptr, d_vals, d_stage, nullptr, s)
+                                             d_slot, slot_pairs, nullptr, d_vals, d_stage, nullptr, s)
               : launch_elasticity<1, float>(n_elem, d_cells, d_coords, d_tables, d_lam, d_mu,
                                             coef_stride_e, coef_stride_q, nullptr, 0, 0, 0,
-                                            d_slot, nullptr, d_vals, d_stage, nullptr, s);
+                                            d_slot, slot_pairs, nullptr, d_vals, d_stage, nullptr, s);
 }
 
 extern "C" int nfx_gather(int fp64, int64_t n, const int32_t* d_idx, const void* d_src, void* d_dst,

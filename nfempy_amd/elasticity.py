@@ -51,6 +51,18 @@ def _vector_coefficient(value, xq, nq, ncomp, device):
     return arr.contiguous()
 
 
+def _pair_slots(slot):
+    """(E, NL (NL + 1) / 2) codes, one per local upper-triangle entry (row-major), from an
+    entry-major (NL * NL, E) map of an upper (SBAIJ) operator: the code of whichever of
+    (I, J), (J, I) the operator keeps (-1 when neither)."""
+    iu, ju = np.triu_indices(NL)
+    a = slot[iu * NL + ju]
+    b = slot[ju * NL + iu]
+    if np.any((a >= 0) & (b >= 0) & (iu != ju)[:, None]):
+        raise RuntimeError("slot map keeps both orientations of an entry: not an upper operator")
+    return np.ascontiguousarray(np.where(a >= 0, a, b).T).astype(np.int32)
+
+
 def _sbaij_mask(vals, ref_dest, ref_sign, upper=True):
     """Entries the reference inserts: not isclose(v, 1e-16) (numpy defaults), row <= col."""
     v = vals * (ref_sign[:, :, None] * ref_sign[:, None, :])
@@ -167,17 +179,18 @@ class ElasticitySystem:
         rhs = t.zeros(max(self.E * NL, 1), dtype=t.float64, device=self.device)
         rows = to_dev(np.ascontiguousarray((np.arange(self.E)[:, None] * NL + np.arange(36, 42)[None, :]).T)
                       .astype(np.int32), device=self.device)
-        self._launch(mode, True, d_loc, rows, out, out, rhs)
+        self._launch(mode, True, d_loc, rows, out, out, rhs, pairs=0)
         self._local_rhs = rhs
         return out.cpu().numpy().reshape(self.E, NL, NL)
 
-    def _launch(self, mode, fp64, slot, rhs_row, vals, stage, rhs, stream=None):
+    def _launch(self, mode, fp64, slot, rhs_row, vals, stage, rhs, stream=None, pairs=1):
         c = self._coef
         lam, mu, f = c["lam"], c["mu"], c["f"]
         nat.call("nfx_assemble_elasticity", mode, self.E, nat.ptr(self.cells),
                  nat.ptr(self.coords64 if fp64 else self.coords), nat.ptr(self.tables64 if fp64 else self.tables32),
                  nat.ptr(lam.get(fp64)), nat.ptr(mu.get(fp64)), lam.se, lam.sq, nat.ptr(f.get(fp64)),
-                 self.nq * 6, 6, 1, nat.ptr(slot), nat.ptr(rhs_row), int(fp64), nat.ptr(vals), nat.ptr(stage),
+                 self.nq * 6, 6, 1, nat.ptr(slot), int(pairs), nat.ptr(rhs_row), int(fp64), nat.ptr(vals),
+                 nat.ptr(stage),
                  nat.ptr(rhs), nat.stream_handle(stream))
 
     def _upload_patterns(self):
@@ -189,8 +202,9 @@ class ElasticitySystem:
             self.ops[key] = {
                 "pattern": pat,
                 "rowptr": to_dev(pat.rowptr, device=dev), "col": to_dev(pat.col, device=dev),
-                # the host pattern is entry-major (NL * NL, E); the kernel reads element-major
-                "slot": to_dev(np.ascontiguousarray(pat.slot.T), device=dev), "segptr": to_dev(pat.segptr, device=dev),
+                # the host pattern is entry-major (NL * NL, E); the kernel reads one code per
+                # upper-triangle entry, element-major
+                "slot": to_dev(_pair_slots(pat.slot), device=dev), "segptr": to_dev(pat.segptr, device=dev),
                 "target": to_dev(pat.target, device=dev),
                 "vals32": t.zeros(pat.nnz, dtype=t.float32, device=dev),
                 "vals64": t.zeros(pat.nnz, dtype=t.float64, device=dev),
