@@ -850,9 +850,15 @@ __global__ __launch_bounds__(256) void k_element_geometry(int n_elem, const int3
                                                           const R* __restrict__ coords, R* __restrict__ geo) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n_elem) return;
-  Geom<DIM, R> g;
   R X[DIM + 1][3];
-  load_geometry<DIM, R>(cells, coords, e, g, X);
+#pragma unroll
+  for (int a = 0; a <= DIM; ++a) {
+    const int v = cells[(int64_t)e * (DIM + 1) + a];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) X[a][c] = coords[3 * (int64_t)v + c];
+  }
+  Geom<DIM, R> g;
+  geometry_from<DIM, R>(X, g);  // the function the vertex path of the row kernels uses
   constexpr int NS = DIM * (DIM + 1) / 2;
   R gd[NS], dj;
   scaled_metric<DIM, R>(g, gd, dj);

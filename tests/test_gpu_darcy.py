@@ -80,6 +80,16 @@ def test_darcy_assembly_parity(case):
         assert dr <= (1e-15 if fp64 else 1e-7)
         rr = r.double().cpu().numpy()
         assert np.abs(rr - rg).max() <= tol * max(np.abs(rg).max(), 1e-300), ("fused", fp64)
+    # the metric read from the per-element array (the default) against the metric derived
+    # from the vertices in the kernel: the same arithmetic up to rounding
+    for fp64, tol in ((True, 1e-14), (False, 2e-6)):
+        vg, rgeo = (a.clone() for a in sysd.assemble(fun["kappa"], fun["rhs"], ex["p"], fp64=fp64))
+        sysd.use_geometry = False
+        vv, rv = (a.clone() for a in sysd.assemble(fun["kappa"], fun["rhs"], ex["p"], fp64=fp64))
+        sysd.use_geometry = True
+        torch.cuda.synchronize()
+        assert float((vg - vv).abs().max()) <= tol * float(vg.abs().max()), ("metric path", fp64)
+        assert float((rgeo - rv).abs().max()) <= tol * max(float(rgeo.abs().max()), 1e-300), ("metric path", fp64)
     # run-to-run determinism: a second assembly is bitwise identical
     v1 = sysd.assemble_elements(fun["kappa"], fun["rhs"], fp64=False)[0].clone()
     v2 = sysd.assemble_elements(fun["kappa"], fun["rhs"], fp64=False)[0].clone()
