@@ -193,6 +193,20 @@ int nfx_assemble_elasticity(int mode, int n_elem, const int32_t* d_cells,
                             const int32_t* d_rhs_row, int fp64, void* d_vals,
                             void* d_stage, void* d_rhs, void* stream);
 
+/* Block preconditioner of the elasticity system from its full FP64 CSR (n = n_s + 2 n_u:
+ * stress rows, then u, then t): the inverse of each 9 x 9 face block of the stress rows
+ * (rows 9b .. 9b+8, columns of the same block) and of each 6 x 6 cell block of the Schur
+ * complement B D^-1 B^T on (u, t) (B the stress columns of the cell's u rows n_s + 3c + k
+ * and t rows n_s + n_u + 3c + k), FP64 Gauss-Jordan, symmetrised. The field-split
+ * preconditioner of mfe_elasticity_example_1.py:238-274 solves with the Riesz map
+ * exactly; this is its block-diagonal stand-in. Output: FP32 values of the CSR with 9
+ * entries per stress row (row 9b + i: columns 9b .. 9b+8) then 6 per (u, t) row (the
+ * cell's u then t DoFs); d_dinv (n_s / 9 x 81 FP64) workspace. A non-positive pivot sets
+ * bits of *d_status (1 face block, 2 more than 4 faces per cell, 4 cell block). */
+int nfx_elasticity_mass_schur_pc(int n, int n_s, int n_u, const int32_t* d_rowptr, const int32_t* d_col,
+                                 const double* d_vals64, double* d_dinv, float* d_pc_val, int* d_status,
+                                 void* stream);
+
 /* Stress Dirichlet term of LEDualWeakFormBCDirichlet.evaluate_form
  * (src/weak_forms/le_dual_weak_form.py:514-578):
  * r[(a, c)] = -sum_q w |detJ_f| u_D,c(x_q) psi_a(x_q) . n for the 3 BDM1 DoFs of

@@ -197,9 +197,162 @@ __global__ __launch_bounds__(256) void k_gather(int64_t n, const int32_t* __rest
   if (i < n) dst[i] = src[idx[i]];
 }
 
+// In-place Gauss-Jordan inverse of an SPD N x N block (FP64, no pivoting: the pivots of
+// an SPD matrix are positive); returns false on a non-positive pivot. The result is
+// symmetrised, (A^-1 + A^-T) / 2.
+template <int N>
+__device__ bool spd_block_inverse(double (&A)[N][N]) {
+  for (int k = 0; k < N; ++k) {
+    const double piv = A[k][k];
+    if (!(piv > 0.0)) return false;
+    const double ip = 1.0 / piv;
+    A[k][k] = 1.0;
+    for (int j = 0; j < N; ++j) A[k][j] *= ip;
+    for (int i = 0; i < N; ++i) {
+      if (i == k) continue;
+      const double f = A[i][k];
+      A[i][k] = 0.0;
+      for (int j = 0; j < N; ++j) A[i][j] -= f * A[k][j];
+    }
+  }
+  for (int i = 0; i < N; ++i)
+    for (int j = i + 1; j < N; ++j) {
+      const double v = 0.5 * (A[i][j] + A[j][i]);
+      A[i][j] = v;
+      A[j][i] = v;
+    }
+  return true;
+}
+
+// Face blocks of the stress mass block: rows 9b .. 9b+8 restricted to columns of the same
+// block, inverted; the inverse goes to dinv (FP64) and to the preconditioner's FP32
+// values (9 per row, columns 9b .. 9b+8).
+__global__ __launch_bounds__(64) void k_face_block_inverse(int nb, const int32_t* __restrict__ rp,
+                                                           const int32_t* __restrict__ ci,
+                                                           const double* __restrict__ va,
+                                                           double* __restrict__ dinv, float* __restrict__ pc_val,
+                                                           int* __restrict__ status) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  double D[9][9];
+  for (int i = 0; i < 9; ++i)
+    for (int j = 0; j < 9; ++j) D[i][j] = 0.0;
+  for (int i = 0; i < 9; ++i) {
+    const int r = 9 * b + i;
+    for (int k = rp[r]; k < rp[r + 1]; ++k) {
+      const int c = ci[k] - 9 * b;
+      if (c >= 0 && c < 9) D[i][c] = va[k];
+    }
+  }
+  if (!spd_block_inverse<9>(D)) {
+    atomicOr(status, 1);
+    return;
+  }
+  for (int i = 0; i < 9; ++i)
+    for (int j = 0; j < 9; ++j) {
+      dinv[(int64_t)b * 81 + i * 9 + j] = D[i][j];
+      pc_val[((int64_t)9 * b + i) * 9 + j] = (float)D[i][j];
+    }
+}
+
+// Cell blocks of the Schur complement S = B D^-1 B^T on (u, t): rows of cell c are
+// ns + 3c + k (u) and ns + nu + 3c + k (t), k < 3; B their stress columns, grouped by
+// face block (at most 4 faces per cell). S is inverted and written as the
+// preconditioner's FP32 values (6 per row, columns the cell's u then t DoFs).
+__global__ __launch_bounds__(64) void k_cell_schur_inverse(int n_cells, int ns, int nu,
+                                                           const int32_t* __restrict__ rp,
+                                                           const int32_t* __restrict__ ci,
+                                                           const double* __restrict__ va,
+                                                           const double* __restrict__ dinv,
+                                                           float* __restrict__ pc_val, int* __restrict__ status) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_cells) return;
+  int rows[6];
+  for (int k = 0; k < 3; ++k) {
+    rows[k] = ns + 3 * c + k;
+    rows[3 + k] = ns + nu + 3 * c + k;
+  }
+  int faces[4];
+  int nf = 0;
+  bool over = false;
+  for (int a = 0; a < 6; ++a)
+    for (int k = rp[rows[a]]; k < rp[rows[a] + 1]; ++k) {
+      const int col = ci[k];
+      if (col >= ns) continue;
+      const int f = col / 9;
+      bool seen = false;
+      for (int t = 0; t < nf; ++t) seen = seen || faces[t] == f;
+      if (seen) continue;
+      if (nf < 4)
+        faces[nf++] = f;
+      else
+        over = true;
+    }
+  if (over) {
+    atomicOr(status, 2);
+    return;
+  }
+  double S[6][6];
+  for (int a = 0; a < 6; ++a)
+    for (int b = 0; b < 6; ++b) S[a][b] = 0.0;
+  for (int t = 0; t < nf; ++t) {
+    const int f = faces[t];
+    double Bf[6][9];
+    for (int a = 0; a < 6; ++a)
+      for (int j = 0; j < 9; ++j) Bf[a][j] = 0.0;
+    for (int a = 0; a < 6; ++a)
+      for (int k = rp[rows[a]]; k < rp[rows[a] + 1]; ++k) {
+        const int col = ci[k];
+        if (col < ns && col / 9 == f) Bf[a][col - 9 * f] = va[k];
+      }
+    const double* Di = dinv + (int64_t)f * 81;
+    for (int a = 0; a < 6; ++a) {
+      double T[9];
+      for (int j = 0; j < 9; ++j) {
+        double sacc = 0.0;
+        for (int i = 0; i < 9; ++i) sacc += Bf[a][i] * Di[i * 9 + j];
+        T[j] = sacc;
+      }
+      for (int b = 0; b < 6; ++b) {
+        double sacc = 0.0;
+        for (int j = 0; j < 9; ++j) sacc += T[j] * Bf[b][j];
+        S[a][b] += sacc;
+      }
+    }
+  }
+  if (!spd_block_inverse<6>(S)) {
+    atomicOr(status, 4);
+    return;
+  }
+  for (int a = 0; a < 6; ++a) {
+    const int64_t r = rows[a] - ns;  // preconditioner row among the (u, t) rows
+    for (int b = 0; b < 6; ++b) pc_val[(int64_t)ns * 9 + r * 6 + b] = (float)S[a][b];
+  }
+}
+
 }  // namespace nfx
 
 using namespace nfx;
+
+extern "C" int nfx_elasticity_mass_schur_pc(int n, int n_s, int n_u, const int32_t* d_rowptr, const int32_t* d_col,
+                                            const double* d_vals64, double* d_dinv, float* d_pc_val,
+                                            int* d_status, void* stream) {
+  NFX_REQUIRE(n_s >= 0 && n_u >= 0 && n_s % 9 == 0 && n_u % 3 == 0 && n == n_s + 2 * n_u,
+              "nfx_elasticity_mass_schur_pc: sizes");
+  if (n == 0) return NFX_OK;
+  NFX_REQUIRE(d_rowptr && d_col && d_vals64 && d_dinv && d_pc_val && d_status,
+              "nfx_elasticity_mass_schur_pc: null pointer");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int nb = n_s / 9, n_cells = n_u / 3;
+  if (nb)
+    hipLaunchKernelGGL(k_face_block_inverse, dim3(grid_for(nb, 64)), dim3(64), 0, st, nb, d_rowptr, d_col, d_vals64,
+                       d_dinv, d_pc_val, d_status);
+  if (n_cells)
+    hipLaunchKernelGGL(k_cell_schur_inverse, dim3(grid_for(n_cells, 64)), dim3(64), 0, st, n_cells, n_s, n_u,
+                       d_rowptr, d_col, d_vals64, d_dinv, d_pc_val, d_status);
+  NFX_LAUNCH_CHECK();
+  return NFX_OK;
+}
 
 extern "C" int nfx_assemble_elasticity(int mode, int n_elem, const int32_t* d_cells,
                                        const void* d_coords, const void* d_tables,

@@ -310,8 +310,12 @@ class ElasticitySystem:
         return sps.csr_matrix((vals, pat.col, pat.rowptr), shape=(self.n, self.n))
 
     # -- preconditioner -----------------------------------------------------------
-    def preconditioner(self, P=None, kind="mass_schur"):
+    def preconditioner(self, P=None, kind="mass_schur", device=True):
         """(sign, pc) for MinresIR: an SPD block-diagonal preconditioner as a CSR matrix.
+
+        device=True builds "mass_schur" on the device from the FP64 values
+        (nfx_elasticity_mass_schur_pc: block gathers and FP64 block inverses, FP32 output);
+        device=False (and kind "riesz_block") build it on the host with numpy/scipy.
 
         kind "mass_schur" (default): sigma rows get the inverse of the 9 x 9 face blocks
         of the compliance block A_ss (each face's 3 BDM1 DoFs x 3 stress rows are
@@ -324,6 +328,8 @@ class ElasticitySystem:
         is a much weaker stand-in)."""
         import scipy.sparse as sps
 
+        if kind == "mass_schur" and device:
+            return None, self._mass_schur_pc_device()
         which = "A" if kind == "mass_schur" else "P"
         if which == "P":
             self.assemble("P", fp64=True)
@@ -377,6 +383,35 @@ class ElasticitySystem:
         pc = (to_dev(M.indptr.astype(np.int32), device=self.device), to_dev(M.indices.astype(np.int32), device=self.device),
               to_dev(M.data.astype(np.float32), device=self.device))
         return None, pc
+
+    def _mass_schur_pc_device(self):
+        """The "mass_schur" preconditioner built on the device: its CSR pattern (9 entries
+        per stress row, the row's face block; 6 per u / t row, the cell's u and t DoFs) is
+        set up once, its values come from the current FP64 values of A."""
+        t = torch()
+        ns, nu, n = self.n_s, self.u_sp.n_dof, self.n
+        if getattr(self, "_pc_dev", None) is None:
+            if ns % 9 or nu % 3 or n != ns + 2 * nu:
+                raise ValueError("unexpected elasticity DoF layout")
+            rp = np.concatenate([9 * np.arange(ns + 1), 9 * ns + 6 * np.arange(1, n - ns + 1)]).astype(np.int32)
+            cols_s = (np.arange(ns)[:, None] // 9) * 9 + np.arange(9)[None, :]
+            cell = (np.arange(n - ns) % nu) // 3
+            cols_ut = np.concatenate([ns + cell[:, None] * 3 + np.arange(3), ns + nu + cell[:, None] * 3 + np.arange(3)],
+                                     axis=1)
+            col = np.concatenate([cols_s.ravel(), cols_ut.ravel()]).astype(np.int32)
+            self._pc_dev = {"rowptr": to_dev(rp, device=self.device), "col": to_dev(col, device=self.device),
+                            "val": t.empty(len(col), dtype=t.float32, device=self.device),
+                            "dinv": t.empty(max(ns // 9, 1) * 81, dtype=t.float64, device=self.device),
+                            "status": t.zeros(1, dtype=t.int32, device=self.device)}
+        d = self._pc_dev
+        rpA, colA, _, v64 = self.full("A")
+        d["status"].zero_()
+        nat.call("nfx_elasticity_mass_schur_pc", n, ns, nu, nat.ptr(rpA), nat.ptr(colA), nat.ptr(v64),
+                 nat.ptr(d["dinv"]), nat.ptr(d["val"]), nat.ptr(d["status"]), nat.stream_handle())
+        st = int(d["status"].item())
+        if st:
+            raise RuntimeError(f"elasticity preconditioner: non-SPD block (status {st})")
+        return d["rowptr"], d["col"], d["val"]
 
     def solve(self, rtol=1e-8, inner_rtol=1e-5, max_inner=50000, max_outer=30, pc_kind="mass_schur"):
         from .solvers import MinresIR

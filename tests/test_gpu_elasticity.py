@@ -3,7 +3,9 @@
 SBAIJ patterns of A (value filter + explicit zero diagonals) and of the Riesz map P
 bit-exact; FP64 values to 1e-12 and FP32 to 1e-5 (per row); residual with the stress
 Dirichlet term; element-local evaluate_form; MINRES with the block-Jacobi field split
-to an FP64 relative residual of 1e-8, agreeing with scipy's solution.
+to an FP64 relative residual of 1e-8, agreeing with scipy's solution; the block
+preconditioner built on the device equal (pattern bit-exact, FP32 values 1e-6 of the
+largest) to the host construction.
 """
 
 import numpy as np
@@ -72,6 +74,12 @@ def test_elasticity_parity(case, coef):
     # solve
     sysd.assemble("A", fp64=True, u_dirichlet=ex["u"])
     sysd.assemble("A", fp64=False, u_dirichlet=ex["u"])
+    # the block preconditioner built on the device equals the host construction
+    _, (prp, pcol, pval) = sysd.preconditioner(device=True)
+    _, (hrp, hcol, hval) = sysd.preconditioner(device=False)
+    assert torch.equal(prp, hrp) and torch.equal(pcol, hcol)
+    hv = hval.double()
+    assert float((pval.double() - hv).abs().max()) <= 1e-6 * float(hv.abs().max())
     x, info = sysd.solve(rtol=1e-8)
     assert info["converged"], info
     Af = symmetric_from_upper(A)
