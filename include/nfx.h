@@ -283,6 +283,14 @@ int nfx_schur_diag64(int n_rows, int n_u, const int32_t* d_rowptr,
 typedef struct nfx_solver nfx_solver;
 int nfx_solver_create(nfx_solver** out, int n_rows, int nnz, void* stream);
 int nfx_solver_destroy(nfx_solver* s);
+/* Give the solver a 3 x 3 block CSR copy of the FP32 operator it is then called with
+ * (n_rows a multiple of 3; block row R holds rows 3R .. 3R+2, d_browptr (n_rows/3 + 1),
+ * d_bcol (n_blocks) block columns, d_bval 9 n_blocks values with entry (a, b) of block j
+ * at (3a + b) n_blocks + j): the inner MINRES multiplies with it (one index and three
+ * consecutive vector loads per block); the FP64 refinement keeps the CSR values. n_blocks
+ * = 0 or null pointers: back to the CSR operator. The arrays must outlive the solves. */
+int nfx_solver_set_block3(nfx_solver* s, int n_blocks, const int32_t* d_browptr, const int32_t* d_bcol,
+                          const float* d_bval);
 int nfx_solve_minres_ir(nfx_solver* s, const int32_t* d_rowptr, const int32_t* d_col,
                         const float* d_val32, const double* d_val64,
                         const float* d_sign, const int32_t* d_pc_rowptr,

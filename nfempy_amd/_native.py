@@ -84,6 +84,7 @@ SIGNATURES = {
     "nfx_assemble_projector": (c_int, [c_int] * 8 + [c_void_p] * 6 + [c_int] + [c_void_p] * 5),
     "nfx_field_values": (c_int, [c_int] * 6 + [c_void_p] * 7),
     "nfx_elasticity_mass_schur_pc": (c_int, [c_int] * 3 + [c_void_p] * 7),
+    "nfx_solver_set_block3": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "nfx_gnn_num_params": (c_int, []),
     "nfx_schur_ell": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "nfx_gnn_features": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,

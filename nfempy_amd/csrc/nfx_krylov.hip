@@ -279,6 +279,66 @@ __global__ __launch_bounds__(kBlock) void k_minres_a(
   if (threadIdx.x == 0) partial_a[blockIdx.x] = bs;
 }
 
+// A(k) with the operator as a 3 x 3 block CSR copy (block row R = rows 3R .. 3R+2; block
+// j's entry (a, b) at bva[(3a + b) * nblk + j], so that the lanes of a group load each
+// entry of consecutive blocks with coalesced loads; three consecutive loads of z per block
+// instead of nine scattered ones). Same scalar step and vector updates as k_minres_a.
+template <int LANES>
+__global__ __launch_bounds__(kBlock) void k_minres_a_b3(
+    int nbr, int nblk, const int32_t* __restrict__ brp, const int32_t* __restrict__ bci,
+    const float* __restrict__ bva, const float* __restrict__ sign, const float* __restrict__ z,
+    const float* __restrict__ zprev, const float* __restrict__ r1, float* __restrict__ y,
+    float* __restrict__ x, float* __restrict__ wnew, const float* __restrict__ wa,
+    const float* __restrict__ wb, int first, int final_only, const double* __restrict__ partial_b,
+    int np_b, double* __restrict__ partial_a, const MinresState* __restrict__ sin,
+    MinresState* __restrict__ sout) {
+  __shared__ double sh[kBlock / 64];
+  const double bsum = sum_partials(partial_b, np_b, sh);
+  MinresState st;
+  minres_scalars(*sin, st, sqrt(bsum > 0 ? bsum : 0), first);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *sout = st;
+  const float s = (float)st.s;
+  const float ratio = first ? 0.f : (float)st.ratio;
+  const float sprev = st.oldb > 0 ? (float)(1.0 / st.oldb) : 0.f;
+  const float oldeps = (float)st.oldeps, delta = (float)st.delta, denom = (float)st.denom,
+              phi = (float)st.phi;
+  const int lane = threadIdx.x % LANES;
+  double acc = 0;
+  const int groups_per_grid = gridDim.x * (kBlock / LANES);
+  for (int br = blockIdx.x * (kBlock / LANES) + threadIdx.x / LANES; br < nbr; br += groups_per_grid) {
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+    if (!final_only)
+      for (int j = brp[br] + lane; j < brp[br + 1]; j += LANES) {
+        const int c = 3 * bci[j];
+        const float z0 = z[c], z1 = z[c + 1], z2 = z[c + 2];
+        s0 += bva[0 * (int64_t)nblk + j] * z0 + bva[1 * (int64_t)nblk + j] * z1 + bva[2 * (int64_t)nblk + j] * z2;
+        s1 += bva[3 * (int64_t)nblk + j] * z0 + bva[4 * (int64_t)nblk + j] * z1 + bva[5 * (int64_t)nblk + j] * z2;
+        s2 += bva[6 * (int64_t)nblk + j] * z0 + bva[7 * (int64_t)nblk + j] * z1 + bva[8 * (int64_t)nblk + j] * z2;
+      }
+    s0 = group_sum<LANES>(s0);
+    s1 = group_sum<LANES>(s1);
+    s2 = group_sum<LANES>(s2);
+    if (lane < 3) {
+      const int row = 3 * br + lane;
+      const float sum = lane == 0 ? s0 : (lane == 1 ? s1 : s2);
+      if (!first) {
+        const float v = zprev[row] * sprev;
+        const float w = (v - oldeps * wa[row] - delta * wb[row]) * denom;
+        wnew[row] = w;
+        x[row] += phi * w;
+      }
+      if (!final_only) {
+        const float sg = sign ? sign[row] : 1.f;
+        const float yi = s * sg * sum - ratio * r1[row];
+        y[row] = yi;
+        acc += (double)(s * z[row]) * (double)yi;
+      }
+    }
+  }
+  const double bs = block_sum(acc, sh);
+  if (threadIdx.x == 0) partial_a[blockIdx.x] = bs;
+}
+
 // B(k): alpha_k, y' = y - (alpha/beta) r2 (written into the free buffer ynew),
 // z = P^-1 y' with P^-1 a sparse SPD matrix (CSR; y' of other rows is formed on
 // the fly from the unmodified y and r2), partials of y' . z.
@@ -343,6 +403,11 @@ struct nfx_solver {
   double* partial_r = nullptr;
   double* h_partial = nullptr;  // pinned
   MinresState* st = nullptr;    // 2 states
+  // optional 3 x 3 block CSR copy of the FP32 operator (nfx_solver_set_block3)
+  int b3_nbr = 0, b3_nblk = 0, blocks_a3 = 1;
+  const int32_t* b3_rp = nullptr;
+  const int32_t* b3_ci = nullptr;
+  const float* b3_va = nullptr;
   MinresState* h_st = nullptr;  // pinned
   int blocks_a = 1, blocks_b = 1;
   int lanes = 4;
@@ -416,6 +481,26 @@ extern "C" int nfx_solver_create(nfx_solver** out, int n_rows, int nnz, void* st
   return NFX_OK;
 }
 
+extern "C" int nfx_solver_set_block3(nfx_solver* s, int n_blocks, const int32_t* d_browptr, const int32_t* d_bcol,
+                                     const float* d_bval) {
+  NFX_REQUIRE(s, "nfx_solver_set_block3: null solver");
+  if (n_blocks <= 0 || !d_browptr || !d_bcol || !d_bval) {
+    s->b3_va = nullptr;
+    s->b3_rp = s->b3_ci = nullptr;
+    s->b3_nbr = s->b3_nblk = 0;
+    return NFX_OK;
+  }
+  NFX_REQUIRE(s->n % 3 == 0, "nfx_solver_set_block3: rows not a multiple of 3");
+  s->b3_nbr = s->n / 3;
+  s->b3_nblk = n_blocks;
+  s->b3_rp = d_browptr;
+  s->b3_ci = d_bcol;
+  s->b3_va = d_bval;
+  const int ba = grid_for(s->b3_nbr, kBlock / 16);
+  s->blocks_a3 = ba < kMaxBlocks ? ba : kMaxBlocks;
+  return NFX_OK;
+}
+
 extern "C" int nfx_solver_destroy(nfx_solver* s) {
   if (!s) return NFX_OK;
   if (s->buf) (void)hipFree(s->buf);
@@ -435,6 +520,13 @@ static int launch_a(nfx_solver* s, const int32_t* rp, const int32_t* ci, const f
                     float* y, float* x, float* wnew, const float* wa, const float* wb, int first,
                     int final_only, const MinresState* sin, MinresState* sout) {
   const int n = s->n;
+  if (s->b3_va) {
+    hipLaunchKernelGGL(k_minres_a_b3<16>, dim3(s->blocks_a3), dim3(kBlock), 0, s->stream, s->b3_nbr, s->b3_nblk,
+                       s->b3_rp, s->b3_ci, s->b3_va, sign, z, zprev, r1, y, x, wnew, wa, wb, first, final_only,
+                       s->partial_b, s->blocks_b, s->partial_a, sin, sout);
+    NFX_LAUNCH_CHECK();
+    return NFX_OK;
+  }
 #define NFX_A(L)                                                                                  \
   hipLaunchKernelGGL(k_minres_a<L>, dim3(s->blocks_a), dim3(kBlock), 0, s->stream, n, rp, ci, va, \
                      sign, z, zprev, r1, y, x, wnew, wa, wb, first, final_only, s->partial_b,     \
@@ -482,7 +574,8 @@ static int minres_inner(nfx_solver* s, const int32_t* rp, const int32_t* ci, con
     if (rc) return rc;
     // y' goes into r1's buffer (r1 is dead after A(k)); z_{k+1} into zprev's buffer
     hipLaunchKernelGGL(k_minres_b, dim3(s->blocks_b), dim3(kBlock), 0, s->stream, n, F, r2, prp,
-                       pci, pva, r1, zprev, s->partial_a, s->blocks_a, s->partial_b, S[k & 1]);
+                       pci, pva, r1, zprev, s->partial_a, s->b3_va ? s->blocks_a3 : s->blocks_a, s->partial_b,
+                       S[k & 1]);
     NFX_LAUNCH_CHECK();
     if (k % check == 0 || k == maxit) {
       NFX_CHECK_HIP(hipMemcpyAsync(s->h_st, S[k & 1], sizeof(MinresState), hipMemcpyDeviceToHost,

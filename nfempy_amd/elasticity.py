@@ -51,6 +51,31 @@ def _vector_coefficient(value, xq, nq, ncomp, device):
     return arr.contiguous()
 
 
+def _block3_structure(n, rowptr, col, device):
+    """3 x 3 block CSR structure of a full CSR pattern (n a multiple of 3): block row
+    pointer, block columns, and for every CSR entry its value slot (3a + b) * n_blocks + j
+    in the block copy (entry (a, b) of block j)."""
+    if n % 3:
+        return None
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(rowptr))
+    cols = col.astype(np.int64)
+    nbc = n // 3
+    key = (rows // 3) * nbc + cols // 3
+    ukey, inv = np.unique(key, return_inverse=True)
+    nblk = len(ukey)
+    brp = np.zeros(nbc + 1, np.int64)
+    np.add.at(brp, ukey // nbc + 1, 1)
+    brp = np.cumsum(brp)
+    dest = ((rows % 3) * 3 + cols % 3) * nblk + inv
+    if 9 * nblk > np.iinfo(np.int32).max:
+        return None
+    t = torch()
+    return {"n_blocks": nblk, "browptr": to_dev(brp.astype(np.int32), device=device),
+            "bcol": to_dev((ukey % nbc).astype(np.int32), device=device),
+            "dest": to_dev(dest.astype(np.int32), device=device),
+            "bval": t.zeros(9 * nblk, dtype=t.float32, device=device)}
+
+
 def _pair_slots(slot):
     """(E, NL (NL + 1) / 2) codes, one per local upper-triangle entry (row-major), from an
     entry-major (NL * NL, E) map of an upper (SBAIJ) operator: the code of whichever of
@@ -93,6 +118,8 @@ class ElasticitySystem:
         t = torch()
         self.space = space
         self.device = device
+        # inner MINRES on the 3 x 3 block copy of the operator (False: the scalar CSR)
+        self.use_block3 = True
         names = space.names
         self.s_sp, self.u_sp, self.t_sp = (space.discrete_spaces[n] for n in names)
         s_sp = self.s_sp
@@ -213,6 +240,8 @@ class ElasticitySystem:
                 "full_rowptr": to_dev(frp, device=dev), "full_col": to_dev(fcol, device=dev),
                 "full_src": to_dev(fsrc, device=dev), "full_nnz": int(len(fcol)),
             }
+            if key == "A":
+                self.ops[key]["block3"] = _block3_structure(self.n, frp, fcol, dev)
 
     def _setup_bc(self, mesh, device):
         s_sp = self.s_sp
@@ -418,7 +447,14 @@ class ElasticitySystem:
 
         sign, pc = self.preconditioner(kind=pc_kind)
         rp, col, v32, v64 = self.full("A")
-        solver = MinresIR(rp, col, v32, v64, sign=None, pc=pc)
+        b3 = self.ops["A"].get("block3")
+        block3 = None
+        if b3 is not None and self.use_block3:
+            # the block copy of the FP32 values (explicit zeros inside blocks stay zero)
+            nat.call("nfx_md_scatter", int(v32.numel()), nat.ptr(v32), nat.ptr(b3["dest"]), 0, nat.ptr(b3["bval"]),
+                     nat.stream_handle())
+            block3 = (b3["n_blocks"], b3["browptr"], b3["bcol"], b3["bval"])
+        solver = MinresIR(rp, col, v32, v64, sign=None, pc=pc, block3=block3)
         x, info = solver.solve(-self.rhs64, rtol=rtol, inner_rtol=inner_rtol, max_inner=max_inner,
                                max_outer=max_outer)
         solver.close()

@@ -28,9 +28,11 @@ from .device import torch
 class MinresIR:
     """FP32 MINRES + FP64 refinement on one CSR operator resident on the device."""
 
-    def __init__(self, rowptr, col, vals32, vals64, sign=None, pinv=None, stream=None, pc=None):
+    def __init__(self, rowptr, col, vals32, vals64, sign=None, pinv=None, stream=None, pc=None, block3=None):
         """pinv: diagonal preconditioner (FP32 inverse diagonal) or pc: (rowptr, col, val)
-        CSR of a sparse SPD preconditioner P^-1 (FP32 values)."""
+        CSR of a sparse SPD preconditioner P^-1 (FP32 values). block3: optional
+        (n_blocks, browptr, bcol, bval), a 3 x 3 block CSR copy of the FP32 operator that
+        the inner MINRES multiplies with (nfx_solver_set_block3)."""
         t = torch()
         self.n = int(rowptr.numel() - 1)
         self.nnz = int(col.numel())
@@ -46,6 +48,10 @@ class MinresIR:
         h = ctypes.c_void_p()
         nat.call("nfx_solver_create", ctypes.byref(h), self.n, self.nnz, nat.stream_handle(stream))
         self._h = h
+        self.block3 = block3
+        if block3 is not None:
+            nb, brp, bcol, bval = block3
+            nat.call("nfx_solver_set_block3", h, int(nb), nat.ptr(brp), nat.ptr(bcol), nat.ptr(bval))
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
