@@ -46,7 +46,9 @@ extern "C" {
 
 /* ---- library ------------------------------------------------------------ */
 
-/* Version as 10000*major + 100*minor + patch. */
+/* Version as 10000*major + 100*minor + patch (0.2.0: per-element metric argument of the
+ * Darcy row assembly, slot_pairs of the elasticity assembly, the block-LU tail of
+ * nfx_twogrid_pc, block operator of the MINRES solver). */
 int nfx_version(void);
 /* Thread-local description of the last error on this thread. */
 const char* nfx_last_error(void);
