@@ -350,17 +350,21 @@ __global__ __launch_bounds__(256) void k_facet_bc(
 // every column (first incidence then second: the reference's addv order). The
 // block then stores its contiguous range of CSR values with coalesced writes.
 //   inc[2][n_rows]  packed incidence e << 5 | a, -1 = none
-//   pos[n_rows][2][NLP] column position of reference DoF b inside the row (255 = none);
-//                   NLP = row_pos_stride(NL): NL, or NL rounded up to whole words when NL > 8
+//   pos             column position of reference DoF b inside the row, per (row, incidence):
+//                   NL <= 7: one word, 4-bit field b (15 = none); otherwise NLP bytes
+//                   (255 = none), NLP = NL, or NL rounded up to whole words when NL > 8
 //   signs[E]        bit b set: reference DoF b of element e carries a minus sign;
 //                   entry (a, b) enters with the sign (-1)^(bit a xor bit b)
 // Element row a < NB: M[a][b] (b < NB) and -B[k][a] (b = NB + k);
 // row a = NB + k: B[k][b] (b < NB), 0 (b >= NB), residual -sum_q w detJ f psi_k.
 constexpr int kRowsPerBlock = 128;
 
-// Bytes per incidence in the column-position array: the larger elements pad to whole
-// 32-bit words so that a row reads its positions with word loads.
-constexpr int row_pos_stride(int nl) { return nl > 8 ? (nl + 3) / 4 * 4 : nl; }
+// Bytes per incidence in the column-position array: the lowest-order elements (nl <= 7,
+// rows of at most 14 entries) pack the nl positions as 4-bit fields of one 32-bit word
+// (15 = not present: one word load per incidence); the larger elements keep one byte per
+// position, padded to whole words so that a row reads them with word loads.
+constexpr bool row_pos_packed(int nl) { return nl <= 7; }
+constexpr int row_pos_stride(int nl) { return nl <= 7 ? 4 : (nl > 8 ? (nl + 3) / 4 * 4 : nl); }
 
 // Table layout: w[NQ] | phi[NQ][NB][DIM] | div[NQ][NB] | psi[NQ][NP] | Rs[NB][NB][NS] |
 // Bc[NB][NP] | ck[NP]  (NS = DIM (DIM + 1) / 2 symmetric pairs of G = J^T J).
@@ -430,6 +434,7 @@ struct RowLoad {
   R X[GEO ? 1 : 2][DIM + 1][3];
   R gd[2][NS], dj[2];
   R kv[2], fv[2];
+  uint32_t pw[2];  // packed column positions (row_pos_packed)
 };
 
 template <int DIM, int NB, int NP, int NQ, int TILE, typename R, bool GEO = false, int RPT = 1>
@@ -472,6 +477,8 @@ __device__ __forceinline__ void darcy_row_tile(
       in[h].code[j] = c;
       in[h].e[j] = c < 0 ? 0 : c >> 5;
       in[h].a[j] = c & 31;
+      if constexpr (row_pos_packed(NL))
+        in[h].pw[j] = c >= 0 ? reinterpret_cast<const uint32_t*>(pos)[(int64_t)r * 2 + j] : 0u;
     }
   }
 #pragma unroll
@@ -602,7 +609,14 @@ __device__ __forceinline__ void darcy_row_tile(
           res -= wp * L.dj[j] * f[(int64_t)ej * fs_e + q * fs_q];
         }
       }
-      if constexpr (row_pos_stride(NL) % 4 == 0) {
+      if constexpr (row_pos_packed(NL)) {
+#pragma unroll
+        for (int b = 0; b < NL; ++b) {
+          const int p = (L.pw[j] >> (4 * b)) & 15;
+          if (p == 15) continue;
+          sv[off + p] += ((L.sg[j] >> b) & 1u) ? -row[b] : row[b];
+        }
+      } else if constexpr (row_pos_stride(NL) % 4 == 0) {
         // padded to whole words: NLP / 4 word loads instead of NL byte loads
         constexpr int NW = row_pos_stride(NL) / 4;
         const uint32_t* pw = reinterpret_cast<const uint32_t*>(pos) + ((int64_t)r * 2 + j) * NW;

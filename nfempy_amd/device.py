@@ -191,6 +191,22 @@ def check_coefficient(arg, E, nq, what):
         raise ValueError(f"{what}: coefficient array too short ({n} <= {last})")
 
 
+def row_pos_device(pos, nl):
+    """The row kernel's column-position array from the (n, 2, nlp) byte positions (255 =
+    not present): nl <= 7 packs each incidence's nl positions as 4-bit fields of one
+    uint32 (15 = not present; rows of at most 14 entries), otherwise the bytes as they
+    are (the kernel's row_pos_stride)."""
+    if nl > 7:
+        return pos.reshape(-1)
+    p = pos[:, :, :nl].astype(np.uint32)
+    if np.any((p != 255) & (p >= 15)):
+        raise ValueError("row too long for 4-bit column positions")
+    nib = np.where(p == 255, 15, p)
+    words = (nib << (4 * np.arange(nl, dtype=np.uint32))[None, None, :]).sum(axis=2, dtype=np.uint64)
+    words |= np.uint64(0xFFFFFFFF) & ~np.uint64((1 << (4 * nl)) - 1)  # unused fields: 15
+    return np.ascontiguousarray(words.astype(np.uint32)).view(np.uint8).reshape(-1)
+
+
 class DarcySystem:
     """Mixed Darcy RT_k - P_{k-1} on one product space, resident on the GPU."""
 
@@ -310,8 +326,8 @@ class DarcySystem:
         # column positions of every reference DoF b of each incident element, inside row r
         rowptr = self.pattern.rowptr.astype(np.int64)
         keys = np.repeat(np.arange(n, dtype=np.int64), np.diff(rowptr)) * n + self.pattern.col
-        # bytes per incidence: nl, padded to whole words for the larger elements (the
-        # kernel's row_pos_stride)
+        # bytes per incidence: nl, padded to whole words for the larger elements; the
+        # device copy packs nl <= 7 into 4-bit fields (row_pos_device)
         nlp = nl if nl <= 8 else (nl + 3) // 4 * 4
         pos = np.full((n, 2, nlp), 255, np.uint8)
         if (np.diff(rowptr).max(initial=0)) >= 255:
@@ -328,7 +344,7 @@ class DarcySystem:
         signs = ((self.ref_sign < 0).astype(np.uint64) << np.arange(nl, dtype=np.uint64)[None, :]).sum(axis=1)
         self.row_inc_np, self.row_pos_np = packed.astype(np.int32), pos
         self.row_inc = to_dev(np.ascontiguousarray(packed.astype(np.int32)), device=device)
-        self.row_pos = to_dev(pos.reshape(-1), device=device)
+        self.row_pos = to_dev(row_pos_device(pos, nl), device=device)
         self.row_signs = to_dev(signs.astype(np.uint32).view(np.int32), device=device)
 
     def _validate(self, mesh):
@@ -366,6 +382,9 @@ class DarcySystem:
             raise ValueError("row column positions shape")
         if np.any((pp != 255) & (pp >= lens[:, None, None])):
             raise ValueError("row column positions out of range")
+        stride = 4 if self.nl <= 7 else pp.shape[2]
+        if self.row_pos.numel() != self.n * 2 * stride:
+            raise ValueError("device row column positions size")
         if self.F:
             if self.nqf != NQF:
                 raise ValueError("facet quadrature does not match the compiled kernel")

@@ -104,6 +104,30 @@ def test_launch_validation(case):
         sysd._validate(sysd.u_sp.mesh)
 
 
+@pytest.mark.parametrize("case", DARCY_CASES[:5], ids=[case_id(c) for c in DARCY_CASES[:5]])
+def test_row_positions_device_layout(case):
+    """The device copy of the row kernel's column positions decodes back to the byte
+    positions: 4-bit fields of one little-endian word per incidence for nl <= 7
+    (15 = absent, unused fields 15), the bytes themselves otherwise."""
+    from nfempy_amd.device import DarcySystem
+
+    sysd = DarcySystem(product_darcy_space(*case), device="cpu")
+    pp, nl = sysd.row_pos_np, sysd.nl
+    dev = sysd.row_pos.numpy()
+    if nl > 7:
+        assert np.array_equal(dev, pp.reshape(-1))
+        return
+    assert dev.size == sysd.n * 2 * 4
+    words = dev.reshape(sysd.n, 2, 4).astype(np.int64)
+    words = words[..., 0] | words[..., 1] << 8 | words[..., 2] << 16 | words[..., 3] << 24
+    for b in range(8):
+        field = (words >> (4 * b)) & 15
+        if b < nl:
+            assert np.array_equal(np.where(field == 15, 255, field), pp[:, :, b])
+        else:
+            assert np.all(field == 15)
+
+
 @pytest.mark.parametrize("dim,k", [(2, 1), (2, 2), (3, 1), (3, 2)])
 def test_reference_tensors_match_quadrature_sums(dim, k):
     """The per-element fast path of the row assembly kernel (reference tensors Rs, Bc,
