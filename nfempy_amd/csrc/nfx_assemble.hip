@@ -351,20 +351,22 @@ __global__ __launch_bounds__(256) void k_facet_bc(
 // block then stores its contiguous range of CSR values with coalesced writes.
 //   inc[2][n_rows]  packed incidence e << 5 | a, -1 = none
 //   pos             column position of reference DoF b inside the row, per (row, incidence):
-//                   NL <= 7: one word, 4-bit field b (15 = none); otherwise NLP bytes
-//                   (255 = none), NLP = NL, or NL rounded up to whole words when NL > 8
+//                   NL <= 6: one word, bits 4b..4b+3 the position of b (15 = none) and bit
+//                   4 NL + b the sign of entry (a, b); otherwise NLP bytes (255 = none),
+//                   NLP = NL, or NL rounded up to whole words when NL > 8
 //   signs[E]        bit b set: reference DoF b of element e carries a minus sign;
 //                   entry (a, b) enters with the sign (-1)^(bit a xor bit b)
 // Element row a < NB: M[a][b] (b < NB) and -B[k][a] (b = NB + k);
 // row a = NB + k: B[k][b] (b < NB), 0 (b >= NB), residual -sum_q w detJ f psi_k.
 constexpr int kRowsPerBlock = 128;
 
-// Bytes per incidence in the column-position array: the lowest-order elements (nl <= 7,
-// rows of at most 14 entries) pack the nl positions as 4-bit fields of one 32-bit word
-// (15 = not present: one word load per incidence); the larger elements keep one byte per
+// Bytes per incidence in the column-position array: the lowest-order elements (nl <= 6,
+// rows of at most 12 entries) pack the nl positions as 4-bit fields of one 32-bit word and
+// the nl entry signs of the incidence's element row above them (one word load per
+// incidence, no gather of the element's sign bits); the larger elements keep one byte per
 // position, padded to whole words so that a row reads them with word loads.
-constexpr bool row_pos_packed(int nl) { return nl <= 7; }
-constexpr int row_pos_stride(int nl) { return nl <= 7 ? 4 : (nl > 8 ? (nl + 3) / 4 * 4 : nl); }
+constexpr bool row_pos_packed(int nl) { return nl <= 6; }
+constexpr int row_pos_stride(int nl) { return nl <= 6 ? 4 : (nl > 8 ? (nl + 3) / 4 * 4 : nl); }
 
 // Table layout: w[NQ] | phi[NQ][NB][DIM] | div[NQ][NB] | psi[NQ][NP] | Rs[NB][NB][NS] |
 // Bc[NB][NP] | ck[NP]  (NS = DIM (DIM + 1) / 2 symmetric pairs of G = J^T J).
@@ -489,8 +491,12 @@ __device__ __forceinline__ void darcy_row_tile(
       L.sg[j] = 0;
       L.kv[j] = L.fv[j] = R(0);
       if (L.code[j] >= 0) {
-        const uint32_t s0 = signs[L.e[j]];
-        L.sg[j] = s0 ^ (((s0 >> L.a[j]) & 1u) ? 0xffffffffu : 0u);
+        if constexpr (row_pos_packed(NL)) {
+          L.sg[j] = L.pw[j] >> (4 * NL);
+        } else {
+          const uint32_t s0 = signs[L.e[j]];
+          L.sg[j] = s0 ^ (((s0 >> L.a[j]) & 1u) ? 0xffffffffu : 0u);
+        }
         if constexpr (GEO && DIM == 3 && sizeof(R) == 4) {
           // two 16-byte loads (the per-element stride is 32 bytes, the array 256-byte aligned)
           const float4* gp = reinterpret_cast<const float4*>(geo + (int64_t)L.e[j] * geo_stride<DIM>());

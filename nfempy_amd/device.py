@@ -191,19 +191,21 @@ def check_coefficient(arg, E, nq, what):
         raise ValueError(f"{what}: coefficient array too short ({n} <= {last})")
 
 
-def row_pos_device(pos, nl):
+def row_pos_device(pos, nl, sign_mask):
     """The row kernel's column-position array from the (n, 2, nlp) byte positions (255 =
-    not present): nl <= 7 packs each incidence's nl positions as 4-bit fields of one
-    uint32 (15 = not present; rows of at most 14 entries), otherwise the bytes as they
-    are (the kernel's row_pos_stride)."""
-    if nl > 7:
+    not present) and the (n, 2) entry-sign masks (bit b: entry (a, b) of the incident
+    element row is negated). nl <= 6 packs each incidence into one uint32: bits 4b..4b+3
+    the position of b (15 = not present; rows of at most 12 entries), bits 4 nl + b the
+    sign mask, the rest zero. Larger elements keep the bytes as they are (the kernel's
+    row_pos_stride)."""
+    if nl > 6:
         return pos.reshape(-1)
-    p = pos[:, :, :nl].astype(np.uint32)
+    p = pos[:, :, :nl].astype(np.uint64)
     if np.any((p != 255) & (p >= 15)):
         raise ValueError("row too long for 4-bit column positions")
     nib = np.where(p == 255, 15, p)
-    words = (nib << (4 * np.arange(nl, dtype=np.uint32))[None, None, :]).sum(axis=2, dtype=np.uint64)
-    words |= np.uint64(0xFFFFFFFF) & ~np.uint64((1 << (4 * nl)) - 1)  # unused fields: 15
+    words = (nib << (4 * np.arange(nl, dtype=np.uint64))[None, None, :]).sum(axis=2, dtype=np.uint64)
+    words |= (sign_mask.astype(np.uint64) & np.uint64((1 << nl) - 1)) << np.uint64(4 * nl)
     return np.ascontiguousarray(words.astype(np.uint32)).view(np.uint8).reshape(-1)
 
 
@@ -342,9 +344,17 @@ class DarcySystem:
                 raise ValueError("incident column missing from the pattern")
             pos[r, j, :nl] = (idx - rowptr[r][:, None]).astype(np.uint8)
         signs = ((self.ref_sign < 0).astype(np.uint64) << np.arange(nl, dtype=np.uint64)[None, :]).sum(axis=1)
+        # entry (a, b) of element e's row a enters with (-1)^(sign bit a xor sign bit b)
+        sign_mask = np.zeros((n, 2), np.uint64)
+        for j in range(2):
+            live = inc[j] >= 0
+            s_e = signs[e_of[j, live]]
+            own = (s_e >> a_of[j, live].astype(np.uint64)) & np.uint64(1)
+            sign_mask[live, j] = s_e ^ (own * np.uint64((1 << nl) - 1))
         self.row_inc_np, self.row_pos_np = packed.astype(np.int32), pos
+        self.row_sign_mask_np = sign_mask
         self.row_inc = to_dev(np.ascontiguousarray(packed.astype(np.int32)), device=device)
-        self.row_pos = to_dev(row_pos_device(pos, nl), device=device)
+        self.row_pos = to_dev(row_pos_device(pos, nl, sign_mask), device=device)
         self.row_signs = to_dev(signs.astype(np.uint32).view(np.int32), device=device)
 
     def _validate(self, mesh):
@@ -382,7 +392,7 @@ class DarcySystem:
             raise ValueError("row column positions shape")
         if np.any((pp != 255) & (pp >= lens[:, None, None])):
             raise ValueError("row column positions out of range")
-        stride = 4 if self.nl <= 7 else pp.shape[2]
+        stride = 4 if self.nl <= 6 else pp.shape[2]
         if self.row_pos.numel() != self.n * 2 * stride:
             raise ValueError("device row column positions size")
         if self.F:

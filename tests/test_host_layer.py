@@ -107,25 +107,34 @@ def test_launch_validation(case):
 @pytest.mark.parametrize("case", DARCY_CASES[:5], ids=[case_id(c) for c in DARCY_CASES[:5]])
 def test_row_positions_device_layout(case):
     """The device copy of the row kernel's column positions decodes back to the byte
-    positions: 4-bit fields of one little-endian word per incidence for nl <= 7
-    (15 = absent, unused fields 15), the bytes themselves otherwise."""
+    positions and the entry signs: for nl <= 6 one little-endian word per incidence,
+    4-bit position fields (15 = absent), then one bit per column b equal to
+    ref_sign[e, a] * ref_sign[e, b] < 0, zeros above; the bytes themselves otherwise."""
     from nfempy_amd.device import DarcySystem
 
     sysd = DarcySystem(product_darcy_space(*case), device="cpu")
     pp, nl = sysd.row_pos_np, sysd.nl
     dev = sysd.row_pos.numpy()
-    if nl > 7:
+    if nl > 6:
         assert np.array_equal(dev, pp.reshape(-1))
         return
     assert dev.size == sysd.n * 2 * 4
     words = dev.reshape(sysd.n, 2, 4).astype(np.int64)
     words = words[..., 0] | words[..., 1] << 8 | words[..., 2] << 16 | words[..., 3] << 24
-    for b in range(8):
+    for b in range(nl):
         field = (words >> (4 * b)) & 15
-        if b < nl:
-            assert np.array_equal(np.where(field == 15, 255, field), pp[:, :, b])
-        else:
-            assert np.all(field == 15)
+        assert np.array_equal(np.where(field == 15, 255, field), pp[:, :, b])
+    assert np.all(words >> (5 * nl) == 0)
+    inc = sysd.row_inc_np
+    for j in range(2):
+        live = np.flatnonzero(inc[j] >= 0)
+        e, a = inc[j][live] >> 5, inc[j][live] & 31
+        for b in range(nl):
+            bit = (words[live, j] >> (4 * nl + b)) & 1
+            want = sysd.ref_sign[e, a] * sysd.ref_sign[e, b] < 0
+            assert np.array_equal(bit.astype(bool), want)
+        dead = np.flatnonzero(inc[j] < 0)
+        assert np.all(words[dead, j] >> (4 * nl) == 0)
 
 
 @pytest.mark.parametrize("dim,k", [(2, 1), (2, 2), (3, 1), (3, 2)])
