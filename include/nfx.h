@@ -89,10 +89,13 @@ int nfx_assemble_darcy(int dim, int k, int n_elem, const int32_t* d_cells,
  * flux rows: 0, before the Dirichlet facet term). No slot map, no staging.
  *  d_inc    (2, n_rows) int32 incidences e*32 + a (reference DoF a), negative = none
  *  d_pos    column position of reference DoF b inside the row, per (row, incidence):
- *           nl <= 7 (lowest-order elements): one uint32 with 4-bit fields, field b the
- *           position (15 = not present); otherwise nlp uint8 (255 = not present), nlp =
- *           nl rounded up to a multiple of 4 when nl > 8 (RT2 elements: word loads)
- *  d_rowptr CSR row pointer; d_signs (E) uint32, bit a = reference DoF a negated
+ *           nl <= 6 (lowest-order elements): one uint32, bits 4b..4b+3 the position
+ *           (15 = not present), bit 4 nl + b set when entry (a, b) is negated (sign bit a
+ *           xor sign bit b of d_signs), higher bits zero; otherwise nlp uint8 (255 = not
+ *           present), nlp = nl, or nl rounded up to a multiple of 4 when nl > 8 (RT2
+ *           elements: word loads)
+ *  d_rowptr CSR row pointer; d_signs (E) uint32, bit a = reference DoF a negated (read
+ *           for nl > 6 only)
  *  d_geo    optional per-element metric from nfx_element_geometry (same precision as
  *           the values): the kernel then reads it instead of deriving it from d_cells /
  *           d_coords (same values); NULL derives it; 16-byte aligned
