@@ -496,7 +496,8 @@ def main():
                      nat.ptr(sysd.tables32), nat.ptr(kinv.get(False)), kinv.se, kinv.sq, nat.ptr(fsrc.get(False)),
                      fsrc.se, fsrc.sq, sysd.F, nat.ptr(sysd.bc_facet), nat.ptr(sysd.bc_combo),
                      nat.ptr(sysd.bc_trace32), nat.ptr(sysd.bc_fw32), sysd.nqf, nat.ptr(pdv.get(False)),
-                     nat.ptr(sysd.bc_rows), nat.ptr(geo), 0, nat.ptr(sysd.vals32), nat.ptr(sysd.rhs32), s)
+                     nat.ptr(sysd.bc_rows), nat.ptr(geo), nat.ptr(sysd.row_tile_map), sysd.row_tile_map.numel(), 0,
+                     nat.ptr(sysd.vals32), nat.ptr(sysd.rhs32), s)
         ev1.record()
         torch.cuda.synchronize()
         return ev0.elapsed_time(ev1) / reps

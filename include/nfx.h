@@ -99,6 +99,12 @@ int nfx_assemble_darcy(int dim, int k, int n_elem, const int32_t* d_cells,
  *  d_geo    optional per-element metric from nfx_element_geometry (same precision as
  *           the values): the kernel then reads it instead of deriving it from d_cells /
  *           d_coords (same values); NULL derives it; 16-byte aligned
+ *  d_tile_map optional (n_tile_map) int32 launch order of the row tiles (tile t = CSR
+ *           rows [128 t, 128 t + 128)): slot i holds tile d_tile_map[i], -1 (or any value
+ *           outside the tile range) an empty slot; every tile must appear once. Workgroups
+ *           go to the 8 XCDs round-robin, so slots x, x + 8, ... run on XCD x: a map that
+ *           gives each XCD the tiles of one element range keeps each element's data in one
+ *           L2. NULL = tiles in row order. Results do not depend on the map.
  * Other arguments as nfx_assemble_darcy. */
 int nfx_assemble_darcy_rows(int dim, int k, int n_rows, const int32_t* d_inc,
                             const uint8_t* d_pos, const int32_t* d_rowptr,
@@ -106,7 +112,8 @@ int nfx_assemble_darcy_rows(int dim, int k, int n_rows, const int32_t* d_inc,
                             const void* d_coords, const void* d_tables,
                             const void* d_kinv, int kinv_stride_e, int kinv_stride_q,
                             const void* d_f, int f_stride_e, int f_stride_q, const void* d_geo,
-                            int fp64, void* d_vals, void* d_rhs, void* stream);
+                            const int32_t* d_tile_map, int n_tile_map, int fp64, void* d_vals,
+                            void* d_rhs, void* stream);
 
 /* Per-element metric of the row kernels, computed once per mesh (the mapping of
  * src/geometry/mapping.py evaluated on the vertices): for element e, geo[e * gs + s] =
@@ -173,7 +180,8 @@ int nfx_assemble_darcy_rows_bc(int dim, int k, int n_rows, const int32_t* d_inc,
                                int f_stride_e, int f_stride_q, int n_facets,
                                const int32_t* d_facet, const int32_t* d_combo,
                                const void* d_trace, const void* d_fw, int nqf, const void* d_pD,
-                               const int32_t* d_rows, const void* d_geo, int fp64, void* d_vals,
+                               const int32_t* d_rows, const void* d_geo,
+                               const int32_t* d_tile_map, int n_tile_map, int fp64, void* d_vals,
                                void* d_rhs, void* stream);
 
 /* ---- assembly: dual-mixed elasticity (BDM1 - P0 - P0, tetrahedra) --------

@@ -44,11 +44,12 @@ SIGNATURES = {
                                    c_void_p, c_void_p]),
     "nfx_assemble_darcy_rows": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                         c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_int,
-                                        c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+                                        c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     "nfx_assemble_darcy_rows_bc": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                            c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_int,
                                            c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p,
-                                           c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+                                           c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p,
+                                           c_void_p]),
     "nfx_element_geometry": (c_int, [c_int, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     "nfx_md_coupling": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                 c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p]),

@@ -688,29 +688,38 @@ __attribute__((amdgpu_waves_per_eu(row_kernel_min_waves<NB, R>(), 8))) void k_da
     const int32_t* __restrict__ rowptr, const uint32_t* __restrict__ signs,
     const int32_t* __restrict__ cells, const R* __restrict__ coords, const R* __restrict__ tab,
     const R* __restrict__ kinv, int ks_e, int ks_q, const R* __restrict__ f, int fs_e, int fs_q,
-    R* __restrict__ vals, R* __restrict__ rhs, DarcyRowBc<R> bc, const R* __restrict__ geo) {
+    R* __restrict__ vals, R* __restrict__ rhs, DarcyRowBc<R> bc, const R* __restrict__ geo,
+    const int32_t* __restrict__ tile_map, int n_map) {
   constexpr int NL = NB + NP;
   constexpr int NT = DarcyTab<DIM, NB, NP, NQ>::NT;
   constexpr int RPT = row_kernel_rpt<NB, R>();
   __shared__ R st[NT];
   __shared__ R sv[kRowsPerBlock * RPT * 2 * NL];
-  // Tiles in launch order, block b taking tiles b, b + grid, ... (one tile per block for
-  // the lowest-order elements; the larger elements run a persistent grid that loads the
-  // reference table into LDS once). An XCD-aware order (block b taking tile
-  // (b % 8) * (nb / 8) + b / 8, so that each of the 8 XCDs walks a contiguous range and
-  // keeps the shared elements in its own L2) measured 7 % slower at 32^3 x 6 (16.9-17.6
-  // us against 15.6-16.1 us, alternating runs of both builds).
+  // Slot i of the launch holds tile tile_map[i] (tile i without a map; entries outside
+  // [0, n_tiles) are empty slots). Block b takes slots b, b + grid, ... (one slot per
+  // block for the lowest-order elements; the larger elements run a persistent grid that
+  // loads the reference table into LDS once). Workgroups are dispatched to the 8 XCDs
+  // round-robin, so a map whose slots congruent to x mod 8 hold the tiles of one element
+  // range keeps each element's data in one XCD's L2.
+  const int n_tiles = (n_rows + kRowsPerBlock * RPT - 1) / (kRowsPerBlock * RPT);
   if constexpr (NB <= 4) {
-    darcy_row_tile<DIM, NB, NP, NQ, kRowsPerBlock, R, GEO, RPT>(blockIdx.x * kRowsPerBlock * RPT, n_rows, inc,
-                                                                pos, rowptr, signs, cells, coords, tab, kinv, ks_e,
-                                                                ks_q, f, fs_e, fs_q, vals, rhs, st, sv, bc, true, geo);
+    const int t = tile_map ? tile_map[blockIdx.x] : (int)blockIdx.x;
+    if (t < 0 || t >= n_tiles) return;
+    darcy_row_tile<DIM, NB, NP, NQ, kRowsPerBlock, R, GEO, RPT>(t * kRowsPerBlock * RPT, n_rows, inc, pos, rowptr,
+                                                                signs, cells, coords, tab, kinv, ks_e, ks_q, f,
+                                                                fs_e, fs_q, vals, rhs, st, sv, bc, true, geo);
   } else {
-    const int n_tiles = (n_rows + kRowsPerBlock * RPT - 1) / (kRowsPerBlock * RPT);
-    for (int t = blockIdx.x; t < n_tiles; t += gridDim.x)
+    const int n_slots = tile_map ? n_map : n_tiles;
+    bool first = true;
+    for (int i = blockIdx.x; i < n_slots; i += gridDim.x) {
+      const int t = tile_map ? tile_map[i] : i;
+      if (t < 0 || t >= n_tiles) continue;
       darcy_row_tile<DIM, NB, NP, NQ, kRowsPerBlock, R, GEO, RPT>(t * kRowsPerBlock * RPT, n_rows, inc, pos,
                                                                   rowptr, signs, cells, coords, tab, kinv, ks_e,
                                                                   ks_q, f, fs_e, fs_q, vals, rhs, st, sv, bc,
-                                                                  t == (int)blockIdx.x, geo);
+                                                                  first, geo);
+      first = false;
+    }
   }
 }
 
@@ -806,23 +815,27 @@ static int launch_darcy_rows(int n_rows, const int32_t* inc, const uint8_t* pos,
                              hipStream_t s, int n_bc = 0, const int32_t* facet = nullptr,
                              const int32_t* combo = nullptr, const void* trace = nullptr,
                              const void* fw = nullptr, const void* pD = nullptr,
-                             const int32_t* bc_rows = nullptr, const void* geo = nullptr) {
+                             const int32_t* bc_rows = nullptr, const void* geo = nullptr,
+                             const int32_t* tile_map = nullptr, int n_map = 0) {
   const DarcyRowBc<R> bc{n_bc, facet, combo, static_cast<const R*>(trace), static_cast<const R*>(fw),
                          static_cast<const R*>(pD), bc_rows};
-  // one tile per block for the lowest-order elements; a persistent grid (8 blocks per
-  // CU of 256) for the larger tables
-  const int n_tiles = grid_for(n_rows, kRowsPerBlock * row_kernel_rpt<NB, R>());
-  const int grid = NB <= 4 ? n_tiles : std::min(n_tiles, 2048);
+  // one slot per block for the lowest-order elements; a persistent grid (8 blocks per
+  // CU of 256, a multiple of the 8 XCDs) for the larger tables
+  const int n_slots = tile_map ? n_map : grid_for(n_rows, kRowsPerBlock * row_kernel_rpt<NB, R>());
+  const int grid = NB <= 4 ? n_slots : std::min(n_slots, 2048);
+  if (grid == 0) return NFX_OK;
   if (geo)
     hipLaunchKernelGGL((k_darcy_rows<DIM, NB, NP, NQ, R, true>), dim3(grid), dim3(kRowsPerBlock), 0, s, n_rows, inc,
                        pos, rowptr, signs, cells, static_cast<const R*>(coords), static_cast<const R*>(tab),
                        static_cast<const R*>(kinv), kse, ksq, static_cast<const R*>(f), fse, fsq,
-                       static_cast<R*>(vals), static_cast<R*>(rhs), bc, static_cast<const R*>(geo));
+                       static_cast<R*>(vals), static_cast<R*>(rhs), bc, static_cast<const R*>(geo), tile_map,
+                       n_map);
   else
     hipLaunchKernelGGL((k_darcy_rows<DIM, NB, NP, NQ, R, false>), dim3(grid), dim3(kRowsPerBlock), 0, s, n_rows,
                        inc, pos, rowptr, signs, cells, static_cast<const R*>(coords), static_cast<const R*>(tab),
                        static_cast<const R*>(kinv), kse, ksq, static_cast<const R*>(f), fse, fsq,
-                       static_cast<R*>(vals), static_cast<R*>(rhs), bc, static_cast<const R*>(nullptr));
+                       static_cast<R*>(vals), static_cast<R*>(rhs), bc, static_cast<const R*>(nullptr), tile_map,
+                       n_map);
   NFX_LAUNCH_CHECK();
   return NFX_OK;
 }
@@ -1137,8 +1150,10 @@ extern "C" int nfx_assemble_darcy_rows(int dim, int k, int n_rows, const int32_t
                                        const void* d_coords, const void* d_tables,
                                        const void* d_kinv, int kinv_stride_e, int kinv_stride_q,
                                        const void* d_f, int f_stride_e, int f_stride_q, const void* d_geo,
-                                       int fp64, void* d_vals, void* d_rhs, void* stream) {
+                                       const int32_t* d_tile_map, int n_tile_map, int fp64, void* d_vals,
+                                       void* d_rhs, void* stream) {
   NFX_REQUIRE(n_rows >= 0, "nfx_assemble_darcy_rows: n_rows < 0");
+  NFX_REQUIRE(d_tile_map == nullptr || n_tile_map >= 0, "nfx_assemble_darcy_rows: n_tile_map < 0");
   NFX_REQUIRE(reinterpret_cast<uintptr_t>(d_geo) % 16 == 0, "nfx_assemble_darcy_rows: d_geo not 16-byte aligned");
   if (n_rows == 0) return NFX_OK;
   NFX_REQUIRE(d_inc && d_pos && d_rowptr && d_signs && d_cells && d_coords && d_tables && d_kinv &&
@@ -1149,11 +1164,13 @@ extern "C" int nfx_assemble_darcy_rows(int dim, int k, int n_rows, const int32_t
   return fp64 ? launch_darcy_rows<DIM, NB, NP, NQ, double>(                                        \
                     n_rows, d_inc, d_pos, d_rowptr, d_signs, d_cells, d_coords, d_tables, d_kinv,  \
                     kinv_stride_e, kinv_stride_q, d_f, f_stride_e, f_stride_q, d_vals, d_rhs, s,   \
-                    0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, d_geo)                \
+                    0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, d_geo, d_tile_map,    \
+                    n_tile_map)                                                                    \
               : launch_darcy_rows<DIM, NB, NP, NQ, float>(                                         \
                     n_rows, d_inc, d_pos, d_rowptr, d_signs, d_cells, d_coords, d_tables, d_kinv,  \
                     kinv_stride_e, kinv_stride_q, d_f, f_stride_e, f_stride_q, d_vals, d_rhs, s,   \
-                    0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, d_geo)
+                    0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, d_geo, d_tile_map,    \
+                    n_tile_map)
   if (dim == 2 && k == 1) NFX_DROWS(2, 3, 1, 4);
   if (dim == 2 && k == 2) NFX_DROWS(2, 8, 3, 9);
   if (dim == 3 && k == 1) NFX_DROWS(3, 4, 1, 8);
@@ -1168,9 +1185,10 @@ extern "C" int nfx_assemble_darcy_rows_bc(
     const uint32_t* d_signs, const int32_t* d_cells, const void* d_coords, const void* d_tables,
     const void* d_kinv, int kinv_stride_e, int kinv_stride_q, const void* d_f, int f_stride_e,
     int f_stride_q, int n_facets, const int32_t* d_facet, const int32_t* d_combo, const void* d_trace,
-    const void* d_fw, int nqf, const void* d_pD, const int32_t* d_rows, const void* d_geo, int fp64,
-    void* d_vals, void* d_rhs, void* stream) {
+    const void* d_fw, int nqf, const void* d_pD, const int32_t* d_rows, const void* d_geo,
+    const int32_t* d_tile_map, int n_tile_map, int fp64, void* d_vals, void* d_rhs, void* stream) {
   NFX_REQUIRE(n_rows >= 0 && n_facets >= 0, "nfx_assemble_darcy_rows_bc: negative size");
+  NFX_REQUIRE(d_tile_map == nullptr || n_tile_map >= 0, "nfx_assemble_darcy_rows_bc: n_tile_map < 0");
   if (n_rows == 0) return NFX_OK;
   NFX_REQUIRE(d_inc && d_pos && d_rowptr && d_signs && d_cells && d_coords && d_tables && d_kinv &&
                   d_f && d_vals && d_rhs,
@@ -1189,11 +1207,13 @@ extern "C" int nfx_assemble_darcy_rows_bc(
     return fp64 ? launch_darcy_rows<DIM, NB, NP, NQ, double>(                                         \
                       n_rows, d_inc, d_pos, d_rowptr, d_signs, d_cells, d_coords, d_tables, d_kinv,   \
                       kinv_stride_e, kinv_stride_q, d_f, f_stride_e, f_stride_q, d_vals, d_rhs, s,     \
-                      n_facets * FD, d_facet, d_combo, d_trace, d_fw, pD, d_rows, d_geo)               \
+                      n_facets * FD, d_facet, d_combo, d_trace, d_fw, pD, d_rows, d_geo, d_tile_map,   \
+                      n_tile_map)                                                                     \
                 : launch_darcy_rows<DIM, NB, NP, NQ, float>(                                          \
                       n_rows, d_inc, d_pos, d_rowptr, d_signs, d_cells, d_coords, d_tables, d_kinv,   \
                       kinv_stride_e, kinv_stride_q, d_f, f_stride_e, f_stride_q, d_vals, d_rhs, s,     \
-                      n_facets * FD, d_facet, d_combo, d_trace, d_fw, pD, d_rows, d_geo);              \
+                      n_facets * FD, d_facet, d_combo, d_trace, d_fw, pD, d_rows, d_geo, d_tile_map,   \
+                      n_tile_map);                                                                    \
   } while (0)
   if (dim == 2 && k == 1) NFX_DROWSBC(2, 3, 1, 4);
   if (dim == 2 && k == 2) NFX_DROWSBC(2, 8, 3, 9);

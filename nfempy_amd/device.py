@@ -209,6 +209,30 @@ def row_pos_device(pos, nl, sign_mask):
     return np.ascontiguousarray(words.astype(np.uint32)).view(np.uint8).reshape(-1)
 
 
+ROW_TILE = 128  # rows per workgroup of the row kernel (kRowsPerBlock)
+N_XCD = 8  # accelerator complex dies; workgroups are dispatched to them round-robin
+
+
+def xcd_tile_map(first_elem, n_rows, tile=ROW_TILE, n_xcd=N_XCD):
+    """Launch slots of the row kernel: tiles (runs of ``tile`` consecutive CSR rows)
+    ordered by the median element of their rows' first incidences and cut into
+    ``n_xcd`` equal runs; run x fills slots x, x + n_xcd, ... (ascending tile order,
+    -1 = empty slot). Round-robin dispatch then places the face rows and the cell rows of
+    one element range on the same XCD, so each element's metric and coefficient are
+    fetched into one L2 instead of up to five."""
+    nt = -(-n_rows // tile)
+    if nt == 0:
+        return np.zeros(0, np.int32)
+    idx = np.minimum(np.arange(nt * tile), n_rows - 1).reshape(nt, tile)
+    key = np.median(first_elem[idx], axis=1)
+    order = np.argsort(key, kind="stable")
+    runs = np.array_split(order, n_xcd)
+    slots = np.full((len(runs[0]), n_xcd), -1, np.int32)
+    for x, run in enumerate(runs):
+        slots[: len(run), x] = np.sort(run)
+    return slots.reshape(-1)
+
+
 class DarcySystem:
     """Mixed Darcy RT_k - P_{k-1} on one product space, resident on the GPU."""
 
@@ -353,6 +377,9 @@ class DarcySystem:
             sign_mask[live, j] = s_e ^ (own * np.uint64((1 << nl) - 1))
         self.row_inc_np, self.row_pos_np = packed.astype(np.int32), pos
         self.row_sign_mask_np = sign_mask
+        self.row_tile_map_np = xcd_tile_map(e_of[0], n)
+        self.row_tile_map = to_dev(self.row_tile_map_np, device=device)
+        self.use_tile_map = True
         self.row_inc = to_dev(np.ascontiguousarray(packed.astype(np.int32)), device=device)
         self.row_pos = to_dev(row_pos_device(pos, nl, sign_mask), device=device)
         self.row_signs = to_dev(signs.astype(np.uint32).view(np.int32), device=device)
@@ -392,6 +419,10 @@ class DarcySystem:
             raise ValueError("row column positions shape")
         if np.any((pp != 255) & (pp >= lens[:, None, None])):
             raise ValueError("row column positions out of range")
+        tm = self.row_tile_map_np
+        nt = -(-self.n // ROW_TILE)
+        if not np.array_equal(np.sort(tm[tm >= 0]), np.arange(nt)) or np.any(tm < -1):
+            raise ValueError("row tile map is not a permutation of the tiles")
         stride = 4 if self.nl <= 6 else pp.shape[2]
         if self.row_pos.numel() != self.n * 2 * stride:
             raise ValueError("device row column positions size")
@@ -483,11 +514,12 @@ class DarcySystem:
         coords = nat.ptr(self.coords64 if fp64 else self.coords)
         tables = nat.ptr(self.tables64 if fp64 else self.tables32)
         geo = nat.ptr(self.geo64 if fp64 else self.geo32) if self.use_geometry else None
+        tmap, n_map = (nat.ptr(self.row_tile_map), self.row_tile_map.numel()) if self.use_tile_map else (None, 0)
         if p_dirichlet is None:
             nat.call("nfx_assemble_darcy_rows", self.dim, self.k, self.n, nat.ptr(self.row_inc),
                      nat.ptr(self.row_pos), nat.ptr(self.rowptr), nat.ptr(self.row_signs), nat.ptr(self.cells),
                      coords, tables, nat.ptr(ka.get(fp64)), ka.se, ka.sq, nat.ptr(fa.get(fp64)), fa.se, fa.sq,
-                     geo, int(fp64), nat.ptr(vals), nat.ptr(rg), s)
+                     geo, tmap, n_map, int(fp64), nat.ptr(vals), nat.ptr(rg), s)
             return vals, rg
         pd = self.dirichlet_values(p_dirichlet)
         nat.call("nfx_assemble_darcy_rows_bc", self.dim, self.k, self.n, nat.ptr(self.row_inc),
@@ -496,7 +528,7 @@ class DarcySystem:
                  self.F, nat.ptr(self.bc_facet), nat.ptr(self.bc_combo),
                  nat.ptr(self.bc_trace64 if fp64 else self.bc_trace32),
                  nat.ptr(self.bc_fw64 if fp64 else self.bc_fw32), self.nqf, nat.ptr(pd.get(fp64)),
-                 nat.ptr(self.bc_rows), geo, int(fp64), nat.ptr(vals), nat.ptr(rg), s)
+                 nat.ptr(self.bc_rows), geo, tmap, n_map, int(fp64), nat.ptr(vals), nat.ptr(rg), s)
         return vals, rg
 
     def dirichlet_values(self, p_dirichlet):

@@ -90,6 +90,12 @@ def test_darcy_assembly_parity(case):
         torch.cuda.synchronize()
         assert float((vg - vv).abs().max()) <= tol * float(vg.abs().max()), ("metric path", fp64)
         assert float((rgeo - rv).abs().max()) <= tol * max(float(rgeo.abs().max()), 1e-300), ("metric path", fp64)
+        # the launch order of the row tiles (XCD-grouped map or row order) changes nothing
+        sysd.use_tile_map = False
+        vt, rt = (a.clone() for a in sysd.assemble(fun["kappa"], fun["rhs"], ex["p"], fp64=fp64))
+        sysd.use_tile_map = True
+        torch.cuda.synchronize()
+        assert torch.equal(vt, vg) and torch.equal(rt, rgeo), ("tile map", fp64)
     # run-to-run determinism: a second assembly is bitwise identical
     v1 = sysd.assemble_elements(fun["kappa"], fun["rhs"], fp64=False)[0].clone()
     v2 = sysd.assemble_elements(fun["kappa"], fun["rhs"], fp64=False)[0].clone()
