@@ -192,7 +192,13 @@ def ptr(t) -> int | None:
 
 
 def stream_handle(stream=None) -> int | None:
+    """HIP stream handle of ``stream`` (a torch stream), or of torch's current stream on
+    the current device (read without building a stream object: ~0.1 us instead of ~3)."""
     import torch
 
-    s = stream if stream is not None else torch.cuda.current_stream()
-    return s.cuda_stream
+    if stream is not None:
+        return stream.cuda_stream
+    raw = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+    if raw is not None:
+        return raw(torch._C._cuda_getDevice())
+    return torch.cuda.current_stream().cuda_stream

@@ -273,6 +273,7 @@ class DarcySystem:
         # derives it in the kernel; same values)
         self.geo32 = self.geo64 = None
         self.use_geometry = True
+        self._launch_cache = {}
         if str(device).startswith("cuda"):
             t = torch()
             gs = 8 if D == 3 else 4
@@ -507,6 +508,14 @@ class DarcySystem:
         s = nat.stream_handle(stream)
         vals = self.vals64 if fp64 else self.vals32
         rg = self.rhs64 if fp64 else self.rhs32
+        # repeated launches with the same device coefficients (CoefficientArg objects) reuse
+        # the prepared argument list; the entry holds the objects, so an identity match is
+        # the same arrays (callables and host arrays are re-evaluated every call)
+        key = (fp64, self.use_geometry, self.use_tile_map, p_dirichlet is None)
+        ent = self._launch_cache.get(key)
+        if ent is not None and ent[0] is kappa and ent[1] is rhs and ent[2] is p_dirichlet:
+            nat.check(ent[3](*ent[4], s), ent[5])
+            return vals, rg
         ka = kappa if isinstance(kappa, CoefficientArg) else coefficient(kappa, self.xq, self.nq, invert=True, device=self.device)
         fa = rhs if isinstance(rhs, CoefficientArg) else coefficient(rhs, self.xq, self.nq, device=self.device)
         check_coefficient(ka, self.E, self.nq, "kappa")
@@ -516,19 +525,27 @@ class DarcySystem:
         geo = nat.ptr(self.geo64 if fp64 else self.geo32) if self.use_geometry else None
         tmap, n_map = (nat.ptr(self.row_tile_map), self.row_tile_map.numel()) if self.use_tile_map else (None, 0)
         if p_dirichlet is None:
-            nat.call("nfx_assemble_darcy_rows", self.dim, self.k, self.n, nat.ptr(self.row_inc),
-                     nat.ptr(self.row_pos), nat.ptr(self.rowptr), nat.ptr(self.row_signs), nat.ptr(self.cells),
-                     coords, tables, nat.ptr(ka.get(fp64)), ka.se, ka.sq, nat.ptr(fa.get(fp64)), fa.se, fa.sq,
-                     geo, tmap, n_map, int(fp64), nat.ptr(vals), nat.ptr(rg), s)
-            return vals, rg
-        pd = self.dirichlet_values(p_dirichlet)
-        nat.call("nfx_assemble_darcy_rows_bc", self.dim, self.k, self.n, nat.ptr(self.row_inc),
-                 nat.ptr(self.row_pos), nat.ptr(self.rowptr), nat.ptr(self.row_signs), nat.ptr(self.cells),
-                 coords, tables, nat.ptr(ka.get(fp64)), ka.se, ka.sq, nat.ptr(fa.get(fp64)), fa.se, fa.sq,
-                 self.F, nat.ptr(self.bc_facet), nat.ptr(self.bc_combo),
-                 nat.ptr(self.bc_trace64 if fp64 else self.bc_trace32),
-                 nat.ptr(self.bc_fw64 if fp64 else self.bc_fw32), self.nqf, nat.ptr(pd.get(fp64)),
-                 nat.ptr(self.bc_rows), geo, tmap, n_map, int(fp64), nat.ptr(vals), nat.ptr(rg), s)
+            name = "nfx_assemble_darcy_rows"
+            args = (self.dim, self.k, self.n, nat.ptr(self.row_inc), nat.ptr(self.row_pos), nat.ptr(self.rowptr),
+                    nat.ptr(self.row_signs), nat.ptr(self.cells), coords, tables, nat.ptr(ka.get(fp64)), ka.se,
+                    ka.sq, nat.ptr(fa.get(fp64)), fa.se, fa.sq, geo, tmap, n_map, int(fp64), nat.ptr(vals),
+                    nat.ptr(rg))
+            held = (ka, fa)
+        else:
+            pd = self.dirichlet_values(p_dirichlet)
+            name = "nfx_assemble_darcy_rows_bc"
+            args = (self.dim, self.k, self.n, nat.ptr(self.row_inc), nat.ptr(self.row_pos), nat.ptr(self.rowptr),
+                    nat.ptr(self.row_signs), nat.ptr(self.cells), coords, tables, nat.ptr(ka.get(fp64)), ka.se,
+                    ka.sq, nat.ptr(fa.get(fp64)), fa.se, fa.sq, self.F, nat.ptr(self.bc_facet),
+                    nat.ptr(self.bc_combo), nat.ptr(self.bc_trace64 if fp64 else self.bc_trace32),
+                    nat.ptr(self.bc_fw64 if fp64 else self.bc_fw32), self.nqf, nat.ptr(pd.get(fp64)),
+                    nat.ptr(self.bc_rows), geo, tmap, n_map, int(fp64), nat.ptr(vals), nat.ptr(rg))
+            held = (ka, fa, pd)
+        fn = getattr(nat.lib(), name)
+        nat.check(fn(*args, s), name)
+        if all(isinstance(c, CoefficientArg) for c in (kappa, rhs, p_dirichlet) if c is not None):
+            cargs = tuple(t(v) for t, v in zip(fn.argtypes, args))
+            self._launch_cache[key] = (kappa, rhs, p_dirichlet, fn, cargs, name, held)
         return vals, rg
 
     def dirichlet_values(self, p_dirichlet):

@@ -128,3 +128,37 @@ def test_darcy_solve_parity(case):
     assert res <= 1.2e-8, res
     xref = spla.spsolve(A.tocsc(), bref)
     assert np.linalg.norm(xs - xref) <= 1e-6 * np.linalg.norm(xref)
+
+
+@pytest.mark.parametrize("dim,k", [(3, 1), (2, 2)])
+def test_repeated_assembly_follows_coefficients(dim, k):
+    """Repeated warm assemblies with device coefficients (the prepared-launch path):
+    alternating two permeability fields gives each field's own values bitwise, and a
+    fresh system assembled once gives the same."""
+    import torch
+
+    from nfempy_amd.device import CoefficientArg, DarcySystem
+
+    def system():
+        return DarcySystem(product_darcy_space(dim, n=3, k=k))
+
+    sysd = system()
+    dev = sysd.vals32.device
+    g = torch.Generator().manual_seed(7)
+    k1 = CoefficientArg(torch.rand(sysd.E, generator=g, dtype=torch.float64).add_(0.5).to(dev), 1, 0)
+    k2 = CoefficientArg(torch.rand(sysd.E, generator=g, dtype=torch.float64).add_(0.5).to(dev), 1, 0)
+    f = CoefficientArg(torch.ones(1, dtype=torch.float64, device=dev), 0, 0)
+    pd = CoefficientArg(torch.linspace(0, 1, sysd.F * sysd.nqf, dtype=torch.float64, device=dev), sysd.nqf, 1)
+    outs = []
+    for kk in (k1, k2, k1, k2, k1):
+        for fp64 in (False, True):
+            v, r = sysd.assemble(kk, f, pd, fp64=fp64)
+            outs.append((v.clone(), r.clone()))
+    torch.cuda.synchronize()
+    for i in range(4, len(outs)):
+        assert torch.equal(outs[i][0], outs[i - 4][0]) and torch.equal(outs[i][1], outs[i - 4][1]), i
+    assert not torch.equal(outs[0][0], outs[2][0])
+    fresh = system()
+    v, r = fresh.assemble(k2, f, pd, fp64=False)
+    torch.cuda.synchronize()
+    assert torch.equal(v, outs[2][0]) and torch.equal(r, outs[2][1])
