@@ -67,8 +67,17 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int n, const int32_t* __restric
   const int rows_per_grid = gridDim.x * (kBlock / LANES);
   for (int row = blockIdx.x * (kBlock / LANES) + threadIdx.x / LANES; row < n;
        row += rows_per_grid) {
+    // three entries per lane per pass, predicated: the column / value loads of a pass are
+    // issued together, then the gathers of x (one pass covers rows of up to 3 LANES entries)
+    const int j1 = rp[row + 1];
     float s = 0.f;
-    for (int j = rp[row] + lane; j < rp[row + 1]; j += LANES) s += va[j] * x[ci[j]];
+    for (int j = rp[row] + lane; j < j1; j += 3 * LANES) {
+      const bool pb = j + LANES < j1, pc = j + 2 * LANES < j1;
+      const int ca = ci[j], cb = pb ? ci[j + LANES] : 0, cc = pc ? ci[j + 2 * LANES] : 0;
+      const float va_ = va[j], vb = pb ? va[j + LANES] : 0.f, vc = pc ? va[j + 2 * LANES] : 0.f;
+      const float xa = x[ca], xb = pb ? x[cb] : 0.f, xc = pc ? x[cc] : 0.f;
+      s += va_ * xa + vb * xb + vc * xc;
+    }
     s = group_sum<LANES>(s);
     if (lane == 0) y[row] = alpha * s + (beta == 0.f ? 0.f : beta * y[row]);
   }
@@ -258,7 +267,14 @@ __global__ __launch_bounds__(kBlock) void k_minres_a(
        row += rows_per_grid) {
     float sum = 0.f;
     if (!final_only)
-      for (int j = rp[row] + lane; j < rp[row + 1]; j += LANES) sum += va[j] * z[ci[j]];
+      for (int j = rp[row] + lane, j1 = rp[row + 1]; j < j1; j += 3 * LANES) {
+        // three entries per lane per pass, loads issued together (as k_spmv)
+        const bool pb = j + LANES < j1, pc = j + 2 * LANES < j1;
+        const int ca = ci[j], cb = pb ? ci[j + LANES] : 0, cc = pc ? ci[j + 2 * LANES] : 0;
+        const float va_ = va[j], vb = pb ? va[j + LANES] : 0.f, vc = pc ? va[j + 2 * LANES] : 0.f;
+        const float za = z[ca], zb = pb ? z[cb] : 0.f, zc = pc ? z[cc] : 0.f;
+        sum += va_ * za + vb * zb + vc * zc;
+      }
     sum = group_sum<LANES>(sum);
     if (lane == 0) {
       if (!first) {
