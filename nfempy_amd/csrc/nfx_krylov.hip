@@ -324,12 +324,23 @@ __global__ __launch_bounds__(kBlock) void k_minres_a_b3(
   for (int br = blockIdx.x * (kBlock / LANES) + threadIdx.x / LANES; br < nbr; br += groups_per_grid) {
     float s0 = 0.f, s1 = 0.f, s2 = 0.f;
     if (!final_only)
-      for (int j = brp[br] + lane; j < brp[br + 1]; j += LANES) {
-        const int c = 3 * bci[j];
-        const float z0 = z[c], z1 = z[c + 1], z2 = z[c + 2];
-        s0 += bva[0 * (int64_t)nblk + j] * z0 + bva[1 * (int64_t)nblk + j] * z1 + bva[2 * (int64_t)nblk + j] * z2;
-        s1 += bva[3 * (int64_t)nblk + j] * z0 + bva[4 * (int64_t)nblk + j] * z1 + bva[5 * (int64_t)nblk + j] * z2;
-        s2 += bva[6 * (int64_t)nblk + j] * z0 + bva[7 * (int64_t)nblk + j] * z1 + bva[8 * (int64_t)nblk + j] * z2;
+      for (int j = brp[br] + lane, j1 = brp[br + 1]; j < j1; j += 2 * LANES) {
+        // two blocks per lane per pass, the second predicated (its loads repeat block j
+        // when absent, with its input zeroed): a row of up to 2 LANES blocks in one pass
+        const bool pb = j + LANES < j1;
+        const int jb = pb ? j + LANES : j;
+        const int ca = 3 * bci[j], cb = 3 * bci[jb];
+        float a[9], b[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+          a[k] = bva[k * (int64_t)nblk + j];
+          b[k] = bva[k * (int64_t)nblk + jb];
+        }
+        const float z0 = z[ca], z1 = z[ca + 1], z2 = z[ca + 2];
+        const float w0 = pb ? z[cb] : 0.f, w1 = pb ? z[cb + 1] : 0.f, w2 = pb ? z[cb + 2] : 0.f;
+        s0 += a[0] * z0 + a[1] * z1 + a[2] * z2 + (b[0] * w0 + b[1] * w1 + b[2] * w2);
+        s1 += a[3] * z0 + a[4] * z1 + a[5] * z2 + (b[3] * w0 + b[4] * w1 + b[5] * w2);
+        s2 += a[6] * z0 + a[7] * z1 + a[8] * z2 + (b[6] * w0 + b[7] * w1 + b[8] * w2);
       }
     s0 = group_sum<LANES>(s0);
     s1 = group_sum<LANES>(s1);
@@ -372,10 +383,14 @@ __global__ __launch_bounds__(kBlock) void k_minres_b(
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     const float yi = y[i] - c * r2[i];
     ynew[i] = yi;
+    // three entries per pass, predicated: the loads of a pass are issued together
     float zi = 0.f;
-    for (int j = prp[i]; j < prp[i + 1]; ++j) {
-      const int col = pci[j];
-      zi += pva[j] * (y[col] - c * r2[col]);
+    for (int j = prp[i], j1 = prp[i + 1]; j < j1; j += 3) {
+      const bool pb = j + 1 < j1, pc = j + 2 < j1;
+      const int ca = pci[j], cb = pb ? pci[j + 1] : ca, cc = pc ? pci[j + 2] : ca;
+      const float wa = pva[j], wb = pb ? pva[j + 1] : 0.f, wc = pc ? pva[j + 2] : 0.f;
+      const float ya = y[ca] - c * r2[ca], yb = y[cb] - c * r2[cb], yc = y[cc] - c * r2[cc];
+      zi += wa * ya + wb * yb + wc * yc;
     }
     znew[i] = zi;
     acc += (double)yi * (double)zi;
