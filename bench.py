@@ -421,7 +421,8 @@ def main():
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        print(json.dumps({"error": "the benchmark measures one GPU (one system per device, no "
+                                   "multi-GPU path); --gpus must be 1"}))
         sys.exit(2)
 
     if args.batch:
