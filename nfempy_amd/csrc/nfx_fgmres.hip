@@ -105,6 +105,39 @@ __global__ __launch_bounds__(kFB) void k_fg_mupdate(int n, int j1, const float* 
   if (threadIdx.x == 0) partial[blockIdx.x] = ss;
 }
 
+// The first update and the second dot pass of the reorthogonalised Gram-Schmidt in one
+// kernel: w -= sum_i h32[i] V_i over block b's rows (the arithmetic of k_fg_mupdate), then
+// partial[b][i] = <V_i, w> with the updated rows (the arithmetic of k_fg_mdot), so that the
+// block's rows of V come from HBM once and are re-read from cache for the dots.
+__global__ __launch_bounds__(kFB) void k_fg_mupdate_dot(int n, int j1, int ld, const float* __restrict__ V,
+                                                        const float* __restrict__ h32, float* __restrict__ w,
+                                                        double* __restrict__ partial) {
+  static_assert(kChunkDot == kChunkUpd, "one row range per block for both phases");
+  __shared__ float sh[512];
+  __shared__ float sw[kChunkDot];
+  for (int i = threadIdx.x; i < j1; i += kFB) sh[i] = h32[i];
+  __syncthreads();
+  const int start = blockIdx.x * kChunkDot;
+  const int len = min(kChunkDot, n - start);
+  for (int k = threadIdx.x; k < len; k += kFB) {
+    float v = w[start + k];
+#pragma unroll 4
+    for (int i = 0; i < j1; ++i) v -= sh[i] * V[(int64_t)i * n + start + k];
+    w[start + k] = v;
+    sw[k] = v;
+  }
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int i = wave; i < j1; i += kFB / 64) {
+    const float* v = V + (int64_t)i * n + start;
+    float a = 0.f;
+#pragma unroll 4
+    for (int k = lane; k < len; k += 64) a += v[k] * sw[k];
+    const double d = fg_warp_sum((double)a);
+    if (lane == 0) partial[(int64_t)blockIdx.x * ld + i] = d;
+  }
+}
+
 // hn = sqrt(sum_b partial[b]) (one block). mode 1: hnv[j+1] = hn, h_{j+1,j} = hn,
 // Givens on column j, residual estimate |g_{j+1}|. mode 2 (start of a cycle):
 // hnv[0] = g[0] = hn.
@@ -362,12 +395,17 @@ static int fg_inner(nfx_fgmres* s, const int32_t* rp, const int32_t* ci, const f
       if (rc) return rc;
       double* Hcol = s->H + (size_t)j * ld;
       const int passes = reorth ? 2 : 1;
+      // pass p: dots (the first pass's own kernel, later ones fused into the previous
+      // update), coefficients, update (fused with the next pass's dots, the last with the norm)
+      hipLaunchKernelGGL(k_fg_mdot, dim3(B), dim3(kFB), 0, st, n, j + 1, ld, s->V, w, s->partial);
       for (int pass = 0; pass < passes; ++pass) {
-        hipLaunchKernelGGL(k_fg_mdot, dim3(B), dim3(kFB), 0, st, n, j + 1, ld, s->V, w, s->partial);
         hipLaunchKernelGGL(k_fg_hreduce, dim3(j + 1), dim3(kFB), 0, st, B, ld, s->partial, s->hnv, Hcol, s->h32,
                            pass);
-        hipLaunchKernelGGL(k_fg_mupdate, dim3(BU), dim3(kFB), 0, st, n, j + 1, s->V, s->h32, w, s->upartial,
-                           pass == passes - 1 ? 1 : 0);
+        if (pass < passes - 1)
+          hipLaunchKernelGGL(k_fg_mupdate_dot, dim3(B), dim3(kFB), 0, st, n, j + 1, ld, s->V, s->h32, w,
+                             s->partial);
+        else
+          hipLaunchKernelGGL(k_fg_mupdate, dim3(BU), dim3(kFB), 0, st, n, j + 1, s->V, s->h32, w, s->upartial, 1);
       }
       hipLaunchKernelGGL(k_fg_finish, dim3(1), dim3(kFB), 0, st, BU, s->upartial, 1, j, m, s->H, s->cs, s->sn,
                          s->g, s->res, s->hnv);
