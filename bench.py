@@ -173,8 +173,9 @@ def bench_batch(args, log):
     t_setup = time.perf_counter() - t0
     log(f"batch: {m} instances, {b.n_elements} tets, set-up {t_setup:.1f} s")
     trials = b.autotune(reps=3)
-    tuned = {f"rt{d}": {"tile_rows": g["tile_rows"], "grid": g["grid"], "ms": round(g["ms"], 4)}
-             for d, g in b.groups.items()}
+    label = {0: "both", 1: "rt1", 2: "rt2", 3: "rt2"}  # 3: the compact RT2 launch
+    tuned = {label[d]: {"tile_rows": g["tile_rows"], "grid": g["grid"], "ms": round(g["ms"], 4),
+                        "degree_code": d} for d, g in b.groups.items()}
     log(f"autotune (one launch per degree): {tuned}")
     b.capture()
     for _ in range(args.warmup):

@@ -140,7 +140,12 @@ int nfx_element_geometry(int dim, int n_elem, const int32_t* d_cells, const void
  * consecutive tiles carry the same tables pointer (give all instances of one degree
  * one table to benefit; any pointers are correct). degree = 0: tiles of both
  * degrees; degree = 1 or 2: every tile is of that degree (tiles of the other degree
- * are skipped) and the kernel is specialised for it (RT1: 8 waves per SIMD). */
+ * are skipped) and the kernel is specialised for it (RT1: 8 waves per SIMD).
+ * degree = 3: RT2 tiles whose items have per-element coefficients (kinv and f with
+ * stride_q = 0), run with only the reference tensors in LDS and a staging slice of 33
+ * entries per tile row (4 waves per SIMD): a tile's item field is item | rows << 16
+ * (rows <= tile_rows; 0 = tile_rows) and its CSR span must be at most 33 * tile_rows
+ * (other tiles are skipped). */
 int nfx_assemble_darcy_batched(int n_items, const void* d_items, int n_tiles,
                                const int32_t* d_tiles, int tile_rows, int grid, int degree,
                                void* stream);

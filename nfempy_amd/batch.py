@@ -49,6 +49,26 @@ def darcy_space(n, k):
     return ps
 
 
+RT2_SPAN_PER_ROW = 33  # staging entries per tile row of the compact RT2 launch (kRt2SpanPerRow)
+
+
+def span_tiles(rowptr, tile_rows, span_per_row=RT2_SPAN_PER_ROW):
+    """Greedy cut of rows [0, n) into tiles of at most ``tile_rows`` rows whose CSR span
+    (rowptr[end] - rowptr[start]) is at most ``tile_rows * span_per_row``: (starts, counts)."""
+    rp = np.asarray(rowptr, dtype=np.int64)
+    n, cap = len(rp) - 1, tile_rows * span_per_row
+    starts, counts = [], []
+    r0 = 0
+    while r0 < n:
+        end = min(int(np.searchsorted(rp, rp[r0] + cap, side="right")) - 1, r0 + tile_rows, n)
+        if end <= r0:
+            raise ValueError("a CSR row is longer than the compact staging slice")
+        starts.append(r0)
+        counts.append(end - r0)
+        r0 = end
+    return np.asarray(starts, np.int64), np.asarray(counts, np.int64)
+
+
 class DarcyBatch:
     def __init__(self, specs, seed=0, ell=0.1, sigma=1.0, device="cuda", log=None, split=True):
         """specs: list of (n, k) per instance; log: optional progress callable; split: one
@@ -83,6 +103,11 @@ class DarcyBatch:
         # launch groups: degree -> tiles, tile rows, grid (degree 0: both degrees in one launch)
         self.split = bool(split)
         degrees = sorted({k for _, k in self.specs}, reverse=True) if self.split else [0]
+        # RT2 with per-element coefficients only: the compact launch (degree code 3: LDS holds
+        # the reference tensors alone and a staging slice sized for the average row)
+        rt2 = [inst for inst in self.instances if inst["system"].k == 2]
+        if self.split and rt2 and all(inst["kinv"].sq == 0 and inst["f"].sq == 0 for inst in rt2):
+            degrees = [3 if d == 2 else d for d in degrees]
         self.groups = {d: {"tile_rows": 128, "grid": None} for d in degrees}
         self._build_items()
         for d in degrees:
@@ -111,16 +136,25 @@ class DarcyBatch:
         self._items = torch().from_numpy(raw).to(self.device)
 
     def _build_tiles(self, degree, tile_rows):
-        """Tiles of the instances of `degree` (0: all), heaviest first."""
+        """Tiles of the instances of `degree` (0: all; 3: RT2 compact), heaviest first.
+        Degree 3 cuts each tile where its CSR span would pass tile_rows * RT2_SPAN_PER_ROW
+        staging entries and carries the row count in bits 16.. of the item field."""
         tiles, cost = [], []
+        if len(self.instances) >= 1 << 16:
+            raise ValueError("at most 65535 instances per batch")
         for i, inst in enumerate(self.instances):
             s = inst["system"]
-            if degree and s.k != degree:
+            if degree and s.k != (2 if degree == 3 else degree):
                 continue
             c = s.nl * s.nq
-            for r0 in range(0, s.n, tile_rows):
-                tiles.append((i, r0))
-                cost.append(min(tile_rows, s.n - r0) * c)
+            if degree != 3:
+                for r0 in range(0, s.n, tile_rows):
+                    tiles.append((i, r0))
+                    cost.append(min(tile_rows, s.n - r0) * c)
+                continue
+            for r0, cnt in zip(*span_tiles(s.pattern.rowptr, tile_rows)):
+                tiles.append((i | int(cnt) << 16, int(r0)))
+                cost.append(int(cnt) * c)
         order = np.argsort(-np.asarray(cost), kind="stable")
         tl = np.asarray(tiles, dtype=np.int32).reshape(-1, 2)[order]
         g = self.groups[degree]

@@ -439,24 +439,33 @@ struct RowLoad {
   uint32_t pw[2];  // packed column positions (row_pos_packed)
 };
 
-template <int DIM, int NB, int NP, int NQ, int TILE, typename R, bool GEO = false, int RPT = 1>
+// RS_ONLY: st holds only Rs | Bc | ck (the per-element path; tiles with per-quadrature
+// coefficients are not processed). n_in_tile: rows of this tile (at most TILE * RPT).
+template <int DIM, int NB, int NP, int NQ, int TILE, typename R, bool GEO = false, int RPT = 1,
+          bool RS_ONLY = false>
 __device__ __forceinline__ void darcy_row_tile(
     int r0, int n_rows, const int32_t* __restrict__ inc, const uint8_t* __restrict__ pos,
     const int32_t* __restrict__ rowptr, const uint32_t* __restrict__ signs,
     const int32_t* __restrict__ cells, const R* __restrict__ coords, const R* __restrict__ tab,
     const R* __restrict__ kinv, int ks_e, int ks_q, const R* __restrict__ f, int fs_e, int fs_q,
     R* __restrict__ vals, R* __restrict__ rhs, R* st, R* sv, const DarcyRowBc<R>& bc,
-    bool load_table = true, const R* __restrict__ geo = nullptr) {
+    bool load_table = true, const R* __restrict__ geo = nullptr, int n_in_tile = TILE * RPT) {
   constexpr int NL = NB + NP;
   constexpr int NS = DarcyTab<DIM, NB, NP, NQ>::NS;
   constexpr int NT = DarcyTab<DIM, NB, NP, NQ>::NT;
   constexpr int NT_RS = NQ * (1 + NB * DIM + NB + NP);  // start of Rs | Bc | ck
-  const int rl = min(r0 + TILE * RPT, n_rows);
+  const int rl = min(r0 + n_in_tile, n_rows);
   const bool per_element = (ks_q == 0 && fs_q == 0);
+  if (RS_ONLY && !per_element) return;
   // the reference-tensor path reads only Rs | Bc | ck; a caller that already holds the
   // table in st (same element, same path) passes load_table = false
-  if (load_table)
-    for (int i = (per_element ? NT_RS : 0) + threadIdx.x; i < NT; i += TILE) st[i] = tab[i];
+  if (load_table) {
+    if constexpr (RS_ONLY) {
+      for (int i = threadIdx.x; i < NT - NT_RS; i += TILE) st[i] = tab[NT_RS + i];
+    } else {
+      for (int i = (per_element ? NT_RS : 0) + threadIdx.x; i < NT; i += TILE) st[i] = tab[i];
+    }
+  }
   const int base = rowptr[r0];
   const int span = rowptr[rl] - base;
   for (int i = threadIdx.x; i < span; i += TILE) sv[i] = R(0);
@@ -465,7 +474,7 @@ __device__ __forceinline__ void darcy_row_tile(
   const R* tphi = st + NQ;
   const R* tdiv = tphi + NQ * NB * DIM;
   const R* tpsi = tdiv + NQ * NB;
-  const R* tRs = tpsi + NQ * NP;
+  const R* tRs = RS_ONLY ? st : tpsi + NQ * NP;
   const R* tBc = tRs + NB * NB * NS;
   const R* tck = tBc + NB * NP;
   RowLoad<DIM, R, GEO> in[RPT];
@@ -475,7 +484,7 @@ __device__ __forceinline__ void darcy_row_tile(
     const int r = r0 + h * TILE + threadIdx.x;
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-      const int c = r < n_rows ? inc[(int64_t)j * n_rows + r] : -1;
+      const int c = r < rl ? inc[(int64_t)j * n_rows + r] : -1;
       in[h].code[j] = c;
       in[h].e[j] = c < 0 ? 0 : c >> 5;
       in[h].a[j] = c & 31;
@@ -533,7 +542,7 @@ __device__ __forceinline__ void darcy_row_tile(
 #pragma unroll
   for (int h = 0; h < RPT; ++h) {
     const int r = r0 + h * TILE + threadIdx.x;
-    if (r >= n_rows) continue;
+    if (r >= rl) continue;
     RowLoad<DIM, R, GEO>& L = in[h];
     const int off = rowptr[r] - base;
     R res = 0;
@@ -549,7 +558,7 @@ __device__ __forceinline__ void darcy_row_tile(
       R row[NL];
 #pragma unroll
       for (int b = 0; b < NL; ++b) row[b] = 0;
-      if (per_element) {
+      if (RS_ONLY || per_element) {
         if (aj < NB) {
           R gs[NS];
 #pragma unroll
@@ -570,7 +579,7 @@ __device__ __forceinline__ void darcy_row_tile(
           for (int b = 0; b < NB; ++b) row[b] = tBc[b * NP + k];
           res -= L.dj[j] * L.fv[j] * tck[k];
         }
-      } else if (aj < NB) {
+      } else if (!RS_ONLY && aj < NB) {
         // symmetric G / detJ from the packed upper triangle
         R Gd[DIM][DIM];
         {
@@ -606,7 +615,7 @@ __device__ __forceinline__ void darcy_row_tile(
 #pragma unroll
           for (int k = 0; k < NP; ++k) row[NB + k] -= wd * tpsi[q * NP + k];
         }
-      } else {
+      } else if (!RS_ONLY) {
         const int k = aj - NB;
         for (int q = 0; q < NQ; ++q) {
           const R wp = tw[q] * tpsi[q * NP + k];
@@ -751,20 +760,26 @@ struct DarcyBatchItem {
 // tiles: (item, first row) pairs, heaviest first; block b takes tiles b, b + G, ...
 // KD = 0: tiles of both degrees; KD = 1 or 2: every tile is of that degree (a tile of
 // the other degree is skipped), so the kernel carries only that element's registers
-// and LDS (RT1: 8 waves per SIMD, as the single-system kernel).
+// and LDS (RT1: 8 waves per SIMD, as the single-system kernel). KD = 3: RT2 tiles with
+// per-element coefficients, LDS holding only Rs | Bc | ck and a staging slice of
+// kRt2SpanPerRow entries per tile row; the tile's row count rides in bits 16.. of its item
+// field (0 = TILE) and the host cuts tiles so that each CSR span fits, which brings a
+// 256-row block to 39.4 KB of LDS: 4 blocks, 4 waves per SIMD.
+constexpr int kRt2SpanPerRow = 33;
 template <int KD>
 constexpr int batched_min_waves() {
-  return KD == 1 ? 8 : 1;
+  return KD == 1 ? 8 : (KD == 3 ? 4 : 1);
 }
 
 template <int TILE, int KD>
 __global__ __launch_bounds__(TILE)
 __attribute__((amdgpu_waves_per_eu(batched_min_waves<KD>(), 8))) void k_darcy_rows_batched(
     const DarcyBatchItem* __restrict__ items, const int2* __restrict__ tiles, int n_tiles) {
-  constexpr int NTS = KD == 1 ? DarcyTab<3, 4, 1, 8>::NT : DarcyTab<3, 15, 4, 27>::NT;
-  constexpr int NLS = KD == 1 ? 5 : 19;
+  constexpr int NT2 = DarcyTab<3, 15, 4, 27>::NT, NT2_RS = 27 * (1 + 15 * 3 + 15 + 4);
+  constexpr int NTS = KD == 1 ? DarcyTab<3, 4, 1, 8>::NT : (KD == 3 ? NT2 - NT2_RS : NT2);
+  constexpr int NSV = KD == 1 ? TILE * 2 * 5 : (KD == 3 ? TILE * kRt2SpanPerRow : TILE * 2 * 19);
   __shared__ float st[NTS];
-  __shared__ float sv[TILE * 2 * NLS];
+  __shared__ float sv[NSV];
   const DarcyRowBc<float> no_bc{0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   // The table stays in LDS across consecutive tiles with the same table pointer and
   // path (the host gives all instances of one degree one table; tiles come grouped by
@@ -773,7 +788,9 @@ __attribute__((amdgpu_waves_per_eu(batched_min_waves<KD>(), 8))) void k_darcy_ro
   int held_path = -1;
   for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {
     const int2 tl = tiles[t];
-    const DarcyBatchItem& it = items[tl.x];
+    const int item = KD == 3 ? (tl.x & 0xFFFF) : tl.x;
+    const int tile_n = KD == 3 && (tl.x >> 16) > 0 ? min(tl.x >> 16, TILE) : TILE;
+    const DarcyBatchItem& it = items[item];
     // explicit casts from the global-typed fields: the compiler then knows every access
     // below is a global one
     const int32_t* inc = (const int32_t*)it.inc;
@@ -798,11 +815,23 @@ __attribute__((amdgpu_waves_per_eu(batched_min_waves<KD>(), 8))) void k_darcy_ro
                                                       tab, kinv, it.ks_e, it.ks_q, f, it.fs_e, it.fs_q, vals, rhs,
                                                       st, sv, no_bc, load, geo);
     }
-    if constexpr (KD != 1) {
+    if constexpr (KD == 0 || KD == 2) {
       if (it.k == 2)
         darcy_row_tile<3, 15, 4, 27, TILE, float, true>(tl.y, it.n_rows, inc, pos, rowptr, signs, cells,
                                                         coords, tab, kinv, it.ks_e, it.ks_q, f, it.fs_e, it.fs_q,
                                                         vals, rhs, st, sv, no_bc, load, geo);
+    }
+    if constexpr (KD == 3) {
+      // a tile whose CSR span exceeds the staging slice is not processed (the host's
+      // tiling rules it out)
+      const int rl = min(tl.y + tile_n, it.n_rows);
+      if (it.k != 2 || path != 1 || rowptr[rl] - rowptr[tl.y] > NSV) {
+        held_tab = nullptr;  // nothing loaded: the next tile reloads the table
+        continue;
+      }
+      darcy_row_tile<3, 15, 4, 27, TILE, float, true, 1, true>(
+            tl.y, it.n_rows, inc, pos, rowptr, signs, cells, coords, tab, kinv, it.ks_e, it.ks_q, f, it.fs_e,
+            it.fs_q, vals, rhs, st, sv, no_bc, load, geo, tile_n);
     }
   }
 }
@@ -1292,7 +1321,7 @@ extern "C" int nfx_assemble_darcy_batched(int n_items, const void* d_items, int 
                                           const int32_t* d_tiles, int tile_rows, int grid, int degree,
                                           void* stream) {
   NFX_REQUIRE(n_items >= 0 && n_tiles >= 0 && grid > 0, "nfx_assemble_darcy_batched: sizes");
-  NFX_REQUIRE(degree >= 0 && degree <= 2, "nfx_assemble_darcy_batched: degree must be 0, 1 or 2");
+  NFX_REQUIRE(degree >= 0 && degree <= 3, "nfx_assemble_darcy_batched: degree must be 0, 1, 2 or 3");
   if (n_tiles == 0) return NFX_OK;
   NFX_REQUIRE(d_items && d_tiles, "nfx_assemble_darcy_batched: null pointer");
   hipStream_t s = static_cast<hipStream_t>(stream);
@@ -1300,6 +1329,7 @@ extern "C" int nfx_assemble_darcy_batched(int n_items, const void* d_items, int 
   const int2* tiles = reinterpret_cast<const int2*>(d_tiles);
   if (degree == 1) return launch_batched<1>(items, tiles, n_tiles, tile_rows, grid, s);
   if (degree == 2) return launch_batched<2>(items, tiles, n_tiles, tile_rows, grid, s);
+  if (degree == 3) return launch_batched<3>(items, tiles, n_tiles, tile_rows, grid, s);
   return launch_batched<0>(items, tiles, n_tiles, tile_rows, grid, s);
 }
 

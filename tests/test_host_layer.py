@@ -170,3 +170,23 @@ def test_reference_tensors_match_quadrature_sums(dim, k):
     assert np.allclose(M_t, M_q, rtol=1e-12, atol=1e-12 * np.abs(M_q).max())
     assert np.allclose(Bc, np.einsum("q,qa,qk->ak", w, div, psi), rtol=1e-12, atol=1e-14)
     assert np.allclose(ck, np.einsum("q,qk->k", w, psi), rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("tile_rows", [64, 128, 256])
+def test_span_tiles_cover_rows_within_staging(tile_rows):
+    """The compact RT2 tiling: consecutive tiles cover every row once, each holds at most
+    tile_rows rows and a CSR span within tile_rows * 33 entries, and a tile stops short of
+    tile_rows only where one more row would pass the span limit."""
+    from nfempy_amd.batch import RT2_SPAN_PER_ROW, span_tiles
+    from nfempy_amd.device import DarcySystem
+
+    sysd = DarcySystem(product_darcy_space(3, n=3, k=2), device="cpu")
+    rp = np.asarray(sysd.pattern.rowptr, np.int64)
+    starts, counts = span_tiles(rp, tile_rows)
+    assert starts[0] == 0 and np.array_equal(starts[1:], (starts + counts)[:-1]) and starts[-1] + counts[-1] == sysd.n
+    cap = tile_rows * RT2_SPAN_PER_ROW
+    ends = starts + counts
+    assert np.all(counts <= tile_rows) and np.all(rp[ends] - rp[starts] <= cap)
+    short = (counts < tile_rows) & (ends < sysd.n)
+    assert np.all(rp[ends[short] + 1] - rp[starts[short]] > cap)
+    assert np.diff(rp).max() > RT2_SPAN_PER_ROW  # the RT2 face rows are longer than the mean
