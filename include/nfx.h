@@ -92,7 +92,9 @@ int nfx_assemble_darcy(int dim, int k, int n_elem, const int32_t* d_cells,
  *  d_pos    column position of reference DoF b inside the row, per (row, incidence):
  *           nl <= 6 (lowest-order elements): one uint32, bits 4b..4b+3 the position
  *           (15 = not present), bit 4 nl + b set when entry (a, b) is negated (sign bit a
- *           xor sign bit b of d_signs), higher bits zero; otherwise nlp uint8 (255 = not
+ *           xor sign bit b of d_signs); for nl <= 5 bits 25..31 of the row's two words hold
+ *           its CSR offset rowptr[r] - rowptr[128 (r / 128)] (low 7 bits in the first word,
+ *           high 7 in the second), read instead of d_rowptr[r]; other bits zero; otherwise nlp uint8 (255 = not
  *           present), nlp = nl, or nl rounded up to a multiple of 4 when nl > 8 (RT2
  *           elements: word loads)
  *  d_rowptr CSR row pointer; d_signs (E) uint32, bit a = reference DoF a negated (read

@@ -366,6 +366,10 @@ constexpr int kRowsPerBlock = 128;
 // incidence, no gather of the element's sign bits); the larger elements keep one byte per
 // position, padded to whole words so that a row reads them with word loads.
 constexpr bool row_pos_packed(int nl) { return nl <= 6; }
+// nl <= 5: bits 25..31 of an incidence's word are free; the two words of a row carry the
+// row's CSR offset inside its 128-row tile (low 7 bits in word 0, high 7 in word 1), read
+// by the single-system kernel instead of rowptr[r].
+constexpr bool row_off_packed(int nl) { return nl <= 5; }
 constexpr int row_pos_stride(int nl) { return nl <= 6 ? 4 : (nl > 8 ? (nl + 3) / 4 * 4 : nl); }
 
 // Table layout: w[NQ] | phi[NQ][NB][DIM] | div[NQ][NB] | psi[NQ][NP] | Rs[NB][NB][NS] |
@@ -442,7 +446,7 @@ struct RowLoad {
 // RS_ONLY: st holds only Rs | Bc | ck (the per-element path; tiles with per-quadrature
 // coefficients are not processed). n_in_tile: rows of this tile (at most TILE * RPT).
 template <int DIM, int NB, int NP, int NQ, int TILE, typename R, bool GEO = false, int RPT = 1,
-          bool RS_ONLY = false>
+          bool RS_ONLY = false, bool OFF_PACKED = false>
 __device__ __forceinline__ void darcy_row_tile(
     int r0, int n_rows, const int32_t* __restrict__ inc, const uint8_t* __restrict__ pos,
     const int32_t* __restrict__ rowptr, const uint32_t* __restrict__ signs,
@@ -489,7 +493,7 @@ __device__ __forceinline__ void darcy_row_tile(
       in[h].e[j] = c < 0 ? 0 : c >> 5;
       in[h].a[j] = c & 31;
       if constexpr (row_pos_packed(NL))
-        in[h].pw[j] = c >= 0 ? reinterpret_cast<const uint32_t*>(pos)[(int64_t)r * 2 + j] : 0u;
+        in[h].pw[j] = (OFF_PACKED ? r < rl : c >= 0) ? reinterpret_cast<const uint32_t*>(pos)[(int64_t)r * 2 + j] : 0u;
     }
   }
 #pragma unroll
@@ -544,7 +548,13 @@ __device__ __forceinline__ void darcy_row_tile(
     const int r = r0 + h * TILE + threadIdx.x;
     if (r >= rl) continue;
     RowLoad<DIM, R, GEO>& L = in[h];
-    const int off = rowptr[r] - base;
+    int off;
+    if constexpr (OFF_PACKED) {
+      static_assert(row_off_packed(NL) && TILE == 128 && RPT == 1, "packed offsets: 128-row tiles");
+      off = (int)((L.pw[0] >> 25) | ((L.pw[1] >> 25) << 7));
+    } else {
+      off = rowptr[r] - base;
+    }
     R res = 0;
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
@@ -714,9 +724,9 @@ __attribute__((amdgpu_waves_per_eu(row_kernel_min_waves<NB, R>(), 8))) void k_da
   if constexpr (NB <= 4) {
     const int t = tile_map ? tile_map[blockIdx.x] : (int)blockIdx.x;
     if (t < 0 || t >= n_tiles) return;
-    darcy_row_tile<DIM, NB, NP, NQ, kRowsPerBlock, R, GEO, RPT>(t * kRowsPerBlock * RPT, n_rows, inc, pos, rowptr,
-                                                                signs, cells, coords, tab, kinv, ks_e, ks_q, f,
-                                                                fs_e, fs_q, vals, rhs, st, sv, bc, true, geo);
+    darcy_row_tile<DIM, NB, NP, NQ, kRowsPerBlock, R, GEO, RPT, false, row_off_packed(NB + NP) && RPT == 1>(
+        t * kRowsPerBlock * RPT, n_rows, inc, pos, rowptr, signs, cells, coords, tab, kinv, ks_e, ks_q, f, fs_e,
+        fs_q, vals, rhs, st, sv, bc, true, geo);
   } else {
     const int n_slots = tile_map ? n_map : n_tiles;
     bool first = true;

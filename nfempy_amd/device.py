@@ -191,13 +191,15 @@ def check_coefficient(arg, E, nq, what):
         raise ValueError(f"{what}: coefficient array too short ({n} <= {last})")
 
 
-def row_pos_device(pos, nl, sign_mask):
+def row_pos_device(pos, nl, sign_mask, rowptr=None, tile=128):
     """The row kernel's column-position array from the (n, 2, nlp) byte positions (255 =
     not present) and the (n, 2) entry-sign masks (bit b: entry (a, b) of the incident
     element row is negated). nl <= 6 packs each incidence into one uint32: bits 4b..4b+3
     the position of b (15 = not present; rows of at most 12 entries), bits 4 nl + b the
-    sign mask, the rest zero. Larger elements keep the bytes as they are (the kernel's
-    row_pos_stride)."""
+    sign mask, the rest zero. For nl <= 5 with ``rowptr`` given, bits 25..31 of the two
+    words of row r hold its CSR offset inside its ``tile``-row tile, rowptr[r] -
+    rowptr[tile * (r // tile)] (low 7 bits in word 0, high 7 in word 1). Larger elements keep
+    the bytes as they are (the kernel's row_pos_stride)."""
     if nl > 6:
         return pos.reshape(-1)
     p = pos[:, :, :nl].astype(np.uint64)
@@ -206,6 +208,14 @@ def row_pos_device(pos, nl, sign_mask):
     nib = np.where(p == 255, 15, p)
     words = (nib << (4 * np.arange(nl, dtype=np.uint64))[None, None, :]).sum(axis=2, dtype=np.uint64)
     words |= (sign_mask.astype(np.uint64) & np.uint64((1 << nl) - 1)) << np.uint64(4 * nl)
+    if rowptr is not None and nl <= 5:
+        rp = np.asarray(rowptr, dtype=np.int64)
+        r = np.arange(len(rp) - 1)
+        off = rp[:-1] - rp[(r // tile) * tile]
+        if off.size and off.max() >= 1 << 14:
+            raise ValueError("row offset inside its tile does not fit 14 bits")
+        words[:, 0] |= (off.astype(np.uint64) & np.uint64(0x7F)) << np.uint64(25)
+        words[:, 1] |= (off.astype(np.uint64) >> np.uint64(7)) << np.uint64(25)
     return np.ascontiguousarray(words.astype(np.uint32)).view(np.uint8).reshape(-1)
 
 
@@ -382,7 +392,7 @@ class DarcySystem:
         self.row_tile_map = to_dev(self.row_tile_map_np, device=device)
         self.use_tile_map = True
         self.row_inc = to_dev(np.ascontiguousarray(packed.astype(np.int32)), device=device)
-        self.row_pos = to_dev(row_pos_device(pos, nl, sign_mask), device=device)
+        self.row_pos = to_dev(row_pos_device(pos, nl, sign_mask, self.pattern.rowptr, ROW_TILE), device=device)
         self.row_signs = to_dev(signs.astype(np.uint32).view(np.int32), device=device)
 
     def _validate(self, mesh):

@@ -109,7 +109,8 @@ def test_row_positions_device_layout(case):
     """The device copy of the row kernel's column positions decodes back to the byte
     positions and the entry signs: for nl <= 6 one little-endian word per incidence,
     4-bit position fields (15 = absent), then one bit per column b equal to
-    ref_sign[e, a] * ref_sign[e, b] < 0, zeros above; the bytes themselves otherwise."""
+    ref_sign[e, a] * ref_sign[e, b] < 0, and for nl <= 5 the row's offset inside its
+    128-row tile in bits 25..31 of the two words; the bytes themselves otherwise."""
     from nfempy_amd.device import DarcySystem
 
     sysd = DarcySystem(product_darcy_space(*case), device="cpu")
@@ -124,7 +125,15 @@ def test_row_positions_device_layout(case):
     for b in range(nl):
         field = (words >> (4 * b)) & 15
         assert np.array_equal(np.where(field == 15, 255, field), pp[:, :, b])
-    assert np.all(words >> (5 * nl) == 0)
+    hi = words >> 25
+    if nl <= 5:
+        # bits 25..31: the row's CSR offset inside its 128-row tile, split over the two words
+        rp = np.asarray(sysd.pattern.rowptr, np.int64)
+        r = np.arange(sysd.n)
+        assert np.array_equal(hi[:, 0] | hi[:, 1] << 7, rp[:-1] - rp[(r // 128) * 128])
+        assert np.all((words >> (5 * nl)) & ((1 << (25 - 5 * nl)) - 1) == 0)
+    else:
+        assert np.all(words >> (5 * nl) == 0)
     inc = sysd.row_inc_np
     for j in range(2):
         live = np.flatnonzero(inc[j] >= 0)
@@ -134,7 +143,7 @@ def test_row_positions_device_layout(case):
             want = sysd.ref_sign[e, a] * sysd.ref_sign[e, b] < 0
             assert np.array_equal(bit.astype(bool), want)
         dead = np.flatnonzero(inc[j] < 0)
-        assert np.all(words[dead, j] >> (4 * nl) == 0)
+        assert np.all((words[dead, j] >> (4 * nl)) & ((1 << nl) - 1) == 0)
 
 
 @pytest.mark.parametrize("dim,k", [(2, 1), (2, 2), (3, 1), (3, 2)])
