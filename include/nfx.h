@@ -49,7 +49,8 @@ extern "C" {
 /* Version as 10000*major + 100*minor + patch (0.2.0: per-element metric argument of the
  * Darcy row assembly, slot_pairs of the elasticity assembly, the block-LU tail of
  * nfx_twogrid_pc, block operator of the MINRES solver; 0.3.0: tile launch map of the Darcy
- * row assembly, packed positions and entry signs of the lowest-order rows). */
+ * row assembly, packed positions and entry signs of the lowest-order rows; 0.4.0: in-tile
+ * row offsets in the lowest-order position words, compact RT2 batched launch, degree 3). */
 int nfx_version(void);
 /* Thread-local description of the last error on this thread. */
 const char* nfx_last_error(void);

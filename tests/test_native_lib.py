@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
     lib = _native.lib()
     for name in declared_functions():
         assert hasattr(lib, name), name
-    assert lib.nfx_version() >= 300
+    assert lib.nfx_version() >= 400
 
 
 def test_errors_are_reported_without_a_gpu():
