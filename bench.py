@@ -395,8 +395,8 @@ def bench_solve(args, sysd, cent, fsrc, pd, log):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=2000, help="timed warm steps (11-12 us each at 32^3)")
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--n", type=int, default=32, help="cube divisions per axis")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner solve")
