@@ -189,6 +189,330 @@ static int launch_elasticity(int n, const int32_t* cells, const void* coords, co
   return NFX_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Face-owner assembly: one wavefront owns the 9 CSR rows of one face (its 3 BDM1 DoFs x 3
+// stress rows, consecutive in the DoF numbering and therefore one contiguous CSR slice),
+// evaluates the rows of those DoFs in the element matrices of the face's (at most two)
+// incident tetrahedra, sums them in element order and writes the slice with coalesced
+// stores. No slot codes per element entry, no staging, no atomics. Each CSR entry of a
+// stress row carries a 16-bit source code: bits 0..3 the row inside the face (3 j + c),
+// bits 4..9 and 10..15 the element-local reference column (3 b + d, 36 + d for u_d,
+// 39 + d for t_d) of the entry in the first / second incident element, 63 = no
+// contribution (entries dropped by the value filter, explicit zero diagonals).
+//
+// Per incident element the wave computes only the columns the upper triangle needs: the
+// blocks of the element's faces whose DoF block index is >= the owning face's, plus the
+// u / t columns. The (s, s) contraction V = sum_q k2_q psi_ac(q) psi_bd(q) (9 rows x 9 nb
+// columns, K = 8 quadrature points) runs on v_mfma_f32_16x16x4_f32 in FP32; FP64 builds
+// evaluate the same tiles on the vector ALU. The diagonal-in-component part
+// D_ab = sum_q k1_q psi_a . psi_b (and the Riesz map's div-div term) is a 3 x 12 block per
+// element on the vector ALU.
+//
+// Cell rows (u_c, t_c of each tetrahedron) hold one entry, their diagonal: 0 in A (the
+// explicit zero of the SBAIJ scatter), the mass |K| in P; their residual rows take
+// -sum_q w detJ f. One thread per cell, in blocks launched after the face blocks.
+//
+// Table layout (R): w[8] | phi[8][12][3] | div[8][12] | Uref[12] | Phi[12][3] | DDref[12][12]
+// with Uref_b = sum_q w_q divref_b(q), Phi_be = sum_q w_q phihat_be(q),
+// DDref_ab = sum_q w_q divref_a(q) divref_b(q).
+constexpr int kFW = 4;  // faces (waves) per block of the face kernel
+constexpr int kCellBlock = 256;
+constexpr int kNoSrc = 63;
+
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+template <typename R>
+struct FaceLds {
+  R psi[2][12][3][kEQ];      // psi_{b d}(q)
+  R kpsi[2][2][3][3][kEQ];   // [i][0][j][c][q] = k2_q psi_{a_j c}(q); [i][1] k1_q psi_{a_j c}(q)
+  R D[2][3][12];             // D_{a_j b} (+ div-div, Riesz map)
+  R K[2][9][kEL];            // signed element rows of the face: s(a) s(b) K_e[(a, c), col]
+};
+
+template <typename R>
+__device__ __forceinline__ void face_geometry(const int32_t* __restrict__ cells, const R* __restrict__ coords,
+                                              int e, R (&J)[3][3], R& detJ) {
+  R X[4][3];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int v = cells[e * 4 + a];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) X[a][c] = coords[3 * v + c];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) J[c][d] = X[d + 1][c] - X[0][c];
+  R dt = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+         J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+  detJ = dt < R(0) ? -dt : dt;
+}
+
+// Per incident element of a face: Jacobian and |det J|, the element's 4 face DoF blocks,
+// orientation word (bits 3b, 3b + 1: face slot of reference basis b; bit 3b + 2: its sign),
+// the reference basis functions at the owning face's slots j (4 bits each) and the list of
+// the element's local faces whose block is >= the owning face's (2 bits each, nb of them).
+template <typename R>
+struct FaceInc {
+  R J[3][3];
+  R detJ, invdet;
+  uint64_t orient;
+  int ajpack, blpack, nb;
+};
+
+template <typename R>
+__device__ __forceinline__ R face_psi(const FaceInc<R>& g, int d, R p0, R p1, R p2) {
+  return (g.J[d][0] * p0 + g.J[d][1] * p1 + g.J[d][2] * p2) * g.invdet;
+}
+
+template <typename R>
+__device__ __forceinline__ R basis_sign(const FaceInc<R>& g, int b) {
+  return ((g.orient >> (3 * b + 2)) & 1) ? R(-1) : R(1);
+}
+
+// rows of incidence i (uniform): (s, s) blocks of the needed faces into L.K[i]
+template <typename R>
+__device__ __forceinline__ void face_ss_rows(FaceLds<R>& L, const FaceInc<R>& g, int i, int l) {
+  const int nc = 9 * g.nb;
+  const int ntile = (nc + 15) >> 4;
+  const int lr = l & 15, lq = l >> 4;
+#pragma unroll
+  for (int t = 0; t < 3; ++t) {
+    if (t >= ntile) break;
+    const int n = 16 * t + lr;
+    const bool cn = n < nc;
+    const int bi = cn ? n / 9 : 0, w = n - 9 * bi;
+    const int lf = (g.blpack >> (2 * bi)) & 3;
+    const int b = 3 * lf + w / 3, d = w % 3;
+    R v[4];
+    if constexpr (sizeof(R) == 4) {
+      f4_t acc = f4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        const int q = 4 * s + lq;
+        const float av = lr < 9 ? L.kpsi[i][0][lr / 3][lr % 3][q] : 0.f;
+        const float bv = cn ? L.psi[i][b][d][q] : 0.f;
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) v[r] = acc[r];
+    } else {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = 4 * lq + r;
+        R s = 0;
+        if (m < 9 && cn)
+#pragma unroll
+          for (int q = 0; q < kEQ; ++q) s += L.kpsi[i][0][m / 3][m % 3][q] * L.psi[i][b][d][q];
+        v[r] = s;
+      }
+    }
+    if (cn) {
+      const R sb = basis_sign(g, b);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = 4 * lq + r;
+        if (m < 9) {
+          const int j = m / 3, c = m % 3, a = (g.ajpack >> (4 * j)) & 15;
+          const R kv = (c == d ? L.D[i][j][b] : R(0)) - v[r];
+          L.K[i][m][9 * lf + w] = basis_sign(g, a) * sb * kv;
+        }
+      }
+    }
+  }
+}
+
+template <int MODE, typename R>
+__global__ __launch_bounds__(64 * kFW) void k_elasticity_faces(
+    int n_faces, int n_elem, int n_s, int n_u, const int32_t* __restrict__ face_inc,
+    const int32_t* __restrict__ face_lf, const int32_t* __restrict__ el_face,
+    const uint64_t* __restrict__ el_orient, const int32_t* __restrict__ cells, const R* __restrict__ coords,
+    const R* __restrict__ tab, const R* __restrict__ lam, const R* __restrict__ mu, int cs_e, int cs_q,
+    const R* __restrict__ f, int fs_e, int fs_q, int fs_c, const int32_t* __restrict__ rowptr,
+    const uint16_t* __restrict__ codes, R* __restrict__ vals, R* __restrict__ rhs) {
+  const R* tw = tab;
+  const R* tphi = tab + kEQ;
+  const R* tdiv = tphi + kEQ * kEB * 3;
+  const R* tU = tdiv + kEQ * kEB;
+  const R* tPhi = tU + kEB;
+  const R* tDD = tPhi + kEB * 3;
+  const int n_face_blocks = (n_faces + kFW - 1) / kFW;
+  if ((int)blockIdx.x >= n_face_blocks) {
+    // cell rows: u_c = n_s + 3 e + c, t_c = n_s + n_u + 3 e + c
+    const int e = ((int)blockIdx.x - n_face_blocks) * kCellBlock + threadIdx.x;
+    if (e >= n_elem) return;
+    R J[3][3], detJ;
+    face_geometry<R>(cells, coords, e, J, detJ);
+    R mass = 0;
+#pragma unroll
+    for (int q = 0; q < kEQ; ++q) mass += tw[q] * detJ;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int ru = n_s + 3 * e + c, rt = n_s + n_u + 3 * e + c;
+      vals[rowptr[ru]] = MODE == 1 ? mass : R(0);
+      vals[rowptr[rt]] = MODE == 1 ? mass : R(0);
+      if (MODE == 0) {
+        R su = 0, st = 0;
+#pragma unroll
+        for (int q = 0; q < kEQ; ++q) {
+          const R wd = tw[q] * detJ;
+          su -= wd * f[(int64_t)e * fs_e + q * fs_q + c * fs_c];
+          st -= wd * f[(int64_t)e * fs_e + q * fs_q + (3 + c) * fs_c];
+        }
+        rhs[ru] = su;
+        rhs[rt] = st;
+      }
+    }
+    return;
+  }
+  __shared__ FaceLds<R> lds_all[kFW];
+  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int p = __builtin_amdgcn_readfirstlane((int)blockIdx.x * kFW + wv);
+  if (p >= n_faces) return;
+  FaceLds<R>& L = lds_all[wv];
+  const int e0 = face_inc[2 * p], e1r = face_inc[2 * p + 1];
+  const int ni = e1r >= 0 ? 2 : 1;
+  const int e1 = ni == 2 ? e1r : e0;
+  const int lfw = face_lf[p];
+  FaceInc<R> g0, g1;
+  {
+    FaceInc<R>* gs[2] = {&g0, &g1};
+    const int es[2] = {e0, e1};
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      FaceInc<R>& g = *gs[i];
+      const int e = es[i];
+      face_geometry<R>(cells, coords, e, g.J, g.detJ);
+      g.invdet = R(1) / g.detJ;
+      g.orient = el_orient[e];
+      const int lf = (lfw >> (2 * i)) & 3;
+      g.ajpack = 0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const int b = 3 * lf + k;
+        g.ajpack |= b << (4 * (int)((g.orient >> (3 * b)) & 3));
+      }
+      g.nb = 0;
+      g.blpack = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (el_face[e * 4 + k] >= p) {
+          g.blpack |= k << (2 * g.nb);
+          ++g.nb;
+        }
+    }
+  }
+  // pushed-forward basis psi_{b d}(q) of both elements
+  for (int t = l; t < 2 * kEB * kEQ; t += 64) {
+    const int i = t / (kEB * kEQ), rem = t % (kEB * kEQ), b = rem / kEQ, q = rem % kEQ;
+    if (i >= ni) continue;
+    const R p0 = tphi[(q * kEB + b) * 3], p1 = tphi[(q * kEB + b) * 3 + 1], p2 = tphi[(q * kEB + b) * 3 + 2];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) L.psi[i][b][d][q] = i ? face_psi(g1, d, p0, p1, p2) : face_psi(g0, d, p0, p1, p2);
+  }
+  // coefficient-weighted rows of the owning face's basis functions
+  if (l < 2 * 3 * kEQ) {
+    const int i = l / (3 * kEQ), j = (l / kEQ) % 3, q = l % kEQ;
+    if (i < ni) {
+      const int e = i ? e1 : e0;
+      const int a = ((i ? g1.ajpack : g0.ajpack) >> (4 * j)) & 15;
+      const R m = mu[(int64_t)e * cs_e + q * cs_q];
+      const R lm = lam[(int64_t)e * cs_e + q * cs_q];
+      const R wd = tw[q] * (i ? g1.detJ : g0.detJ);
+      const R k1 = wd / (R(2) * m);
+      const R k2 = wd * (R(1) / (R(2) * m)) * (lm / (R(2) * m + R(3) * lm));
+      const R p0 = tphi[(q * kEB + a) * 3], p1 = tphi[(q * kEB + a) * 3 + 1], p2 = tphi[(q * kEB + a) * 3 + 2];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const R ps = i ? face_psi(g1, c, p0, p1, p2) : face_psi(g0, c, p0, p1, p2);
+        L.kpsi[i][0][j][c][q] = k2 * ps;
+        L.kpsi[i][1][j][c][q] = k1 * ps;
+      }
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+  // D_{a_j b} = sum_q k1_q psi_{a_j} . psi_b  (+ DDref_{a_j b} / detJ for the Riesz map)
+  for (int t = l; t < 2 * 3 * kEB; t += 64) {
+    const int i = t / (3 * kEB), j = (t / kEB) % 3, b = t % kEB;
+    if (i >= ni) continue;
+    R s = 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int q = 0; q < kEQ; ++q) s += L.kpsi[i][1][j][c][q] * L.psi[i][b][c][q];
+    if (MODE == 1) {
+      const int a = ((i ? g1.ajpack : g0.ajpack) >> (4 * j)) & 15;
+      s += tDD[a * kEB + b] * (i ? g1.invdet : g0.invdet);
+    }
+    L.D[i][j][b] = s;
+  }
+  __builtin_amdgcn_wave_barrier();
+  // element rows of the face: (s, s) blocks of the needed faces, then the u / t columns
+  face_ss_rows<R>(L, g0, 0, l);
+  if (ni == 2) face_ss_rows<R>(L, g1, 1, l);
+  if (MODE == 0) {
+    for (int t = l; t < 2 * 9 * 6; t += 64) {
+      const int i = t / 54, m = (t / 6) % 9, col = t % 6;
+      if (i >= ni) continue;
+      const int j = m / 3, c = m % 3;
+      const int a = ((i ? g1.ajpack : g0.ajpack) >> (4 * j)) & 15;
+      const R sa = i ? basis_sign(g1, a) : basis_sign(g0, a);
+      R v = 0;
+      if (col < 3) {
+        v = col == c ? tU[a] : R(0);
+      } else {
+        const int ct = col - 3;
+        if (ct != c) {
+          const int k = 3 - c - ct;
+          const R P0 = tPhi[a * 3], P1 = tPhi[a * 3 + 1], P2 = tPhi[a * 3 + 2];
+          // row k of J by selects (a runtime index into a register array would go to scratch)
+          R Jk[3];
+#pragma unroll
+          for (int x = 0; x < 3; ++x) {
+            const R u0 = k == 0 ? g0.J[0][x] : (k == 1 ? g0.J[1][x] : g0.J[2][x]);
+            const R u1 = k == 0 ? g1.J[0][x] : (k == 1 ? g1.J[1][x] : g1.J[2][x]);
+            Jk[x] = i ? u1 : u0;
+          }
+          const R Aak = Jk[0] * P0 + Jk[1] * P1 + Jk[2] * P2;
+          const bool even = (ct == 0 && c == 1) || (ct == 1 && c == 2) || (ct == 2 && c == 0);
+          v = even ? Aak : -Aak;
+        }
+      }
+      L.K[i][m][36 + col] = sa * v;
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+  // the face's CSR slice: contiguous, coalesced stores
+  const int base = rowptr[9 * p];
+  const int cnt = rowptr[9 * p + 9] - base;
+  for (int k = l; k < cnt; k += 64) {
+    const int code = codes[base + k];
+    const int r = code & 15, s0 = (code >> 4) & 63, s1 = (code >> 10) & 63;
+    R v = 0;
+    if (s0 != kNoSrc) v += L.K[0][r][s0];
+    if (s1 != kNoSrc) v += L.K[1][r][s1];
+    vals[base + k] = v;
+  }
+  if (MODE == 0 && l < 9) rhs[9 * p + l] = R(0);
+}
+
+template <int MODE, typename R>
+static int launch_elasticity_faces(int n_faces, int n_elem, int n_s, int n_u, const int32_t* face_inc,
+                                   const int32_t* face_lf, const int32_t* el_face, const uint64_t* el_orient,
+                                   const int32_t* cells, const void* coords, const void* tab, const void* lam,
+                                   const void* mu, int cse, int csq, const void* f, int fse, int fsq, int fsc,
+                                   const int32_t* rowptr, const uint16_t* codes, void* vals, void* rhs,
+                                   hipStream_t s) {
+  const int blocks = grid_for(n_faces, kFW) + grid_for(n_elem, kCellBlock);
+  hipLaunchKernelGGL((k_elasticity_faces<MODE, R>), dim3(blocks), dim3(64 * kFW), 0, s, n_faces, n_elem, n_s, n_u,
+                     face_inc, face_lf, el_face, el_orient, cells, static_cast<const R*>(coords),
+                     static_cast<const R*>(tab), static_cast<const R*>(lam), static_cast<const R*>(mu), cse, csq,
+                     static_cast<const R*>(f), fse, fsq, fsc, rowptr, codes, static_cast<R*>(vals),
+                     static_cast<R*>(rhs));
+  NFX_LAUNCH_CHECK();
+  return NFX_OK;
+}
+
 // gather: dst[i] = src[idx[i]] (FP32 or FP64)
 template <typename R>
 __global__ __launch_bounds__(256) void k_gather(int64_t n, const int32_t* __restrict__ idx,
@@ -383,6 +707,32 @@ extern "C" int nfx_assemble_elasticity(int mode, int n_elem, const int32_t* d_ce
               : launch_elasticity<1, float>(n_elem, d_cells, d_coords, d_tables, d_lam, d_mu,
                                             coef_stride_e, coef_stride_q, nullptr, 0, 0, 0,
                                             d_slot, slot_pairs, nullptr, d_vals, d_stage, nullptr, s);
+}
+
+extern "C" int nfx_assemble_elasticity_faces(int mode, int n_faces, int n_elem, int n_s, int n_u,
+                                             const int32_t* d_face_inc, const int32_t* d_face_lf,
+                                             const int32_t* d_el_face, const uint64_t* d_el_orient,
+                                             const int32_t* d_cells, const void* d_coords, const void* d_tables,
+                                             const void* d_lam, const void* d_mu, int coef_stride_e,
+                                             int coef_stride_q, const void* d_f, int f_stride_e, int f_stride_q,
+                                             int f_stride_c, const int32_t* d_rowptr, const uint16_t* d_codes,
+                                             int fp64, void* d_vals, void* d_rhs, void* stream) {
+  NFX_REQUIRE(mode == 0 || mode == 1, "nfx_assemble_elasticity_faces: mode");
+  NFX_REQUIRE(n_faces >= 0 && n_elem >= 0 && n_s == 9 * n_faces && n_u == 3 * n_elem,
+              "nfx_assemble_elasticity_faces: sizes (n_s = 9 n_faces, n_u = 3 n_elem)");
+  if (n_faces == 0 && n_elem == 0) return NFX_OK;
+  NFX_REQUIRE(d_face_inc && d_face_lf && d_el_face && d_el_orient && d_cells && d_coords && d_tables && d_lam &&
+                  d_mu && d_rowptr && d_codes && d_vals,
+              "nfx_assemble_elasticity_faces: null pointer");
+  NFX_REQUIRE(mode == 1 || (d_f && d_rhs), "nfx_assemble_elasticity_faces: residual args");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+#define NFX_EF(M, T)                                                                                            \
+  launch_elasticity_faces<M, T>(n_faces, n_elem, n_s, n_u, d_face_inc, d_face_lf, d_el_face, d_el_orient, d_cells, \
+                                d_coords, d_tables, d_lam, d_mu, coef_stride_e, coef_stride_q, d_f, f_stride_e,   \
+                                f_stride_q, f_stride_c, d_rowptr, d_codes, d_vals, d_rhs, s)
+  if (mode == 0) return fp64 ? NFX_EF(0, double) : NFX_EF(0, float);
+  return fp64 ? NFX_EF(1, double) : NFX_EF(1, float);
+#undef NFX_EF
 }
 
 extern "C" int nfx_gather(int fp64, int64_t n, const int32_t* d_idx, const void* d_src, void* d_dst,

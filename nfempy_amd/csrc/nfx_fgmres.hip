@@ -20,11 +20,13 @@
 #include <vector>
 
 #include "nfx_common.h"
+#include "nfx_residual.h"
 
 namespace nfx {
 namespace {
 
 constexpr int kFB = 256;
+constexpr int kFgResLanes = 4;  // lanes per row of the FP64 refinement residual
 constexpr int kChunkDot = 1024;  // rows per block of the dot-product kernel
 constexpr int kChunkUpd = 1024;  // rows per block of the update/norm kernel
 
@@ -227,35 +229,6 @@ __global__ void k_fg_diag(int n, const float* __restrict__ d, const float* __res
 }
 
 // FP64 residual r = b - A x with block partials of r.r and b.b
-__global__ __launch_bounds__(kFB) void k_fg_residual64(int n, const int32_t* __restrict__ rp,
-                                                       const int32_t* __restrict__ ci,
-                                                       const double* __restrict__ va,
-                                                       const double* __restrict__ x,
-                                                       const double* __restrict__ b, double* __restrict__ r,
-                                                       double* __restrict__ partial) {
-  __shared__ double red[2][kFB / 64];
-  double rr = 0, bb = 0;
-  for (int row = blockIdx.x * kFB + threadIdx.x; row < n; row += gridDim.x * kFB) {
-    double s = 0;
-    for (int k = rp[row]; k < rp[row + 1]; ++k) s += va[k] * x[ci[k]];
-    const double ri = b[row] - s;
-    r[row] = ri;
-    rr += ri * ri;
-    bb += b[row] * b[row];
-  }
-  rr = fg_warp_sum(rr);
-  bb = fg_warp_sum(bb);
-  if ((threadIdx.x & 63) == 0) {
-    red[0][threadIdx.x >> 6] = rr;
-    red[1][threadIdx.x >> 6] = bb;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    partial[2 * blockIdx.x] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
-    partial[2 * blockIdx.x + 1] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
-  }
-}
-
 __global__ void k_fg_to_f32(int n, const double* __restrict__ r, double scale, float* __restrict__ rf) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p < n) rf[p] = (float)(r[p] / scale);
@@ -312,7 +285,9 @@ extern "C" int nfx_fgmres_create(nfx_fgmres** out, int n_rows, int restart, void
   s->stream = static_cast<hipStream_t>(stream);
   s->blocks = grid_for(n_rows, kChunkDot);
   s->ublocks = grid_for(n_rows, kChunkUpd);
-  s->rblocks = std::min(grid_for(n_rows, kFB), 1024);
+  // residual lanes per row: the Darcy saddle points FGMRES serves have rows of at most 12
+  // entries (RT1-P0), 4 lanes x 3 entries cover one in a single pass
+  s->rblocks = residual_blocks(n_rows, kFgResLanes);
   const size_t n = n_rows, m = restart;
   hipError_t e = hipMalloc(&s->V, sizeof(float) * n * (m + 1));
   if (e == hipSuccess) e = hipMalloc(&s->Z, sizeof(float) * n * m);
@@ -456,9 +431,8 @@ extern "C" int nfx_solve_fgmres_ir(nfx_fgmres* s, const int32_t* d_rowptr, const
   int total = 0, outer = 0;
   double rel = 0;
   for (; outer <= max_outer; ++outer) {
-    hipLaunchKernelGGL(k_fg_residual64, dim3(s->rblocks), dim3(kFB), 0, st, n, d_rowptr, d_col, d_val64, d_x, d_b,
-                       s->r64, s->partial_r);
-    NFX_LAUNCH_CHECK();
+    NFX_CHECK_HIP(launch_residual64(kFgResLanes, s->rblocks, st, n, d_rowptr, d_col, d_val64, d_x, d_b, s->r64,
+                                    s->partial_r));
     NFX_CHECK_HIP(hipMemcpyAsync(s->h_partial, s->partial_r, sizeof(double) * 2 * s->rblocks,
                                  hipMemcpyDeviceToHost, st));
     NFX_CHECK_HIP(hipStreamSynchronize(st));

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "nfx_common.h"
+#include "nfx_residual.h"
 
 namespace nfx {
 
@@ -80,30 +81,6 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int n, const int32_t* __restric
     }
     s = group_sum<LANES>(s);
     if (lane == 0) y[row] = alpha * s + (beta == 0.f ? 0.f : beta * y[row]);
-  }
-}
-
-// FP64 residual r = b - A x; block partials of r.r and b.b (summed on the host
-// in block order).
-__global__ __launch_bounds__(kBlock) void k_residual64(
-    int n, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
-    const double* __restrict__ va, const double* __restrict__ x, const double* __restrict__ b,
-    double* __restrict__ r, double* __restrict__ partial) {
-  __shared__ double sh[kBlock / 64];
-  double rr = 0, bb = 0;
-  for (int row = blockIdx.x * kBlock + threadIdx.x; row < n; row += gridDim.x * kBlock) {
-    double s = 0;
-    for (int j = rp[row]; j < rp[row + 1]; ++j) s += va[j] * x[ci[j]];
-    const double ri = b[row] - s;
-    r[row] = ri;
-    rr += ri * ri;
-    bb += b[row] * b[row];
-  }
-  const double brr = block_sum(rr, sh);
-  const double bbb = block_sum(bb, sh);
-  if (threadIdx.x == 0) {
-    partial[2 * blockIdx.x] = brr;
-    partial[2 * blockIdx.x + 1] = bbb;
   }
 }
 
@@ -442,6 +419,7 @@ struct nfx_solver {
   MinresState* h_st = nullptr;  // pinned
   int blocks_a = 1, blocks_b = 1;
   int lanes = 4;
+  int blocks_r = 1;  // grid of the FP64 residual kernel
 };
 
 extern "C" int nfx_spmv(int n_rows, const int32_t* d_rowptr, const int32_t* d_col,
@@ -490,7 +468,8 @@ extern "C" int nfx_solver_create(nfx_solver** out, int n_rows, int nnz, void* st
   s->nnz = nnz;
   s->stream = static_cast<hipStream_t>(stream);
   const double avg = (double)nnz / n_rows;
-  s->lanes = avg > 24 ? 16 : (avg > 10 ? 8 : 4);
+  s->lanes = residual_lanes(avg);
+  s->blocks_r = residual_blocks(n_rows, s->lanes);
   int ba = grid_for(n_rows, kBlock / s->lanes);
   int bb = grid_for(n_rows, kBlock);
   s->blocks_a = ba < kMaxBlocks ? ba : kMaxBlocks;
@@ -499,8 +478,8 @@ extern "C" int nfx_solver_create(nfx_solver** out, int n_rows, int nnz, void* st
   if (e == hipSuccess) e = hipMalloc(&s->r64, sizeof(double) * (size_t)n_rows);
   if (e == hipSuccess) e = hipMalloc(&s->partial_a, sizeof(double) * kMaxBlocks);
   if (e == hipSuccess) e = hipMalloc(&s->partial_b, sizeof(double) * kMaxBlocks);
-  if (e == hipSuccess) e = hipMalloc(&s->partial_r, sizeof(double) * 2 * kMaxBlocks);
-  if (e == hipSuccess) e = hipHostMalloc(&s->h_partial, sizeof(double) * 2 * kMaxBlocks, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMalloc(&s->partial_r, sizeof(double) * 2 * kResMaxBlocks);
+  if (e == hipSuccess) e = hipHostMalloc(&s->h_partial, sizeof(double) * 2 * kResMaxBlocks, hipHostMallocDefault);
   if (e == hipSuccess) e = hipMalloc(&s->st, 2 * sizeof(MinresState));
   if (e == hipSuccess) e = hipHostMalloc(&s->h_st, sizeof(MinresState), hipHostMallocDefault);
   if (e != hipSuccess) {
@@ -637,10 +616,8 @@ static int minres_inner(nfx_solver* s, const int32_t* rp, const int32_t* ci, con
 // r = b - A x in FP64; returns ||r||^2 and ||b||^2 (host sum of block partials).
 static int residual64(nfx_solver* s, const int32_t* rp, const int32_t* ci, const double* va,
                       const double* x, const double* b, double* rr, double* bb) {
-  const int blocks = s->blocks_b;
-  hipLaunchKernelGGL(k_residual64, dim3(blocks), dim3(kBlock), 0, s->stream, s->n, rp, ci, va, x,
-                     b, s->r64, s->partial_r);
-  NFX_LAUNCH_CHECK();
+  const int blocks = s->blocks_r;
+  NFX_CHECK_HIP(launch_residual64(s->lanes, blocks, s->stream, s->n, rp, ci, va, x, b, s->r64, s->partial_r));
   NFX_CHECK_HIP(hipMemcpyAsync(s->h_partial, s->partial_r, sizeof(double) * 2 * blocks,
                                hipMemcpyDeviceToHost, s->stream));
   NFX_CHECK_HIP(hipStreamSynchronize(s->stream));
@@ -705,12 +682,11 @@ extern "C" int nfx_residual64(int n_rows, const int32_t* d_rowptr, const int32_t
   NFX_REQUIRE(d_rowptr && d_col && d_val && d_x && d_b && d_r, "nfx_residual64: null pointer");
   hipStream_t sm = static_cast<hipStream_t>(stream);
   double* partial = nullptr;
-  int blocks = grid_for(n_rows, kBlock);
-  if (blocks > kMaxBlocks) blocks = kMaxBlocks;
+  // 8 lanes per row: the row lengths are not read here (no synchronisation)
+  const int blocks = residual_blocks(n_rows, kResidualLanesDefault);
   NFX_CHECK_HIP(hipMallocAsync(reinterpret_cast<void**>(&partial), sizeof(double) * 2 * blocks, sm));
-  hipLaunchKernelGGL(k_residual64, dim3(blocks), dim3(kBlock), 0, sm, n_rows, d_rowptr, d_col,
-                     d_val, d_x, d_b, d_r, partial);
-  NFX_LAUNCH_CHECK();
+  NFX_CHECK_HIP(launch_residual64(kResidualLanesDefault, blocks, sm, n_rows, d_rowptr, d_col, d_val, d_x,
+                                  d_b, d_r, partial));
   NFX_CHECK_HIP(hipFreeAsync(partial, sm));
   return NFX_OK;
 }

@@ -208,7 +208,11 @@ def row_pos_device(pos, nl, sign_mask, rowptr=None, tile=128):
     nib = np.where(p == 255, 15, p)
     words = (nib << (4 * np.arange(nl, dtype=np.uint64))[None, None, :]).sum(axis=2, dtype=np.uint64)
     words |= (sign_mask.astype(np.uint64) & np.uint64((1 << nl) - 1)) << np.uint64(4 * nl)
-    if rowptr is not None and nl <= 5:
+    if nl <= 5:
+        # the row kernel reads these offsets unconditionally for such elements
+        if rowptr is None:
+            raise ValueError("rowptr is required for nl <= 5: the row kernel reads each row's "
+                             "in-tile CSR offset from the position words")
         rp = np.asarray(rowptr, dtype=np.int64)
         r = np.arange(len(rp) - 1)
         off = rp[:-1] - rp[(r // tile) * tile]
@@ -437,6 +441,14 @@ class DarcySystem:
         stride = 4 if self.nl <= 6 else pp.shape[2]
         if self.row_pos.numel() != self.n * 2 * stride:
             raise ValueError("device row column positions size")
+        if self.nl <= 5:
+            # the in-tile CSR offsets the kernel decodes from bits 25..31 of the two words
+            w = self.row_pos.cpu().numpy().view(np.uint32).reshape(self.n, 2).astype(np.int64)
+            off = (w[:, 0] >> 25) | ((w[:, 1] >> 25) << 7)
+            rp = pat.rowptr.astype(np.int64)
+            r = np.arange(self.n)
+            if not np.array_equal(off, rp[:-1] - rp[(r // ROW_TILE) * ROW_TILE]):
+                raise ValueError("row position words carry wrong in-tile CSR offsets")
         if self.F:
             if self.nqf != NQF:
                 raise ValueError("facet quadrature does not match the compiled kernel")
@@ -512,6 +524,8 @@ class DarcySystem:
                  nat.ptr(rg), s)
         nat.call("nfx_stage_reduce", len(self.pattern.target), nat.ptr(self.segptr), nat.ptr(self.target),
                  int(fp64), nat.ptr(stage), nat.ptr(vals), s)
+        # the coefficient arrays stay referenced until the next launch of this system
+        self._held = (ka, fa)
         return vals, rg
 
     def _assemble_rows(self, kappa, rhs, fp64, stream, p_dirichlet=None):
@@ -553,6 +567,7 @@ class DarcySystem:
             held = (ka, fa, pd)
         fn = getattr(nat.lib(), name)
         nat.check(fn(*args, s), name)
+        self._held = held
         if all(isinstance(c, CoefficientArg) for c in (kappa, rhs, p_dirichlet) if c is not None):
             cargs = tuple(t(v) for t, v in zip(fn.argtypes, args))
             self._launch_cache[key] = (kappa, rhs, p_dirichlet, fn, cargs, name, held)
@@ -579,6 +594,7 @@ class DarcySystem:
         pd = self.dirichlet_values(p_dirichlet)
         rg = out if out is not None else (self.rhs64 if fp64 else self.rhs32)
         coords = self.coords64 if fp64 else self.coords
+        self._held = (pd, rows)
         nat.call("nfx_assemble_darcy_bc", self.dim, self.k, self.F, nat.ptr(self.bc_facet),
                  nat.ptr(self.bc_neigh), nat.ptr(self.bc_combo), nat.ptr(self.cells), nat.ptr(coords),
                  nat.ptr(self.bc_trace64 if fp64 else self.bc_trace32),
@@ -614,6 +630,7 @@ class DarcySystem:
                  nat.ptr(ka.get(fp64)), ka.se, ka.sq, nat.ptr(fa.get(fp64)), fa.se, fa.sq,
                  nat.ptr(self._local_slot), nat.ptr(self._local_rows), int(fp64), nat.ptr(out), nat.ptr(out),
                  nat.ptr(res), s)
+        self._held = (ka, fa)
         return (out.cpu().numpy().reshape(self.E, self.nl, self.nl), res.cpu().numpy().reshape(self.E, self.nl))
 
     def local_index_of_reference(self):

@@ -30,9 +30,16 @@ NL = 42
 
 
 def elasticity_tables(s_el, points, weights):
+    """w (nq) | phi (nq, 12, 3) | reference divergence (nq, 12) | Uref (12) | Phi (12, 3) |
+    DDref (12, 12): Uref_b = sum_q w divref_b, Phi_be = sum_q w phi_be,
+    DDref_ab = sum_q w divref_a divref_b (the element-independent sums of the face kernel)."""
     tab = s_el.tabulate(points)
     div = np.einsum("kqjk->qj", tab[1:])
-    return np.concatenate([weights, tab[0].ravel(), div.ravel()])
+    phi = tab[0]
+    uref = weights @ div
+    Phi = np.einsum("q,qbe->be", weights, phi)
+    dd = np.einsum("q,qa,qb->ab", weights, div, div)
+    return np.concatenate([weights, phi.ravel(), div.ravel(), uref, Phi.ravel(), dd.ravel()])
 
 
 def _vector_coefficient(value, xq, nq, ncomp, device):
@@ -97,6 +104,110 @@ def _sbaij_mask(vals, ref_dest, ref_sign, upper=True):
     return keep
 
 
+FACE_NO_SRC = 63
+
+
+def face_owner_layout(n, ref_dest, ref_sign):
+    """Per-face and per-element data of the face-owner kernel (nfx_assemble_elasticity_faces).
+
+    The stress DoFs come in blocks of 9 per face (3 BDM1 DoFs x 3 rows, ``j * 3 + c``), the
+    displacement and rotation DoFs 3 per cell (``n_s + 3 e + c``, ``n_s + n_u + 3 e + c``).
+    Returns a dict with face_inc (n_f, 2) (incident elements in element order, -1 none),
+    face_lf (n_f,) (local face index of each incidence, 2 bits each), el_face (E, 4) (the
+    DoF block of each local face) and el_orient (E,) uint64 (bits 3 b, 3 b + 1: face slot of
+    reference basis b; bit 3 b + 2: its orientation sign), or raises NotImplementedError
+    when the DoF layout is not this one."""
+    E = ref_dest.shape[0]
+    if ref_dest.shape[1] != NL:
+        raise NotImplementedError("face-owner layout: 42 local DoFs expected")
+    sd = ref_dest[:, :36].reshape(E, 12, 3).astype(np.int64)
+    if np.any(sd != sd[:, :, :1] + np.arange(3)):
+        raise NotImplementedError("face-owner layout: stress components must be consecutive")
+    base = sd[:, :, 0]
+    if np.any(base % 3):
+        raise NotImplementedError("face-owner layout: stress DoF blocks")
+    block, slot = base // 9, (base % 9) // 3
+    el_face = block[:, ::3]
+    if np.any(block != np.repeat(el_face, 3, axis=1)):
+        raise NotImplementedError("face-owner layout: a face's BDM1 DoFs must share one DoF block")
+    if np.any(np.sort(slot.reshape(E, 4, 3), axis=2) != np.arange(3)):
+        raise NotImplementedError("face-owner layout: face slots")
+    sg = ref_sign[:, :36].reshape(E, 12, 3)
+    if np.any(sg != sg[:, :, :1]):
+        raise NotImplementedError("face-owner layout: signs must not depend on the stress row")
+    neg = (sg[:, :, 0] < 0).astype(np.uint64)
+    bits = slot.astype(np.uint64) | (neg << np.uint64(2))
+    orient = (bits << (np.uint64(3) * np.arange(12, dtype=np.uint64))[None, :]).sum(axis=1, dtype=np.uint64)
+    n_f = int(el_face.max()) + 1 if E else 0
+    n_s = 9 * n_f
+    n_u = 3 * E
+    if n != n_s + 2 * n_u:
+        raise NotImplementedError("face-owner layout: n != 9 n_faces + 6 n_cells")
+    e_idx = np.arange(E)[:, None]
+    if np.any(ref_dest[:, 36:39] != n_s + 3 * e_idx + np.arange(3)) or \
+            np.any(ref_dest[:, 39:42] != n_s + n_u + 3 * e_idx + np.arange(3)):
+        raise NotImplementedError("face-owner layout: cell DoFs must follow the element order")
+    flat = el_face.reshape(-1)
+    order = np.argsort(flat, kind="stable")  # element order inside each face
+    counts = np.bincount(flat, minlength=n_f)
+    if counts.min(initial=1) < 1 or counts.max(initial=1) > 2:
+        raise NotImplementedError("face-owner layout: every face needs 1 or 2 incident elements")
+    first = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    face_inc = np.full((n_f, 2), -1, np.int64)
+    lf = np.zeros((n_f, 2), np.int64)
+    face_inc[:, 0] = order[first] // 4
+    lf[:, 0] = order[first] % 4
+    two = counts == 2
+    face_inc[two, 1] = order[first[two] + 1] // 4
+    lf[two, 1] = order[first[two] + 1] % 4
+    return {"n_faces": n_f, "n_s": n_s, "n_u": n_u, "face_inc": face_inc, "face_lf": lf[:, 0] | (lf[:, 1] << 2),
+            "el_face": el_face, "el_orient": orient}
+
+
+def face_owner_codes(n, lay, ref_dest, keep, pattern, zero_diag):
+    """16-bit source code of every CSR entry of the stress rows (bits 0..3: row inside the
+    face, 3 j + c; bits 4..9 / 10..15: element-local reference column of the entry in the
+    face's first / second incident element, 63 = none), after checking that every cell row
+    holds exactly its diagonal (its only entry in both SBAIJ operators)."""
+    E = ref_dest.shape[0]
+    n_s = lay["n_s"]
+    rowptr = pattern.rowptr.astype(np.int64)
+    col = pattern.col.astype(np.int64)
+    if np.any(np.diff(rowptr[n_s:]) != 1) or np.any(col[rowptr[n_s:n]] != np.arange(n_s, n)):
+        raise NotImplementedError("face-owner layout: cell rows must hold only their diagonal")
+    rows = np.repeat(ref_dest[:, :, None], NL, axis=2)
+    cols = np.repeat(ref_dest[:, None, :], NL, axis=1)
+    src = keep & (rows <= cols) & (rows < n_s)
+    e, I, Jl = np.nonzero(src)
+    R = ref_dest[e, I].astype(np.int64)
+    C = ref_dest[e, Jl].astype(np.int64)
+    keys = np.repeat(np.arange(n, dtype=np.int64), np.diff(rowptr)) * n + col
+    k = np.searchsorted(keys, R * n + C)
+    if np.any(k >= len(keys)) or np.any(keys[np.minimum(k, len(keys) - 1)] != R * n + C):
+        raise ValueError("face-owner codes: an element entry is missing from the pattern")
+    p = R // 9
+    inc = (lay["face_inc"][p, 1] == e).astype(np.int64)
+    if np.any((inc == 0) & (lay["face_inc"][p, 0] != e)):
+        raise ValueError("face-owner codes: an element entry of a face row from a non-incident element")
+    ns_entries = int(rowptr[n_s])
+    srcs = np.full((ns_entries, 2), FACE_NO_SRC, np.int64)
+    flat = k * 2 + inc
+    if len(np.unique(flat)) != len(flat):
+        raise ValueError("face-owner codes: two contributions of one element to one entry")
+    srcs.reshape(-1)[flat] = Jl
+    row_of = np.repeat(np.arange(n_s, dtype=np.int64), np.diff(rowptr[: n_s + 1]))
+    none = (srcs[:, 0] == FACE_NO_SRC) & (srcs[:, 1] == FACE_NO_SRC)
+    if np.any(none & (row_of != col[:ns_entries])) or (not zero_diag and np.any(none)):
+        raise ValueError("face-owner codes: an entry without a contribution that is not an explicit zero diagonal")
+    # the kernel evaluates the stress columns of faces whose block is >= the row's face
+    sc = srcs < 36
+    blk_col = np.where(sc, col[:ns_entries, None] // 9, 1 << 40)
+    if np.any(blk_col < (row_of // 9)[:, None]):
+        raise ValueError("face-owner codes: a stress column below the row's face block")
+    code = (row_of % 9) | (srcs[:, 0] << 4) | (srcs[:, 1] << 10)
+    return code.astype(np.uint16)
+
+
 def full_from_upper(n, rowptr, col):
     """Full symmetric CSR pattern from an upper-triangle pattern + gather map into it."""
     r = np.repeat(np.arange(n, dtype=np.int64), np.diff(rowptr))
@@ -131,6 +242,8 @@ class ElasticitySystem:
         self.E = len(s_sp.element_ids)
         self.n = space.n_dof
         self.n_s = s_sp.n_dof
+        # DoF ranges of the fields (stress | displacement | rotation), in the space's order
+        self.field_bounds = [0, self.n_s, self.n_s + self.u_sp.n_dof, self.n]
         points, weights = space.quadrature[3]
         if len(weights) != 8:
             raise ValueError("expected the degree-3 tetrahedral rule (8 points)")
@@ -161,12 +274,31 @@ class ElasticitySystem:
         d_loc = to_dev(loc_slot, device=device)
         KA = self._local(0, d_loc)
         KP = self._local(1, d_loc)
-        self.pattern_A = build_pattern(self.n, ref_dest, ref_sign, keep=_sbaij_mask(KA, ref_dest, ref_sign),
-                                       zero_diag_dest=ref_dest, upper=True)
-        self.pattern_P = build_pattern(self.n, ref_dest, ref_sign, keep=_sbaij_mask(KP, ref_dest, ref_sign),
-                                       upper=True)
+        keepA = _sbaij_mask(KA, ref_dest, ref_sign, upper=False)
+        keepP = _sbaij_mask(KP, ref_dest, ref_sign, upper=False)
         del KA, KP
+        self.pattern_A = build_pattern(self.n, ref_dest, ref_sign, keep=keepA, zero_diag_dest=ref_dest, upper=True)
+        self.pattern_P = build_pattern(self.n, ref_dest, ref_sign, keep=keepP, upper=True)
         self._upload_patterns()
+        # face-owner kernel (the default warm path): one wave per face block of 9 CSR rows
+        self.face_layout = None
+        try:
+            lay = face_owner_layout(self.n, ref_dest, ref_sign)
+            codes = {"A": face_owner_codes(self.n, lay, ref_dest, keepA, self.pattern_A, zero_diag=True),
+                     "P": face_owner_codes(self.n, lay, ref_dest, keepP, self.pattern_P, zero_diag=False)}
+        except NotImplementedError:
+            lay = None
+        del keepA, keepP
+        if lay is not None:
+            self.face_layout = lay
+            self.face_inc = to_dev(lay["face_inc"].astype(np.int32), device=device)
+            self.face_lf = to_dev(lay["face_lf"].astype(np.int32), device=device)
+            self.el_face = to_dev(lay["el_face"].astype(np.int32), device=device)
+            self.el_orient = to_dev(lay["el_orient"].view(np.int64), device=device)
+            for key in ("A", "P"):
+                self.ops[key]["codes_np"] = codes[key]
+                self.ops[key]["codes"] = to_dev(codes[key].view(np.int16), device=device)
+        self.use_faces = lay is not None
         self.rhs_row = to_dev(np.ascontiguousarray(row_codes(ref_dest[:, 36:]).T), device=device)
         self.rhs32 = t.zeros(self.n, dtype=t.float32, device=device)
         self.rhs64 = t.zeros(self.n, dtype=t.float64, device=device)
@@ -207,6 +339,7 @@ class ElasticitySystem:
         rows = to_dev(np.ascontiguousarray((np.arange(self.E)[:, None] * NL + np.arange(36, 42)[None, :]).T)
                       .astype(np.int32), device=self.device)
         self._launch(mode, True, d_loc, rows, out, out, rhs, pairs=0)
+        self._held = rows
         self._local_rhs = rhs
         return out.cpu().numpy().reshape(self.E, NL, NL)
 
@@ -286,15 +419,62 @@ class ElasticitySystem:
                 raise ValueError("facet tables do not match the compiled kernel")
             if self.bc_rows_np.shape != (self.F, 9) or (self.bc_rows_np & nat.SLOT_INDEX_MASK).max() >= self.n:
                 raise ValueError("facet rows out of range")
+        if self.use_faces:
+            # every index the face-owner kernel forms stays inside its buffer
+            lay = self.face_layout
+            nf, E = lay["n_faces"], self.E
+            fi = lay["face_inc"]
+            if fi.shape != (nf, 2) or fi[:, 0].min(initial=0) < 0 or fi.max(initial=-1) >= E or \
+                    np.any(fi[:, 1] < -1) or np.any((fi[:, 1] >= 0) & (fi[:, 1] <= fi[:, 0])):
+                raise ValueError("face incidences out of range or not in element order")
+            if lay["el_face"].shape != (E, 4) or lay["el_face"].min(initial=0) < 0 or \
+                    lay["el_face"].max(initial=-1) >= nf or np.any(lay["face_lf"] >> 4):
+                raise ValueError("element face blocks out of range")
+            if lay["n_s"] != self.n_s or lay["n_u"] != self.u_sp.n_dof or self.n != lay["n_s"] + 2 * lay["n_u"]:
+                raise ValueError("face-owner layout sizes")
+            if self.tables32.numel() != 8 + 8 * 36 + 8 * 12 + 12 + 36 + 144:
+                raise ValueError("elasticity table size")
+            for key in ("A", "P"):
+                op = self.ops[key]
+                code = op["codes_np"].astype(np.int64)
+                rp = op["pattern"].rowptr
+                if len(code) != int(rp[self.n_s]) or len(rp) != self.n + 1:
+                    raise ValueError(f"face codes of {key}: size")
+                s0, s1 = (code >> 4) & 63, code >> 10
+                if np.any((code & 15) >= 9) or np.any((s0 >= NL) & (s0 != FACE_NO_SRC)) or \
+                        np.any((s1 >= NL) & (s1 != FACE_NO_SRC)):
+                    raise ValueError(f"face codes of {key} out of range")
+                p_of = np.repeat(np.arange(self.n_s) // 9, np.diff(rp[: self.n_s + 1]))
+                if np.any((s1 != FACE_NO_SRC) & (fi[p_of, 1] < 0)):
+                    raise ValueError(f"face codes of {key}: a second source on a one-element face")
 
     # -- warm assembly ---------------------------------------------------------------
-    def assemble(self, which="A", fp64=False, stream=None, u_dirichlet=None):
+    def assemble(self, which="A", fp64=False, stream=None, u_dirichlet=None, method=None):
+        """Values of operator ``which`` ("A" or "P") and, for A, the residual. method "faces"
+        (the default when the DoF layout allows it): the face-owner kernel; "element": one
+        wave per element through the slot map and the staged segment sum."""
         op = self.ops[which]
         s = nat.stream_handle(stream)
         vals = op["vals64" if fp64 else "vals32"]
         stage = op["stage64" if fp64 else "stage32"]
         rg = self.rhs64 if fp64 else self.rhs32
         mode = 0 if which == "A" else 1
+        method = method or ("faces" if self.use_faces else "element")
+        if method == "faces":
+            if not self.use_faces:
+                raise NotImplementedError("face-owner assembly: DoF layout not supported")
+            c = self._coef
+            lam, mu, f = c["lam"], c["mu"], c["f"]
+            lay = self.face_layout
+            nat.call("nfx_assemble_elasticity_faces", mode, lay["n_faces"], self.E, lay["n_s"], lay["n_u"],
+                     nat.ptr(self.face_inc), nat.ptr(self.face_lf), nat.ptr(self.el_face), nat.ptr(self.el_orient),
+                     nat.ptr(self.cells), nat.ptr(self.coords64 if fp64 else self.coords),
+                     nat.ptr(self.tables64 if fp64 else self.tables32), nat.ptr(lam.get(fp64)),
+                     nat.ptr(mu.get(fp64)), lam.se, lam.sq, nat.ptr(f.get(fp64)), self.nq * 6, 6, 1,
+                     nat.ptr(op["rowptr"]), nat.ptr(op["codes"]), int(fp64), nat.ptr(vals), nat.ptr(rg), s)
+            if mode == 0 and u_dirichlet is not None and self.F:
+                self.assemble_dirichlet(u_dirichlet, fp64=fp64, stream=stream)
+            return vals, rg
         if mode == 0:
             nat.call("nfx_fill", int(fp64), self.n, 0.0, nat.ptr(rg), s)
         self._launch(mode, fp64, op["slot"], self.rhs_row, vals, stage, rg, stream)

@@ -48,3 +48,19 @@ def grf_device(points, seed, ell, sigma, n_features=2048, chunk=16384):
 
 def centroids(points, cells):
     return points[cells].mean(axis=1)
+
+
+def lame_lognormal(n_elements, seed=2024, sigma=0.5):
+    """Per-element Lame parameters of the 3-D elasticity workload: ``lambda_e, mu_e =
+    exp(N(0, sigma^2))``, independent, drawn in that order from ``default_rng(seed)``."""
+    rng = np.random.default_rng(seed)
+    lam = np.exp(sigma * rng.standard_normal(n_elements))
+    mu = np.exp(sigma * rng.standard_normal(n_elements))
+    return lam, mu
+
+
+def elasticity_body_force(x, y, z):
+    """Constant body force (1, 1/2, 1/4) on the displacement rows, none on the rotation rows."""
+    one = np.ones_like(np.asarray(x, dtype=np.float64))
+    zero = np.zeros_like(one)
+    return np.array([one, 0.5 * one, 0.25 * one, zero, zero, zero])

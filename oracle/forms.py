@@ -175,13 +175,29 @@ def _elasticity_common(ps, i, functions):
     return s_sp, u_sp, t_sp, dim, points, weights, x, det, s_tab, u_tab, t_tab, lam, mu
 
 
+def volumetric_factor(lam, mu, dim):
+    """``(1 / 2 mu) (lambda / (2 mu + dim lambda))`` (``le_dual_weak_form.py:434``): the
+    trace part of the compliance ``A sigma = (sigma - lambda / (2 mu + dim lambda) tr(sigma) I) / 2 mu``."""
+    return (1.0 / (2.0 * mu)) * (lam / (2.0 * mu + dim * lam))
+
+
+# rotation component c -> [(stress row, direction d, sign)]: the (t, s) block entry is
+# -sign * int t_k phi_{j,d} on stress row cs (``le_dual_weak_form.py:473-491``); rotation c
+# pairs with tau_{r1 r2} - tau_{r2 r1} for (r1, r2) = (y, z), (z, x), (x, y) in 3-D and
+# (x, y) in 2-D
+SKEW_MAP = {
+    3: {0: [(2, 1, 1.0), (1, 2, -1.0)], 1: [(0, 2, 1.0), (2, 0, -1.0)], 2: [(1, 0, 1.0), (0, 1, -1.0)]},
+    2: {0: [(1, 0, 1.0), (0, 1, -1.0)]},
+}
+
+
 def _ss_blocks(J, s_tab, dim, sc, weights, det, lam, mu):
     ns = s_tab.shape[2] * sc
     phi = s_tab[0, :, :, 0:dim]
     ss = np.einsum("qic,qjc,q->ij", phi, phi, (1.0 / (2.0 * mu)) * det * weights)
     for c in range(sc):
         J[c:ns:sc, c:ns:sc] += ss
-    vol = (1.0 / (2.0 * mu)) * (lam / (2.0 * mu + dim * lam))
+    vol = volumetric_factor(lam, mu, dim)
     flat = phi.reshape(phi.shape[0], -1)
     J[:ns, :ns] -= np.einsum("qa,qb,q->ab", flat, flat, vol * det * weights)
     return ns
@@ -207,15 +223,11 @@ def elasticity_element(ps, i, functions):
     phi = s_tab[0, :, :, 0:dim]
     # A_d[k, j] = sum_q detJ w t_k(q) phi_{j,d}(q)
     A = [np.einsum("qk,qj,q->kj", t_tab[0, :, :, 0], phi[:, :, d], det * weights) for d in range(dim)]
-    if dim == 3:
-        cmap = {0: [(2, A[1]), (1, -A[2])], 1: [(0, A[2]), (2, -A[0])], 2: [(1, A[0]), (0, -A[1])]}
-    else:
-        cmap = {0: [(1, A[0]), (0, -A[1])]}
     for c in range(tc):
         b = ns + nu + c
-        for cs, op in cmap[c]:
-            J[b:n:tc, cs:ns:sc] += -op
-            J[cs:ns:sc, b:n:tc] += -op.T
+        for cs, d, sgn in SKEW_MAP[dim][c]:
+            J[b:n:tc, cs:ns:sc] += -sgn * A[d]
+            J[cs:ns:sc, b:n:tc] += -sgn * A[d].T
     div = np.trace(s_tab[1:, :, :, 0:dim], axis1=0, axis2=3)  # (nq, nphi)
     ub = (weights * u_tab[0, :, :, 0].T) @ div
     for c in range(min(uc, sc)):

@@ -75,38 +75,62 @@ def case_id(c):
     return f"d{d}-k{k}-" + (msh.replace(".msh", "") if msh else f"n{n}")
 
 
-def elasticity_functions(lam=1.0, mu=1.0):
-    """Manufactured 3-D solution: sigma = 2 mu eps(u) + lam tr(eps) I, f_u = div sigma, f_t = 0."""
+def elasticity_functions(lam=1.0, mu=1.0, dim=3, u_expr=None):
+    """Manufactured solution of the dual-mixed elasticity forms from a displacement u:
+    sigma = 2 mu eps(u) + lam tr(eps) I, f_u = div sigma (the forms' sign: the u rows read
+    (div sigma, v) = -r_u = (f_u, v)), f_t = 0, and the rotation the (t, s) coupling of
+    ``le_dual_weak_form.py:473-491`` enforces, t = skew(grad u) as
+    (T_yz, T_zx, T_xy) with T = (grad u - grad u^T) / 2 in 3-D and T_xy in 2-D.
+
+    3-D default u = (sin(pi x) sin(pi y) sin(pi z), x y z (1 - x), cos(y) z^2); 2-D
+    default u is the displacement of ``examples/cauchy_elasticity/
+    strong_solution_elasticity_example_1.py:4-21`` (zero on the unit square's boundary).
+    Returns (functions, exact) with exact fields "s", "u", "t", "div_s"."""
     import sympy as sp
 
     X = sp.symbols("x y z")
-    u = sp.Matrix([sp.sin(sp.pi * X[0]) * sp.sin(sp.pi * X[1]) * sp.sin(sp.pi * X[2]),
-                   X[0] * X[1] * X[2] * (1 - X[0]), sp.cos(X[1]) * X[2] ** 2])
-    G = u.jacobian(X)
+    if u_expr is None:
+        if dim == 3:
+            u_expr = [sp.sin(sp.pi * X[0]) * sp.sin(sp.pi * X[1]) * sp.sin(sp.pi * X[2]),
+                      X[0] * X[1] * X[2] * (1 - X[0]), sp.cos(X[1]) * X[2] ** 2]
+        else:
+            s2x, s2y = sp.sin(2 * sp.pi * X[0]), sp.sin(2 * sp.pi * X[1])
+            c2x, c2y = sp.cos(2 * sp.pi * X[0]), sp.cos(2 * sp.pi * X[1])
+            u_expr = [4 * sp.pi * c2y * s2x ** 2 * s2y, -4 * sp.pi * c2x * s2x * s2y ** 2]
+    Xd = X[:dim]
+    u = sp.Matrix(u_expr)
+    G = u.jacobian(Xd)  # G[c, j] = d u_c / d x_j
     eps = (G + G.T) / 2
-    sig = 2 * mu * eps + lam * eps.trace() * sp.eye(3)
-    f = sp.Matrix([sum(sp.diff(sig[i, j], X[j]) for j in range(3)) for i in range(3)])
-    fu = sp.lambdify(X, list(f), "numpy")
-    us = sp.lambdify(X, list(u), "numpy")
-    ss = sp.lambdify(X, sig.tolist(), "numpy")
+    sig = 2 * mu * eps + lam * eps.trace() * sp.eye(dim)
+    f = [sum(sp.diff(sig[i, j], Xd[j]) for j in range(dim)) for i in range(dim)]
+    T = (G - G.T) / 2
+    t = [T[1, 2], T[2, 0], T[0, 1]] if dim == 3 else [T[0, 1]]
 
-    def vec(fun, n):
-        def g(x, y, z):
-            out = fun(x, y, z)
-            return np.array([np.broadcast_to(np.asarray(v, dtype=float), np.shape(x)) for v in out][:n])
-        return g
+    def lamb(exprs):
+        g = sp.lambdify(X, exprs, "numpy")
 
-    fu6 = vec(fu, 3)
+        def out(x, y, z):
+            shape = np.shape(x)
+            v = g(x, y, z)
+            return np.array([np.broadcast_to(np.asarray(e, dtype=float), shape) for e in v])
+        return out
+
+    fu = lamb(f)
+    us = lamb(list(u))
+    ts = lamb(t)
+    sflat = lamb([sig[i, j] for i in range(dim) for j in range(dim)])
+    zeros_t = len(t)
 
     def rhs(x, y, z):
-        a = fu6(x, y, z)
-        return np.concatenate([a, np.zeros((3,) + np.shape(x))], axis=0)
+        a = fu(x, y, z)
+        return np.concatenate([a, np.zeros((zeros_t,) + np.shape(x))], axis=0)
 
     def s_exact(x, y, z):
-        return np.array([[np.broadcast_to(np.asarray(v, dtype=float), np.shape(x)) for v in row] for row in ss(x, y, z)])
+        v = sflat(x, y, z)
+        return v.reshape((dim, dim) + v.shape[1:])
 
     fun = {"rhs": rhs, "lambda": lambda x, y, z: lam, "mu": lambda x, y, z: mu}
-    exact = {"u": vec(us, 3), "s": s_exact, "t": lambda x, y, z: np.zeros((3,) + np.shape(x))}
+    exact = {"u": us, "s": s_exact, "t": ts, "div_s": fu}
     return fun, exact
 
 

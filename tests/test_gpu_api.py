@@ -88,3 +88,76 @@ def test_driver_flow(case):
     dv_gpu = div_error(d, space, dfun, alpha)
     dv_ref = onorms.div_error(ops, dfun, alpha)
     assert np.allclose(dv_gpu, dv_ref, rtol=1e-10, atol=1e-14), (dv_gpu, dv_ref)
+
+
+def test_elasticity_driver_ksp():
+    """The reference elasticity driver's solver set-up (``mfe_elasticity_example_1.py:238-276``):
+    MINRES + additive field split (stress | displacement and rotation), ``rtol = 0``,
+    ``atol = 1e-12``, ``divtol = 5000``, ``max_it = 20000``. The facade must stop on the
+    absolute test (``||b - A x|| <= 1e-12``, reason CONVERGED_ATOL), a relative test gives
+    CONVERGED_RTOL, and a total iteration cap gives DIVERGED_ITS within the cap."""
+    import warnings
+
+    from nfempy_amd.assembly import Mat, SequentialAssembler
+    from nfempy_amd.ksp import CONVERGED_ATOL, CONVERGED_RTOL, DIVERGED_ITS, IS, KSP
+    from nfempy_amd.weak_forms import LEDualWeakForm, LEDualWeakFormBCDirichlet, LERieszMapWeakForm
+    from oracle.assemble import assemble_elasticity, symmetric_from_upper
+    from tests._cases import elasticity_functions, oracle_elasticity_space, product_elasticity_space
+
+    print("[api] elasticity driver ksp", flush=True)
+    fun, ex = elasticity_functions()
+    space = product_elasticity_space(2)
+    n = space.n_dof
+    A, P = Mat().createAIJ([n, n]), Mat().createAIJ([n, n])
+    A.setType("sbaij")
+    P.setType("sbaij")
+    rg = np.zeros(n)
+    wf, pf = LEDualWeakForm(space), LERieszMapWeakForm(space)
+    wf.functions = pf.functions = fun
+    bc = LEDualWeakFormBCDirichlet(space, fun)
+    bc.functions = ex
+    ne = len(space.discrete_spaces["s"].elements)
+    nbc = len(space.discrete_spaces["s"].bc_elements)
+    asm = SequentialAssembler(space, A, rg)
+    asm.form_to_input_list = {"le": ["form", range(ne), wf, np.zeros(n)],
+                              "bc": ["bc_form", range(nbc), bc, np.zeros(n)]}
+    asm.scatter_forms()
+    pf.scatter(P, None)
+    Aref, _, rgref = assemble_elasticity(oracle_elasticity_space(2), fun, ex)
+    assert np.abs(rg - rgref).max() <= 1e-12 * np.abs(rgref).max()
+    Af = symmetric_from_upper(Aref)
+    bounds = np.add.accumulate([0] + list(space.discrete_spaces_dofs.values()))
+
+    def run(rtol, atol, max_it):
+        ksp = KSP().create()
+        ksp.setOperators(A, P)
+        b = A.createVecLeft()
+        b.array[:] = -rg
+        x = A.createVecRight()
+        ksp.setType("minres")
+        ksp.getPC().setType("fieldsplit")
+        ksp.getPC().setFieldSplitIS(("gen_sigma", IS().createGeneral(np.arange(bounds[0], bounds[1]))),
+                                    ("gen_u", IS().createGeneral(np.arange(bounds[1], bounds[3]))))
+        ksp.getPC().setFieldSplitType("additive")
+        ks, ku = ksp.getPC().getFieldSplitSubKSP()
+        ks.setType("preonly")
+        ks.getPC().setType("cholesky")
+        ks.getPC().setFactorSolverType("mumps")
+        ku.setType("preonly")
+        ku.getPC().setType("icc")
+        ksp.setTolerances(rtol=rtol, atol=atol, divtol=5000, max_it=max_it)
+        ksp.setConvergenceHistory()
+        ksp.setFromOptions()
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore", RuntimeWarning)
+            ksp.solve(b, x)
+        res = np.linalg.norm(-rgref - Af @ x.array)
+        return ksp, res
+
+    ksp, res = run(0.0, 1e-12, 20000)
+    assert ksp.getConvergedReason() == CONVERGED_ATOL and res <= 1.5e-12, (ksp.getConvergedReason(), res)
+    assert ksp.getConvergenceHistory()[-1] <= 1e-12
+    ksp, res = run(1e-8, 0.0, 20000)
+    assert ksp.getConvergedReason() == CONVERGED_RTOL and res <= 1.2e-8 * np.linalg.norm(rgref)
+    ksp, res = run(0.0, 1e-12, 5)
+    assert ksp.getConvergedReason() == DIVERGED_ITS and ksp.getIterationNumber() <= 5

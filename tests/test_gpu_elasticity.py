@@ -88,3 +88,89 @@ def test_elasticity_parity(case, coef):
     assert res <= 1.2e-8, res
     xref = spla.spsolve(Af.tocsc(), -rg)
     assert np.linalg.norm(xs - xref) <= 1e-5 * np.linalg.norm(xref)
+
+
+def test_elasticity_workload_model_at_8():
+    """The elasticity workload's coefficient model (per-element log-normal Lame parameters,
+    sigma 0.5, seed 2024; constant body force; zero Dirichlet displacement) at 8^3 x 6
+    (3,072 tets, 77,184 DoFs): A and P against the oracle's assembly (patterns bit-exact,
+    FP64 1e-12, FP32 1e-5 per row), and the solve to 1e-8 against the committed FP64
+    oracle solution (scipy LU of the oracle-assembled system), per field."""
+    import os
+
+    import torch
+    from nfempy_amd.elasticity import ElasticitySystem
+    from nfempy_amd.fields import elasticity_body_force, lame_lognormal
+    from oracle.assemble import assemble_elasticity
+
+    print("[elasticity] workload model 8^3", flush=True)
+    ops = oracle_elasticity_space(8)
+    E = len(ops.spaces["s"].element_ids)
+    lam, mu = lame_lognormal(E, 2024)
+    fun = {"lambda": lam, "mu": mu, "rhs": elasticity_body_force}
+    uD = lambda x, y, z: np.zeros((3,) + np.shape(x))
+    A, P, rg = assemble_elasticity(ops, fun, {"u": uD})
+    ps = product_elasticity_space(8)
+    sysd = ElasticitySystem(ps, fun)
+    for which, R in (("A", A), ("P", P)):
+        pat = sysd.ops[which]["pattern"]
+        assert np.array_equal(pat.rowptr, R.indptr) and np.array_equal(pat.col, R.indices), which
+        for fp64, tol in ((True, 1e-12), (False, 1e-5)):
+            sysd.assemble(which, fp64=fp64, u_dirichlet=uD)
+            torch.cuda.synchronize()
+            err = _row_err(R, sysd.csr_numpy(which, fp64))
+            print(f"[elasticity] 8^3 {which} fp64={fp64}: {err:.1e}", flush=True)
+            assert err <= tol, (which, fp64, err)
+            if which == "A":
+                r = (sysd.rhs64 if fp64 else sysd.rhs32).double().cpu().numpy()
+                assert np.abs(r - rg).max() <= tol * np.abs(rg).max(), fp64
+    ref = np.load(os.path.join(os.path.dirname(__file__), "golden", "elasticity_c2_n8.npz"))
+    sysd.assemble("A", fp64=True, u_dirichlet=uD)
+    sysd.assemble("A", fp64=False, u_dirichlet=uD)
+    x, info = sysd.solve(rtol=1e-8)
+    assert info["converged"], info
+    xs = x.cpu().numpy()
+    xr = ref["x"]
+    off = 0
+    for name, nd in zip(("s", "u", "t"), ref["n_dofs"].tolist()):
+        e = np.linalg.norm(xs[off:off + nd] - xr[off:off + nd]) / np.linalg.norm(xr[off:off + nd])
+        print(f"[elasticity] 8^3 field {name}: rel error vs oracle solution {e:.1e}", flush=True)
+        assert e <= 1e-6, (name, e)
+        off += nd
+
+
+@pytest.mark.parametrize("case", [(1, None), (3, None), (None, "example_3_3d_l_1.msh")], ids=lambda c: str(c[1] or c[0]))
+def test_face_kernel_matches_element_kernel(case):
+    """The face-owner launch (the default) against the per-element launch with its slot map
+    and staged sum: A and P, FP32 and FP64, with per-element and per-point (callable) Lame
+    parameters; values within a few ulps of each other (different summation order), the
+    residual rows likewise, and two face launches bitwise equal."""
+    import torch
+    from nfempy_amd.elasticity import ElasticitySystem
+
+    n, msh = case
+    print(f"[elasticity faces] {case}", flush=True)
+    ps = product_elasticity_space(n, msh)
+    E = len(ps.discrete_spaces["s"].element_ids)
+    for coef in ("lognormal", "callable"):
+        fun, ex = _coefs("lognormal", E)
+        if coef == "callable":
+            fun = dict(fun)
+            fun["lambda"] = lambda x, y, z: 1.0 + 0.5 * x * y
+            fun["mu"] = lambda x, y, z: 0.7 + 0.2 * z
+        sysd = ElasticitySystem(ps, fun)
+        assert sysd.use_faces
+        for which in ("A", "P"):
+            for fp64, tol in ((True, 1e-13), (False, 2e-6)):
+                vf, rf = (t.clone() for t in sysd.assemble(which, fp64=fp64, u_dirichlet=ex["u"], method="faces"))
+                vf2, rf2 = (t.clone() for t in sysd.assemble(which, fp64=fp64, u_dirichlet=ex["u"], method="faces"))
+                ve, re = (t.clone() for t in sysd.assemble(which, fp64=fp64, u_dirichlet=ex["u"], method="element"))
+                torch.cuda.synchronize()
+                assert torch.equal(vf, vf2) and torch.equal(rf, rf2)
+                scale = float(ve.abs().max())
+                err = float((vf.double() - ve.double()).abs().max()) / scale
+                print(f"[elasticity faces] {coef} {which} fp64={fp64}: {err:.1e}", flush=True)
+                assert err <= tol, (coef, which, fp64, err)
+                if which == "A":
+                    er = float((rf.double() - re.double()).abs().max()) / max(float(re.abs().max()), 1e-300)
+                    assert er <= tol, (coef, fp64, er)

@@ -1,0 +1,142 @@
+"""Host data of the face-owner elasticity kernel (CPU only).
+
+``face_owner_layout`` / ``face_owner_codes`` (incident elements per face, element face
+blocks and orientation words, the 16-bit source code of every stress-row CSR entry) are
+checked by replaying the kernel's arithmetic in numpy: reference-order element matrices
+from the same formulas and table constants the kernel uses, the value filter decided on
+them, the SBAIJ patterns, then every CSR entry summed from its coded sources with the
+orientation signs. The result must be the oracle's assembled A and P (patterns bit-exact,
+values within 1e-12), on a structured cube and on the reference's own gmsh
+tetrahedral mesh.
+"""
+
+import numpy as np
+import pytest
+
+from nfempy_amd.elasticity import NL, elasticity_tables, face_owner_codes, face_owner_layout, _sbaij_mask
+from nfempy_amd.pattern import build_pattern
+from tests._cases import oracle_elasticity_space, product_elasticity_space
+
+
+def reference_order_matrices(ps, lam, mu, mode):
+    """(E, 42, 42) element matrices in reference-basis order with the face kernel's formulas."""
+    s = ps.discrete_spaces["s"]
+    mesh = s.mesh
+    pts, w = ps.quadrature[3]
+    tab = elasticity_tables(s.element, pts, w)
+    nq = len(w)
+    phi = tab[nq:nq + nq * 36].reshape(nq, 12, 3)
+    o = nq + nq * 36
+    div = tab[o:o + nq * 12].reshape(nq, 12)
+    o += nq * 12
+    uref = tab[o:o + 12]
+    Phi = tab[o + 12:o + 48].reshape(12, 3)
+    ddref = tab[o + 48:o + 192].reshape(12, 12)
+    cells = mesh.entities[3].nodes[s.topology.cell_index]
+    X = mesh.points[cells]
+    J = np.transpose(X[:, 1:] - X[:, :1], (0, 2, 1))  # J[c, d] = X[d + 1, c] - X[0, c]
+    det = np.abs(np.linalg.det(J))
+    psi = np.einsum("ecd,qbd->ebcq", J, phi) / det[:, None, None, None]  # (E, 12, 3, q)
+    k1 = w[None, :] * det[:, None] / (2 * mu[:, None])
+    k2 = w[None, :] * det[:, None] / (2 * mu[:, None]) * (lam / (2 * mu + 3 * lam))[:, None]
+    E = len(det)
+    K = np.zeros((E, NL, NL))
+    D = np.einsum("eacq,ebcq,eq->eab", psi, psi, k1)
+    V = np.einsum("eacq,ebdq,eq->eacbd", psi, psi, k2)
+    ss = -V
+    for c in range(3):
+        ss[:, :, c, :, c] += D
+    if mode == 1:
+        for c in range(3):
+            ss[:, :, c, :, c] += ddref[None] / det[:, None, None]
+    K[:, :36, :36] = ss.reshape(E, 36, 36)
+    if mode == 0:
+        A = np.einsum("ekx,bx->ebk", J, Phi)  # A[e, b, k] = sum_x J_kx Phi_bx
+        for b in range(12):
+            for d in range(3):
+                K[:, 3 * b + d, 36 + d] = uref[b]
+                for ct in range(3):
+                    if ct == d:
+                        continue
+                    k = 3 - d - ct
+                    even = (ct, d) in ((0, 1), (1, 2), (2, 0))
+                    K[:, 3 * b + d, 39 + ct] = A[:, b, k] if even else -A[:, b, k]
+        K[:, 36:, :36] = np.transpose(K[:, :36, 36:], (0, 2, 1))
+    else:
+        for r in range(36, 42):
+            K[:, r, r] = det * w.sum()
+    return K, det, w
+
+
+def replay(n, lay, codes, pattern, K, ref_sign):
+    """CSR values of the stress rows summed from the coded sources (the kernel's write phase)."""
+    rowptr = pattern.rowptr.astype(np.int64)
+    n_s = lay["n_s"]
+    vals = np.zeros(pattern.nnz)
+    E = K.shape[0]
+    sgn = ref_sign  # (E, 42) per reference DoF
+    # local row of slot (j, c) in incidence i: the reference basis a with face slot j
+    orient = lay["el_orient"]
+    slot = ((orient[:, None] >> (np.uint64(3) * np.arange(12, dtype=np.uint64))[None, :]) & np.uint64(3)).astype(int)
+    for p in range(lay["n_faces"]):
+        for k in range(rowptr[9 * p], rowptr[9 * p + 9]):
+            code = int(codes[k])
+            r, srcs = code & 15, ((code >> 4) & 63, code >> 10)
+            j, c = divmod(r, 3)
+            v = 0.0
+            for i in range(2):
+                if srcs[i] == 63:
+                    continue
+                e = lay["face_inc"][p, i]
+                lf = (lay["face_lf"][p] >> (2 * i)) & 3
+                a = [b for b in range(3 * lf, 3 * lf + 3) if slot[e, b] == j][0]
+                v += sgn[e, 3 * a + c] * sgn[e, srcs[i]] * K[e, 3 * a + c, srcs[i]]
+            vals[k] = v
+    for e in range(E):
+        for rr in list(range(n_s + 3 * e, n_s + 3 * e + 3)) + list(range(n_s + lay["n_u"] + 3 * e,
+                                                                         n_s + lay["n_u"] + 3 * e + 3)):
+            vals[rowptr[rr]] = K[e, 36 + (rr - n_s) % 3 + (3 if rr >= n_s + lay["n_u"] else 0),
+                                 36 + (rr - n_s) % 3 + (3 if rr >= n_s + lay["n_u"] else 0)]
+    return vals
+
+
+@pytest.mark.parametrize("case", [(2, None), (None, "example_3_3d_l_0.msh")], ids=["cube2", "gmsh"])
+def test_face_owner_replay_matches_oracle(case):
+    import scipy.sparse as sps
+
+    from oracle.assemble import assemble_elasticity
+
+    n_div, msh = case
+    ps = product_elasticity_space(n_div, msh)
+    ops = oracle_elasticity_space(n_div, msh)
+    E = len(ps.discrete_spaces["s"].element_ids)
+    rng = np.random.default_rng(5)
+    lam = np.exp(0.5 * rng.standard_normal(E))
+    mu = np.exp(0.5 * rng.standard_normal(E))
+    fun = {"lambda": lam, "mu": mu, "rhs": lambda x, y, z: np.zeros((6,) + np.shape(x))}
+    A, P, _ = assemble_elasticity(ops, fun)
+    ref_dest, ref_sign = ps.reference_destinations()
+    n = ps.n_dof
+    lay = face_owner_layout(n, ref_dest, ref_sign)
+    assert lay["n_s"] == ps.discrete_spaces_dofs["s"]
+    for mode, R in ((0, A), (1, P)):
+        K, det, w = reference_order_matrices(ps, lam, mu, mode)
+        keep = _sbaij_mask(K, ref_dest, ref_sign, upper=False)
+        pat = build_pattern(n, ref_dest, ref_sign, keep=keep, zero_diag_dest=ref_dest if mode == 0 else None,
+                            upper=True)
+        assert np.array_equal(pat.rowptr, R.indptr) and np.array_equal(pat.col, R.indices), mode
+        codes = face_owner_codes(n, lay, ref_dest, keep, pat, zero_diag=(mode == 0))
+        vals = replay(n, lay, codes, pat, K, ref_sign)
+        G = sps.csr_matrix((vals, pat.col, pat.rowptr), shape=(n, n))
+        D = abs(G - R).tocsr()
+        rowmax = np.maximum(abs(R).max(axis=1).toarray().ravel(), 1e-300)
+        assert (D.max(axis=1).toarray().ravel() / rowmax).max() <= 1e-12, mode
+
+
+def test_face_owner_layout_rejects_other_layouts():
+    ps = product_elasticity_space(1)
+    ref_dest, ref_sign = ps.reference_destinations()
+    bad = ref_dest.copy()
+    bad[:, [0, 1]] = bad[:, [1, 0]]  # stress components no longer consecutive
+    with pytest.raises(NotImplementedError):
+        face_owner_layout(ps.n_dof, bad, ref_sign)

@@ -199,3 +199,25 @@ def test_span_tiles_cover_rows_within_staging(tile_rows):
     short = (counts < tile_rows) & (ends < sysd.n)
     assert np.all(rp[ends[short] + 1] - rp[starts[short]] > cap)
     assert np.diff(rp).max() > RT2_SPAN_PER_ROW  # the RT2 face rows are longer than the mean
+
+
+def test_packed_offsets_required_and_validated():
+    """For nl <= 5 the row kernel reads each row's in-tile CSR offset from its position
+    words: building the words without rowptr is refused, and words whose offsets disagree
+    with rowptr fail the launch validation."""
+    import torch
+
+    from nfempy_amd.device import DarcySystem, row_pos_device
+
+    sysd = DarcySystem(product_darcy_space(3, n=3, k=1), device="cpu")
+    assert sysd.nl <= 5
+    with pytest.raises(ValueError):
+        row_pos_device(sysd.row_pos_np, sysd.nl, sysd.row_sign_mask_np)
+    sysd._validate(sysd.u_sp.mesh)
+    good = sysd.row_pos
+    w = good.numpy().view(np.uint32).copy()
+    w[2 * 5] ^= np.uint32(1 << 25)  # flip one offset bit of row 5
+    sysd.row_pos = torch.from_numpy(w.view(np.uint8).copy())
+    with pytest.raises(ValueError):
+        sysd._validate(sysd.u_sp.mesh)
+    sysd.row_pos = good
