@@ -14,7 +14,8 @@ def _random_csr(n, max_len, seed):
 
     rng = np.random.default_rng(seed)
     lens = rng.integers(0, max_len + 1, n)
-    lens[: min(n, 5)] = 0  # empty rows
+    lens[: min(n - 1, 5)] = 0  # empty rows
+    lens[-1] = max(lens[-1], 1)  # at least one entry (col / val pointers not null)
     rows = np.repeat(np.arange(n), lens)
     cols = np.concatenate([np.sort(rng.choice(n, size=l, replace=False)) for l in lens]) if lens.sum() else \
         np.zeros(0, np.int64)
