@@ -216,31 +216,33 @@ int nfx_assemble_elasticity(int mode, int n_elem, const int32_t* d_cells,
                             const int32_t* d_rhs_row, int fp64, void* d_vals,
                             void* d_stage, void* d_rhs, void* stream);
 
-/* The same operators (A: mode 0 with the residual, P: mode 1), face-owner form:
- * replaces the element contributions of LEDualWeakForm.evaluate_form_vectorized and
- * LERieszMapWeakForm.evaluate_form_vectorized (src/weak_forms/le_dual_weak_form.py:354-510,
- * src/weak_forms/le_riesz_map_weak_form.py:258-360) together with their SBAIJ scatter
- * (examples/cauchy_elasticity/mfe_elasticity_example_1.py:128-192). One wavefront owns the
- * 9 CSR rows of one face (DoFs 9 p .. 9 p + 8: 3 BDM1 DoFs x 3 stress rows), sums the rows
- * of its (at most two) incident elements in element order and stores the contiguous CSR
- * slice; cell rows (u: n_s + 3 e + c, t: n_s + n_u + 3 e + c) hold only their diagonal
- * (0 in A, the mass in P). n_s = 9 n_faces, n_u = 3 n_elem.
+/* The same operators (A: mode 0 with the residual, P: mode 1), facet-owner form, for
+ * tetrahedra (dim 3) and triangles (dim 2): replaces the element contributions of
+ * LEDualWeakForm.evaluate_form_vectorized and LERieszMapWeakForm.evaluate_form_vectorized
+ * (src/weak_forms/le_dual_weak_form.py:354-510, src/weak_forms/le_riesz_map_weak_form.py:258-360)
+ * together with their SBAIJ scatter (examples/cauchy_elasticity/mfe_elasticity_example_1.py:128-192).
+ * One wavefront owns the CSR rows of one facet (a face: 3 BDM1 DoFs x 3 stress rows, rows
+ * 9 p .. 9 p + 8; an edge: 2 x 2, rows 4 p .. 4 p + 3), sums the rows of its (at most two)
+ * incident elements in element order and stores the contiguous CSR slice; cell rows
+ * (u: n_s + NC e + c, t: n_s + n_u + NT e + c; NC = dim, NT = 3 or 1) hold only their diagonal
+ * (0 in A, the mass in P). n_s = (9 | 4) n_faces, n_u = NC n_elem.
  *   d_face_inc   (n_faces, 2) incident elements in element order (-1 none)
- *   d_face_lf    (n_faces) local face index of each incidence (bits 0-1, 2-3)
- *   d_el_face    (n_elem, 4) DoF block of each local face
- *   d_el_orient  (n_elem) uint64: bits 3b, 3b+1 face slot of reference basis b, 3b+2 its sign
- *   d_tables     nfx_assemble_elasticity's table followed by Uref (12), Phi (12 x 3),
- *                DDref (12 x 12): sum_q w divref_b, sum_q w phihat_be, sum_q w divref_a divref_b
+ *   d_face_lf    (n_faces) local facet index of each incidence (bits 0-1, 2-3)
+ *   d_el_face    (n_elem, dim + 1) DoF block of each local facet
+ *   d_el_orient  (n_elem) uint64: bits 2b, 2b+1 facet slot of reference basis b, 32+b its sign
+ *   d_geom       (n_elem, dim dim + 1): J / |det J| row-major, then |det J| (J[c][d] = X_{d+1,c} - X_{0,c})
+ *   d_tables     w[NQ] | phi[NQ][NB][dim] | divref[NQ][NB] | Uref[NB] | Phi[NB][dim] |
+ *                DDref[NB][NB]: sum_q w divref_b, sum_q w phihat_be, sum_q w divref_a divref_b
  *   d_rowptr     CSR row pointer of the operator (stress rows first)
- *   d_codes      uint16 per CSR entry of the stress rows: bits 0-3 row in the face (3 j + c),
- *                4-9 / 10-15 element-local reference column in the first / second incident
- *                element (63: no contribution)
- * Coefficient and right-hand-side arguments as nfx_assemble_elasticity; d_rhs gets every
- * row (0 on stress rows) in mode 0. */
-int nfx_assemble_elasticity_faces(int mode, int n_faces, int n_elem, int n_s, int n_u,
+ *   d_codes      uint16 per CSR entry of the stress rows: bits 0-3 row in the facet block
+ *                (j NC + c), 4-9 / 10-15 element-local reference column in the first / second
+ *                incident element (63: no contribution)
+ * Coefficients as nfx_assemble_elasticity (f has NC + NT components); d_rhs gets every row
+ * (0 on stress rows) in mode 0. */
+int nfx_assemble_elasticity_faces(int dim, int mode, int n_faces, int n_elem,
                                   const int32_t* d_face_inc, const int32_t* d_face_lf,
                                   const int32_t* d_el_face, const uint64_t* d_el_orient,
-                                  const int32_t* d_cells, const void* d_coords, const void* d_tables,
+                                  const void* d_geom, const void* d_tables,
                                   const void* d_lam, const void* d_mu, int coef_stride_e,
                                   int coef_stride_q, const void* d_f, int f_stride_e, int f_stride_q,
                                   int f_stride_c, const int32_t* d_rowptr, const uint16_t* d_codes,

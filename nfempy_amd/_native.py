@@ -65,8 +65,8 @@ SIGNATURES = {
     "nfx_assemble_elasticity": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
                                         c_int, c_void_p, c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_int,
                                         c_void_p, c_void_p, c_void_p, c_void_p]),
-    "nfx_assemble_elasticity_faces": (c_int, [c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
-                                              c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+    "nfx_assemble_elasticity_faces": (c_int, [c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
+                                              c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
                                               c_int, c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_int,
                                               c_void_p, c_void_p, c_void_p]),
     "nfx_assemble_elasticity_bc": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,

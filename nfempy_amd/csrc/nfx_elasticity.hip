@@ -190,108 +190,291 @@ static int launch_elasticity(int n, const int32_t* cells, const void* coords, co
 }
 
 // ---------------------------------------------------------------------------
-// Face-owner assembly: one wavefront owns the 9 CSR rows of one face (its 3 BDM1 DoFs x 3
-// stress rows, consecutive in the DoF numbering and therefore one contiguous CSR slice),
-// evaluates the rows of those DoFs in the element matrices of the face's (at most two)
-// incident tetrahedra, sums them in element order and writes the slice with coalesced
-// stores. No slot codes per element entry, no staging, no atomics. Each CSR entry of a
-// stress row carries a 16-bit source code: bits 0..3 the row inside the face (3 j + c),
-// bits 4..9 and 10..15 the element-local reference column (3 b + d, 36 + d for u_d,
-// 39 + d for t_d) of the entry in the first / second incident element, 63 = no
-// contribution (entries dropped by the value filter, explicit zero diagonals).
+// Face-owner assembly: one wavefront owns the CSR rows of one facet (a face of a tetrahedron
+// or an edge of a triangle): its BDM1 DoFs x stress rows, consecutive in the DoF numbering
+// and therefore one contiguous CSR slice. It evaluates the rows of those DoFs in the element
+// matrices of the facet's (at most two) incident elements, sums them in element order and
+// writes the slice with coalesced stores. No slot codes per element entry, no staging, no
+// atomics. Each CSR entry of a stress row carries a 16-bit source code: bits 0..3 the row
+// inside the facet block (j * NC + c), bits 4..9 and 10..15 the element-local reference
+// column (b * NC + d for stress, NSS + d for u_d, NSS + NC + d for t_d) of the entry in the
+// first / second incident element, 63 = no contribution (entries dropped by the value
+// filter, explicit zero diagonals).
 //
 // Per incident element the wave computes only the columns the upper triangle needs: the
-// blocks of the element's faces whose DoF block index is >= the owning face's, plus the
-// u / t columns. The (s, s) contraction V = sum_q k2_q psi_ac(q) psi_bd(q) (9 rows x 9 nb
-// columns, K = 8 quadrature points) runs on v_mfma_f32_16x16x4_f32 in FP32; FP64 builds
+// blocks of the element's facets whose DoF block index is >= the owning facet's, plus the
+// u / t columns. The (s, s) contraction V = sum_q k2_q psi_ac(q) psi_bd(q) (rows x needed
+// columns, K = quadrature points) runs on v_mfma_f32_16x16x4_f32 in FP32; FP64 builds
 // evaluate the same tiles on the vector ALU. The diagonal-in-component part
-// D_ab = sum_q k1_q psi_a . psi_b (and the Riesz map's div-div term) is a 3 x 12 block per
+// D_ab = sum_q k1_q psi_a . psi_b (and the Riesz map's div-div term) is a small block per
 // element on the vector ALU.
 //
-// Cell rows (u_c, t_c of each tetrahedron) hold one entry, their diagonal: 0 in A (the
-// explicit zero of the SBAIJ scatter), the mass |K| in P; their residual rows take
-// -sum_q w detJ f. One thread per cell, in blocks launched after the face blocks.
+// Cell rows (u_c, t_c of each element) hold one entry, their diagonal: 0 in A (the explicit
+// zero of the SBAIJ scatter), the mass |K| in P; their residual rows take -sum_q w detJ f.
+// One thread per cell, in blocks launched after the facet blocks.
 //
-// Table layout (R): w[8] | phi[8][12][3] | div[8][12] | Uref[12] | Phi[12][3] | DDref[12][12]
-// with Uref_b = sum_q w_q divref_b(q), Phi_be = sum_q w_q phihat_be(q),
+// Table layout (R): w[NQ] | phi[NQ][NB][DIM] | div[NQ][NB] | Uref[NB] | Phi[NB][DIM] |
+// DDref[NB][NB] with Uref_b = sum_q w_q divref_b(q), Phi_be = sum_q w_q phihat_be(q),
 // DDref_ab = sum_q w_q divref_a(q) divref_b(q).
-constexpr int kFW = 4;  // faces (waves) per block of the face kernel
+constexpr int kFW = 4;  // facets (waves) per block of the face kernel
 constexpr int kCellBlock = 256;
 constexpr int kNoSrc = 63;
 
+// BDM1 - P0 - P0 sizes: NB basis functions, NQ quadrature points (degree 3), NC stress rows
+// (= displacement components), NT rotation components, NBF basis functions per facet, NFC
+// facets per element, NL local DoFs.
+template <int DIM>
+struct ElSizes;
+template <>
+struct ElSizes<3> {
+  static constexpr int NB = 12, NQ = 8, NC = 3, NT = 3, NBF = 3, NFC = 4;
+};
+template <>
+struct ElSizes<2> {
+  static constexpr int NB = 6, NQ = 4, NC = 2, NT = 1, NBF = 2, NFC = 3;
+};
+
 typedef float f4_t __attribute__((ext_vector_type(4)));
 
-template <typename R>
+template <int DIM, typename R>
 struct FaceLds {
-  R psi[2][12][3][kEQ];      // psi_{b d}(q)
-  R kpsi[2][2][3][3][kEQ];   // [i][0][j][c][q] = k2_q psi_{a_j c}(q); [i][1] k1_q psi_{a_j c}(q)
-  R D[2][3][12];             // D_{a_j b} (+ div-div, Riesz map)
-  R K[2][9][kEL];            // signed element rows of the face: s(a) s(b) K_e[(a, c), col]
+  using S = ElSizes<DIM>;
+  static constexpr int ROWS = S::NBF * S::NC, NL = S::NB * S::NC + S::NC + S::NT;
+  // psi, kpsi and D serve one incidence at a time (the second reuses them after the first's
+  // rows are in K)
+  R psi[S::NB][DIM][S::NQ];       // s(b) psi_{b d}(q)
+  R kpsi[2][S::NBF][DIM][S::NQ];  // [0][j][c][q] = k2_q s(a_j) psi_{a_j c}(q); [1] with k1_q
+  R D[S::NBF][S::NB];             // s(a) s(b) D_{a_j b} (+ div-div, Riesz map)
+  R K[2][ROWS][NL];               // signed element rows of the facet: s(a) s(b) K_e[(a, c), col]
 };
 
-template <typename R>
-__device__ __forceinline__ void face_geometry(const int32_t* __restrict__ cells, const R* __restrict__ coords,
-                                              int e, R (&J)[3][3], R& detJ) {
-  R X[4][3];
+// the element-independent table, staged in LDS once per block
+template <int DIM>
+struct TabSizes {
+  using S = ElSizes<DIM>;
+  static constexpr int W = 0, PHI = S::NQ, DIV = PHI + S::NQ * S::NB * DIM, U = DIV + S::NQ * S::NB,
+                       PH = U + S::NB, DD = PH + S::NB * DIM, N = DD + S::NB * S::NB;
+};
+
+// Per-element geometry record (computed once per mesh): J / |det J| row-major, then |det J|.
+template <int DIM, typename R>
+__device__ __forceinline__ void face_geometry(const R* __restrict__ geom, int e, R (&Jn)[DIM][DIM], R& detJ) {
+  const R* gp = geom + (int64_t)e * (DIM * DIM + 1);
 #pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    const int v = cells[e * 4 + a];
+  for (int c = 0; c < DIM; ++c)
 #pragma unroll
-    for (int c = 0; c < 3; ++c) X[a][c] = coords[3 * v + c];
-  }
-#pragma unroll
-  for (int c = 0; c < 3; ++c)
-#pragma unroll
-    for (int d = 0; d < 3; ++d) J[c][d] = X[d + 1][c] - X[0][c];
-  R dt = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
-         J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
-  detJ = dt < R(0) ? -dt : dt;
+    for (int d = 0; d < DIM; ++d) Jn[c][d] = gp[c * DIM + d];
+  detJ = gp[DIM * DIM];
 }
 
-// Per incident element of a face: Jacobian and |det J|, the element's 4 face DoF blocks,
-// orientation word (bits 3b, 3b + 1: face slot of reference basis b; bit 3b + 2: its sign),
-// the reference basis functions at the owning face's slots j (4 bits each) and the list of
-// the element's local faces whose block is >= the owning face's (2 bits each, nb of them).
-template <typename R>
+// Per incident element of a facet: element, Jacobian and |det J|, the facet slots (2 bits per
+// reference basis function) and orientation signs (1 bit per function) of the element, the
+// reference basis functions at the owning facet's slots j (4 bits each) and the list of the
+// element's local facets whose block is >= the owning facet's (2 bits each, nb of them).
+template <int DIM, typename R>
 struct FaceInc {
-  R J[3][3];
-  R detJ, invdet;
-  uint64_t orient;
+  int e;
+  R J[DIM][DIM];  // J / |det J|
+  R detJ;
+  uint32_t slots, signs;
   int ajpack, blpack, nb;
+  R lam_q, mu_q;  // coefficients at quadrature point q = lane mod NQ
 };
 
-template <typename R>
-__device__ __forceinline__ R face_psi(const FaceInc<R>& g, int d, R p0, R p1, R p2) {
-  return (g.J[d][0] * p0 + g.J[d][1] * p1 + g.J[d][2] * p2) * g.invdet;
+template <int DIM, typename R>
+__device__ __forceinline__ R basis_sign(const FaceInc<DIM, R>& g, int b) {
+  return ((g.signs >> b) & 1) ? R(-1) : R(1);
 }
 
-template <typename R>
-__device__ __forceinline__ R basis_sign(const FaceInc<R>& g, int b) {
-  return ((g.orient >> (3 * b + 2)) & 1) ? R(-1) : R(1);
-}
-
-// rows of incidence i (uniform): (s, s) blocks of the needed faces into L.K[i]
-template <typename R>
-__device__ __forceinline__ void face_ss_rows(FaceLds<R>& L, const FaceInc<R>& g, int i, int l) {
-  const int nc = 9 * g.nb;
-  const int ntile = (nc + 15) >> 4;
-  const int lr = l & 15, lq = l >> 4;
+// Every global load of an incidence issued together (one latency wait per facet instead of a
+// chain per phase): geometry record, orientation word, facet blocks, the lane's coefficients.
+template <int DIM, typename R>
+__device__ __forceinline__ FaceInc<DIM, R> face_inc_data(const R* __restrict__ geom,
+                                                          const uint64_t* __restrict__ el_orient,
+                                                          const int32_t* __restrict__ el_face,
+                                                          const R* __restrict__ lam, const R* __restrict__ mu,
+                                                          int cs_e, int cs_q, int e, int lf, int p, int l) {
+  using S = ElSizes<DIM>;
+  FaceInc<DIM, R> g;
+  g.e = e;
+  const int q = l & (S::NQ - 1);
+  g.mu_q = mu[(int64_t)e * cs_e + q * cs_q];
+  g.lam_q = lam[(int64_t)e * cs_e + q * cs_q];
+  const uint64_t o = el_orient[e];
+  int fb[S::NFC];
 #pragma unroll
-  for (int t = 0; t < 3; ++t) {
+  for (int k = 0; k < S::NFC; ++k) fb[k] = el_face[e * S::NFC + k];
+  face_geometry<DIM, R>(geom, e, g.J, g.detJ);
+  g.slots = (uint32_t)o;
+  g.signs = (uint32_t)(o >> 32);
+  g.ajpack = 0;
+#pragma unroll
+  for (int k = 0; k < S::NBF; ++k) {
+    const int b = S::NBF * lf + k;
+    g.ajpack |= b << (4 * (int)((g.slots >> (2 * b)) & 3));
+  }
+  g.nb = 0;
+  g.blpack = 0;
+#pragma unroll
+  for (int k = 0; k < S::NFC; ++k)
+    if (fb[k] >= p) {
+      g.blpack |= k << (2 * g.nb);
+      ++g.nb;
+    }
+  return g;
+}
+
+// All element rows of the owning facet in incidence I (uniform): psi, coefficient-weighted
+// rows, the D block, the u / t columns and the (s, s) blocks of the needed facets into L.K[I].
+template <int DIM, int MODE, int I, typename R>
+__device__ __forceinline__ void face_incidence(FaceLds<DIM, R>& L, const R* __restrict__ T,
+                                               const FaceInc<DIM, R>& g, int l) {
+  using S = ElSizes<DIM>;
+  using TS = TabSizes<DIM>;
+  constexpr int NB = S::NB, NQ = S::NQ, NC = S::NC, NT = S::NT, NBF = S::NBF, NFC = S::NFC;
+  constexpr int ROWS = FaceLds<DIM, R>::ROWS, NSS = NB * NC;
+  constexpr int LQ = NQ == 8 ? 3 : 2;  // log2 NQ
+  __builtin_amdgcn_wave_barrier();  // a previous incidence's reads of psi / kpsi / D are done
+  // psi_{b d}(q) = (J phihat_b(q))_d / detJ, lane <-> (b, q)
+#pragma unroll
+  for (int pass = 0; pass < (NB * NQ + 63) / 64; ++pass) {
+    const int idx = l + 64 * pass;
+    if (idx < NB * NQ) {
+      const int b = idx >> LQ, q = idx & (NQ - 1);
+      R ph[DIM];
+#pragma unroll
+      for (int x = 0; x < DIM; ++x) ph[x] = T[TS::PHI + (q * NB + b) * DIM + x];
+      // the orientation sign of b folded in: every product below is then signed
+      const R sc = basis_sign(g, b);
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        R v = 0;
+#pragma unroll
+        for (int x = 0; x < DIM; ++x) v += g.J[d][x] * ph[x];
+        L.psi[b][d][q] = v * sc;
+      }
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+  // k2_q psi_{a_j c}(q) and k1_q psi_{a_j c}(q), lane <-> (j, q)
+  if (l < NBF * NQ) {
+    const int j = l >> LQ, q = l & (NQ - 1);
+    const int a = (g.ajpack >> (4 * j)) & 15;
+    const R m = g.mu_q, lm = g.lam_q;
+    const R wd = T[TS::W + q] * g.detJ;
+    const R k1 = wd / (R(2) * m);
+    const R k2 = wd * (R(1) / (R(2) * m)) * (lm / (R(2) * m + R(DIM) * lm));
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+      const R ps = L.psi[a][c][q];
+      L.kpsi[0][j][c][q] = k2 * ps;
+      L.kpsi[1][j][c][q] = k1 * ps;
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+  const int lr = l & 15, lq = l >> 4;
+  // D_{a_j b} = sum_{c, q} k1_q psi_{a_j c}(q) psi_{b c}(q) (+ DDref / detJ for the Riesz map):
+  // an NBF x NB block, K = NC NQ
+  {
+    R dv[4];
+    if constexpr (sizeof(R) == 4) {
+      f4_t acc = f4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int s = 0; s < NC * NQ / 4; ++s) {
+        const int k = 4 * s + lq, c = k >> LQ, q = k & (NQ - 1);
+        const float av = lr < NBF ? L.kpsi[1][lr < NBF ? lr : 0][c][q] : 0.f;
+        const float bv = lr < NB ? L.psi[lr < NB ? lr : 0][c][q] : 0.f;
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) dv[r] = acc[r];
+    } else {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        R sacc = 0;
+        if (lq == 0 && r < NBF && lr < NB)
+#pragma unroll
+          for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) sacc += L.kpsi[1][r][c][q] * L.psi[lr][c][q];
+        dv[r] = sacc;
+      }
+    }
+    if (lq == 0 && lr < NB) {
+#pragma unroll
+      for (int r = 0; r < NBF; ++r) {
+        R v = dv[r];
+        if (MODE == 1) {
+          const int a = (g.ajpack >> (4 * r)) & 15;
+          v += basis_sign(g, a) * basis_sign(g, lr) * T[TS::DD + a * NB + lr] / g.detJ;
+        }
+        L.D[r][lr] = v;
+      }
+    }
+  }
+  // u / t columns of the facet rows, lane <-> (row m, column)
+  if (MODE == 0) {
+    constexpr int NUT = NC + NT;
+    if (l < ROWS * NUT) {
+      const int m = l / NUT, col = l - NUT * m;
+      const int j = m / NC, c = m - NC * j;
+      const int a = (g.ajpack >> (4 * j)) & 15;
+      R v = 0;
+      if (col < NC) {
+        v = col == c ? T[TS::U + a] : R(0);
+      } else {
+        // rotation component; in 2-D the single rotation is the one about z
+        const int ct = DIM == 3 ? col - NC : 2;
+        if (ct != c) {
+          const int k = 3 - c - ct;  // < DIM
+          // A_{a k} = sum_x J_kx Phi_ax for every row k, then a select (no runtime index into J)
+          R Ar[DIM];
+#pragma unroll
+          for (int y = 0; y < DIM; ++y) {
+            Ar[y] = 0;
+#pragma unroll
+            for (int x = 0; x < DIM; ++x) Ar[y] += g.J[y][x] * T[TS::PH + a * DIM + x];
+            Ar[y] *= g.detJ;  // J = (J / |det J|) |det J|
+          }
+          R Aak = Ar[0];
+#pragma unroll
+          for (int y = 1; y < DIM; ++y) Aak = k == y ? Ar[y] : Aak;
+          const bool even = (ct == 0 && c == 1) || (ct == 1 && c == 2) || (ct == 2 && c == 0);
+          v = even ? Aak : -Aak;
+        }
+      }
+      L.K[I][m][NSS + col] = basis_sign(g, a) * v;
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+  // (s, s) blocks: V = sum_q k2_q psi_ac psi_bd on MFMA (rows m = j NC + c, columns of the
+  // needed facet blocks), then K = delta_cd D - V with the orientation signs
+  int rj[4], rc[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int m = 4 * lq + r;
+    rj[r] = m / NC;
+    rc[r] = m - NC * rj[r];
+  }
+  const int nc = ROWS * g.nb;
+  const int ntile = (nc + 15) >> 4;
+  const int am = lr < ROWS ? lr : 0;
+#pragma unroll
+  for (int t = 0; t < (ROWS * NFC + 15) / 16; ++t) {
     if (t >= ntile) break;
     const int n = 16 * t + lr;
     const bool cn = n < nc;
-    const int bi = cn ? n / 9 : 0, w = n - 9 * bi;
+    const int bi = cn ? n / ROWS : 0, w = n - ROWS * bi;
     const int lf = (g.blpack >> (2 * bi)) & 3;
-    const int b = 3 * lf + w / 3, d = w % 3;
+    const int b = NBF * lf + w / NC, d = w - NC * (w / NC);
     R v[4];
     if constexpr (sizeof(R) == 4) {
       f4_t acc = f4_t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int s = 0; s < 2; ++s) {
+      for (int s = 0; s < NQ / 4; ++s) {
         const int q = 4 * s + lq;
-        const float av = lr < 9 ? L.kpsi[i][0][lr / 3][lr % 3][q] : 0.f;
-        const float bv = cn ? L.psi[i][b][d][q] : 0.f;
+        const float av = lr < ROWS ? L.kpsi[0][am / NC][am % NC][q] : 0.f;
+        const float bv = cn ? L.psi[b][d][q] : 0.f;
         acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
       }
 #pragma unroll
@@ -299,193 +482,98 @@ __device__ __forceinline__ void face_ss_rows(FaceLds<R>& L, const FaceInc<R>& g,
     } else {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int m = 4 * lq + r;
-        R s = 0;
-        if (m < 9 && cn)
+        R sacc = 0;
+        if (4 * lq + r < ROWS && cn)
 #pragma unroll
-          for (int q = 0; q < kEQ; ++q) s += L.kpsi[i][0][m / 3][m % 3][q] * L.psi[i][b][d][q];
-        v[r] = s;
+          for (int q = 0; q < NQ; ++q) sacc += L.kpsi[0][rj[r]][rc[r]][q] * L.psi[b][d][q];
+        v[r] = sacc;
       }
     }
     if (cn) {
-      const R sb = basis_sign(g, b);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int m = 4 * lq + r;
-        if (m < 9) {
-          const int j = m / 3, c = m % 3, a = (g.ajpack >> (4 * j)) & 15;
-          const R kv = (c == d ? L.D[i][j][b] : R(0)) - v[r];
-          L.K[i][m][9 * lf + w] = basis_sign(g, a) * sb * kv;
-        }
+        if (m < ROWS) L.K[I][m][ROWS * lf + w] = (rc[r] == d ? L.D[rj[r]][b] : R(0)) - v[r];
       }
     }
   }
 }
 
-template <int MODE, typename R>
+template <int DIM, int MODE, typename R>
 __global__ __launch_bounds__(64 * kFW) void k_elasticity_faces(
-    int n_faces, int n_elem, int n_s, int n_u, const int32_t* __restrict__ face_inc,
-    const int32_t* __restrict__ face_lf, const int32_t* __restrict__ el_face,
-    const uint64_t* __restrict__ el_orient, const int32_t* __restrict__ cells, const R* __restrict__ coords,
-    const R* __restrict__ tab, const R* __restrict__ lam, const R* __restrict__ mu, int cs_e, int cs_q,
-    const R* __restrict__ f, int fs_e, int fs_q, int fs_c, const int32_t* __restrict__ rowptr,
+    int n_faces, int n_elem, const int32_t* __restrict__ face_inc, const int32_t* __restrict__ face_lf,
+    const int32_t* __restrict__ el_face, const uint64_t* __restrict__ el_orient, const R* __restrict__ geom,
+    const R* __restrict__ tab, const R* __restrict__ lam, const R* __restrict__ mu,
+    int cs_e, int cs_q, const R* __restrict__ f, int fs_e, int fs_q, int fs_c, const int32_t* __restrict__ rowptr,
     const uint16_t* __restrict__ codes, R* __restrict__ vals, R* __restrict__ rhs) {
-  const R* tw = tab;
-  const R* tphi = tab + kEQ;
-  const R* tdiv = tphi + kEQ * kEB * 3;
-  const R* tU = tdiv + kEQ * kEB;
-  const R* tPhi = tU + kEB;
-  const R* tDD = tPhi + kEB * 3;
+  using S = ElSizes<DIM>;
+  using TS = TabSizes<DIM>;
+  constexpr int NQ = S::NQ, NC = S::NC, NT = S::NT, NBF = S::NBF, NFC = S::NFC;
+  constexpr int ROWS = FaceLds<DIM, R>::ROWS;
+  const int n_s = ROWS * n_faces, n_u = NC * n_elem;
   const int n_face_blocks = (n_faces + kFW - 1) / kFW;
   if ((int)blockIdx.x >= n_face_blocks) {
-    // cell rows: u_c = n_s + 3 e + c, t_c = n_s + n_u + 3 e + c
+    // cell rows: u_c = n_s + NC e + c, t_c = n_s + n_u + NT e + c
     const int e = ((int)blockIdx.x - n_face_blocks) * kCellBlock + threadIdx.x;
     if (e >= n_elem) return;
-    R J[3][3], detJ;
-    face_geometry<R>(cells, coords, e, J, detJ);
+    R J[DIM][DIM], detJ;
+    face_geometry<DIM, R>(geom, e, J, detJ);
     R mass = 0;
 #pragma unroll
-    for (int q = 0; q < kEQ; ++q) mass += tw[q] * detJ;
+    for (int q = 0; q < NQ; ++q) mass += tab[TS::W + q] * detJ;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const int ru = n_s + 3 * e + c, rt = n_s + n_u + 3 * e + c;
-      vals[rowptr[ru]] = MODE == 1 ? mass : R(0);
-      vals[rowptr[rt]] = MODE == 1 ? mass : R(0);
+    for (int c = 0; c < NC + NT; ++c) {
+      const int row = c < NC ? n_s + NC * e + c : n_s + n_u + NT * e + (c - NC);
+      vals[rowptr[row]] = MODE == 1 ? mass : R(0);
       if (MODE == 0) {
-        R su = 0, st = 0;
+        R sr = 0;
 #pragma unroll
-        for (int q = 0; q < kEQ; ++q) {
-          const R wd = tw[q] * detJ;
-          su -= wd * f[(int64_t)e * fs_e + q * fs_q + c * fs_c];
-          st -= wd * f[(int64_t)e * fs_e + q * fs_q + (3 + c) * fs_c];
-        }
-        rhs[ru] = su;
-        rhs[rt] = st;
+        for (int q = 0; q < NQ; ++q) sr -= tab[TS::W + q] * detJ * f[(int64_t)e * fs_e + q * fs_q + c * fs_c];
+        rhs[row] = sr;
       }
     }
     return;
   }
-  __shared__ FaceLds<R> lds_all[kFW];
+  __shared__ R T[TS::N];
+  __shared__ FaceLds<DIM, R> lds_all[kFW];
+  for (int i = threadIdx.x; i < TS::N; i += 64 * kFW) T[i] = tab[i];
+  __syncthreads();
   const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int p = __builtin_amdgcn_readfirstlane((int)blockIdx.x * kFW + wv);
   if (p >= n_faces) return;
-  FaceLds<R>& L = lds_all[wv];
+  FaceLds<DIM, R>& L = lds_all[wv];
   const int e0 = face_inc[2 * p], e1r = face_inc[2 * p + 1];
   const int ni = e1r >= 0 ? 2 : 1;
-  const int e1 = ni == 2 ? e1r : e0;
   const int lfw = face_lf[p];
-  FaceInc<R> g0, g1;
-  {
-    FaceInc<R>* gs[2] = {&g0, &g1};
-    const int es[2] = {e0, e1};
+  // the facet's CSR slice; its first source codes are loaded while the rows are computed
+  const int base = rowptr[ROWS * p];
+  const int cnt = rowptr[ROWS * p + ROWS] - base;
+  constexpr int kPre = 6;
+  int pre[kPre];
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      FaceInc<R>& g = *gs[i];
-      const int e = es[i];
-      face_geometry<R>(cells, coords, e, g.J, g.detJ);
-      g.invdet = R(1) / g.detJ;
-      g.orient = el_orient[e];
-      const int lf = (lfw >> (2 * i)) & 3;
-      g.ajpack = 0;
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const int b = 3 * lf + k;
-        g.ajpack |= b << (4 * (int)((g.orient >> (3 * b)) & 3));
-      }
-      g.nb = 0;
-      g.blpack = 0;
-#pragma unroll
-      for (int k = 0; k < 4; ++k)
-        if (el_face[e * 4 + k] >= p) {
-          g.blpack |= k << (2 * g.nb);
-          ++g.nb;
-        }
-    }
-  }
-  // pushed-forward basis psi_{b d}(q) of both elements
-  for (int t = l; t < 2 * kEB * kEQ; t += 64) {
-    const int i = t / (kEB * kEQ), rem = t % (kEB * kEQ), b = rem / kEQ, q = rem % kEQ;
-    if (i >= ni) continue;
-    const R p0 = tphi[(q * kEB + b) * 3], p1 = tphi[(q * kEB + b) * 3 + 1], p2 = tphi[(q * kEB + b) * 3 + 2];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) L.psi[i][b][d][q] = i ? face_psi(g1, d, p0, p1, p2) : face_psi(g0, d, p0, p1, p2);
-  }
-  // coefficient-weighted rows of the owning face's basis functions
-  if (l < 2 * 3 * kEQ) {
-    const int i = l / (3 * kEQ), j = (l / kEQ) % 3, q = l % kEQ;
-    if (i < ni) {
-      const int e = i ? e1 : e0;
-      const int a = ((i ? g1.ajpack : g0.ajpack) >> (4 * j)) & 15;
-      const R m = mu[(int64_t)e * cs_e + q * cs_q];
-      const R lm = lam[(int64_t)e * cs_e + q * cs_q];
-      const R wd = tw[q] * (i ? g1.detJ : g0.detJ);
-      const R k1 = wd / (R(2) * m);
-      const R k2 = wd * (R(1) / (R(2) * m)) * (lm / (R(2) * m + R(3) * lm));
-      const R p0 = tphi[(q * kEB + a) * 3], p1 = tphi[(q * kEB + a) * 3 + 1], p2 = tphi[(q * kEB + a) * 3 + 2];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const R ps = i ? face_psi(g1, c, p0, p1, p2) : face_psi(g0, c, p0, p1, p2);
-        L.kpsi[i][0][j][c][q] = k2 * ps;
-        L.kpsi[i][1][j][c][q] = k1 * ps;
-      }
-    }
-  }
+  for (int u = 0; u < kPre; ++u) pre[u] = l + 64 * u < cnt ? codes[base + l + 64 * u] : 0;
+  // both incidences' loads before any arithmetic (the second reuses the first element on a
+  // boundary facet; its rows are then not used)
+  const FaceInc<DIM, R> g0 = face_inc_data<DIM, R>(geom, el_orient, el_face, lam, mu, cs_e, cs_q, e0, lfw & 3, p, l);
+  const FaceInc<DIM, R> g1 = face_inc_data<DIM, R>(geom, el_orient, el_face, lam, mu, cs_e, cs_q,
+                                                   ni == 2 ? e1r : e0, (lfw >> 2) & 3, p, l);
+  face_incidence<DIM, MODE, 0, R>(L, T, g0, l);
+  if (ni == 2) face_incidence<DIM, MODE, 1, R>(L, T, g1, l);
   __builtin_amdgcn_wave_barrier();
-  // D_{a_j b} = sum_q k1_q psi_{a_j} . psi_b  (+ DDref_{a_j b} / detJ for the Riesz map)
-  for (int t = l; t < 2 * 3 * kEB; t += 64) {
-    const int i = t / (3 * kEB), j = (t / kEB) % 3, b = t % kEB;
-    if (i >= ni) continue;
-    R s = 0;
+  // the facet's CSR slice: contiguous, coalesced stores
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-      for (int q = 0; q < kEQ; ++q) s += L.kpsi[i][1][j][c][q] * L.psi[i][b][c][q];
-    if (MODE == 1) {
-      const int a = ((i ? g1.ajpack : g0.ajpack) >> (4 * j)) & 15;
-      s += tDD[a * kEB + b] * (i ? g1.invdet : g0.invdet);
-    }
-    L.D[i][j][b] = s;
-  }
-  __builtin_amdgcn_wave_barrier();
-  // element rows of the face: (s, s) blocks of the needed faces, then the u / t columns
-  face_ss_rows<R>(L, g0, 0, l);
-  if (ni == 2) face_ss_rows<R>(L, g1, 1, l);
-  if (MODE == 0) {
-    for (int t = l; t < 2 * 9 * 6; t += 64) {
-      const int i = t / 54, m = (t / 6) % 9, col = t % 6;
-      if (i >= ni) continue;
-      const int j = m / 3, c = m % 3;
-      const int a = ((i ? g1.ajpack : g0.ajpack) >> (4 * j)) & 15;
-      const R sa = i ? basis_sign(g1, a) : basis_sign(g0, a);
+  for (int u = 0; u < kPre; ++u) {
+    const int k = l + 64 * u;
+    if (k < cnt) {
+      const int code = pre[u];
+      const int r = code & 15, s0 = (code >> 4) & 63, s1 = (code >> 10) & 63;
       R v = 0;
-      if (col < 3) {
-        v = col == c ? tU[a] : R(0);
-      } else {
-        const int ct = col - 3;
-        if (ct != c) {
-          const int k = 3 - c - ct;
-          const R P0 = tPhi[a * 3], P1 = tPhi[a * 3 + 1], P2 = tPhi[a * 3 + 2];
-          // row k of J by selects (a runtime index into a register array would go to scratch)
-          R Jk[3];
-#pragma unroll
-          for (int x = 0; x < 3; ++x) {
-            const R u0 = k == 0 ? g0.J[0][x] : (k == 1 ? g0.J[1][x] : g0.J[2][x]);
-            const R u1 = k == 0 ? g1.J[0][x] : (k == 1 ? g1.J[1][x] : g1.J[2][x]);
-            Jk[x] = i ? u1 : u0;
-          }
-          const R Aak = Jk[0] * P0 + Jk[1] * P1 + Jk[2] * P2;
-          const bool even = (ct == 0 && c == 1) || (ct == 1 && c == 2) || (ct == 2 && c == 0);
-          v = even ? Aak : -Aak;
-        }
-      }
-      L.K[i][m][36 + col] = sa * v;
+      if (s0 != kNoSrc) v += L.K[0][r][s0];
+      if (s1 != kNoSrc) v += L.K[1][r][s1];
+      vals[base + k] = v;
     }
   }
-  __builtin_amdgcn_wave_barrier();
-  // the face's CSR slice: contiguous, coalesced stores
-  const int base = rowptr[9 * p];
-  const int cnt = rowptr[9 * p + 9] - base;
-  for (int k = l; k < cnt; k += 64) {
+  for (int k = l + 64 * kPre; k < cnt; k += 64) {
     const int code = codes[base + k];
     const int r = code & 15, s0 = (code >> 4) & 63, s1 = (code >> 10) & 63;
     R v = 0;
@@ -493,20 +581,18 @@ __global__ __launch_bounds__(64 * kFW) void k_elasticity_faces(
     if (s1 != kNoSrc) v += L.K[1][r][s1];
     vals[base + k] = v;
   }
-  if (MODE == 0 && l < 9) rhs[9 * p + l] = R(0);
+  if (MODE == 0 && l < ROWS) rhs[ROWS * p + l] = R(0);
 }
 
-template <int MODE, typename R>
-static int launch_elasticity_faces(int n_faces, int n_elem, int n_s, int n_u, const int32_t* face_inc,
-                                   const int32_t* face_lf, const int32_t* el_face, const uint64_t* el_orient,
-                                   const int32_t* cells, const void* coords, const void* tab, const void* lam,
-                                   const void* mu, int cse, int csq, const void* f, int fse, int fsq, int fsc,
-                                   const int32_t* rowptr, const uint16_t* codes, void* vals, void* rhs,
-                                   hipStream_t s) {
+template <int DIM, int MODE, typename R>
+static int launch_elasticity_faces(int n_faces, int n_elem, const int32_t* face_inc, const int32_t* face_lf,
+                                   const int32_t* el_face, const uint64_t* el_orient, const void* geom,
+                                   const void* tab, const void* lam, const void* mu, int cse,
+                                   int csq, const void* f, int fse, int fsq, int fsc, const int32_t* rowptr,
+                                   const uint16_t* codes, void* vals, void* rhs, hipStream_t s) {
   const int blocks = grid_for(n_faces, kFW) + grid_for(n_elem, kCellBlock);
-  hipLaunchKernelGGL((k_elasticity_faces<MODE, R>), dim3(blocks), dim3(64 * kFW), 0, s, n_faces, n_elem, n_s, n_u,
-                     face_inc, face_lf, el_face, el_orient, cells, static_cast<const R*>(coords),
-                     static_cast<const R*>(tab), static_cast<const R*>(lam), static_cast<const R*>(mu), cse, csq,
+  hipLaunchKernelGGL((k_elasticity_faces<DIM, MODE, R>), dim3(blocks), dim3(64 * kFW), 0, s, n_faces, n_elem,
+                     face_inc, face_lf, el_face, el_orient, static_cast<const R*>(geom), static_cast<const R*>(tab), static_cast<const R*>(lam), static_cast<const R*>(mu), cse, csq,
                      static_cast<const R*>(f), fse, fsq, fsc, rowptr, codes, static_cast<R*>(vals),
                      static_cast<R*>(rhs));
   NFX_LAUNCH_CHECK();
@@ -709,29 +795,33 @@ extern "C" int nfx_assemble_elasticity(int mode, int n_elem, const int32_t* d_ce
                                             d_slot, slot_pairs, nullptr, d_vals, d_stage, nullptr, s);
 }
 
-extern "C" int nfx_assemble_elasticity_faces(int mode, int n_faces, int n_elem, int n_s, int n_u,
+extern "C" int nfx_assemble_elasticity_faces(int dim, int mode, int n_faces, int n_elem,
                                              const int32_t* d_face_inc, const int32_t* d_face_lf,
                                              const int32_t* d_el_face, const uint64_t* d_el_orient,
-                                             const int32_t* d_cells, const void* d_coords, const void* d_tables,
+                                             const void* d_geom, const void* d_tables,
                                              const void* d_lam, const void* d_mu, int coef_stride_e,
                                              int coef_stride_q, const void* d_f, int f_stride_e, int f_stride_q,
                                              int f_stride_c, const int32_t* d_rowptr, const uint16_t* d_codes,
                                              int fp64, void* d_vals, void* d_rhs, void* stream) {
+  NFX_REQUIRE(dim == 2 || dim == 3, "nfx_assemble_elasticity_faces: dim must be 2 or 3");
   NFX_REQUIRE(mode == 0 || mode == 1, "nfx_assemble_elasticity_faces: mode");
-  NFX_REQUIRE(n_faces >= 0 && n_elem >= 0 && n_s == 9 * n_faces && n_u == 3 * n_elem,
-              "nfx_assemble_elasticity_faces: sizes (n_s = 9 n_faces, n_u = 3 n_elem)");
+  NFX_REQUIRE(n_faces >= 0 && n_elem >= 0, "nfx_assemble_elasticity_faces: sizes");
   if (n_faces == 0 && n_elem == 0) return NFX_OK;
-  NFX_REQUIRE(d_face_inc && d_face_lf && d_el_face && d_el_orient && d_cells && d_coords && d_tables && d_lam &&
+  NFX_REQUIRE(d_face_inc && d_face_lf && d_el_face && d_el_orient && d_geom && d_tables && d_lam &&
                   d_mu && d_rowptr && d_codes && d_vals,
               "nfx_assemble_elasticity_faces: null pointer");
   NFX_REQUIRE(mode == 1 || (d_f && d_rhs), "nfx_assemble_elasticity_faces: residual args");
   hipStream_t s = static_cast<hipStream_t>(stream);
-#define NFX_EF(M, T)                                                                                            \
-  launch_elasticity_faces<M, T>(n_faces, n_elem, n_s, n_u, d_face_inc, d_face_lf, d_el_face, d_el_orient, d_cells, \
-                                d_coords, d_tables, d_lam, d_mu, coef_stride_e, coef_stride_q, d_f, f_stride_e,   \
-                                f_stride_q, f_stride_c, d_rowptr, d_codes, d_vals, d_rhs, s)
-  if (mode == 0) return fp64 ? NFX_EF(0, double) : NFX_EF(0, float);
-  return fp64 ? NFX_EF(1, double) : NFX_EF(1, float);
+#define NFX_EF(D, M, T)                                                                                         \
+  launch_elasticity_faces<D, M, T>(n_faces, n_elem, d_face_inc, d_face_lf, d_el_face, d_el_orient, d_geom,      \
+                                   d_tables, d_lam, d_mu, coef_stride_e, coef_stride_q, d_f, f_stride_e,          \
+                                   f_stride_q, f_stride_c, d_rowptr, d_codes, d_vals, d_rhs, s)
+  if (dim == 3) {
+    if (mode == 0) return fp64 ? NFX_EF(3, 0, double) : NFX_EF(3, 0, float);
+    return fp64 ? NFX_EF(3, 1, double) : NFX_EF(3, 1, float);
+  }
+  if (mode == 0) return fp64 ? NFX_EF(2, 0, double) : NFX_EF(2, 0, float);
+  return fp64 ? NFX_EF(2, 1, double) : NFX_EF(2, 1, float);
 #undef NFX_EF
 }
 

@@ -114,8 +114,8 @@ def face_owner_layout(n, ref_dest, ref_sign):
     displacement and rotation DoFs 3 per cell (``n_s + 3 e + c``, ``n_s + n_u + 3 e + c``).
     Returns a dict with face_inc (n_f, 2) (incident elements in element order, -1 none),
     face_lf (n_f,) (local face index of each incidence, 2 bits each), el_face (E, 4) (the
-    DoF block of each local face) and el_orient (E,) uint64 (bits 3 b, 3 b + 1: face slot of
-    reference basis b; bit 3 b + 2: its orientation sign), or raises NotImplementedError
+    DoF block of each local face) and el_orient (E,) uint64 (bits 2 b, 2 b + 1: face slot of
+    reference basis b; bit 32 + b: its orientation sign), or raises NotImplementedError
     when the DoF layout is not this one."""
     E = ref_dest.shape[0]
     if ref_dest.shape[1] != NL:
@@ -136,8 +136,9 @@ def face_owner_layout(n, ref_dest, ref_sign):
     if np.any(sg != sg[:, :, :1]):
         raise NotImplementedError("face-owner layout: signs must not depend on the stress row")
     neg = (sg[:, :, 0] < 0).astype(np.uint64)
-    bits = slot.astype(np.uint64) | (neg << np.uint64(2))
-    orient = (bits << (np.uint64(3) * np.arange(12, dtype=np.uint64))[None, :]).sum(axis=1, dtype=np.uint64)
+    b = np.arange(12, dtype=np.uint64)[None, :]
+    orient = ((slot.astype(np.uint64) << (np.uint64(2) * b)) | (neg << (np.uint64(32) + b))).sum(axis=1,
+                                                                                                dtype=np.uint64)
     n_f = int(el_face.max()) + 1 if E else 0
     n_s = 9 * n_f
     n_u = 3 * E
@@ -260,6 +261,15 @@ class ElasticitySystem:
         self.tables32 = to_dev(tab.astype(np.float32), device=device)
         X = mesh.points[cells]
         self.xq = np.einsum("qa,eac->eqc", p1_bary(points), X)
+        # per-element geometry record of the face kernel: J / |det J| row-major, then |det J|,
+        # with J[c][d] = X_{d+1,c} - X_{0,c}
+        J = np.transpose(X[:, 1:] - X[:, :1], (0, 2, 1))
+        det = np.abs(np.linalg.det(J))
+        if self.E and det.min() <= 0:
+            raise ValueError("degenerate element")
+        geom = np.concatenate([(J / det[:, None, None]).reshape(self.E, 9), det[:, None]], axis=1)
+        self.geom64 = to_dev(np.ascontiguousarray(geom), device=device)
+        self.geom32 = to_dev(np.ascontiguousarray(geom.astype(np.float32)), device=device)
         if self.E * NL * NL > nat.SLOT_INDEX_MASK:
             raise ValueError("mesh too large for the dense cold pass (28-bit local codes)")
         ref_dest, ref_sign = space.reference_destinations()
@@ -434,6 +444,8 @@ class ElasticitySystem:
                 raise ValueError("face-owner layout sizes")
             if self.tables32.numel() != 8 + 8 * 36 + 8 * 12 + 12 + 36 + 144:
                 raise ValueError("elasticity table size")
+            if tuple(self.geom32.shape) != (E, 10) or tuple(self.geom64.shape) != (E, 10):
+                raise ValueError("element geometry record size")
             for key in ("A", "P"):
                 op = self.ops[key]
                 code = op["codes_np"].astype(np.int64)
@@ -466,9 +478,9 @@ class ElasticitySystem:
             c = self._coef
             lam, mu, f = c["lam"], c["mu"], c["f"]
             lay = self.face_layout
-            nat.call("nfx_assemble_elasticity_faces", mode, lay["n_faces"], self.E, lay["n_s"], lay["n_u"],
+            nat.call("nfx_assemble_elasticity_faces", self.dim, mode, lay["n_faces"], self.E,
                      nat.ptr(self.face_inc), nat.ptr(self.face_lf), nat.ptr(self.el_face), nat.ptr(self.el_orient),
-                     nat.ptr(self.cells), nat.ptr(self.coords64 if fp64 else self.coords),
+                     nat.ptr(self.geom64 if fp64 else self.geom32),
                      nat.ptr(self.tables64 if fp64 else self.tables32), nat.ptr(lam.get(fp64)),
                      nat.ptr(mu.get(fp64)), lam.se, lam.sq, nat.ptr(f.get(fp64)), self.nq * 6, 6, 1,
                      nat.ptr(op["rowptr"]), nat.ptr(op["codes"]), int(fp64), nat.ptr(vals), nat.ptr(rg), s)

@@ -77,7 +77,7 @@ def replay(n, lay, codes, pattern, K, ref_sign):
     sgn = ref_sign  # (E, 42) per reference DoF
     # local row of slot (j, c) in incidence i: the reference basis a with face slot j
     orient = lay["el_orient"]
-    slot = ((orient[:, None] >> (np.uint64(3) * np.arange(12, dtype=np.uint64))[None, :]) & np.uint64(3)).astype(int)
+    slot = ((orient[:, None] >> (np.uint64(2) * np.arange(12, dtype=np.uint64))[None, :]) & np.uint64(3)).astype(int)
     for p in range(lay["n_faces"]):
         for k in range(rowptr[9 * p], rowptr[9 * p + 9]):
             code = int(codes[k])
