@@ -310,6 +310,16 @@ int nfx_residual64(int n_rows, const int32_t* d_rowptr, const int32_t* d_col,
                    const double* d_val, const double* d_x, const double* d_b,
                    double* d_r, void* stream);
 
+/* nfx_residual64 without any allocation: no counterpart in the reference; for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). The residual kernel's partial sums go to a static device
+ * array of 2 x 2048 doubles (32 KiB per GPU, held until the process ends); the kernel grid is
+ * capped at 2048 blocks, so any n_rows fits. Same outputs as nfx_residual64, bit for bit.
+ * Calls that overlap on one GPU share that array and race. */
+int nfx_residual64_noalloc(int n_rows, const int32_t* d_rowptr, const int32_t* d_col,
+                           const double* d_val, const double* d_x, const double* d_b,
+                           double* d_r, void* stream);
+
 /* FP64 row diagonal of a CSR matrix, and the FP64 Schur-complement diagonal
  * S[k - n_u] = sum_{j < n_u} A[k, j] A[j, k] / A[j, j] for rows k >= n_u (the
  * pressure part of the block preconditioner; pattern must be structurally
