@@ -46,6 +46,18 @@ def profiled_traffic(kernel, n):
     return None if rec is None else round(rec["fetch_bytes"] + rec["write_bytes"])
 
 
+def calibration_copy(log):
+    """Three 256 MiB device copies: the known-size kernel the counter runs scale by."""
+    import torch
+
+    a = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log("calibration copy: 3 x 256 MiB")
+
+
 def _one_core():
     try:
         from threadpoolctl import threadpool_limits
@@ -271,6 +283,38 @@ def bench_elasticity(args, log):
         sysd.assemble("A", fp64=False)
     torch.cuda.synchronize()
     ms = (time.perf_counter() - ts) * 1e3 / steps
+    # the assembly kernel alone: events on its launch stream (torch's current stream) around
+    # back-to-back launches (the host enqueues faster than a launch runs)
+    reps = 20
+    k_times = []
+    for _ in range(3):
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record()
+        for _ in range(reps):
+            sysd.assemble("A", fp64=False)
+        ev1.record()
+        torch.cuda.synchronize()
+        k_times.append(ev0.elapsed_time(ev1) / reps)
+    k_ms = min(k_times)
+    nnz_u = sysd.pattern_A.nnz
+    # algorithmic bytes per tet: 241 in (85 as for the RT1 row: 4 vertex ids, 12 coordinates,
+    # a coefficient, 4 face ids, an orientation byte; + 12 for lambda, mu, f; + 144 for the 36
+    # face-DoF ids) and out the FP32 values of the upper-triangle CSR (4 nnz / E) and the
+    # residual (4 N / E)
+    alg = 241.0 + 4.0 * nnz_u / E + 4.0 * sysd.n / E
+    achieved = alg * E / (k_ms * 1e-3) / 1e9
+    kname = "k_elasticity_faces<3,0,float>" if sysd.use_faces else "k_elasticity<0,float>"
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": 8000.0, "unit": "GB/s",
+                "frac": round(achieved / 8000.0, 4), "traffic": profiled_traffic(kname, n),
+                "kernel": kname, "kernel_ms": round(k_ms, 5), "alg_bytes_per_tet": round(alg, 2)}
+    log(f"elasticity: {kname} {k_ms * 1e3:.2f} us/launch, {achieved:.1f} GB/s algorithmic")
+    out = {"workload": f"elasticity3d-bdm1p0p0-kuhn{n}x6-lognormal-lame(sigma=0.5)", "elements": E, "dofs": sysd.n,
+           "nnz_upper": nnz_u, "cold_setup_s": round(t1 - t0, 3),
+           "ms_per_assembly": round(ms, 4), "tets_per_s": round(E / (ms * 1e-3), 1),
+           "kernel_tets_per_s": round(E / (k_ms * 1e-3), 1), "roofline": roofline}
+    if getattr(args, "no_elasticity_solve", False):
+        return out
     sysd.assemble("A", fp64=True, u_dirichlet=uD)
     sysd.assemble("A", fp64=False, u_dirichlet=uD)
     torch.cuda.synchronize()
@@ -278,8 +322,6 @@ def bench_elasticity(args, log):
     x, info = sysd.solve(rtol=1e-8, max_inner=100000)
     torch.cuda.synchronize()
     tsolve = time.perf_counter() - tsolve
-    # error per field against the FP64 reference solution of the same system: the FP64
-    # assembly (equal to the oracle's to 1e-12 in the parity tests) solved to 1e-12
     x = x.clone()
     # the 1e-8 solution against the same system solved to 1e-12 on the device (a GPU-side
     # consistency figure; the FP64 assembly equals the oracle's to 1e-12 in the parity tests)
@@ -289,13 +331,10 @@ def bench_elasticity(args, log):
         a, b = ps.strides[name], ps.strides[name] + sp.n_dof
         field_err[name] = float(torch.linalg.norm(x[a:b] - xr[a:b]) / torch.linalg.norm(xr[a:b]))
     oracle_check = elasticity_oracle_check(log)
-    out = {"workload": f"elasticity3d-bdm1p0p0-kuhn{n}x6-lognormal-lame(sigma=0.5)", "elements": E, "dofs": sysd.n,
-           "nnz_upper": sysd.pattern_A.nnz, "cold_setup_s": round(t1 - t0, 3),
-           "ms_per_assembly": round(ms, 4), "tets_per_s": round(E / (ms * 1e-3), 1),
-           "time_to_solve_s": round(tsolve, 4), "iterations": info["iterations"],
-           "refinements": info["outer"], "residual": info["residual"], "converged": info["converged"],
-           "rel_diff_vs_gpu_fp64_solve_1e-12": field_err, "gpu_fp64_solve_residual": info_r["residual"],
-           "oracle_check": oracle_check}
+    out.update({"time_to_solve_s": round(tsolve, 4), "iterations": info["iterations"],
+                "refinements": info["outer"], "residual": info["residual"], "converged": info["converged"],
+                "rel_diff_vs_gpu_fp64_solve_1e-12": field_err, "gpu_fp64_solve_residual": info_r["residual"],
+                "oracle_check": oracle_check})
     log(f"elasticity: {ms:.3f} ms/assembly ({out['tets_per_s']:.3e} tets/s), solve {tsolve:.2f}s "
         f"{info['iterations']} it, residual {info['residual']:.2e}; per-field difference from the 1e-12 solve "
         f"{field_err}; oracle check {oracle_check}")
@@ -592,6 +631,9 @@ def main():
                     help="cube divisions of the system the CPU time-to-solve leg assembles and solves (0: skip)")
     ap.add_argument("--no-elasticity", action="store_true")
     ap.add_argument("--elasticity-n", type=int, default=16)
+    ap.add_argument("--elasticity-only", action="store_true",
+                    help="run only the elasticity workload (assembly steps; solve unless --no-elasticity-solve)")
+    ap.add_argument("--no-elasticity-solve", action="store_true")
     ap.add_argument("--batch", action="store_true",
                     help="run the batched workload instead: 256 instances, n ~ U{8..32}, k in {1, 2}")
     ap.add_argument("--batch-instances", type=int, default=256)
@@ -617,6 +659,12 @@ def main():
         return
     if args.md:
         print(json.dumps(bench_md(args, log)), flush=True)
+        return
+    if args.elasticity_only:
+        out = bench_elasticity(args, log)
+        if args.calibrate_copy:
+            calibration_copy(log)
+        print(json.dumps(out), flush=True)
         return
 
     import torch
@@ -712,12 +760,7 @@ def main():
     log(f"{kname} {k_ms * 1e3:.2f} us/launch, {achieved:.1f} GB/s algorithmic")
 
     if args.calibrate_copy:
-        a = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device=dev).fill_(1.0)
-        b = torch.empty_like(a)
-        for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        log("calibration copy: 3 x 256 MiB")
+        calibration_copy(log)
 
     solve = None
     if not args.no_solve:

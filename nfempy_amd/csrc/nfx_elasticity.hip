@@ -240,12 +240,21 @@ template <int DIM, typename R>
 struct FaceLds {
   using S = ElSizes<DIM>;
   static constexpr int ROWS = S::NBF * S::NC, NL = S::NB * S::NC + S::NC + S::NT;
+  // psi staging passes of 64 lanes over (b, q): lanes past NB rows store to the junk slot
+  static constexpr int NBP = (S::NB * S::NQ + 63) / 64 * 64 / S::NQ;
+  // a K row holds NL columns and one junk slot (column NL) that masked lanes store to
+  static constexpr int NLP = NL + 1, DJ = S::NBF * S::NB;
   // psi, kpsi and D serve one incidence at a time (the second reuses them after the first's
   // rows are in K)
-  R psi[S::NB][DIM][S::NQ];       // s(b) psi_{b d}(q)
+  // s(b) psi_{b d}(q); rows padded by two (a stride of 26 / 10 words): the reads of the tiles
+  // (rows b and b + 1 in adjacent lanes, q and q + 1 in the other half) hit distinct banks
+  struct PsiRow {
+    R v[DIM][S::NQ];
+    R pad[2];
+  } psi[S::NB];
   R kpsi[2][S::NBF][DIM][S::NQ];  // [0][j][c][q] = k2_q s(a_j) psi_{a_j c}(q); [1] with k1_q
-  R D[S::NBF][S::NB];             // s(a) s(b) D_{a_j b} (+ div-div, Riesz map)
-  R K[2][ROWS][NL];               // signed element rows of the facet: s(a) s(b) K_e[(a, c), col]
+  R D[DJ + 1];                    // s(a) s(b) D_{a_j b} at j NB + b (+ div-div, Riesz map); DJ junk
+  R K[2][ROWS][NLP];              // signed element rows of the facet: s(a) s(b) K_e[(a, c), col]
 };
 
 // the element-independent table, staged in LDS once per block
@@ -326,33 +335,42 @@ __device__ __forceinline__ FaceInc<DIM, R> face_inc_data(const R* __restrict__ g
 
 // All element rows of the owning facet in incidence I (uniform): psi, coefficient-weighted
 // rows, the D block, the u / t columns and the (s, s) blocks of the needed facets into L.K[I].
+// The matrix-core tiles use power-of-two lane layouts, so no lane index needs a division:
+// accumulator row 4 j + c (facet slot j < NBF, component c < NC), column 4 w + d (basis w of
+// one needed facet, component d); one 16 x 16 tile per needed facet. Lanes without an entry
+// to store write to the junk slots (D[DJ], column NL of K row 0) instead of branching.
 template <int DIM, int MODE, int I, typename R>
 __device__ __forceinline__ void face_incidence(FaceLds<DIM, R>& L, const R* __restrict__ T,
                                                const FaceInc<DIM, R>& g, int l) {
   using S = ElSizes<DIM>;
   using TS = TabSizes<DIM>;
+  using LD = FaceLds<DIM, R>;
   constexpr int NB = S::NB, NQ = S::NQ, NC = S::NC, NT = S::NT, NBF = S::NBF, NFC = S::NFC;
-  constexpr int ROWS = FaceLds<DIM, R>::ROWS, NSS = NB * NC;
+  constexpr int ROWS = LD::ROWS, NSS = NB * NC, NLP = LD::NLP, NL = LD::NL;
   constexpr int LQ = NQ == 8 ? 3 : 2;  // log2 NQ
+  static_assert(NBF <= 4 && NC <= 4 && NB < LD::NBP, "lane layout");
+  R* const Kf = &L.K[I][0][0];
+  constexpr int JUNK = NL;  // column NL of row 0: written by masked lanes, never read
   __builtin_amdgcn_wave_barrier();  // a previous incidence's reads of psi / kpsi / D are done
-  // psi_{b d}(q) = (J phihat_b(q))_d / detJ, lane <-> (b, q)
+  // psi_{b d}(q) = (J phihat_b(q))_d / detJ, lane <-> (b, q); lanes past the NB rows read
+  // the table at b = 0 and store to the junk slot
 #pragma unroll
-  for (int pass = 0; pass < (NB * NQ + 63) / 64; ++pass) {
+  for (int pass = 0; pass < LD::NBP * NQ / 64; ++pass) {
     const int idx = l + 64 * pass;
-    if (idx < NB * NQ) {
-      const int b = idx >> LQ, q = idx & (NQ - 1);
-      R ph[DIM];
+    const bool in = idx < NB * NQ;
+    const int b = in ? idx >> LQ : 0, q = idx & (NQ - 1);
+    R ph[DIM];
 #pragma unroll
-      for (int x = 0; x < DIM; ++x) ph[x] = T[TS::PHI + (q * NB + b) * DIM + x];
-      // the orientation sign of b folded in: every product below is then signed
-      const R sc = basis_sign(g, b);
+    for (int x = 0; x < DIM; ++x) ph[x] = T[TS::PHI + (q * NB + b) * DIM + x];
+    // the orientation sign of b folded in: every product below is then signed
+    const R sc = basis_sign(g, b);
+    R* const dst = in ? &L.psi[b].v[0][q] : Kf + JUNK;
 #pragma unroll
-      for (int d = 0; d < DIM; ++d) {
-        R v = 0;
+    for (int d = 0; d < DIM; ++d) {
+      R v = 0;
 #pragma unroll
-        for (int x = 0; x < DIM; ++x) v += g.J[d][x] * ph[x];
-        L.psi[b][d][q] = v * sc;
-      }
+      for (int x = 0; x < DIM; ++x) v += g.J[d][x] * ph[x];
+      dst[in ? d * NQ : 0] = v * sc;
     }
   }
   __builtin_amdgcn_wave_barrier();
@@ -362,11 +380,19 @@ __device__ __forceinline__ void face_incidence(FaceLds<DIM, R>& L, const R* __re
     const int a = (g.ajpack >> (4 * j)) & 15;
     const R m = g.mu_q, lm = g.lam_q;
     const R wd = T[TS::W + q] * g.detJ;
-    const R k1 = wd / (R(2) * m);
-    const R k2 = wd * (R(1) / (R(2) * m)) * (lm / (R(2) * m + R(DIM) * lm));
+    R k1, k2;
+    if constexpr (sizeof(R) == 4) {
+      // FP32 rows: hardware reciprocals (1 ulp), well inside the FP32 value tolerance
+      const float r2m = __builtin_amdgcn_rcpf(2.f * m);
+      k1 = wd * r2m;
+      k2 = k1 * (lm * __builtin_amdgcn_rcpf(2.f * m + float(DIM) * lm));
+    } else {
+      k1 = wd / (R(2) * m);
+      k2 = wd * (R(1) / (R(2) * m)) * (lm / (R(2) * m + R(DIM) * lm));
+    }
 #pragma unroll
     for (int c = 0; c < DIM; ++c) {
-      const R ps = L.psi[a][c][q];
+      const R ps = L.psi[a].v[c][q];
       L.kpsi[0][j][c][q] = k2 * ps;
       L.kpsi[1][j][c][q] = k1 * ps;
     }
@@ -374,17 +400,18 @@ __device__ __forceinline__ void face_incidence(FaceLds<DIM, R>& L, const R* __re
   __builtin_amdgcn_wave_barrier();
   const int lr = l & 15, lq = l >> 4;
   // D_{a_j b} = sum_{c, q} k1_q psi_{a_j c}(q) psi_{b c}(q) (+ DDref / detJ for the Riesz map):
-  // an NBF x NB block, K = NC NQ
+  // an NBF x NB block, K = NC NQ; accumulator row j = r (lq = 0), column b = lr
   {
     R dv[4];
     if constexpr (sizeof(R) == 4) {
       f4_t acc = f4_t{0.f, 0.f, 0.f, 0.f};
+      // rows j >= NBF and columns b >= NB of the tile read clamped entries: they only reach
+      // accumulator rows / columns that are never stored
+      const int jr = lr < NBF ? lr : 0, br = lr < NB ? lr : 0;
 #pragma unroll
       for (int s = 0; s < NC * NQ / 4; ++s) {
         const int k = 4 * s + lq, c = k >> LQ, q = k & (NQ - 1);
-        const float av = lr < NBF ? L.kpsi[1][lr < NBF ? lr : 0][c][q] : 0.f;
-        const float bv = lr < NB ? L.psi[lr < NB ? lr : 0][c][q] : 0.f;
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(L.kpsi[1][jr][c][q], L.psi[br].v[c][q], acc, 0, 0, 0);
       }
 #pragma unroll
       for (int r = 0; r < 4; ++r) dv[r] = acc[r];
@@ -392,115 +419,111 @@ __device__ __forceinline__ void face_incidence(FaceLds<DIM, R>& L, const R* __re
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         R sacc = 0;
-        if (lq == 0 && r < NBF && lr < NB)
+        if (r < NBF)
 #pragma unroll
           for (int c = 0; c < NC; ++c)
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) sacc += L.kpsi[1][r][c][q] * L.psi[lr][c][q];
+            for (int q = 0; q < NQ; ++q) sacc += L.kpsi[1][r < NBF ? r : 0][c][q] * L.psi[lr < NB ? lr : 0].v[c][q];
         dv[r] = sacc;
       }
     }
-    if (lq == 0 && lr < NB) {
+    const bool dl = lq == 0 && lr < NB;
+    const int lrc = lr < NB ? lr : 0;
 #pragma unroll
-      for (int r = 0; r < NBF; ++r) {
-        R v = dv[r];
-        if (MODE == 1) {
-          const int a = (g.ajpack >> (4 * r)) & 15;
-          v += basis_sign(g, a) * basis_sign(g, lr) * T[TS::DD + a * NB + lr] / g.detJ;
-        }
-        L.D[r][lr] = v;
+    for (int r = 0; r < NBF; ++r) {
+      R v = dv[r];
+      if (MODE == 1) {
+        const int a = (g.ajpack >> (4 * r)) & 15;
+        v += basis_sign(g, a) * basis_sign(g, lrc) * T[TS::DD + a * NB + lrc] / g.detJ;
       }
+      L.D[dl ? r * NB + lr : LD::DJ] = v;
     }
   }
-  // u / t columns of the facet rows, lane <-> (row m, column)
+  // u / t columns of the facet rows: lane 16 j + 4 c + x holds row (j, c), columns u_x and
+  // t_x (x < NC, resp. x < NT)
   if (MODE == 0) {
-    constexpr int NUT = NC + NT;
-    if (l < ROWS * NUT) {
-      const int m = l / NUT, col = l - NUT * m;
-      const int j = m / NC, c = m - NC * j;
-      const int a = (g.ajpack >> (4 * j)) & 15;
-      R v = 0;
-      if (col < NC) {
-        v = col == c ? T[TS::U + a] : R(0);
-      } else {
-        // rotation component; in 2-D the single rotation is the one about z
-        const int ct = DIM == 3 ? col - NC : 2;
-        if (ct != c) {
-          const int k = 3 - c - ct;  // < DIM
-          // A_{a k} = sum_x J_kx Phi_ax for every row k, then a select (no runtime index into J)
-          R Ar[DIM];
+    const int j = l >> 4, c = (l >> 2) & 3, x = l & 3;
+    const bool ok = j < NBF && c < NC;
+    const int jj = ok ? j : 0, cc = ok ? c : 0;
+    const int a = (g.ajpack >> (4 * jj)) & 15;
+    const R sa = basis_sign(g, a);
+    const int rowo = (NC * jj + cc) * NLP;
+    // displacement column u_x: sum_q w divref_a on the diagonal component
+    Kf[ok && x < NC ? rowo + NSS + x : NL] = sa * (x == cc ? T[TS::U + a] : R(0));
+    // rotation column t_x; in 2-D the single rotation is the one about z
+    const int ct = DIM == 3 ? x : 2;
+    const int k = 3 - cc - ct;  // < DIM when ct != c
+    // A_{a k} = sum_y J_ky Phi_ay for every row k, then a select (no runtime index into J)
+    R Aak = 0;
 #pragma unroll
-          for (int y = 0; y < DIM; ++y) {
-            Ar[y] = 0;
+    for (int y = 0; y < DIM; ++y) {
+      R ar = 0;
 #pragma unroll
-            for (int x = 0; x < DIM; ++x) Ar[y] += g.J[y][x] * T[TS::PH + a * DIM + x];
-            Ar[y] *= g.detJ;  // J = (J / |det J|) |det J|
-          }
-          R Aak = Ar[0];
-#pragma unroll
-          for (int y = 1; y < DIM; ++y) Aak = k == y ? Ar[y] : Aak;
-          const bool even = (ct == 0 && c == 1) || (ct == 1 && c == 2) || (ct == 2 && c == 0);
-          v = even ? Aak : -Aak;
-        }
-      }
-      L.K[I][m][NSS + col] = basis_sign(g, a) * v;
+      for (int z = 0; z < DIM; ++z) ar += g.J[y][z] * T[TS::PH + a * DIM + z];
+      Aak = k == y ? ar * g.detJ : Aak;  // J = (J / |det J|) |det J|
     }
+    const bool even = (ct == 0 && cc == 1) || (ct == 1 && cc == 2) || (ct == 2 && cc == 0);
+    const R vt = ct == cc ? R(0) : (even ? Aak : -Aak);
+    Kf[ok && x < NT ? rowo + NSS + NC + x : NL] = sa * vt;
   }
   __builtin_amdgcn_wave_barrier();
-  // (s, s) blocks: V = sum_q k2_q psi_ac psi_bd on MFMA (rows m = j NC + c, columns of the
-  // needed facet blocks), then K = delta_cd D - V with the orientation signs
-  int rj[4], rc[4];
+  // (s, s) blocks: V = sum_q k2_q psi_ac psi_bd on the matrix cores, one tile per needed facet
+  // (rows 4 j + c, columns 4 w + d), then K = delta_cd D - V with the orientation signs
+  const int aj = lr >> 2, ac = lr & 3;   // A operand row (j, c)
+  const int wj = lr >> 2, dd = lr & 3;   // accumulator column (w, d)
+  const bool arow = aj < NBF && ac < NC;
+  const bool ccol = wj < NBF && dd < DIM;
+  const int ajc = arow ? aj : 0, acc_ = arow ? ac : 0;
+  const int wjc = ccol ? wj : 0, ddc = ccol ? dd : 0;
+  const bool vrow = lq < NBF;  // accumulator rows 4 lq + r, r < NC
+  // row offsets of the three stored rows (junk when the lane holds no entry)
+  int rowo[NC];
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int m = 4 * lq + r;
-    rj[r] = m / NC;
-    rc[r] = m - NC * rj[r];
+  for (int r = 0; r < NC; ++r) rowo[r] = (vrow && ccol) ? (NC * lq + r) * NLP : JUNK;
+  // the A operand is the same for every tile (rows (j, c) past NBF x NC read a clamped entry:
+  // their accumulator rows are never stored)
+  R a_op[NQ / 4];
+  if constexpr (sizeof(R) == 4) {
+#pragma unroll
+    for (int s = 0; s < NQ / 4; ++s) a_op[s] = L.kpsi[0][ajc][acc_][4 * s + lq];
   }
-  const int nc = ROWS * g.nb;
-  const int ntile = (nc + 15) >> 4;
-  const int am = lr < ROWS ? lr : 0;
 #pragma unroll
-  for (int t = 0; t < (ROWS * NFC + 15) / 16; ++t) {
-    if (t >= ntile) break;
-    const int n = 16 * t + lr;
-    const bool cn = n < nc;
-    const int bi = cn ? n / ROWS : 0, w = n - ROWS * bi;
-    const int lf = (g.blpack >> (2 * bi)) & 3;
-    const int b = NBF * lf + w / NC, d = w - NC * (w / NC);
+  for (int t = 0; t < NFC; ++t) {
+    if (t >= g.nb) break;
+    const int lf = (g.blpack >> (2 * t)) & 3;  // uniform
+    const int b = NBF * lf + wjc;
     R v[4];
     if constexpr (sizeof(R) == 4) {
-      f4_t acc = f4_t{0.f, 0.f, 0.f, 0.f};
+      f4_t accv = f4_t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int s = 0; s < NQ / 4; ++s) {
-        const int q = 4 * s + lq;
-        const float av = lr < ROWS ? L.kpsi[0][am / NC][am % NC][q] : 0.f;
-        const float bv = cn ? L.psi[b][d][q] : 0.f;
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
+        const float bv = L.psi[b].v[ddc][4 * s + lq];
+        accv = __builtin_amdgcn_mfma_f32_16x16x4f32(a_op[s], bv, accv, 0, 0, 0);
       }
 #pragma unroll
-      for (int r = 0; r < 4; ++r) v[r] = acc[r];
+      for (int r = 0; r < 4; ++r) v[r] = accv[r];
     } else {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
+      for (int r = 0; r < NC; ++r) {
         R sacc = 0;
-        if (4 * lq + r < ROWS && cn)
 #pragma unroll
-          for (int q = 0; q < NQ; ++q) sacc += L.kpsi[0][rj[r]][rc[r]][q] * L.psi[b][d][q];
+        for (int q = 0; q < NQ; ++q) sacc += L.kpsi[0][vrow ? lq : 0][r][q] * L.psi[b].v[ddc][q];
         v[r] = sacc;
       }
     }
-    if (cn) {
+    // K column of (b, d) inside the needed-facet blocks: NBF NC lf + NC w + d
+    const int colk = NBF * NC * lf + NC * wjc + ddc;
+    const R dlv = L.D[(vrow ? lq : 0) * NB + b];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int m = 4 * lq + r;
-        if (m < ROWS) L.K[I][m][ROWS * lf + w] = (rc[r] == d ? L.D[rj[r]][b] : R(0)) - v[r];
-      }
+    for (int r = 0; r < NC; ++r) {
+      const int ro = rowo[r];
+      Kf[ro == JUNK ? JUNK : ro + colk] = (r == ddc ? dlv : R(0)) - v[r];
     }
   }
 }
 
 template <int DIM, int MODE, typename R>
-__global__ __launch_bounds__(64 * kFW) void k_elasticity_faces(
+__global__ __launch_bounds__(64 * kFW, sizeof(R) == 4 ? 7 : 3) void k_elasticity_faces(
     int n_faces, int n_elem, const int32_t* __restrict__ face_inc, const int32_t* __restrict__ face_lf,
     const int32_t* __restrict__ el_face, const uint64_t* __restrict__ el_orient, const R* __restrict__ geom,
     const R* __restrict__ tab, const R* __restrict__ lam, const R* __restrict__ mu,
@@ -560,27 +583,23 @@ __global__ __launch_bounds__(64 * kFW) void k_elasticity_faces(
   face_incidence<DIM, MODE, 0, R>(L, T, g0, l);
   if (ni == 2) face_incidence<DIM, MODE, 1, R>(L, T, g1, l);
   __builtin_amdgcn_wave_barrier();
-  // the facet's CSR slice: contiguous, coalesced stores
+  // the facet's CSR slice: contiguous, coalesced stores; a missing source (code 63) reads a
+  // clamped slot and is selected away (0 + K0 + K1 in that order, a missing term as 0)
+  constexpr int NLP = FaceLds<DIM, R>::NLP, NL = FaceLds<DIM, R>::NL;
+  const R* const K0 = &L.K[0][0][0];
+  const R* const K1 = &L.K[1][0][0];
+  auto entry = [&](int code) -> R {
+    const int r = code & 15, s0 = (code >> 4) & 63, s1 = (code >> 10) & 63;
+    const R a0 = K0[r * NLP + (s0 < NL ? s0 : 0)];
+    const R a1 = K1[r * NLP + (s1 < NL ? s1 : 0)];
+    return (R(0) + (s0 < NL ? a0 : R(0))) + (s1 < NL ? a1 : R(0));
+  };
 #pragma unroll
   for (int u = 0; u < kPre; ++u) {
     const int k = l + 64 * u;
-    if (k < cnt) {
-      const int code = pre[u];
-      const int r = code & 15, s0 = (code >> 4) & 63, s1 = (code >> 10) & 63;
-      R v = 0;
-      if (s0 != kNoSrc) v += L.K[0][r][s0];
-      if (s1 != kNoSrc) v += L.K[1][r][s1];
-      vals[base + k] = v;
-    }
+    if (k < cnt) vals[base + k] = entry(pre[u]);
   }
-  for (int k = l + 64 * kPre; k < cnt; k += 64) {
-    const int code = codes[base + k];
-    const int r = code & 15, s0 = (code >> 4) & 63, s1 = (code >> 10) & 63;
-    R v = 0;
-    if (s0 != kNoSrc) v += L.K[0][r][s0];
-    if (s1 != kNoSrc) v += L.K[1][r][s1];
-    vals[base + k] = v;
-  }
+  for (int k = l + 64 * kPre; k < cnt; k += 64) vals[base + k] = entry(codes[base + k]);
   if (MODE == 0 && l < ROWS) rhs[ROWS * p + l] = R(0);
 }
 
