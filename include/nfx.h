@@ -364,6 +364,61 @@ int nfx_solve_minres_ir(nfx_solver* s, const int32_t* d_rowptr, const int32_t* d
                         double rtol, double inner_rtol, int max_inner, int max_outer,
                         int* h_info, double* h_resid, double* h_hist, int hist_cap);
 
+/* Two-grid block preconditioner of the dual-mixed elasticity operator (stress rows first,
+ * 9 per face; then u rows n_s + 3c + k and t rows n_s + n_u + 3c + k of cell c), the
+ * GPU replacement of the reference's fieldsplit preconditioner built on the Riesz map
+ * (exact Cholesky of its stress block, ICC on the rest:
+ * examples/cauchy_elasticity/mfe_elasticity_example_1.py:238-269). P^-1 = diag(D^-1,
+ * psign TG(S)): D^-1 the inverted 9 x 9 face blocks of the compliance block (FP32, 81 per
+ * face), S = B D^-1 B^T on the cells (6 x 6 blocks: the cell's own, then one per local face
+ * neighbour), TG(S) one symmetric two-grid cycle with the block-Jacobi smoother W = omega
+ * S_cc^-1 and piecewise-constant aggregation with the dense coarse inverse ainv ((6 n_agg)^2
+ * FP32, row-major). Cell data are in aggregate order: position m holds cell agg_mem[m], the
+ * positions of aggregate A are agg_ptr[A] .. agg_ptr[A+1]-1, agg_of[m] its aggregate,
+ * nbr[4m+k] the position of the neighbour across local face k (-1 none); sblk holds entry e
+ * of block b (0: own, 1 + k: neighbour k) of position m at (36 b + e) n_cells + m, winv
+ * entry e at e n_cells + m (the layout nfx_elasticity_tg_blocks writes). work: 12 n_cells +
+ * 12 n_agg floats, 16-byte aligned. psign = +1 gives an SPD preconditioner (MINRES). */
+typedef struct nfx_elasticity_tg {
+  int n_s, n_u, n_cells, n_agg;
+  const float* dinv;
+  const int32_t* nbr;
+  const float* sblk;
+  const float* winv;
+  const int32_t* agg_ptr;
+  const int32_t* agg_mem;
+  const int32_t* agg_of;
+  const float* ainv;
+  float psign;
+  float* work;
+} nfx_elasticity_tg;
+int nfx_elasticity_tg_apply(const nfx_elasticity_tg* pc, const float* d_r, float* d_z, void* stream);
+/* Set-up of the blocks (FP64), cells in mesh order: for cell c and local face k (face
+ * d_efac[4c+k], neighbour cell d_nbr[4c+k] or -1, the neighbour's local index of that face
+ * d_nbrk[4c+k]) with B_ck the 6 x 9 block of the (u, t) x stress coupling (d_bblk, 54 per
+ * (c, k), coefficient-independent) and d_dinv the FP64 face-block inverses: d_S (n_cells x
+ * 5 x 36, cell order) = S blocks; d_S32 and d_W32 = omega S_cc^-1 in the entry-major
+ * aggregate-order layout of nfx_elasticity_tg (cell c at position d_pos[c]); d_part
+ * (n_cells x 4 x 36) scratch. Bit 0 of *d_status is set when a cell block is not SPD. */
+int nfx_elasticity_tg_blocks(int n_cells, const int32_t* d_efac, const int32_t* d_nbr,
+                             const int32_t* d_nbrk, const double* d_bblk, const double* d_dinv,
+                             const int32_t* d_pos, double* d_part, double* d_S, float* d_S32,
+                             float* d_W32, double omega, int* d_status, void* stream);
+/* d_Ac += P^T S P (FP64, (6 n_agg)^2 row-major; zero it first), deterministic; cells in
+ * mesh order: d_agg_mem[agg_ptr[A] ..] the cells of aggregate A, d_agg_of / d_nbr per cell
+ * (d_S as nfx_elasticity_tg_blocks writes it). */
+int nfx_elasticity_tg_coarse(int n_agg, const int32_t* d_agg_ptr, const int32_t* d_agg_mem,
+                             const int32_t* d_agg_of, const int32_t* d_nbr, const double* d_S,
+                             double* d_Ac, void* stream);
+/* nfx_solve_minres_ir with the elasticity two-grid preconditioner (applied as a function,
+ * psign = +1) in place of a sparse P^-1; the solver's 3 x 3 block copy of the operator is
+ * used when set. */
+int nfx_solve_minres_ir_tg(nfx_solver* s, const int32_t* d_rowptr, const int32_t* d_col,
+                           const float* d_val32, const double* d_val64, const nfx_elasticity_tg* pc,
+                           const double* d_b, double* d_x, double rtol, double inner_rtol,
+                           int max_inner, int max_outer, int* h_info, double* h_resid, double* h_hist,
+                           int hist_cap);
+
 /* ---- L2 projection ---------------------------------------------------------
  * Element mass matrix and right-hand side of one field of an L2 projection, as
  * L2ProjectorWeakForm.evaluate_form (src/weak_forms/l2_projector_weak_form.py:8-93)

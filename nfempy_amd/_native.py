@@ -110,6 +110,14 @@ SIGNATURES = {
     "nfx_coarse_dense": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                  c_void_p]),
     "nfx_twogrid_apply": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "nfx_elasticity_tg_apply": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "nfx_elasticity_tg_blocks": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                         c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p]),
+    "nfx_elasticity_tg_coarse": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                         c_void_p]),
+    "nfx_solve_minres_ir_tg": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                       c_void_p, c_double, c_double, c_int, c_int, c_void_p, c_void_p, c_void_p,
+                                       c_int]),
     "nfx_half_rows": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "nfx_sgemm": (c_int, [c_int, c_int, c_int, c_float, c_void_p, c_int, c_int, c_void_p, c_int, c_int, c_float,
                           c_void_p, c_int, c_void_p, c_int, c_void_p]),
@@ -132,6 +140,15 @@ class TwoGridPC(ctypes.Structure):
                 ("n_tail", c_int), ("tail_dinv", c_void_p),
                 ("slot_tail", c_void_p), ("slot_coef", c_void_p), ("tail_cells", c_void_p),
                 ("tail_coef", c_void_p)]
+
+
+class ElasticityTG(ctypes.Structure):
+    """Mirror of ``nfx_elasticity_tg`` (include/nfx.h)."""
+
+    _fields_ = [("n_s", c_int), ("n_u", c_int), ("n_cells", c_int), ("n_agg", c_int), ("dinv", c_void_p),
+                ("nbr", c_void_p), ("sblk", c_void_p), ("winv", c_void_p), ("agg_ptr", c_void_p),
+                ("agg_mem", c_void_p), ("agg_of", c_void_p), ("ainv", c_void_p), ("psign", c_float),
+                ("work", c_void_p)]
 
 
 # entry points whose return value is a count, not a status

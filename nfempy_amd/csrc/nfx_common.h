@@ -80,4 +80,9 @@ inline int grid_for(int64_t n, int block) { return static_cast<int>((n + block -
 int twogrid_apply_scaled(const nfx_twogrid_pc* pc, const float* r, float* z, const double* dscale,
                          hipStream_t stream);
 
+// Elasticity two-grid block preconditioner z = P^-1 r (nfx_elasticity_tg_apply), output
+// divided by *dscale when dscale is not null.
+int el_twogrid_apply_scaled(const nfx_elasticity_tg* pc, const float* r, float* z, const double* dscale,
+                            hipStream_t stream);
+
 }  // namespace nfx

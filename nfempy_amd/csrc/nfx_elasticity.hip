@@ -21,6 +21,7 @@
 //   (t_c, s_bd)  = eps(c, d, k) sum_q w detJ psi_bk                  (mode 0)
 //   (u_c, u_c), (t_c, t_c) = sum_q w detJ                            (mode 1)
 //   r(u_c) = -sum_q w detJ f_c,  r(t_c) = -sum_q w detJ f_{3+c}      (mode 0)
+#include "nfx_blockinv.h"
 #include "nfx_common.h"
 
 namespace nfx {
@@ -624,33 +625,6 @@ __global__ __launch_bounds__(256) void k_gather(int64_t n, const int32_t* __rest
                                                 const R* __restrict__ src, R* __restrict__ dst) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i < n) dst[i] = src[idx[i]];
-}
-
-// In-place Gauss-Jordan inverse of an SPD N x N block (FP64, no pivoting: the pivots of
-// an SPD matrix are positive); returns false on a non-positive pivot. The result is
-// symmetrised, (A^-1 + A^-T) / 2.
-template <int N>
-__device__ bool spd_block_inverse(double (&A)[N][N]) {
-  for (int k = 0; k < N; ++k) {
-    const double piv = A[k][k];
-    if (!(piv > 0.0)) return false;
-    const double ip = 1.0 / piv;
-    A[k][k] = 1.0;
-    for (int j = 0; j < N; ++j) A[k][j] *= ip;
-    for (int i = 0; i < N; ++i) {
-      if (i == k) continue;
-      const double f = A[i][k];
-      A[i][k] = 0.0;
-      for (int j = 0; j < N; ++j) A[i][j] -= f * A[k][j];
-    }
-  }
-  for (int i = 0; i < N; ++i)
-    for (int j = i + 1; j < N; ++j) {
-      const double v = 0.5 * (A[i][j] + A[j][i]);
-      A[i][j] = v;
-      A[j][i] = v;
-    }
-  return true;
 }
 
 // Face blocks of the stress mass block: rows 9b .. 9b+8 restricted to columns of the same

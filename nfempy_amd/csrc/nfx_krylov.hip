@@ -7,10 +7,12 @@
 // iterations, so an iteration is two kernel launches:
 //   A(k): x/w update of iteration k-1, y = D A z / beta - (beta/oldb) r1, alpha = v.y
 //   B(k): y -= (alpha/beta) r2, z = P^-1 y, beta' = sqrt(y.z), Givens update
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
 #include "nfx_common.h"
+#include "nfx_el_twogrid.h"
 #include "nfx_residual.h"
 
 namespace nfx {
@@ -396,6 +398,111 @@ __global__ __launch_bounds__(kBlock) void k_axpy64(int n, double scale, const fl
   x[i] += scale * (double)d[i];
 }
 
+// Variants for a preconditioner applied as a function (a sequence of launches between
+// B1(k) and B2(k)): init without z, B1 = alpha and y' only, B2 = partials of y' . z.
+__global__ __launch_bounds__(kBlock) void k_minres_init_fn(int n, const float* __restrict__ rhs, float* r1,
+                                                           float* r2, float* x, float* w0, float* w1, float* w2) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const float b = rhs[i];
+    r1[i] = b;
+    r2[i] = b;
+    x[i] = 0.f;
+    w0[i] = 0.f;
+    w1[i] = 0.f;
+    w2[i] = 0.f;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_minres_b1(int n, const float* __restrict__ y,
+                                                      const float* __restrict__ r2, float* __restrict__ ynew,
+                                                      const double* __restrict__ partial_a, int np_a,
+                                                      MinresState* __restrict__ st) {
+  __shared__ double sh[kBlock / 64];
+  const double alfa = sum_partials(partial_a, np_a, sh);
+  const float c = (float)(alfa / st->beta);
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) ynew[i] = y[i] - c * r2[i];
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->alfa = alfa;
+}
+
+__global__ __launch_bounds__(kBlock) void k_dot_partials(int n, const float* __restrict__ a,
+                                                         const float* __restrict__ b, double* __restrict__ partial) {
+  __shared__ double sh[kBlock / 64];
+  double acc = 0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) acc += (double)a[i] * (double)b[i];
+  const double bs = block_sum(acc, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = bs;
+}
+
+// B(k) fused with the first steps of the elasticity two-grid preconditioner: alpha,
+// y' = y - (alpha / beta) r2 into ynew, z = D^-1 y' on the stress rows, x1 = W y' on the
+// cells (aggregate order), and the partials of y' . z over the stress rows (grid-stride,
+// fixed order per thread).
+__global__ __launch_bounds__(kBlock) void k_minres_tg_b(int ns, int nu, int n_cells, const float* __restrict__ y,
+                                                        const float* __restrict__ r2, float* __restrict__ ynew,
+                                                        float* __restrict__ z, const float* __restrict__ dinv,
+                                                        const float* __restrict__ W,
+                                                        const int32_t* __restrict__ agg_mem, float* __restrict__ x1,
+                                                        const double* __restrict__ partial_a, int np_a,
+                                                        double* __restrict__ partial_b,
+                                                        MinresState* __restrict__ st) {
+  __shared__ double sh[kBlock / 64];
+  const double alfa = sum_partials(partial_a, np_a, sh);
+  const float c = (float)(alfa / st->beta);
+  double acc = 0;
+  const int total = ns + n_cells;
+  for (int g = blockIdx.x * kBlock + threadIdx.x; g < total; g += gridDim.x * kBlock) {
+    if (g < ns) {
+      const float yv = y[g] - c * r2[g];
+      ynew[g] = yv;
+      const float zv = el_stress_row(g, dinv, [&](int j) { return y[j] - c * r2[j]; });
+      z[g] = zv;
+      acc += (double)yv * (double)zv;
+    } else {
+      const int m = g - ns;
+      const int cell = agg_mem[m];
+      float rc[6];
+#pragma unroll
+      for (int i = 0; i < 6; ++i) {
+        const int64_t row = el_row(cell, ns, nu, i);
+        rc[i] = y[row] - c * r2[row];
+        ynew[row] = rc[i];
+      }
+      el_smooth(m, n_cells, W, rc, x1);
+    }
+  }
+  const double bs = block_sum(acc, sh);
+  if (threadIdx.x == 0) partial_b[blockIdx.x] = bs;
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->alfa = alfa;
+}
+
+// The last step of the two-grid cycle on the cells (z = psign (x2 + W (r - S x2))) and the
+// partials of y' . z over the cell rows.
+__global__ __launch_bounds__(kBlock) void k_minres_tg_post(int ns, int nu, int n_cells,
+                                                           const int32_t* __restrict__ agg_mem,
+                                                           const int32_t* __restrict__ nbr,
+                                                           const int32_t* __restrict__ agg_of,
+                                                           const float* __restrict__ S, const float* __restrict__ W,
+                                                           const float* __restrict__ r, const float* __restrict__ x1,
+                                                           const float* __restrict__ yc, float psign,
+                                                           float* __restrict__ z, double* __restrict__ partial_b) {
+  __shared__ double sh[kBlock / 64];
+  double acc = 0;
+  for (int m = blockIdx.x * kBlock + threadIdx.x; m < n_cells; m += gridDim.x * kBlock) {
+    const int cell = agg_mem[m];
+    float rv[6], zc[6];
+    el_gather(cell, ns, nu, r, rv);
+    el_post(m, n_cells, nbr, agg_of, S, W, rv, x1, yc, zc);
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      const float zv = psign * zc[i];
+      z[el_row(cell, ns, nu, i)] = zv;
+      acc += (double)rv[i] * (double)zv;
+    }
+  }
+  const double bs = block_sum(acc, sh);
+  if (threadIdx.x == 0) partial_b[blockIdx.x] = bs;
+}
+
 }  // namespace nfx
 
 using namespace nfx;
@@ -528,19 +635,20 @@ extern "C" int nfx_solver_destroy(nfx_solver* s) {
 static int launch_a(nfx_solver* s, const int32_t* rp, const int32_t* ci, const float* va,
                     const float* sign, const float* z, const float* zprev, const float* r1,
                     float* y, float* x, float* wnew, const float* wa, const float* wb, int first,
-                    int final_only, const MinresState* sin, MinresState* sout) {
+                    int final_only, const MinresState* sin, MinresState* sout, int np_b = -1) {
   const int n = s->n;
+  if (np_b < 0) np_b = s->blocks_b;
   if (s->b3_va) {
     hipLaunchKernelGGL(k_minres_a_b3<16>, dim3(s->blocks_a3), dim3(kBlock), 0, s->stream, s->b3_nbr, s->b3_nblk,
                        s->b3_rp, s->b3_ci, s->b3_va, sign, z, zprev, r1, y, x, wnew, wa, wb, first, final_only,
-                       s->partial_b, s->blocks_b, s->partial_a, sin, sout);
+                       s->partial_b, np_b, s->partial_a, sin, sout);
     NFX_LAUNCH_CHECK();
     return NFX_OK;
   }
 #define NFX_A(L)                                                                                  \
   hipLaunchKernelGGL(k_minres_a<L>, dim3(s->blocks_a), dim3(kBlock), 0, s->stream, n, rp, ci, va, \
                      sign, z, zprev, r1, y, x, wnew, wa, wb, first, final_only, s->partial_b,     \
-                     s->blocks_b, s->partial_a, sin, sout)
+                     np_b, s->partial_a, sin, sout)
   if (s->lanes == 16)
     NFX_A(16);
   else if (s->lanes == 8)
@@ -655,12 +763,124 @@ extern "C" int nfx_solve_minres_ir(nfx_solver* s, const int32_t* d_rowptr, const
     const double rn = sqrt(rr), bn = sqrt(bb);
     rel = bn > 0 ? rn / bn : rn;
     if (h_hist && outer < hist_cap) h_hist[outer] = rel;
-    if (rel <= rtol || outer == max_outer || rn == 0) break;
+    if (!(rel == rel) || rel <= rtol || outer == max_outer || rn == 0) break;
     hipLaunchKernelGGL(k_to_f32, dim3(g), dim3(kBlock), 0, s->stream, n, s->r64, d_sign, rn, rf);
     NFX_LAUNCH_CHECK();
     int it = 0;
     rc = minres_inner(s, d_rowptr, d_col, d_val32, d_sign, d_pc_rowptr, d_pc_col, d_pc_val, rf, dx,
                       inner_rtol, max_inner, &it);
+    if (rc) return rc;
+    total += it;
+    hipLaunchKernelGGL(k_axpy64, dim3(g), dim3(kBlock), 0, s->stream, n, rn, dx, d_x);
+    NFX_LAUNCH_CHECK();
+  }
+  NFX_CHECK_HIP(hipStreamSynchronize(s->stream));
+  if (h_info) {
+    h_info[0] = outer;
+    h_info[1] = total;
+  }
+  if (h_resid) h_resid[0] = rel;
+  return rel <= rtol ? NFX_OK : NFX_E_NOT_CONVERGED;
+}
+
+// minres_inner with the elasticity two-grid preconditioner: B(k) fused with its first
+// steps (k_minres_tg_b), the coarse correction, then its last step fused with the second
+// half of the y' . z partials (k_minres_tg_post); the partials of the two kernels follow
+// each other in partial_b.
+static int minres_inner_tg(nfx_solver* s, const int32_t* rp, const int32_t* ci, const float* va,
+                           const nfx_elasticity_tg* pc, const float* rhs, float* x, double tol, int maxit,
+                           int* iters) {
+  const int n = s->n;
+  float* V = s->buf;
+  float* P[2] = {V + (size_t)n * 0, V + (size_t)n * 1};
+  float* F = V + (size_t)n * 2;
+  float* Z[2] = {V + (size_t)n * 3, V + (size_t)n * 4};
+  float* W[3] = {V + (size_t)n * 5, V + (size_t)n * 6, V + (size_t)n * 7};
+  MinresState* S[2] = {s->st, s->st + 1};
+  const int ns = pc->n_s, nu = pc->n_u, E = pc->n_cells;
+  const int b1 = std::min(grid_for(ns + E, kBlock), kMaxBlocks / 2);
+  const int b2 = std::min(grid_for(E, kBlock), kMaxBlocks / 2);
+  hipLaunchKernelGGL(k_minres_init_fn, dim3(s->blocks_b), dim3(kBlock), 0, s->stream, n, rhs, P[0], P[1], x, W[0],
+                     W[1], W[2]);
+  NFX_LAUNCH_CHECK();
+  int rc = el_twogrid_apply_scaled(pc, rhs, Z[1], nullptr, s->stream);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_dot_partials, dim3(s->blocks_b), dim3(kBlock), 0, s->stream, n, rhs, Z[1], s->partial_b);
+  NFX_LAUNCH_CHECK();
+  int np_b = s->blocks_b;
+  const int check = 16;
+  int k = 1;
+  for (; k <= maxit; ++k) {
+    float* r1 = P[(k + 1) % 2];
+    float* r2 = P[k % 2];
+    float* z = Z[k % 2];
+    float* zprev = Z[(k + 1) % 2];
+    float* wnew = W[(k + 2) % 3];
+    const float* wa = W[k % 3];
+    const float* wb = W[(k + 1) % 3];
+    rc = launch_a(s, rp, ci, va, nullptr, z, zprev, r1, F, x, wnew, wa, wb, k == 1, 0, S[(k + 1) & 1], S[k & 1],
+                  np_b);
+    if (rc) return rc;
+    // y' into r1's buffer, z_{k+1} = P^-1 y' into zprev's buffer
+    hipLaunchKernelGGL(k_minres_tg_b, dim3(b1), dim3(kBlock), 0, s->stream, ns, nu, E, F, r2, r1, zprev, pc->dinv,
+                       pc->winv, pc->agg_mem, el_tg_x1(pc), s->partial_a, s->b3_va ? s->blocks_a3 : s->blocks_a,
+                       s->partial_b, S[k & 1]);
+    NFX_LAUNCH_CHECK();
+    rc = el_tg_launch_mid(pc, r1, s->stream);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_minres_tg_post, dim3(b2), dim3(kBlock), 0, s->stream, ns, nu, E, pc->agg_mem, pc->nbr,
+                       pc->agg_of, pc->sblk, pc->winv, r1, el_tg_x1(pc), el_tg_y(pc), pc->psign, zprev,
+                       s->partial_b + b1);
+    NFX_LAUNCH_CHECK();
+    np_b = b1 + b2;
+    if (k % check == 0 || k == maxit) {
+      NFX_CHECK_HIP(hipMemcpyAsync(s->h_st, S[k & 1], sizeof(MinresState), hipMemcpyDeviceToHost, s->stream));
+      NFX_CHECK_HIP(hipStreamSynchronize(s->stream));
+      const double beta1 = s->h_st->beta1;
+      if (s->h_st->stop || !(beta1 > 0) || s->h_st->phibar <= tol * beta1) break;
+    }
+  }
+  if (k > maxit) k = maxit;
+  {
+    const int kk = k + 1;
+    float* z = Z[kk % 2];
+    float* zprev = Z[(kk + 1) % 2];
+    float* wnew = W[(kk + 2) % 3];
+    const float* wa = W[kk % 3];
+    const float* wb = W[(kk + 1) % 3];
+    rc = launch_a(s, rp, ci, va, nullptr, z, zprev, nullptr, nullptr, x, wnew, wa, wb, 0, 1, S[(kk + 1) & 1],
+                  S[kk & 1], np_b);
+    if (rc) return rc;
+  }
+  *iters = k;
+  return NFX_OK;
+}
+
+extern "C" int nfx_solve_minres_ir_tg(nfx_solver* s, const int32_t* d_rowptr, const int32_t* d_col,
+                                      const float* d_val32, const double* d_val64, const nfx_elasticity_tg* pc,
+                                      const double* d_b, double* d_x, double rtol, double inner_rtol, int max_inner,
+                                      int max_outer, int* h_info, double* h_resid, double* h_hist, int hist_cap) {
+  NFX_REQUIRE(s && d_rowptr && d_col && d_val32 && d_val64 && pc && d_b && d_x, "nfx_solve_minres_ir_tg: null pointer");
+  NFX_REQUIRE(max_inner > 0 && max_outer >= 0, "nfx_solve_minres_ir_tg: iteration limits");
+  NFX_REQUIRE(pc->n_s + 2 * pc->n_u == s->n, "nfx_solve_minres_ir_tg: preconditioner size differs from the matrix");
+  const int n = s->n;
+  float* rf = s->buf + (size_t)n * 8;
+  float* dx = s->buf + (size_t)n * 9;
+  int total = 0, outer = 0;
+  double rel = 0;
+  const int g = grid_for(n, kBlock);
+  for (; outer <= max_outer; ++outer) {
+    double rr = 0, bb = 0;
+    int rc = residual64(s, d_rowptr, d_col, d_val64, d_x, d_b, &rr, &bb);
+    if (rc) return rc;
+    const double rn = sqrt(rr), bn = sqrt(bb);
+    rel = bn > 0 ? rn / bn : rn;
+    if (h_hist && outer < hist_cap) h_hist[outer] = rel;
+    if (!(rel == rel) || rel <= rtol || outer == max_outer || rn == 0) break;
+    hipLaunchKernelGGL(k_to_f32, dim3(g), dim3(kBlock), 0, s->stream, n, s->r64, nullptr, rn, rf);
+    NFX_LAUNCH_CHECK();
+    int it = 0;
+    rc = minres_inner_tg(s, d_rowptr, d_col, d_val32, pc, rf, dx, inner_rtol, max_inner, &it);
     if (rc) return rc;
     total += it;
     hipLaunchKernelGGL(k_axpy64, dim3(g), dim3(kBlock), 0, s->stream, n, rn, dx, d_x);

@@ -28,11 +28,13 @@ from .device import torch
 class MinresIR:
     """FP32 MINRES + FP64 refinement on one CSR operator resident on the device."""
 
-    def __init__(self, rowptr, col, vals32, vals64, sign=None, pinv=None, stream=None, pc=None, block3=None):
+    def __init__(self, rowptr, col, vals32, vals64, sign=None, pinv=None, stream=None, pc=None, block3=None,
+                 tg=None):
         """pinv: diagonal preconditioner (FP32 inverse diagonal) or pc: (rowptr, col, val)
-        CSR of a sparse SPD preconditioner P^-1 (FP32 values). block3: optional
-        (n_blocks, browptr, bcol, bval), a 3 x 3 block CSR copy of the FP32 operator that
-        the inner MINRES multiplies with (nfx_solver_set_block3)."""
+        CSR of a sparse SPD preconditioner P^-1 (FP32 values), or tg: an ``nfx_elasticity_tg``
+        descriptor (the elasticity two-grid preconditioner, applied as a function). block3:
+        optional (n_blocks, browptr, bcol, bval), a 3 x 3 block CSR copy of the FP32 operator
+        that the inner MINRES multiplies with (nfx_solver_set_block3)."""
         t = torch()
         self.n = int(rowptr.numel() - 1)
         self.nnz = int(col.numel())
@@ -44,6 +46,9 @@ class MinresIR:
             ar = t.arange(self.n + 1, dtype=t.int32, device=dev)
             pc = (ar, ar[:-1].contiguous(), diag.to(t.float32).contiguous())
         self.pc = pc
+        self.tg = tg
+        if tg is not None and sign is not None:
+            raise ValueError("the two-grid preconditioner solves the symmetric operator (sign must be None)")
         self.stream = stream
         h = ctypes.c_void_p()
         nat.call("nfx_solver_create", ctypes.byref(h), self.n, self.nnz, nat.stream_handle(stream))
@@ -72,13 +77,21 @@ class MinresIR:
         info = (ctypes.c_int * 2)()
         resid = (ctypes.c_double * 1)()
         hist = (ctypes.c_double * (max_outer + 1))()
-        rc = nat.lib().nfx_solve_minres_ir(
-            self._h, nat.ptr(self.rowptr), nat.ptr(self.col), nat.ptr(self.vals32), nat.ptr(self.vals64),
-            nat.ptr(self.sign), nat.ptr(self.pc[0]), nat.ptr(self.pc[1]), nat.ptr(self.pc[2]), nat.ptr(b),
-            nat.ptr(x), rtol, inner_rtol, max_inner,
-            max_outer, info, resid, hist, max_outer + 1)
+        if self.tg is not None:
+            name = "nfx_solve_minres_ir_tg"
+            rc = nat.lib().nfx_solve_minres_ir_tg(
+                self._h, nat.ptr(self.rowptr), nat.ptr(self.col), nat.ptr(self.vals32), nat.ptr(self.vals64),
+                ctypes.byref(self.tg), nat.ptr(b), nat.ptr(x), rtol, inner_rtol, max_inner, max_outer, info, resid,
+                hist, max_outer + 1)
+        else:
+            name = "nfx_solve_minres_ir"
+            rc = nat.lib().nfx_solve_minres_ir(
+                self._h, nat.ptr(self.rowptr), nat.ptr(self.col), nat.ptr(self.vals32), nat.ptr(self.vals64),
+                nat.ptr(self.sign), nat.ptr(self.pc[0]), nat.ptr(self.pc[1]), nat.ptr(self.pc[2]), nat.ptr(b),
+                nat.ptr(x), rtol, inner_rtol, max_inner,
+                max_outer, info, resid, hist, max_outer + 1)
         if rc not in (nat.NFX_OK, nat.NFX_E_NOT_CONVERGED) or (raise_on_fail and rc != nat.NFX_OK):
-            nat.check(rc, "nfx_solve_minres_ir")
+            nat.check(rc, name)
         out = {
             "converged": rc == nat.NFX_OK,
             "outer": info[0],
