@@ -414,7 +414,21 @@ def bench_batch(args, log):
     ms = (time.perf_counter() - ts) * 1e3 / args.steps
     value = b.n_elements / (ms * 1e-3)
     log(f"batched assembly (graph replay): {ms:.3f} ms/step -> {value:.4e} tets/s")
-    # solve a stratified sample: smallest and largest instances of each degree
+    # every instance solved together by the batched MINRES (FP64 assembly, preconditioner
+    # and refinement per instance inside the timed region)
+    torch.cuda.synchronize()
+    tb = time.perf_counter()
+    _, binfos, t_iter = b.solve_batched(rtol=1e-8)
+    torch.cuda.synchronize()
+    t_batched = time.perf_counter() - tb
+    batched = {"instances": m, "time_s": round(t_batched, 3), "minres_s": round(t_iter, 3),
+               "converged": sum(1 for q in binfos if q["converged"]),
+               "max_residual": max(q["residual"] for q in binfos),
+               "iterations_max": max(q["iterations"] for q in binfos),
+               "iterations_mean": round(float(np.mean([q["iterations"] for q in binfos])), 1)}
+    log(f"batched solve of all {m} instances: {t_batched:.2f} s ({t_iter:.2f} s MINRES), "
+        f"{batched['converged']} converged, max residual {batched['max_residual']:.2e}")
+    # solve a stratified sample one instance at a time: smallest and largest of each degree
     sample = []
     for k in (1, 2):
         idx = sorted([i for i, (n, kk) in enumerate(specs) if kk == k], key=lambda i: specs[i][0])
@@ -438,8 +452,64 @@ def bench_batch(args, log):
         "config": {"workload": f"darcy3d-batch{m}-rt1p0+rt2p1-kuhn(8..32)x6-lognormal(l=0.1,sigma=1)",
                    "instances": m, "elements": b.n_elements, "rt1_instances": sum(1 for _, k in specs if k == 1),
                    "setup_s": round(t_setup, 2), "autotuned": tuned, "graph": True},
-        "autotune_trials": trials, "solves": solves,
+        "autotune_trials": trials, "batched_solve": batched, "solves": solves,
     }
+
+
+def bench_batch_solve(args, log):
+    """Batched uneven Darcy, end to end, on a sample that fits the default run: instances with
+    n ~ U{nmin..nmax} cubes (x6 Kuhn tets), k in {1, 2} at 50/50, log K = GRF(l = 0.1,
+    sigma = 1) seeded per instance. Timed from coefficients on the device to every instance
+    refined to 1e-8: the batched assembly (captured graph replay) and the batched MINRES
+    (FP64 assembly, diagonal block preconditioner and FP64 refinement per instance), against
+    the same instances assembled and solved one at a time with the same plain method."""
+    import torch
+
+    from nfempy_amd.batch import DarcyBatch
+
+    rng = np.random.default_rng(2027)
+    m = args.batch_solve_instances
+    specs = list(zip(rng.integers(args.batch_solve_nmin, args.batch_solve_nmax + 1, m).tolist(),
+                     rng.integers(1, 3, m).tolist()))
+    t0 = time.perf_counter()
+    b = DarcyBatch(specs, seed=300, ell=0.1, sigma=1.0)
+    b.autotune(reps=2)
+    b.capture()
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    b.assemble()
+    b.solve_batched(rtol=1e-8)  # warm-up (allocations, first launches)
+    torch.cuda.synchronize()
+    tb = time.perf_counter()
+    b.assemble()
+    _, infos, t_iter = b.solve_batched(rtol=1e-8)
+    torch.cuda.synchronize()
+    t_batched = time.perf_counter() - tb
+    # the plain GPU method one instance at a time: single-system assembly and solve
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    seq = []
+    for i, inst in enumerate(b.instances):
+        inst["system"].assemble_elements(inst["kinv"], inst["f"], fp64=False)
+        _, info = b.solve(i, rtol=1e-8)
+        seq.append(info)
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - ts
+    out = {"workload": f"darcy3d-batch{m}-rt1p0+rt2p1-kuhn({args.batch_solve_nmin}..{args.batch_solve_nmax})x6-"
+                       "lognormal(l=0.1,sigma=1)",
+           "instances": m, "elements": b.n_elements, "rt1_instances": sum(1 for _, k in specs if k == 1),
+           "setup_s": round(t_setup, 2),
+           "batched_assembly_plus_solve_s": round(t_batched, 4), "batched_minres_s": round(t_iter, 4),
+           "sequential_assembly_plus_solve_s": round(t_seq, 4),
+           "speedup_batched_vs_sequential": round(t_seq / t_batched, 2),
+           "converged": sum(1 for q in infos if q["converged"]),
+           "max_residual": max(q["residual"] for q in infos),
+           "iterations_max": max(q["iterations"] for q in infos),
+           "sequential_converged": sum(1 for q in seq if q["converged"]),
+           "preconditioner": "diagonal block (flux diagonal, Schur diagonal) per instance"}
+    log(f"batch sample: {m} instances, {b.n_elements} tets: batched assembly + solve {t_batched * 1e3:.1f} ms "
+        f"(MINRES {t_iter * 1e3:.1f} ms), one at a time {t_seq * 1e3:.1f} ms")
+    return out
 
 
 def bench_md(args, log):
@@ -641,6 +711,10 @@ def main():
                     help="run the batched workload instead: 256 instances, n ~ U{8..32}, k in {1, 2}")
     ap.add_argument("--batch-instances", type=int, default=256)
     ap.add_argument("--batch-solves", type=int, default=8, help="instances solved after the batched assembly")
+    ap.add_argument("--batch-solve-instances", type=int, default=32,
+                    help="instances of the default run's batched assembly + solve sample (0: skip)")
+    ap.add_argument("--batch-solve-nmin", type=int, default=4)
+    ap.add_argument("--batch-solve-nmax", type=int, default=16)
     ap.add_argument("--md", action="store_true",
                     help="run the mixed-dimensional transient workload instead (24^3 x 6 matrix, 8 fractures)")
     ap.add_argument("--md-steps", type=int, default=200)
@@ -771,6 +845,9 @@ def main():
     elasticity = None
     if not args.no_elasticity:
         elasticity = bench_elasticity(args, log)
+    batch_solve = None
+    if args.batch_solve_instances > 0:
+        batch_solve = bench_batch_solve(args, log)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -804,6 +881,8 @@ def main():
         out["solve"] = solve
     if elasticity:
         out["elasticity"] = elasticity
+    if batch_solve:
+        out["batch_solve"] = batch_solve
     print(json.dumps(out), flush=True)
 
 

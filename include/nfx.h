@@ -419,6 +419,37 @@ int nfx_solve_minres_ir_tg(nfx_solver* s, const int32_t* d_rowptr, const int32_t
                            int max_inner, int max_outer, int* h_info, double* h_resid, double* h_hist,
                            int hist_cap);
 
+/* Batched MINRES: many independent systems (the batched configuration's instances, each
+ * replacing one KSP solve of examples/md_elliptic_scalar/md_elliptic_scalar.py:351-364), one
+ * set of launches per iteration. Each instance has its own CSR (local column indices), FP32
+ * and FP64 values, optional row signs D (the symmetric form D A) and diagonal preconditioner
+ * (FP32 inverse diagonal, NULL = identity), its own MINRES recurrence, stopping test and FP64
+ * refinement to ||b - A x|| / ||b|| <= rtol; vectors are concatenated in instance order
+ * (row_off is filled in by nfx_minres_batch_create). Partial sums are reduced per instance in
+ * a fixed order: results do not depend on the other instances of the batch. */
+typedef struct nfx_minres_batch_item {
+  int n_rows;
+  int lanes; /* filled in by nfx_minres_batch_create (lanes per row of the SpMV) */
+  int64_t row_off;
+  const int32_t* rowptr;
+  const int32_t* col;
+  const float* val32;
+  const double* val64;
+  const float* sign;
+  const float* pinv;
+} nfx_minres_batch_item;
+typedef struct nfx_minres_batch nfx_minres_batch;
+int nfx_minres_batch_create(nfx_minres_batch** out, int n_inst, const nfx_minres_batch_item* h_items,
+                            void* stream);
+int nfx_minres_batch_destroy(nfx_minres_batch* s);
+/* Total rows of the batch (the length of d_b and d_x); a count, not a status. */
+int64_t nfx_minres_batch_rows(const nfx_minres_batch* s);
+/* Synchronous. Per instance: h_iters total MINRES iterations, h_outer refinement steps,
+ * h_resid final relative FP64 residual. NFX_E_NOT_CONVERGED if any instance stays above rtol. */
+int nfx_solve_minres_batch_ir(nfx_minres_batch* s, const double* d_b, double* d_x, double rtol,
+                              double inner_rtol, int max_inner, int max_outer, int* h_iters, int* h_outer,
+                              double* h_resid);
+
 /* ---- L2 projection ---------------------------------------------------------
  * Element mass matrix and right-hand side of one field of an L2 projection, as
  * L2ProjectorWeakForm.evaluate_form (src/weak_forms/l2_projector_weak_form.py:8-93)

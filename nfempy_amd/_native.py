@@ -111,6 +111,11 @@ SIGNATURES = {
                                  c_void_p]),
     "nfx_twogrid_apply": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "nfx_elasticity_tg_apply": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "nfx_minres_batch_create": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
+    "nfx_minres_batch_destroy": (c_int, [c_void_p]),
+    "nfx_minres_batch_rows": (c_i64, [c_void_p]),
+    "nfx_solve_minres_batch_ir": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_double, c_int, c_int, c_void_p,
+                                          c_void_p, c_void_p]),
     "nfx_elasticity_tg_blocks": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                          c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p, c_void_p]),
     "nfx_elasticity_tg_coarse": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
@@ -142,6 +147,13 @@ class TwoGridPC(ctypes.Structure):
                 ("tail_coef", c_void_p)]
 
 
+class MinresBatchItem(ctypes.Structure):
+    """Mirror of ``nfx_minres_batch_item`` (include/nfx.h)."""
+
+    _fields_ = [("n_rows", c_int), ("lanes", c_int), ("row_off", c_i64), ("rowptr", c_void_p), ("col", c_void_p),
+                ("val32", c_void_p), ("val64", c_void_p), ("sign", c_void_p), ("pinv", c_void_p)]
+
+
 class ElasticityTG(ctypes.Structure):
     """Mirror of ``nfx_elasticity_tg`` (include/nfx.h)."""
 
@@ -152,7 +164,8 @@ class ElasticityTG(ctypes.Structure):
 
 
 # entry points whose return value is a count, not a status
-_RAW = ("nfx_version", "nfx_darcy_batch_item_size", "nfx_gnn_num_params", "nfx_gnn_backward_workspace")
+_RAW = ("nfx_version", "nfx_darcy_batch_item_size", "nfx_gnn_num_params", "nfx_gnn_backward_workspace",
+        "nfx_minres_batch_rows")
 
 
 class NfxError(RuntimeError):

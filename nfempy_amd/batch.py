@@ -15,7 +15,9 @@ persistent launch (``nfx_assemble_darcy_batched``):
 - ``capture`` records the whole batched assembly in a CUDA/HIP graph (kernels
   only, no memsets) for replay.
 
-Solves are per instance (FP32 MINRES + FP64 refinement to 1e-8).
+Solves: ``solve_batched`` runs the batched MINRES (one set of launches per iteration for
+all instances, per-instance recurrences and FP64 refinement to 1e-8); ``solve`` solves one
+instance on its own.
 """
 
 from __future__ import annotations
@@ -225,6 +227,39 @@ class DarcyBatch:
             self._launch()
         self.graph = g
         return g
+
+    def fp64_system(self, i):
+        """Instance i's own FP64 values, residual and (sign, pinv) preconditioner (the shared
+        per-(n, k) system's buffers are overwritten by the next instance)."""
+        from .solvers import darcy_preconditioner
+
+        inst = self.instances[i]
+        s = inst["system"]
+        s.assemble_elements(inst["kinv"], inst["f"], fp64=True)
+        sign, pinv = darcy_preconditioner(s.rowptr, s.col, s.vals64, s.n, s.n_u)
+        return {"rowptr": s.rowptr, "col": s.col, "vals32": inst["vals"], "vals64": s.vals64.clone(),
+                "sign": sign, "pinv": pinv, "b": -s.rhs64.clone()}
+
+    def solve_batched(self, indices=None, rtol=1e-8, inner_rtol=1e-5, max_inner=100000, max_outer=30):
+        """Instances ``indices`` (default all) solved together by the batched MINRES: one set
+        of launches per iteration, per-instance recurrences, stopping tests and FP64
+        refinement. The FP32 values are those of the last batched assembly. Returns
+        (list of x, list of infos, seconds of the batched solve alone)."""
+        from .solvers import MinresBatch
+
+        t = torch()
+        idx = list(range(len(self.instances))) if indices is None else list(indices)
+        systems = [self.fp64_system(i) for i in idx]
+        b = t.cat([sy["b"] for sy in systems])
+        mb = MinresBatch(systems)
+        t.cuda.synchronize()
+        t0 = time.perf_counter()
+        x, infos = mb.solve(b, rtol=rtol, inner_rtol=inner_rtol, max_inner=max_inner, max_outer=max_outer)
+        t.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        mb.close()
+        xs = [x[mb.offsets[j]:mb.offsets[j + 1]] for j in range(len(idx))]
+        return xs, infos, dt
 
     def solve(self, i, rtol=1e-8):
         """FP64-refined solve of instance i (FP64 values assembled on demand)."""

@@ -910,3 +910,556 @@ extern "C" int nfx_residual64(int n_rows, const int32_t* d_rowptr, const int32_t
   NFX_CHECK_HIP(hipFreeAsync(partial, sm));
   return NFX_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Batched MINRES: many independent systems, one set of launches per iteration.
+//
+// Every instance keeps its own MINRES recurrence (scalar state, stopping test, iteration
+// count); the vectors of all instances are concatenated (instance i's rows at row_off[i]).
+// Work is cut into tiles of kBmTile rows of one instance, tiles of an instance contiguous;
+// a kernel over the tiles writes one partial per tile, and a reduction kernel (one block
+// per instance) sums an instance's partials in tile order, so every result is reproducible
+// and no block sums more than its instance's partials once. Per iteration: A (w/x update,
+// y = s D A z - ratio r1, partials of alpha), reduce + alpha, B (y', z = P^-1 y' with a
+// diagonal P^-1, partials of y'.z), reduce + the scalar step of the next A. An instance's
+// mode: 0 iterating, 1 final update pending (its next A applies the last rotation only),
+// 2 frozen. The host checks the states every kBmCheck iterations; the tile kernels run over
+// the list of tiles of unfrozen instances (rebuilt when the set changes).
+
+namespace nfx {
+
+constexpr int kBmTile = 64;   // rows per tile (one thread per row; 4, 8 or 16 lanes per row in A)
+constexpr int kBmCheck = 16;  // iterations between host checks
+
+__global__ __launch_bounds__(kBlock) void k_bm_init(const nfx_minres_batch_item* __restrict__ items,
+                                                    const int32_t* __restrict__ tiles,
+                                                    const int32_t* __restrict__ act,
+                                                    const int32_t* __restrict__ mode,
+                                                    const float* __restrict__ rhs, float* r1, float* r2, float* z,
+                                                    float* x, float* w0, float* w1, float* w2,
+                                                    double* __restrict__ partial_b) {
+  __shared__ double sh[kBlock / 64];
+  const int t = act[blockIdx.x], inst = tiles[2 * t], row0 = tiles[2 * t + 1];
+  double acc = 0;
+  if (mode[inst] == 0) {
+    const nfx_minres_batch_item it = items[inst];
+    const int row = row0 + threadIdx.x;
+    if (row < it.n_rows) {
+      const int64_t g = it.row_off + row;
+      const float b = rhs[g];
+      const float zz = it.pinv ? it.pinv[row] * b : b;
+      r1[g] = b;
+      r2[g] = b;
+      z[g] = zz;
+      x[g] = 0.f;
+      w0[g] = 0.f;
+      w1[g] = 0.f;
+      w2[g] = 0.f;
+      acc = (double)b * (double)zz;
+    }
+  }
+  const double bs = block_sum(acc, sh);
+  if (threadIdx.x == 0) partial_b[t] = bs;
+}
+
+// A(k) of every instance in modes 0 (full) and 1 (final update only)
+// rows of one tile of A(k) (LANES lanes per row, kBlock / LANES rows per pass)
+template <int LANES>
+__device__ __forceinline__ double bm_a_rows(const nfx_minres_batch_item& it, int row0, bool first, bool final_only,
+                                            float s, float ratio, float sprev, float oldeps, float delta,
+                                            float denom, float phi, const float* __restrict__ z,
+                                            const float* __restrict__ zprev, const float* __restrict__ r1,
+                                            float* __restrict__ y, float* __restrict__ x, float* __restrict__ wnew,
+                                            const float* __restrict__ wa, const float* __restrict__ wb) {
+  double acc = 0;
+  const int lane = threadIdx.x % LANES;
+  const float* zi = z + it.row_off;
+  for (int rs = threadIdx.x / LANES; rs < kBmTile; rs += kBlock / LANES) {
+    const int row = row0 + rs;
+    float sum = 0.f;
+    if (row < it.n_rows && !final_only)
+      for (int j = it.rowptr[row] + lane, j1 = it.rowptr[row + 1]; j < j1; j += 3 * LANES) {
+        const bool pb = j + LANES < j1, pc = j + 2 * LANES < j1;
+        const int ca = it.col[j], cb = pb ? it.col[j + LANES] : 0, cc = pc ? it.col[j + 2 * LANES] : 0;
+        const float va_ = it.val32[j], vb = pb ? it.val32[j + LANES] : 0.f, vc = pc ? it.val32[j + 2 * LANES] : 0.f;
+        const float za = zi[ca], zb = pb ? zi[cb] : 0.f, zc = pc ? zi[cc] : 0.f;
+        sum += va_ * za + vb * zb + vc * zc;
+      }
+    sum = group_sum<LANES>(sum);
+    if (lane == 0 && row < it.n_rows) {
+      const int64_t g = it.row_off + row;
+      if (!first) {
+        const float v = zprev[g] * sprev;
+        const float w = (v - oldeps * wa[g] - delta * wb[g]) * denom;
+        wnew[g] = w;
+        x[g] += phi * w;
+      }
+      if (!final_only) {
+        const float sg = it.sign ? it.sign[row] : 1.f;
+        const float yi = s * sg * sum - ratio * r1[g];
+        y[g] = yi;
+        acc += (double)(s * z[g]) * (double)yi;
+      }
+    }
+  }
+  return acc;
+}
+
+// A(k) of every instance in modes 0 (full) and 1 (final update only)
+__global__ __launch_bounds__(kBlock) void k_bm_a(const nfx_minres_batch_item* __restrict__ items,
+                                                 const int32_t* __restrict__ tiles, const int32_t* __restrict__ act,
+                                                 const int32_t* __restrict__ mode,
+                                                 const MinresState* __restrict__ st, int first,
+                                                 const float* __restrict__ z, const float* __restrict__ zprev,
+                                                 const float* __restrict__ r1, float* __restrict__ y,
+                                                 float* __restrict__ x, float* __restrict__ wnew,
+                                                 const float* __restrict__ wa, const float* __restrict__ wb,
+                                                 double* __restrict__ partial_a) {
+  __shared__ double sh[kBlock / 64];
+  const int t = act[blockIdx.x], inst = tiles[2 * t], row0 = tiles[2 * t + 1];
+  const int md = mode[inst];
+  double acc = 0;
+  if (md <= 1) {
+    const nfx_minres_batch_item it = items[inst];
+    const MinresState s0 = st[inst];
+    const bool final_only = md == 1;
+    const float s = (float)s0.s;
+    const float ratio = first ? 0.f : (float)s0.ratio;
+    const float sprev = s0.oldb > 0 ? (float)(1.0 / s0.oldb) : 0.f;
+    const float oldeps = (float)s0.oldeps, delta = (float)s0.delta, denom = (float)s0.denom, phi = (float)s0.phi;
+    // the lane count is uniform over the block (one instance per tile)
+    if (it.lanes == 16)
+      acc = bm_a_rows<16>(it, row0, first, final_only, s, ratio, sprev, oldeps, delta, denom, phi, z, zprev, r1, y, x,
+                          wnew, wa, wb);
+    else if (it.lanes == 8)
+      acc = bm_a_rows<8>(it, row0, first, final_only, s, ratio, sprev, oldeps, delta, denom, phi, z, zprev, r1, y, x,
+                         wnew, wa, wb);
+    else
+      acc = bm_a_rows<4>(it, row0, first, final_only, s, ratio, sprev, oldeps, delta, denom, phi, z, zprev, r1, y, x,
+                         wnew, wa, wb);
+  }
+  const double bs = block_sum(acc, sh);
+  if (threadIdx.x == 0) partial_a[t] = bs;
+}
+
+// B(k): y' = y - (alpha / beta) r2 (into ynew), z = P^-1 y', partials of y' . z
+__global__ __launch_bounds__(kBlock) void k_bm_b(const nfx_minres_batch_item* __restrict__ items,
+                                                 const int32_t* __restrict__ tiles, const int32_t* __restrict__ act,
+                                                 const int32_t* __restrict__ mode,
+                                                 const MinresState* __restrict__ st, const float* __restrict__ y,
+                                                 const float* __restrict__ r2, float* __restrict__ ynew,
+                                                 float* __restrict__ znew, double* __restrict__ partial_b) {
+  __shared__ double sh[kBlock / 64];
+  const int t = act[blockIdx.x], inst = tiles[2 * t], row0 = tiles[2 * t + 1];
+  double acc = 0;
+  if (mode[inst] == 0) {
+    const nfx_minres_batch_item it = items[inst];
+    const float c = (float)(st[inst].alfa / st[inst].beta);
+    const int row = row0 + threadIdx.x;
+    if (row < it.n_rows) {
+      const int64_t g = it.row_off + row;
+      const float yi = y[g] - c * r2[g];
+      ynew[g] = yi;
+      const float zi = it.pinv ? it.pinv[row] * yi : yi;
+      znew[g] = zi;
+      acc = (double)yi * (double)zi;
+    }
+  }
+  const double bs = block_sum(acc, sh);
+  if (threadIdx.x == 0) partial_b[t] = bs;
+}
+
+// One block per instance: the sum of its tile partials in tile order. what = 0: alpha of
+// A(k) (and mode 1 -> 2: the final update has been applied); 1: the scalar step that A(k+1)
+// reads (st_out from st_in); 2: nothing but the sum (sum_out).
+__global__ __launch_bounds__(kBlock) void k_bm_reduce(int n_inst, const int32_t* __restrict__ tile_ptr,
+                                                      const double* __restrict__ partial, int what, int first,
+                                                      int32_t* __restrict__ mode, const MinresState* __restrict__ st_in,
+                                                      MinresState* __restrict__ st_out, double* __restrict__ sum_out) {
+  __shared__ double sh[kBlock / 64];
+  const int inst = blockIdx.x;
+  if (inst >= n_inst) return;
+  double v = 0;
+  for (int t = tile_ptr[inst] + threadIdx.x; t < tile_ptr[inst + 1]; t += kBlock) v += partial[t];
+  v = block_sum(v, sh);
+  if (threadIdx.x != 0) return;
+  if (what == 2) {
+    sum_out[inst] = v;
+    return;
+  }
+  const int md = mode[inst];
+  if (what == 0) {
+    if (md == 0) st_out[inst].alfa = v;
+    if (md == 1) mode[inst] = 2;
+    return;
+  }
+  if (md <= 1) {
+    MinresState s;
+    minres_scalars(st_in[inst], s, sqrt(v > 0 ? v : 0), first);
+    st_out[inst] = s;
+  }
+}
+
+// FP64 residual rows of one tile (LANES lanes per row); rr / bb accumulate r.r and b.b
+template <int LANES>
+__device__ __forceinline__ void bm_res_rows(const nfx_minres_batch_item& it, int row0, const double* __restrict__ x,
+                                            const double* __restrict__ b, double* __restrict__ r, double& rr,
+                                            double& bb) {
+  const int lane = threadIdx.x % LANES;
+  const double* xi = x + it.row_off;
+  for (int rs = threadIdx.x / LANES; rs < kBmTile; rs += kBlock / LANES) {
+    const int row = row0 + rs;
+    double s = 0;
+    if (row < it.n_rows)
+      for (int j = it.rowptr[row] + lane; j < it.rowptr[row + 1]; j += LANES) s += it.val64[j] * xi[it.col[j]];
+#pragma unroll
+    for (int off = LANES / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LANES);
+    if (lane == 0 && row < it.n_rows) {
+      const int64_t g = it.row_off + row;
+      const double ri = b[g] - s;
+      r[g] = ri;
+      rr += ri * ri;
+      bb += b[g] * b[g];
+    }
+  }
+}
+
+// FP64 residual r = b - A x of every instance whose mode is 0, partials of r.r and b.b
+__global__ __launch_bounds__(kBlock) void k_bm_res64(const nfx_minres_batch_item* __restrict__ items,
+                                                     const int32_t* __restrict__ tiles,
+                                                     const int32_t* __restrict__ act,
+                                                     const int32_t* __restrict__ mode, const double* __restrict__ x,
+                                                     const double* __restrict__ b, double* __restrict__ r,
+                                                     double* __restrict__ prr, double* __restrict__ pbb) {
+  __shared__ double sh[kBlock / 64];
+  const int t = act[blockIdx.x], inst = tiles[2 * t], row0 = tiles[2 * t + 1];
+  double rr = 0, bb = 0;
+  if (mode[inst] == 0) {
+    const nfx_minres_batch_item it = items[inst];
+    if (it.lanes == 16)
+      bm_res_rows<16>(it, row0, x, b, r, rr, bb);
+    else if (it.lanes == 8)
+      bm_res_rows<8>(it, row0, x, b, r, rr, bb);
+    else
+      bm_res_rows<4>(it, row0, x, b, r, rr, bb);
+  }
+  const double a = block_sum(rr, sh);
+  const double c = block_sum(bb, sh);
+  if (threadIdx.x == 0) {
+    prr[t] = a;
+    pbb[t] = c;
+  }
+}
+
+// rhs = D r / scale (FP32) and, after the inner solve, x += scale dx (FP64), instances in
+// mode 0 only
+__global__ __launch_bounds__(kBlock) void k_bm_scale(const nfx_minres_batch_item* __restrict__ items,
+                                                     const int32_t* __restrict__ tiles,
+                                                     const int32_t* __restrict__ act,
+                                                     const int32_t* __restrict__ mode,
+                                                     const double* __restrict__ scale, int dir,
+                                                     const double* __restrict__ r, float* __restrict__ rhs,
+                                                     const float* __restrict__ dx, double* __restrict__ x) {
+  const int t = act[blockIdx.x], inst = tiles[2 * t], row0 = tiles[2 * t + 1];
+  if (mode[inst] != 0) return;
+  const nfx_minres_batch_item it = items[inst];
+  const int row = row0 + threadIdx.x;
+  if (row >= it.n_rows) return;
+  const int64_t g = it.row_off + row;
+  if (dir == 0) {
+    const double v = r[g] / scale[inst];
+    rhs[g] = (float)(it.sign ? it.sign[row] * v : v);
+  } else {
+    x[g] += scale[inst] * (double)dx[g];
+  }
+}
+
+}  // namespace nfx
+
+struct nfx_minres_batch {
+  int n_inst = 0, n_tiles = 0;
+  int64_t n_total = 0;
+  hipStream_t stream = nullptr;
+  nfx_minres_batch_item* items = nullptr;  // device copy
+  int32_t* tiles = nullptr;
+  int32_t* tile_ptr = nullptr;
+  int32_t* act = nullptr;    // tile ids the kernels run over
+  int32_t* h_act = nullptr;  // pinned staging of act
+  std::vector<int32_t> h_tile_ptr;
+  int n_act = 0;
+  int32_t* mode = nullptr;
+  float* buf = nullptr;  // 10 vectors of n_total floats
+  double* r64 = nullptr;
+  double* pa = nullptr;  // n_tiles partials (alpha / r.r)
+  double* pb = nullptr;  // n_tiles partials (y'.z / b.b)
+  double* sums = nullptr;   // 2 n_inst
+  double* scale = nullptr;  // n_inst
+  MinresState* st = nullptr;  // 2 n_inst
+  MinresState* h_st = nullptr;
+  int32_t* h_mode = nullptr;
+  double* h_sums = nullptr;
+  double* h_scale = nullptr;
+};
+
+static void bm_free(nfx_minres_batch* s) {
+  if (!s) return;
+  (void)hipFree(s->items);
+  (void)hipFree(s->tiles);
+  (void)hipFree(s->tile_ptr);
+  (void)hipFree(s->act);
+  (void)hipHostFree(s->h_act);
+  (void)hipFree(s->mode);
+  (void)hipFree(s->buf);
+  (void)hipFree(s->r64);
+  (void)hipFree(s->pa);
+  (void)hipFree(s->pb);
+  (void)hipFree(s->sums);
+  (void)hipFree(s->scale);
+  (void)hipFree(s->st);
+  (void)hipHostFree(s->h_st);
+  (void)hipHostFree(s->h_mode);
+  (void)hipHostFree(s->h_sums);
+  (void)hipHostFree(s->h_scale);
+  delete s;
+}
+
+extern "C" int nfx_minres_batch_create(nfx_minres_batch** out, int n_inst, const nfx_minres_batch_item* h_items,
+                                       void* stream) {
+  NFX_REQUIRE(out && n_inst > 0 && h_items, "nfx_minres_batch_create: bad arguments");
+  std::vector<nfx_minres_batch_item> items(h_items, h_items + n_inst);
+  std::vector<int32_t> tiles, tile_ptr(1, 0);
+  int64_t off = 0;
+  for (int i = 0; i < n_inst; ++i) {
+    auto& it = items[i];
+    NFX_REQUIRE(it.n_rows > 0 && it.rowptr && it.col && it.val32 && it.val64,
+                "nfx_minres_batch_create: incomplete instance");
+    it.row_off = off;
+    off += it.n_rows;
+    // 4 lanes per row for every instance: on a batch of 32 RT1 / RT2 instances (4..16 cubes)
+    // 8 and 16 lanes for the longer RT2 rows (the single-system choice) made the batched solve
+    // 1.5 x slower (a tile's rows then take several passes, each ending in a one-lane update)
+    it.lanes = 4;
+    for (int r = 0; r < it.n_rows; r += kBmTile) {
+      tiles.push_back(i);
+      tiles.push_back(r);
+    }
+    NFX_REQUIRE(tiles.size() / 2 < (size_t)INT32_MAX, "nfx_minres_batch_create: too many tiles");
+    tile_ptr.push_back((int32_t)(tiles.size() / 2));
+  }
+  nfx_minres_batch* s = new nfx_minres_batch();
+  s->n_inst = n_inst;
+  s->n_tiles = (int)(tiles.size() / 2);
+  s->n_total = off;
+  s->stream = static_cast<hipStream_t>(stream);
+  hipError_t e = hipMalloc(&s->items, sizeof(nfx_minres_batch_item) * n_inst);
+  if (e == hipSuccess) e = hipMalloc(&s->tiles, sizeof(int32_t) * tiles.size());
+  if (e == hipSuccess) e = hipMalloc(&s->tile_ptr, sizeof(int32_t) * tile_ptr.size());
+  if (e == hipSuccess) e = hipMalloc(&s->mode, sizeof(int32_t) * n_inst);
+  if (e == hipSuccess) e = hipMalloc(&s->act, sizeof(int32_t) * s->n_tiles);
+  if (e == hipSuccess) e = hipHostMalloc(&s->h_act, sizeof(int32_t) * s->n_tiles, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMalloc(&s->buf, sizeof(float) * (size_t)off * 10);
+  if (e == hipSuccess) e = hipMalloc(&s->r64, sizeof(double) * (size_t)off);
+  if (e == hipSuccess) e = hipMalloc(&s->pa, sizeof(double) * s->n_tiles);
+  if (e == hipSuccess) e = hipMalloc(&s->pb, sizeof(double) * s->n_tiles);
+  if (e == hipSuccess) e = hipMalloc(&s->sums, sizeof(double) * 2 * n_inst);
+  if (e == hipSuccess) e = hipMalloc(&s->scale, sizeof(double) * n_inst);
+  if (e == hipSuccess) e = hipMalloc(&s->st, sizeof(MinresState) * 2 * n_inst);
+  if (e == hipSuccess) e = hipHostMalloc(&s->h_st, sizeof(MinresState) * n_inst, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostMalloc(&s->h_mode, sizeof(int32_t) * n_inst, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostMalloc(&s->h_sums, sizeof(double) * 2 * n_inst, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostMalloc(&s->h_scale, sizeof(double) * n_inst, hipHostMallocDefault);
+  if (e == hipSuccess)
+    e = hipMemcpy(s->items, items.data(), sizeof(nfx_minres_batch_item) * n_inst, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(s->tiles, tiles.data(), sizeof(int32_t) * tiles.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(s->tile_ptr, tile_ptr.data(), sizeof(int32_t) * tile_ptr.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    set_error(std::string("nfx_minres_batch_create: ") + hipGetErrorString(e));
+    bm_free(s);
+    return NFX_E_HIP;
+  }
+  s->h_tile_ptr = tile_ptr;
+  *out = s;
+  return NFX_OK;
+}
+
+// act = the tiles of the instances whose host mode is < limit (uploaded; the caller's stream
+// must have finished with the previous list: it is synchronised before every rebuild)
+static int bm_set_active(nfx_minres_batch* s, int limit) {
+  NFX_CHECK_HIP(hipStreamSynchronize(s->stream));  // no copy from h_act is pending
+  int n = 0;
+  for (int i = 0; i < s->n_inst; ++i)
+    if (s->h_mode[i] < limit)
+      for (int t = s->h_tile_ptr[i]; t < s->h_tile_ptr[i + 1]; ++t) s->h_act[n++] = t;
+  s->n_act = n;
+  if (n) NFX_CHECK_HIP(hipMemcpyAsync(s->act, s->h_act, sizeof(int32_t) * n, hipMemcpyHostToDevice, s->stream));
+  return NFX_OK;
+}
+
+extern "C" int nfx_minres_batch_destroy(nfx_minres_batch* s) {
+  bm_free(s);
+  return NFX_OK;
+}
+
+extern "C" int64_t nfx_minres_batch_rows(const nfx_minres_batch* s) { return s ? s->n_total : -1; }
+
+// One batched FP32 MINRES on every instance whose h_mode is 0 on entry (rhs already scaled);
+// iters[i] gets the instance's iteration count.
+static int bm_inner(nfx_minres_batch* s, const float* rhs, float* x, double tol, int maxit, int* iters) {
+  const int64_t N = s->n_total;
+  float* V = s->buf;
+  float* P[2] = {V + N * 0, V + N * 1};
+  float* F = V + N * 2;
+  float* Z[2] = {V + N * 3, V + N * 4};
+  float* W[3] = {V + N * 5, V + N * 6, V + N * 7};
+  MinresState* S[2] = {s->st, s->st + s->n_inst};
+  const int m = s->n_inst;
+  hipStream_t st = s->stream;
+  NFX_CHECK_HIP(hipMemcpyAsync(s->mode, s->h_mode, sizeof(int32_t) * m, hipMemcpyHostToDevice, st));
+  int rc = bm_set_active(s, 1);
+  if (rc) return rc;
+  if (s->n_act)
+    hipLaunchKernelGGL(k_bm_init, dim3(s->n_act), dim3(kBmTile), 0, st, s->items, s->tiles, s->act, s->mode, rhs,
+                       P[0], P[1], Z[1], x, W[0], W[1], W[2], s->pb);
+  hipLaunchKernelGGL(k_bm_reduce, dim3(m), dim3(kBlock), 0, st, m, s->tile_ptr, s->pb, 1, 1, s->mode, S[0], S[1],
+                     nullptr);
+  NFX_LAUNCH_CHECK();
+  std::vector<int> active;
+  for (int i = 0; i < m; ++i) {
+    iters[i] = 0;
+    if (s->h_mode[i] == 0) active.push_back(i);
+  }
+  int k = 1;
+  bool pending_final = false;
+  for (; !active.empty() || pending_final; ++k) {
+    float* r1 = P[(k + 1) % 2];
+    float* r2 = P[k % 2];
+    float* z = Z[k % 2];
+    float* zprev = Z[(k + 1) % 2];
+    float* wnew = W[(k + 2) % 3];
+    const float* wa = W[k % 3];
+    const float* wb = W[(k + 1) % 3];
+    hipLaunchKernelGGL(k_bm_a, dim3(s->n_act), dim3(kBlock), 0, st, s->items, s->tiles, s->act, s->mode, S[k & 1],
+                       k == 1, z, zprev, r1, F, x, wnew, wa, wb, s->pa);
+    hipLaunchKernelGGL(k_bm_reduce, dim3(m), dim3(kBlock), 0, st, m, s->tile_ptr, s->pa, 0, 0, s->mode, nullptr,
+                       S[k & 1], nullptr);
+    pending_final = false;
+    if (active.empty()) break;  // the last final updates have been applied
+    hipLaunchKernelGGL(k_bm_b, dim3(s->n_act), dim3(kBmTile), 0, st, s->items, s->tiles, s->act, s->mode, S[k & 1],
+                       F, r2, r1, zprev, s->pb);
+    hipLaunchKernelGGL(k_bm_reduce, dim3(m), dim3(kBlock), 0, st, m, s->tile_ptr, s->pb, 1, 0, s->mode, S[k & 1],
+                       S[(k + 1) & 1], nullptr);
+    NFX_LAUNCH_CHECK();
+    if (k % kBmCheck == 0 || k >= maxit) {
+      // the state A(k + 1) will read (its phibar is the estimate after iteration k)
+      NFX_CHECK_HIP(
+          hipMemcpyAsync(s->h_st, S[(k + 1) & 1], sizeof(MinresState) * m, hipMemcpyDeviceToHost, st));
+      NFX_CHECK_HIP(hipMemcpyAsync(s->h_mode, s->mode, sizeof(int32_t) * m, hipMemcpyDeviceToHost, st));
+      NFX_CHECK_HIP(hipStreamSynchronize(st));
+      std::vector<int> still;
+      bool changed = false;
+      for (int i : active) {
+        const MinresState& q = s->h_st[i];
+        const bool done = q.stop || !(q.beta1 > 0) || q.phibar <= tol * q.beta1 || k >= maxit;
+        if (done) {
+          s->h_mode[i] = 1;
+          iters[i] = k;
+          changed = true;
+        } else {
+          still.push_back(i);
+        }
+      }
+      active.swap(still);
+      if (changed) {
+        NFX_CHECK_HIP(hipMemcpyAsync(s->mode, s->h_mode, sizeof(int32_t) * m, hipMemcpyHostToDevice, st));
+        pending_final = true;
+        // tiles of instances still iterating or with their final update pending
+        rc = bm_set_active(s, 2);
+        if (rc) return rc;
+      }
+    }
+  }
+  // the host mode array is rewritten next: wait for every copy from it
+  NFX_CHECK_HIP(hipStreamSynchronize(st));
+  return NFX_OK;
+}
+
+extern "C" int nfx_solve_minres_batch_ir(nfx_minres_batch* s, const double* d_b, double* d_x, double rtol,
+                                         double inner_rtol, int max_inner, int max_outer, int* h_iters,
+                                         int* h_outer, double* h_resid) {
+  NFX_REQUIRE(s && d_b && d_x && h_iters && h_outer && h_resid, "nfx_solve_minres_batch_ir: null pointer");
+  NFX_REQUIRE(max_inner > 0 && max_outer >= 0, "nfx_solve_minres_batch_ir: iteration limits");
+  const int m = s->n_inst;
+  const int64_t N = s->n_total;
+  hipStream_t st = s->stream;
+  float* rf = s->buf + N * 8;
+  float* dx = s->buf + N * 9;
+  std::vector<int> outer_active(m, 1), it(m);
+  for (int i = 0; i < m; ++i) {
+    h_iters[i] = 0;
+    h_outer[i] = 0;
+    h_resid[i] = 0;
+  }
+  for (int outer = 0; outer <= max_outer; ++outer) {
+    int n_act = 0;
+    for (int i = 0; i < m; ++i) {
+      s->h_mode[i] = outer_active[i] ? 0 : 2;
+      n_act += outer_active[i];
+    }
+    if (!n_act) break;
+    NFX_CHECK_HIP(hipMemcpyAsync(s->mode, s->h_mode, sizeof(int32_t) * m, hipMemcpyHostToDevice, st));
+    int rc = bm_set_active(s, 1);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_bm_res64, dim3(s->n_act), dim3(kBlock), 0, st, s->items, s->tiles, s->act, s->mode, d_x, d_b,
+                       s->r64, s->pa, s->pb);
+    hipLaunchKernelGGL(k_bm_reduce, dim3(m), dim3(kBlock), 0, st, m, s->tile_ptr, s->pa, 2, 0, s->mode, nullptr,
+                       nullptr, s->sums);
+    hipLaunchKernelGGL(k_bm_reduce, dim3(m), dim3(kBlock), 0, st, m, s->tile_ptr, s->pb, 2, 0, s->mode, nullptr,
+                       nullptr, s->sums + m);
+    NFX_LAUNCH_CHECK();
+    NFX_CHECK_HIP(hipMemcpyAsync(s->h_sums, s->sums, sizeof(double) * 2 * m, hipMemcpyDeviceToHost, st));
+    NFX_CHECK_HIP(hipStreamSynchronize(st));
+    n_act = 0;
+    for (int i = 0; i < m; ++i) {
+      if (!outer_active[i]) continue;
+      const double rn = sqrt(s->h_sums[i]), bn = sqrt(s->h_sums[m + i]);
+      const double rel = bn > 0 ? rn / bn : rn;
+      h_resid[i] = rel;
+      h_outer[i] = outer;
+      s->h_scale[i] = rn;
+      if (!(rel == rel) || rel <= rtol || outer == max_outer || rn == 0) {
+        outer_active[i] = 0;
+        s->h_mode[i] = 2;
+      } else {
+        s->h_mode[i] = 0;
+        ++n_act;
+      }
+    }
+    if (!n_act) break;
+    NFX_CHECK_HIP(hipMemcpyAsync(s->scale, s->h_scale, sizeof(double) * m, hipMemcpyHostToDevice, st));
+    NFX_CHECK_HIP(hipMemcpyAsync(s->mode, s->h_mode, sizeof(int32_t) * m, hipMemcpyHostToDevice, st));
+    rc = bm_set_active(s, 1);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_bm_scale, dim3(s->n_act), dim3(kBmTile), 0, st, s->items, s->tiles, s->act, s->mode, s->scale,
+                       0, s->r64, rf, nullptr, nullptr);
+    NFX_LAUNCH_CHECK();
+    // the inner solve changes h_mode: keep the outer set for the update that follows
+    std::vector<int32_t> mode_outer(s->h_mode, s->h_mode + m);
+    rc = bm_inner(s, rf, dx, inner_rtol, max_inner, it.data());
+    if (rc) return rc;
+    for (int i = 0; i < m; ++i) {
+      h_iters[i] += it[i];
+      s->h_mode[i] = mode_outer[i];
+    }
+    NFX_CHECK_HIP(hipMemcpyAsync(s->mode, s->h_mode, sizeof(int32_t) * m, hipMemcpyHostToDevice, st));
+    rc = bm_set_active(s, 1);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_bm_scale, dim3(s->n_act), dim3(kBmTile), 0, st, s->items, s->tiles, s->act, s->mode, s->scale,
+                       1, nullptr, nullptr, dx, d_x);
+    NFX_LAUNCH_CHECK();
+  }
+  NFX_CHECK_HIP(hipStreamSynchronize(st));
+  for (int i = 0; i < m; ++i)
+    if (!(h_resid[i] <= rtol)) return NFX_E_NOT_CONVERGED;
+  return NFX_OK;
+}

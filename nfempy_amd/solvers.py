@@ -102,6 +102,67 @@ class MinresIR:
         return x, out
 
 
+class MinresBatch:
+    """Batched FP32 MINRES + FP64 refinement over independent systems (``nfx_minres_batch``):
+    one set of launches per iteration for all instances, each with its own recurrence,
+    stopping test and refinement. ``systems``: dicts with rowptr, col, vals32, vals64 and
+    optional sign / pinv (the symmetric form D A and a diagonal preconditioner)."""
+
+    def __init__(self, systems, stream=None):
+        self._systems = list(systems)
+        m = len(self._systems)
+        arr = (nat.MinresBatchItem * m)()
+        self.sizes = []
+        for i, sy in enumerate(self._systems):
+            n = int(sy["rowptr"].numel() - 1)
+            for key in ("vals32", "vals64"):
+                if int(sy[key].numel()) != int(sy["col"].numel()):
+                    raise ValueError(f"instance {i}: {key} size differs from the pattern")
+            for key in ("sign", "pinv"):
+                if sy.get(key) is not None and int(sy[key].numel()) != n:
+                    raise ValueError(f"instance {i}: {key} size differs from the rows")
+            arr[i] = nat.MinresBatchItem(n, 0, 0, nat.ptr(sy["rowptr"]), nat.ptr(sy["col"]), nat.ptr(sy["vals32"]),
+                                         nat.ptr(sy["vals64"]), nat.ptr(sy.get("sign")), nat.ptr(sy.get("pinv")))
+            self.sizes.append(n)
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
+        h = ctypes.c_void_p()
+        nat.call("nfx_minres_batch_create", ctypes.byref(h), m, arr, nat.stream_handle(stream))
+        self._h = h
+        if nat.lib().nfx_minres_batch_rows(h) != self.offsets[-1]:
+            raise RuntimeError("batch row count differs between host and library")
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            nat.lib().nfx_minres_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve(self, b, x=None, rtol=1e-8, inner_rtol=1e-5, max_inner=100000, max_outer=30):
+        """b: concatenated FP64 right-hand sides (instance order). Returns (x, infos)."""
+        t = torch()
+        b = b.to(dtype=t.float64).contiguous()
+        if int(b.numel()) != int(self.offsets[-1]):
+            raise ValueError("right-hand side length differs from the batch")
+        if x is None:
+            x = t.zeros_like(b)
+        m = len(self.sizes)
+        iters = (ctypes.c_int * m)()
+        outer = (ctypes.c_int * m)()
+        resid = (ctypes.c_double * m)()
+        rc = nat.lib().nfx_solve_minres_batch_ir(self._h, nat.ptr(b), nat.ptr(x), rtol, inner_rtol, max_inner,
+                                                 max_outer, iters, outer, resid)
+        if rc not in (nat.NFX_OK, nat.NFX_E_NOT_CONVERGED):
+            nat.check(rc, "nfx_solve_minres_batch_ir")
+        infos = [{"iterations": iters[i], "outer": outer[i], "residual": resid[i], "converged": resid[i] <= rtol}
+                 for i in range(m)]
+        return x, infos
+
+
 def darcy_preconditioner(rowptr, col, vals64, n, n_u, stream=None):
     """(sign, pinv) device FP32 vectors of the Darcy block preconditioner."""
     t = torch()
