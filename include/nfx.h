@@ -264,10 +264,10 @@ int nfx_elasticity_mass_schur_pc(int n, int n_s, int n_u, const int32_t* d_rowpt
 
 /* Stress Dirichlet term of LEDualWeakFormBCDirichlet.evaluate_form
  * (src/weak_forms/le_dual_weak_form.py:514-578):
- * r[(a, c)] = -sum_q w |detJ_f| u_D,c(x_q) psi_a(x_q) . n for the 3 BDM1 DoFs of
- * each boundary face; d_uD (F, 4, 3); d_rows (F, 9) row codes (entry a*3 + c).
- * Other arguments as nfx_assemble_darcy_bc. */
-int nfx_assemble_elasticity_bc(int n_facets, const int32_t* d_facet,
+ * r[(a, c)] = -sum_q w |detJ_f| u_D,c(x_q) psi_a(x_q) . n for the dim BDM1 DoFs of each
+ * boundary facet; dim 3: d_uD (F, 4, 3), d_rows (F, 9); dim 2: d_uD (F, 2, 2), d_rows (F, 4)
+ * row codes (entry a * dim + c). Other arguments as nfx_assemble_darcy_bc. */
+int nfx_assemble_elasticity_bc(int dim, int n_facets, const int32_t* d_facet,
                                const int32_t* d_neigh, const int32_t* d_combo,
                                const int32_t* d_cells, const void* d_coords,
                                const void* d_trace, const void* d_fw, int nqf,

@@ -69,7 +69,7 @@ SIGNATURES = {
                                               c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
                                               c_int, c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_int,
                                               c_void_p, c_void_p, c_void_p]),
-    "nfx_assemble_elasticity_bc": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+    "nfx_assemble_elasticity_bc": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                            c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     "nfx_gather": (c_int, [c_int, c_i64, c_void_p, c_void_p, c_void_p, c_void_p]),
     "nfx_stage_reduce": (c_int, [c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),

@@ -1289,19 +1289,25 @@ extern "C" int nfx_element_geometry(int dim, int n_elem, const int32_t* d_cells,
   return NFX_OK;
 }
 
-extern "C" int nfx_assemble_elasticity_bc(int n_facets, const int32_t* d_facet,
+extern "C" int nfx_assemble_elasticity_bc(int dim, int n_facets, const int32_t* d_facet,
                                           const int32_t* d_neigh, const int32_t* d_combo,
                                           const int32_t* d_cells, const void* d_coords,
                                           const void* d_trace, const void* d_fw, int nqf,
                                           const void* d_uD, const int32_t* d_rows, int fp64,
                                           void* d_rhs, void* stream) {
+  NFX_REQUIRE(dim == 2 || dim == 3, "nfx_assemble_elasticity_bc: dim must be 2 or 3");
   NFX_REQUIRE(n_facets >= 0, "nfx_assemble_elasticity_bc: n_facets < 0");
   if (n_facets == 0) return NFX_OK;
-  NFX_REQUIRE(nqf == 4, "nfx_assemble_elasticity_bc: facet rule must have 4 points");
+  NFX_REQUIRE(nqf == (dim == 3 ? 4 : 2), "nfx_assemble_elasticity_bc: facet rule must have 4 (dim 3) / 2 (dim 2) points");
   NFX_REQUIRE(d_facet && d_neigh && d_combo && d_cells && d_coords && d_trace && d_fw && d_uD &&
                   d_rows && d_rhs,
               "nfx_assemble_elasticity_bc: null pointer");
   hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dim == 2)
+    return fp64 ? launch_darcy_bc<2, 2, 2, double, 2>(n_facets, d_facet, d_neigh, d_combo, d_cells, d_coords,
+                                                      d_trace, d_fw, d_uD, d_rows, d_rhs, s, -1.0)
+                : launch_darcy_bc<2, 2, 2, float, 2>(n_facets, d_facet, d_neigh, d_combo, d_cells, d_coords,
+                                                     d_trace, d_fw, d_uD, d_rows, d_rhs, s, -1.0);
   return fp64 ? launch_darcy_bc<3, 3, 4, double, 3>(n_facets, d_facet, d_neigh, d_combo, d_cells,
                                                     d_coords, d_trace, d_fw, d_uD, d_rows, d_rhs,
                                                     s, -1.0)

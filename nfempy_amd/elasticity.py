@@ -107,77 +107,96 @@ def _sbaij_mask(vals, ref_dest, ref_sign, upper=True):
 FACE_NO_SRC = 63
 
 
-def face_owner_layout(n, ref_dest, ref_sign):
-    """Per-face and per-element data of the face-owner kernel (nfx_assemble_elasticity_faces).
+def el_sizes(nl):
+    """BDM1 - P0 - P0 sizes from the local DoF count (42: tetrahedra, 15: triangles): basis
+    functions NB, stress rows / displacement components NC, rotation components NT, BDM1
+    DoFs per facet NBF, facets per element NFC, rows of one facet block ROWS = NBF NC."""
+    if nl == 42:
+        d = dict(dim=3, NB=12, NC=3, NT=3, NBF=3, NFC=4)
+    elif nl == 15:
+        d = dict(dim=2, NB=6, NC=2, NT=1, NBF=2, NFC=3)
+    else:
+        raise NotImplementedError(f"BDM1 - P0 - P0: unexpected local size {nl}")
+    d["ROWS"] = d["NBF"] * d["NC"]
+    d["NSS"] = d["NB"] * d["NC"]
+    d["NL"] = nl
+    return d
 
-    The stress DoFs come in blocks of 9 per face (3 BDM1 DoFs x 3 rows, ``j * 3 + c``), the
-    displacement and rotation DoFs 3 per cell (``n_s + 3 e + c``, ``n_s + n_u + 3 e + c``).
+
+def face_owner_layout(n, ref_dest, ref_sign):
+    """Per-facet and per-element data of the facet-owner kernel (nfx_assemble_elasticity_faces),
+    tetrahedra (faces) or triangles (edges).
+
+    The stress DoFs come in blocks of ROWS per facet (NBF BDM1 DoFs x NC rows, ``j * NC + c``),
+    the displacement and rotation DoFs per cell (``n_s + NC e + c``, ``n_s + n_u + NT e + c``).
     Returns a dict with face_inc (n_f, 2) (incident elements in element order, -1 none),
-    face_lf (n_f,) (local face index of each incidence, 2 bits each), el_face (E, 4) (the
-    DoF block of each local face) and el_orient (E,) uint64 (bits 2 b, 2 b + 1: face slot of
+    face_lf (n_f,) (local facet index of each incidence, 2 bits each), el_face (E, NFC) (the
+    DoF block of each local facet) and el_orient (E,) uint64 (bits 2 b, 2 b + 1: facet slot of
     reference basis b; bit 32 + b: its orientation sign), or raises NotImplementedError
     when the DoF layout is not this one."""
     E = ref_dest.shape[0]
-    if ref_dest.shape[1] != NL:
-        raise NotImplementedError("face-owner layout: 42 local DoFs expected")
-    sd = ref_dest[:, :36].reshape(E, 12, 3).astype(np.int64)
-    if np.any(sd != sd[:, :, :1] + np.arange(3)):
+    z = el_sizes(ref_dest.shape[1])
+    NB, NC, NT, NBF, NFC, ROWS, NSS = z["NB"], z["NC"], z["NT"], z["NBF"], z["NFC"], z["ROWS"], z["NSS"]
+    sd = ref_dest[:, :NSS].reshape(E, NB, NC).astype(np.int64)
+    if np.any(sd != sd[:, :, :1] + np.arange(NC)):
         raise NotImplementedError("face-owner layout: stress components must be consecutive")
     base = sd[:, :, 0]
-    if np.any(base % 3):
+    if np.any(base % NC):
         raise NotImplementedError("face-owner layout: stress DoF blocks")
-    block, slot = base // 9, (base % 9) // 3
-    el_face = block[:, ::3]
-    if np.any(block != np.repeat(el_face, 3, axis=1)):
-        raise NotImplementedError("face-owner layout: a face's BDM1 DoFs must share one DoF block")
-    if np.any(np.sort(slot.reshape(E, 4, 3), axis=2) != np.arange(3)):
-        raise NotImplementedError("face-owner layout: face slots")
-    sg = ref_sign[:, :36].reshape(E, 12, 3)
+    block, slot = base // ROWS, (base % ROWS) // NC
+    el_face = block[:, ::NBF]
+    if np.any(block != np.repeat(el_face, NBF, axis=1)):
+        raise NotImplementedError("face-owner layout: a facet's BDM1 DoFs must share one DoF block")
+    if np.any(np.sort(slot.reshape(E, NFC, NBF), axis=2) != np.arange(NBF)):
+        raise NotImplementedError("face-owner layout: facet slots")
+    sg = ref_sign[:, :NSS].reshape(E, NB, NC)
     if np.any(sg != sg[:, :, :1]):
         raise NotImplementedError("face-owner layout: signs must not depend on the stress row")
     neg = (sg[:, :, 0] < 0).astype(np.uint64)
-    b = np.arange(12, dtype=np.uint64)[None, :]
+    b = np.arange(NB, dtype=np.uint64)[None, :]
     orient = ((slot.astype(np.uint64) << (np.uint64(2) * b)) | (neg << (np.uint64(32) + b))).sum(axis=1,
                                                                                                 dtype=np.uint64)
     n_f = int(el_face.max()) + 1 if E else 0
-    n_s = 9 * n_f
-    n_u = 3 * E
-    if n != n_s + 2 * n_u:
-        raise NotImplementedError("face-owner layout: n != 9 n_faces + 6 n_cells")
+    n_s = ROWS * n_f
+    n_u = NC * E
+    if n != n_s + n_u + NT * E:
+        raise NotImplementedError("face-owner layout: n != ROWS n_facets + (NC + NT) n_cells")
     e_idx = np.arange(E)[:, None]
-    if np.any(ref_dest[:, 36:39] != n_s + 3 * e_idx + np.arange(3)) or \
-            np.any(ref_dest[:, 39:42] != n_s + n_u + 3 * e_idx + np.arange(3)):
+    if np.any(ref_dest[:, NSS:NSS + NC] != n_s + NC * e_idx + np.arange(NC)) or \
+            np.any(ref_dest[:, NSS + NC:] != n_s + n_u + NT * e_idx + np.arange(NT)):
         raise NotImplementedError("face-owner layout: cell DoFs must follow the element order")
     flat = el_face.reshape(-1)
-    order = np.argsort(flat, kind="stable")  # element order inside each face
+    order = np.argsort(flat, kind="stable")  # element order inside each facet
     counts = np.bincount(flat, minlength=n_f)
     if counts.min(initial=1) < 1 or counts.max(initial=1) > 2:
-        raise NotImplementedError("face-owner layout: every face needs 1 or 2 incident elements")
+        raise NotImplementedError("face-owner layout: every facet needs 1 or 2 incident elements")
     first = np.concatenate([[0], np.cumsum(counts)[:-1]])
     face_inc = np.full((n_f, 2), -1, np.int64)
     lf = np.zeros((n_f, 2), np.int64)
-    face_inc[:, 0] = order[first] // 4
-    lf[:, 0] = order[first] % 4
+    face_inc[:, 0] = order[first] // NFC
+    lf[:, 0] = order[first] % NFC
     two = counts == 2
-    face_inc[two, 1] = order[first[two] + 1] // 4
-    lf[two, 1] = order[first[two] + 1] % 4
+    face_inc[two, 1] = order[first[two] + 1] // NFC
+    lf[two, 1] = order[first[two] + 1] % NFC
     return {"n_faces": n_f, "n_s": n_s, "n_u": n_u, "face_inc": face_inc, "face_lf": lf[:, 0] | (lf[:, 1] << 2),
-            "el_face": el_face, "el_orient": orient}
+            "el_face": el_face, "el_orient": orient, "rows": ROWS}
 
 
 def face_owner_codes(n, lay, ref_dest, keep, pattern, zero_diag):
     """16-bit source code of every CSR entry of the stress rows (bits 0..3: row inside the
-    face, 3 j + c; bits 4..9 / 10..15: element-local reference column of the entry in the
-    face's first / second incident element, 63 = none), after checking that every cell row
-    holds exactly its diagonal (its only entry in both SBAIJ operators)."""
+    facet block, j NC + c; bits 4..9 / 10..15: element-local reference column of the entry in
+    the facet's first / second incident element, 63 = none), after checking that every cell
+    row holds exactly its diagonal (its only entry in both SBAIJ operators)."""
     E = ref_dest.shape[0]
+    z = el_sizes(ref_dest.shape[1])
+    nl, ROWS, NSS = z["NL"], z["ROWS"], z["NSS"]
     n_s = lay["n_s"]
     rowptr = pattern.rowptr.astype(np.int64)
     col = pattern.col.astype(np.int64)
     if np.any(np.diff(rowptr[n_s:]) != 1) or np.any(col[rowptr[n_s:n]] != np.arange(n_s, n)):
         raise NotImplementedError("face-owner layout: cell rows must hold only their diagonal")
-    rows = np.repeat(ref_dest[:, :, None], NL, axis=2)
-    cols = np.repeat(ref_dest[:, None, :], NL, axis=1)
+    rows = np.repeat(ref_dest[:, :, None], nl, axis=2)
+    cols = np.repeat(ref_dest[:, None, :], nl, axis=1)
     src = keep & (rows <= cols) & (rows < n_s)
     e, I, Jl = np.nonzero(src)
     R = ref_dest[e, I].astype(np.int64)
@@ -186,7 +205,7 @@ def face_owner_codes(n, lay, ref_dest, keep, pattern, zero_diag):
     k = np.searchsorted(keys, R * n + C)
     if np.any(k >= len(keys)) or np.any(keys[np.minimum(k, len(keys) - 1)] != R * n + C):
         raise ValueError("face-owner codes: an element entry is missing from the pattern")
-    p = R // 9
+    p = R // ROWS
     inc = (lay["face_inc"][p, 1] == e).astype(np.int64)
     if np.any((inc == 0) & (lay["face_inc"][p, 0] != e)):
         raise ValueError("face-owner codes: an element entry of a face row from a non-incident element")
@@ -200,12 +219,12 @@ def face_owner_codes(n, lay, ref_dest, keep, pattern, zero_diag):
     none = (srcs[:, 0] == FACE_NO_SRC) & (srcs[:, 1] == FACE_NO_SRC)
     if np.any(none & (row_of != col[:ns_entries])) or (not zero_diag and np.any(none)):
         raise ValueError("face-owner codes: an entry without a contribution that is not an explicit zero diagonal")
-    # the kernel evaluates the stress columns of faces whose block is >= the row's face
-    sc = srcs < 36
-    blk_col = np.where(sc, col[:ns_entries, None] // 9, 1 << 40)
-    if np.any(blk_col < (row_of // 9)[:, None]):
+    # the kernel evaluates the stress columns of facets whose block is >= the row's facet
+    sc = srcs < NSS
+    blk_col = np.where(sc, col[:ns_entries, None] // ROWS, 1 << 40)
+    if np.any(blk_col < (row_of // ROWS)[:, None]):
         raise ValueError("face-owner codes: a stress column below the row's face block")
-    code = (row_of % 9) | (srcs[:, 0] << 4) | (srcs[:, 1] << 10)
+    code = (row_of % ROWS) | (srcs[:, 0] << 4) | (srcs[:, 1] << 10)
     return code.astype(np.uint16)
 
 
@@ -506,7 +525,7 @@ class ElasticitySystem:
             raise ValueError("u_D must give 3 components per facet point")
         rg = out if out is not None else (self.rhs64 if fp64 else self.rhs32)
         g = (uD if fp64 else uD.to(torch().float32)).contiguous()
-        nat.call("nfx_assemble_elasticity_bc", self.F, nat.ptr(self.bc_facet), nat.ptr(self.bc_neigh),
+        nat.call("nfx_assemble_elasticity_bc", 3, self.F, nat.ptr(self.bc_facet), nat.ptr(self.bc_neigh),
                  nat.ptr(self.bc_combo), nat.ptr(self.cells), nat.ptr(self.coords64 if fp64 else self.coords),
                  nat.ptr(self.bc_trace64 if fp64 else self.bc_trace32), nat.ptr(self.bc_fw64 if fp64 else self.bc_fw32),
                  self.nqf, nat.ptr(g), nat.ptr(self.bc_rows if rows is None else rows), int(fp64),
@@ -827,3 +846,352 @@ class ElasticityTwoGrid:
         z = out if out is not None else torch().empty_like(r)
         nat.call("nfx_elasticity_tg_apply", ctypes.byref(self._desc), nat.ptr(r), nat.ptr(z), nat.stream_handle())
         return z
+
+
+def _local_matrices_host(el, points, weights, X, lam_q, mu_q, mode):
+    """(E, NL, NL) FP64 element matrices in the reference basis order with the facet-owner
+    kernel's formulas (the cold pass of ``ElasticitySystem2D``: the value filter and the
+    patterns are decided on them). X (E, dim + 1, dim) vertices, lam_q / mu_q (E, nq)."""
+    dim = X.shape[2]
+    tab = elasticity_tables(el, points, weights)
+    nq, NB = len(weights), el.tabulate(points).shape[2]
+    z = el_sizes(NB * dim + dim + (3 if dim == 3 else 1))
+    NC, NT, NSS, NL_ = z["NC"], z["NT"], z["NSS"], z["NL"]
+    o = nq
+    phi = tab[o:o + nq * NB * dim].reshape(nq, NB, dim)
+    o += nq * NB * dim + nq * NB
+    uref = tab[o:o + NB]
+    Phi = tab[o + NB:o + NB + NB * dim].reshape(NB, dim)
+    ddref = tab[o + NB + NB * dim:o + NB + NB * dim + NB * NB].reshape(NB, NB)
+    J = np.transpose(X[:, 1:] - X[:, :1], (0, 2, 1))  # J[c, d] = X[d + 1, c] - X[0, c]
+    det = np.abs(np.linalg.det(J))
+    psi = np.einsum("ecd,qbd->ebcq", J, phi) / det[:, None, None, None]
+    k1 = weights[None, :] * det[:, None] / (2 * mu_q)
+    k2 = k1 * lam_q / (2 * mu_q + dim * lam_q)
+    E = len(det)
+    K = np.zeros((E, NL_, NL_))
+    D = np.einsum("eacq,ebcq,eq->eab", psi, psi, k1)
+    V = np.einsum("eacq,ebdq,eq->eacbd", psi, psi, k2)
+    ss = -V
+    for c in range(NC):
+        ss[:, :, c, :, c] += D
+        if mode == 1:
+            ss[:, :, c, :, c] += ddref[None] / det[:, None, None]
+    K[:, :NSS, :NSS] = ss.reshape(E, NSS, NSS)
+    if mode == 0:
+        A = np.einsum("ekx,bx->ebk", J, Phi)  # A[e, b, k] = sum_x J_kx Phi_bx
+        for b in range(NB):
+            for d in range(NC):
+                K[:, NC * b + d, NSS + d] = uref[b]
+                for r in range(NT):
+                    ct = r if dim == 3 else 2  # the rotation about axis ct (2-D: z)
+                    if ct == d:
+                        continue
+                    k = 3 - d - ct
+                    even = (ct, d) in ((0, 1), (1, 2), (2, 0))
+                    K[:, NC * b + d, NSS + NC + r] = A[:, b, k] if even else -A[:, b, k]
+        K[:, NSS:, :NSS] = np.transpose(K[:, :NSS, NSS:], (0, 2, 1))
+    else:
+        for r in range(NSS, NL_):
+            K[:, r, r] = det * weights.sum()
+    return K, det
+
+
+def _host_block_preconditioner(Mh, n_s, rows, cell_dofs):
+    """Block-diagonal SPD preconditioner as a scipy CSR: the inverted ROWS x ROWS facet blocks
+    of the compliance block on the stress rows, the inverted cell blocks of S = B D^-1 B^T
+    (B the cell rows' stress columns, D the facet blocks) on the cell rows ``cell_dofs``
+    (E, k), both symmetrised."""
+    import scipy.sparse as sps
+
+    nb = n_s // rows
+    blocks = np.zeros((nb, rows, rows))
+    Ms = Mh[:n_s, :n_s].tocoo()
+    same = (Ms.row // rows) == (Ms.col // rows)
+    blocks[Ms.row[same] // rows, Ms.row[same] % rows, Ms.col[same] % rows] = Ms.data[same]
+    Dinv = np.linalg.inv(blocks)
+    Dinv = 0.5 * (Dinv + Dinv.transpose(0, 2, 1))
+    Dm = sps.block_diag(list(Dinv), format="csr") if nb else sps.csr_matrix((0, 0))
+    E, k = cell_dofs.shape
+    B = Mh[cell_dofs.ravel()][:, :n_s].tocsr()
+    S = (B @ Dm @ B.T).tocsr()
+    Sb = np.stack([S[k * c:k * c + k, k * c:k * c + k].toarray() for c in range(E)]) if E else np.zeros((0, k, k))
+    Sinv = np.linalg.inv(Sb)
+    Sinv = 0.5 * (Sinv + Sinv.transpose(0, 2, 1))
+    rows_b = np.repeat(np.arange(n_s), rows)
+    cols_b = ((np.arange(n_s) // rows) * rows)[:, None] + np.arange(rows)[None, :]
+    r_list = [rows_b, np.repeat(cell_dofs, k, axis=1).ravel()]
+    c_list = [cols_b.ravel(), np.tile(cell_dofs, (1, k)).ravel()]
+    v_list = [Dinv.reshape(nb, rows, rows).reshape(-1), Sinv.ravel()]
+    n = Mh.shape[0]
+    M = sps.csr_matrix((np.concatenate(v_list), (np.concatenate(r_list), np.concatenate(c_list))), shape=(n, n))
+    M.sort_indices()
+    return M
+
+
+class ElasticitySystem2D:
+    """Dual-mixed elasticity on triangles (BDM1 stress, P0 displacement, P0 rotation about z)
+    resident on the GPU: the 2-D case of the reference's elasticity driver
+    (``examples/cauchy_elasticity/mfe_elasticity_example_1.py:68-294`` with ``dimension = 2``,
+    ``:729``). Operators A (``LEDualWeakForm``) and P (``LERieszMapWeakForm``) stored like
+    PETSc SBAIJ (upper triangle, the reference's value filter, explicit zero diagonals for
+    A), residual with the stress Dirichlet term; warm assemblies by the facet-owner kernel
+    (``nfx_assemble_elasticity_faces`` with dim 2: one wavefront per edge owns its 4 CSR
+    rows); the cold pass (element matrices for the filter and the patterns) on the host in
+    FP64. Solve: FP32 MINRES with the block-diagonal preconditioner (inverted edge blocks of
+    the compliance block, inverted cell blocks of B D^-1 B^T) inside FP64 refinement."""
+
+    def __init__(self, space, functions, device="cuda"):
+        t = torch()
+        self.space = space
+        self.device = device
+        names = space.names
+        self.s_sp, self.u_sp, self.t_sp = (space.discrete_spaces[n] for n in names)
+        s_sp = self.s_sp
+        mesh = s_sp.mesh
+        if s_sp.dimension != 2 or s_sp.element.family != "BDM" or s_sp.k_order != 1 or s_sp.n_comp != 2:
+            raise NotImplementedError("2-D GPU elasticity path: BDM1 - P0 - P0 on triangles")
+        self.dim = 2
+        self.E = len(s_sp.element_ids)
+        self.n = space.n_dof
+        self.n_s = s_sp.n_dof
+        self.field_bounds = [0, self.n_s, self.n_s + self.u_sp.n_dof, self.n]
+        points, weights = space.quadrature[2]
+        if len(weights) != 4:
+            raise ValueError("expected the degree-3 triangle rule (4 points)")
+        self.nq = 4
+        cells = mesh.entities[2].nodes[s_sp.topology.cell_index].astype(np.int64)
+        if cells.min() < 0 or cells.max() >= len(mesh.points):
+            raise ValueError("cells out of range")
+        X = mesh.points[cells]
+        if np.abs(X[:, :, 2]).max() > 0:
+            raise NotImplementedError("2-D elasticity: the mesh must lie in the plane z = 0")
+        self.xq = np.einsum("qa,eac->eqc", p1_bary(points), X)
+        X2 = X[:, :, :2]
+        J = np.transpose(X2[:, 1:] - X2[:, :1], (0, 2, 1))
+        det = np.abs(np.linalg.det(J))
+        if self.E and det.min() <= 0:
+            raise ValueError("degenerate element")
+        geom = np.concatenate([(J / det[:, None, None]).reshape(self.E, 4), det[:, None]], axis=1)
+        self.geom64 = to_dev(np.ascontiguousarray(geom), device=device)
+        self.geom32 = to_dev(np.ascontiguousarray(geom.astype(np.float32)), device=device)
+        tab = elasticity_tables(s_sp.element, points, weights)
+        self.tables64 = to_dev(tab, device=device)
+        self.tables32 = to_dev(tab.astype(np.float32), device=device)
+        ref_dest, ref_sign = space.reference_destinations()
+        if ref_dest.shape[1] != 15:
+            raise ValueError("unexpected local size")
+        self.ref_dest, self.ref_sign = ref_dest, ref_sign
+        self.functions = functions
+        self._coef = self._coefficients(functions)
+        lq = _expand_coef(self._coef["lam"], self.E, self.nq)
+        mq = _expand_coef(self._coef["mu"], self.E, self.nq)
+        # cold pass on the host: FP64 element matrices -> value filter -> SBAIJ patterns
+        self._K = {}
+        KA, _ = _local_matrices_host(s_sp.element, points, weights, X2, lq, mq, 0)
+        KP, _ = _local_matrices_host(s_sp.element, points, weights, X2, lq, mq, 1)
+        self._K = {0: KA, 1: KP}
+        keepA = _sbaij_mask(KA, ref_dest, ref_sign, upper=False)
+        keepP = _sbaij_mask(KP, ref_dest, ref_sign, upper=False)
+        self.pattern_A = build_pattern(self.n, ref_dest, ref_sign, keep=keepA, zero_diag_dest=ref_dest, upper=True)
+        self.pattern_P = build_pattern(self.n, ref_dest, ref_sign, keep=keepP, upper=True)
+        lay = face_owner_layout(self.n, ref_dest, ref_sign)
+        self.face_layout = lay
+        codes = {"A": face_owner_codes(self.n, lay, ref_dest, keepA, self.pattern_A, zero_diag=True),
+                 "P": face_owner_codes(self.n, lay, ref_dest, keepP, self.pattern_P, zero_diag=False)}
+        self.ops = {}
+        for key, pat in (("A", self.pattern_A), ("P", self.pattern_P)):
+            frp, fcol, fsrc = full_from_upper(self.n, pat.rowptr, pat.col)
+            self.ops[key] = {"pattern": pat, "rowptr": to_dev(pat.rowptr, device=device),
+                             "col": to_dev(pat.col, device=device), "codes_np": codes[key],
+                             "codes": to_dev(codes[key].view(np.int16), device=device),
+                             "vals32": t.zeros(pat.nnz, dtype=t.float32, device=device),
+                             "vals64": t.zeros(pat.nnz, dtype=t.float64, device=device),
+                             "full_rowptr": to_dev(frp, device=device), "full_col": to_dev(fcol, device=device),
+                             "full_src": to_dev(fsrc, device=device), "full_nnz": int(len(fcol))}
+        self.face_inc = to_dev(lay["face_inc"].astype(np.int32), device=device)
+        self.face_lf = to_dev(lay["face_lf"].astype(np.int32), device=device)
+        self.el_face = to_dev(lay["el_face"].astype(np.int32), device=device)
+        self.el_orient = to_dev(lay["el_orient"].view(np.int64), device=device)
+        self.rhs32 = t.zeros(self.n, dtype=t.float32, device=device)
+        self.rhs64 = t.zeros(self.n, dtype=t.float64, device=device)
+        self.cells = to_dev(cells.astype(np.int32), device=device)
+        self.coords = to_dev(mesh.points.astype(np.float32), device=device)
+        self.coords64 = to_dev(mesh.points, device=device)
+        self._setup_bc(mesh, device)
+        self._validate()
+
+    def _coefficients(self, functions):
+        lam = coefficient(functions["lambda"], self.xq, self.nq, device=self.device)
+        mu = coefficient(functions["mu"], self.xq, self.nq, device=self.device)
+        if (lam.se, lam.sq) != (mu.se, mu.sq):
+            lam = CoefficientArg(_expand_coef_dev(lam, self.E, self.nq), self.nq, 1)
+            mu = CoefficientArg(_expand_coef_dev(mu, self.E, self.nq), self.nq, 1)
+        check_coefficient(lam, self.E, self.nq, "lambda")
+        check_coefficient(mu, self.E, self.nq, "mu")
+        f = functions.get("rhs")
+        fv = _vector_coefficient(f, self.xq, self.nq, 3, self.device) if f is not None else \
+            torch().zeros((self.E, self.nq, 3), dtype=torch().float64, device=self.device)
+        if fv.shape != (self.E, self.nq, 3):
+            raise ValueError("rhs must give 3 components (2 force + 1 rotation) per point")
+        return {"lam": lam, "mu": mu, "f": CoefficientArg(fv.reshape(-1), self.nq * 3, 3)}
+
+    def _setup_bc(self, mesh, device):
+        s_sp = self.s_sp
+        self.F = len(s_sp.bc_element_ids)
+        if not self.F:
+            return
+        fpts, fw = self.space.bc_quadrature[1]
+        self.nqf = len(fw)
+        if self.nqf != 2:
+            raise ValueError("expected the 2-point edge rule")
+        neigh_cells = s_sp.topology.cell_index[s_sp.bc_neighbour]
+        combo = facet_combo_index(mesh, 2, s_sp.bc_element_ids, neigh_cells, s_sp.bc_local)
+        trace = facet_combo_tables(s_sp.element, 2, fpts)
+        fnodes = mesh.entities[1].nodes[s_sp.bc_element_ids - mesh.entities[1].offset]
+        self.bc_facet = to_dev(fnodes.astype(np.int32), device=device)
+        self.bc_neigh = to_dev(s_sp.bc_neighbour.astype(np.int32), device=device)
+        self.bc_combo = to_dev(combo.astype(np.int32), device=device)
+        self.bc_trace64 = to_dev(trace, device=device)
+        self.bc_trace32 = to_dev(trace.astype(np.float32), device=device)
+        self.bc_fw64 = to_dev(fw, device=device)
+        self.bc_fw32 = to_dev(fw.astype(np.float32), device=device)
+        shift = self.space.strides[self.space.names[0]] + self.space.dof_shift
+        self.bc_rows_np = row_codes(s_sp.bc_ref_dest + shift, s_sp.bc_ref_sign)
+        self.bc_rows = to_dev(self.bc_rows_np, device=device)
+        self.bc_xq = np.einsum("qa,fac->fqc", p1_bary(fpts), mesh.points[fnodes])
+
+    def _validate(self):
+        lay, E = self.face_layout, self.E
+        nf = lay["n_faces"]
+        fi = lay["face_inc"]
+        if fi.shape != (nf, 2) or fi[:, 0].min(initial=0) < 0 or fi.max(initial=-1) >= E or \
+                np.any(fi[:, 1] < -1) or np.any((fi[:, 1] >= 0) & (fi[:, 1] <= fi[:, 0])):
+            raise ValueError("facet incidences out of range or not in element order")
+        if lay["el_face"].shape != (E, 3) or lay["el_face"].min(initial=0) < 0 or \
+                lay["el_face"].max(initial=-1) >= nf or np.any(lay["face_lf"] >> 4):
+            raise ValueError("element facet blocks out of range")
+        if self.tables32.numel() != 4 + 4 * 12 + 4 * 6 + 6 + 12 + 36:
+            raise ValueError("elasticity table size")
+        for key in ("A", "P"):
+            op = self.ops[key]
+            code = op["codes_np"].astype(np.int64)
+            rp = op["pattern"].rowptr
+            if len(code) != int(rp[self.n_s]) or len(rp) != self.n + 1:
+                raise ValueError(f"facet codes of {key}: size")
+            s0, s1 = (code >> 4) & 63, code >> 10
+            if np.any((code & 15) >= 4) or np.any((s0 >= 15) & (s0 != FACE_NO_SRC)) or \
+                    np.any((s1 >= 15) & (s1 != FACE_NO_SRC)):
+                raise ValueError(f"facet codes of {key} out of range")
+            p_of = np.repeat(np.arange(self.n_s) // 4, np.diff(rp[: self.n_s + 1]))
+            if np.any((s1 != FACE_NO_SRC) & (fi[p_of, 1] < 0)):
+                raise ValueError(f"facet codes of {key}: a second source on a one-element facet")
+        if self.F:
+            rr = self.bc_rows_np.astype(np.int64)
+            if rr.shape != (self.F, 4) or np.any((rr >= 0) & ((rr & nat.SLOT_INDEX_MASK) >= self.n)):
+                raise ValueError("facet rows out of range")
+
+    def local_matrices(self, mode):
+        """The cold pass's FP64 element matrices (reference order) and element residuals."""
+        K = self._K[mode]
+        r = np.zeros((self.E, 15))
+        if mode == 0:
+            f = self._coef["f"].get(True).reshape(self.E, self.nq, 3).cpu().numpy()
+            pts, w = self.space.quadrature[2]
+            det = self.geom64[:, -1].cpu().numpy()
+            r[:, 12:] = -np.einsum("q,e,eqc->ec", w, det, f)
+        return K, r
+
+    def assemble(self, which="A", fp64=False, stream=None, u_dirichlet=None):
+        op = self.ops[which]
+        s = nat.stream_handle(stream)
+        vals = op["vals64" if fp64 else "vals32"]
+        rg = self.rhs64 if fp64 else self.rhs32
+        mode = 0 if which == "A" else 1
+        c = self._coef
+        lam, mu, f = c["lam"], c["mu"], c["f"]
+        lay = self.face_layout
+        nat.call("nfx_assemble_elasticity_faces", 2, mode, lay["n_faces"], self.E, nat.ptr(self.face_inc),
+                 nat.ptr(self.face_lf), nat.ptr(self.el_face), nat.ptr(self.el_orient),
+                 nat.ptr(self.geom64 if fp64 else self.geom32), nat.ptr(self.tables64 if fp64 else self.tables32),
+                 nat.ptr(lam.get(fp64)), nat.ptr(mu.get(fp64)), lam.se, lam.sq, nat.ptr(f.get(fp64)), self.nq * 3, 3,
+                 1, nat.ptr(op["rowptr"]), nat.ptr(op["codes"]), int(fp64), nat.ptr(vals), nat.ptr(rg), s)
+        if mode == 0 and u_dirichlet is not None and self.F:
+            self.assemble_dirichlet(u_dirichlet, fp64=fp64, stream=stream)
+        return vals, rg
+
+    def assemble_dirichlet(self, u_dirichlet, fp64=False, stream=None, rows=None, out=None):
+        uD = _vector_coefficient(u_dirichlet, self.bc_xq, self.nqf, 2, self.device)
+        if uD.shape != (self.F, self.nqf, 2):
+            raise ValueError("u_D must give 2 components per facet point")
+        rg = out if out is not None else (self.rhs64 if fp64 else self.rhs32)
+        g = (uD if fp64 else uD.to(torch().float32)).contiguous()
+        self._held = g
+        nat.call("nfx_assemble_elasticity_bc", 2, self.F, nat.ptr(self.bc_facet), nat.ptr(self.bc_neigh),
+                 nat.ptr(self.bc_combo), nat.ptr(self.cells), nat.ptr(self.coords64 if fp64 else self.coords),
+                 nat.ptr(self.bc_trace64 if fp64 else self.bc_trace32),
+                 nat.ptr(self.bc_fw64 if fp64 else self.bc_fw32), self.nqf, nat.ptr(g),
+                 nat.ptr(self.bc_rows if rows is None else rows), int(fp64), nat.ptr(rg), nat.stream_handle(stream))
+
+    def full(self, which="A"):
+        """Full symmetric (rowptr, col, vals32, vals64) for the Krylov solver."""
+        t = torch()
+        op = self.ops[which]
+        n = op["full_nnz"]
+        if "full32" not in op:
+            op["full32"] = t.empty(n, dtype=t.float32, device=self.device)
+            op["full64"] = t.empty(n, dtype=t.float64, device=self.device)
+        s = nat.stream_handle()
+        nat.call("nfx_gather", 0, n, nat.ptr(op["full_src"]), nat.ptr(op["vals32"]), nat.ptr(op["full32"]), s)
+        nat.call("nfx_gather", 1, n, nat.ptr(op["full_src"]), nat.ptr(op["vals64"]), nat.ptr(op["full64"]), s)
+        return op["full_rowptr"], op["full_col"], op["full32"], op["full64"]
+
+    def csr_numpy(self, which="A", fp64=True):
+        import scipy.sparse as sps
+
+        op = self.ops[which]
+        pat = op["pattern"]
+        vals = op["vals64" if fp64 else "vals32"].cpu().numpy()
+        return sps.csr_matrix((vals, pat.col, pat.rowptr), shape=(self.n, self.n))
+
+    def preconditioner(self):
+        """(rowptr, col, val) FP32 CSR of the block-diagonal preconditioner (built on the host
+        from the current FP64 values of A)."""
+        import scipy.sparse as sps
+
+        rp, col, _, v64 = self.full("A")
+        Mh = sps.csr_matrix((v64.cpu().numpy(), col.cpu().numpy(), rp.cpu().numpy()), shape=(self.n, self.n))
+        nu = self.u_sp.n_dof
+        E = self.E
+        cell_dofs = np.concatenate([self.n_s + 2 * np.arange(E)[:, None] + np.arange(2),
+                                    (self.n_s + nu + np.arange(E))[:, None]], axis=1)
+        M = _host_block_preconditioner(Mh, self.n_s, 4, cell_dofs)
+        return (to_dev(M.indptr.astype(np.int32), device=self.device),
+                to_dev(M.indices.astype(np.int32), device=self.device),
+                to_dev(M.data.astype(np.float32), device=self.device))
+
+    def solve(self, rtol=1e-8, inner_rtol=1e-5, max_inner=100000, max_outer=30):
+        from .solvers import MinresIR
+
+        pc = self.preconditioner()
+        rp, col, v32, v64 = self.full("A")
+        solver = MinresIR(rp, col, v32, v64, sign=None, pc=pc)
+        x, info = solver.solve(-self.rhs64, rtol=rtol, inner_rtol=inner_rtol, max_inner=max_inner,
+                               max_outer=max_outer)
+        solver.close()
+        info["pc"] = "mass_schur"
+        return x, info
+
+
+def _expand_coef_dev(c, E, nq):
+    a = c.get(True)
+    if c.se == 0:
+        return a.expand(E * nq).contiguous()
+    if c.sq == 0:
+        return a.reshape(-1, 1).expand(E, nq).reshape(-1).contiguous()
+    return a
+
+
+def _expand_coef(c, E, nq):
+    """(E, nq) FP64 host array of a coefficient argument."""
+    return _expand_coef_dev(c, E, nq).reshape(E, nq).cpu().numpy()

@@ -148,11 +148,12 @@ def _bc_facet_values(space, p_dirichlet):
 
 
 def _elasticity_system(space, functions):
-    from .elasticity import ElasticitySystem
+    from .elasticity import ElasticitySystem, ElasticitySystem2D
 
     sysd = getattr(space, "_nfx_elasticity_system", None)
     if sysd is None or sysd.functions is not functions:
-        sysd = ElasticitySystem(space, functions)
+        two_d = space.discrete_spaces[space.names[0]].dimension == 2
+        sysd = (ElasticitySystem2D if two_d else ElasticitySystem)(space, functions)
         space._nfx_elasticity_system = sysd
     return sysd
 
@@ -175,6 +176,10 @@ class LEDualWeakForm(WeakForm):
             from .elasticity import NL
 
             sysd = _elasticity_system(self.space, self.functions)
+            if sysd.dim == 2:
+                # the 2-D system keeps its FP64 element matrices from the host cold pass
+                self._cache = sysd.local_matrices(self._mode)
+                return self._cache
             d_loc = _dense_local_slots(sysd)
             K = sysd._local(self._mode, d_loc)
             r = sysd._local_rhs.cpu().numpy().reshape(sysd.E, NL)

@@ -134,31 +134,34 @@ def elasticity_functions(lam=1.0, mu=1.0, dim=3, u_expr=None):
     return fun, exact
 
 
-def oracle_elasticity_space(n=None, msh=None):
+def oracle_elasticity_space(n=None, msh=None, dim=3):
     import oracle.mesh as om
     import oracle.spaces as osp
 
     if msh:
         pts, bl = om.read_msh41(os.path.join(MESHES, msh))
     else:
-        pts, bl = om.structured_cube(n)
-    m = om.ConformalMesh(3, pts, bl)
-    ps = osp.ProductSpace({"s": (3, 3, "BDM", 1, m), "u": (3, 3, "Lagrange", 0, m), "t": (3, 3, "Lagrange", 0, m)},
-                          {"u": True, "t": True})
-    ps.build_structures({"s": [1], "u": [1], "t": [1]}, {"s": [2, 3, 4, 5, 6, 7]})
+        pts, bl = om.structured_cube(n) if dim == 3 else om.structured_square(n)
+    m = om.ConformalMesh(dim, pts, bl)
+    tc = 3 if dim == 3 else 1
+    ps = osp.ProductSpace({"s": (dim, dim, "BDM", 1, m), "u": (dim, dim, "Lagrange", 0, m),
+                           "t": (dim, tc, "Lagrange", 0, m)}, {"u": True, "t": True})
+    ps.build_structures({"s": [1], "u": [1], "t": [1]}, {"s": [2, 3, 4, 5, 6, 7] if dim == 3 else [2, 3, 4, 5]})
     return ps
 
 
-def product_elasticity_space(n=None, msh=None):
-    from nfempy_amd.mesh import Mesh, unit_cube
+def product_elasticity_space(n=None, msh=None, dim=3):
+    from nfempy_amd.mesh import Mesh, unit_cube, unit_square
     from nfempy_amd.spaces import ProductSpace
 
     if msh:
-        mesh = Mesh(3, os.path.join(MESHES, msh))
+        mesh = Mesh(dim, os.path.join(MESHES, msh))
         mesh.build_conformal_mesh()
     else:
-        mesh = Mesh.from_blocks(3, *unit_cube(n))
-    ps = ProductSpace({"s": (3, 3, "BDM", 1, mesh), "u": (3, 3, "Lagrange", 0, mesh), "t": (3, 3, "Lagrange", 0, mesh)})
+        mesh = Mesh.from_blocks(dim, *(unit_cube(n) if dim == 3 else unit_square(n)))
+    tc = 3 if dim == 3 else 1
+    ps = ProductSpace({"s": (dim, dim, "BDM", 1, mesh), "u": (dim, dim, "Lagrange", 0, mesh),
+                       "t": (dim, tc, "Lagrange", 0, mesh)})
     ps.make_subspaces_discontinuous({"u": True, "t": True})
-    ps.build_structures({"s": [1], "u": [1], "t": [1]}, {"s": [2, 3, 4, 5, 6, 7]})
+    ps.build_structures({"s": [1], "u": [1], "t": [1]}, {"s": [2, 3, 4, 5, 6, 7] if dim == 3 else [2, 3, 4, 5]})
     return ps

@@ -140,3 +140,79 @@ def test_face_owner_layout_rejects_other_layouts():
     bad[:, [0, 1]] = bad[:, [1, 0]]  # stress components no longer consecutive
     with pytest.raises(NotImplementedError):
         face_owner_layout(ps.n_dof, bad, ref_sign)
+
+
+def replay_generic(n, lay, codes, pattern, K, ref_sign, rows, nc, nt, nfc):
+    """``replay`` for any dimension: ROWS = rows per facet block, nc / nt the displacement /
+    rotation components, nfc facets per element."""
+    rowptr = pattern.rowptr.astype(np.int64)
+    n_s, n_u = lay["n_s"], lay["n_u"]
+    vals = np.zeros(pattern.nnz)
+    nbf = rows // nc
+    nb = nbf * nfc
+    orient = lay["el_orient"]
+    slot = ((orient[:, None] >> (np.uint64(2) * np.arange(nb, dtype=np.uint64))[None, :]) & np.uint64(3)).astype(int)
+    for p in range(lay["n_faces"]):
+        for k in range(rowptr[rows * p], rowptr[rows * p + rows]):
+            code = int(codes[k])
+            r, srcs = code & 15, ((code >> 4) & 63, code >> 10)
+            j, c = divmod(r, nc)
+            v = 0.0
+            for i in range(2):
+                if srcs[i] == 63:
+                    continue
+                e = lay["face_inc"][p, i]
+                lf = (lay["face_lf"][p] >> (2 * i)) & 3
+                a = [b for b in range(nbf * lf, nbf * lf + nbf) if slot[e, b] == j][0]
+                v += ref_sign[e, nc * a + c] * ref_sign[e, srcs[i]] * K[e, nc * a + c, srcs[i]]
+            vals[k] = v
+    nss = nb * nc
+    E = K.shape[0]
+    for e in range(E):
+        for c in range(nc):
+            vals[rowptr[n_s + nc * e + c]] = K[e, nss + c, nss + c]
+        for c in range(nt):
+            vals[rowptr[n_s + n_u + nt * e + c]] = K[e, nss + nc + c, nss + nc + c]
+    return vals
+
+
+@pytest.mark.parametrize("n", [2, 3])
+def test_face_owner_replay_2d_matches_oracle(n):
+    """Triangles: the host cold pass's element matrices (``_local_matrices_host``) against the
+    oracle's assembled A and P, then the edge-owner layout and codes replayed from them."""
+    import scipy.sparse as sps
+
+    from nfempy_amd.elasticity import _local_matrices_host
+    from oracle.assemble import assemble_elasticity
+    from tests._cases import elasticity_functions
+
+    ps = product_elasticity_space(n, dim=2)
+    ops = oracle_elasticity_space(n, dim=2)
+    s = ps.discrete_spaces["s"]
+    mesh = s.mesh
+    X = mesh.points[mesh.entities[2].nodes[s.topology.cell_index]][:, :, :2]
+    pts, w = ps.quadrature[2]
+    E = len(X)
+    rng = np.random.default_rng(7)
+    lam = np.exp(0.5 * rng.standard_normal(E))
+    mu = np.exp(0.5 * rng.standard_normal(E))
+    fun, ex = elasticity_functions(dim=2)
+    fun = dict(fun, **{"lambda": lam, "mu": mu})
+    A, P, _ = assemble_elasticity(ops, fun, {"u": ex["u"]})
+    ref_dest, ref_sign = ps.reference_destinations()
+    ndof = ps.n_dof
+    lay = face_owner_layout(ndof, ref_dest, ref_sign)
+    assert lay["n_s"] == ps.discrete_spaces_dofs["s"] and lay["rows"] == 4
+    for mode, R in ((0, A), (1, P)):
+        K, _ = _local_matrices_host(s.element, pts, w, X, np.repeat(lam[:, None], len(w), 1),
+                                    np.repeat(mu[:, None], len(w), 1), mode)
+        keep = _sbaij_mask(K, ref_dest, ref_sign, upper=False)
+        pat = build_pattern(ndof, ref_dest, ref_sign, keep=keep, zero_diag_dest=ref_dest if mode == 0 else None,
+                            upper=True)
+        assert np.array_equal(pat.rowptr, R.indptr) and np.array_equal(pat.col, R.indices), mode
+        codes = face_owner_codes(ndof, lay, ref_dest, keep, pat, zero_diag=(mode == 0))
+        vals = replay_generic(ndof, lay, codes, pat, K, ref_sign, 4, 2, 1, 3)
+        G = sps.csr_matrix((vals, pat.col, pat.rowptr), shape=(ndof, ndof))
+        D = abs(G - R).tocsr()
+        rowmax = np.maximum(abs(R).max(axis=1).toarray().ravel(), 1e-300)
+        assert (D.max(axis=1).toarray().ravel() / rowmax).max() <= 1e-12, mode
