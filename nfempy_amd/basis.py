@@ -3,14 +3,22 @@
 Replaces the basix calls of the reference (``basix.create_element(family, cell,
 k, lagrange_variant=gll_centroid, discontinuous)`` in
 ``src/basis/finite_element.py:85-91`` and ``src/spaces/dof_map.py:16-22``) for
-the families on the hot path: Lagrange P0/P1, Raviart-Thomas RT1/RT2 and
-Brezzi-Douglas-Marini BDM1 on triangles and tetrahedra.
+the families of ``src/basis/element_family.py:4-12``: Lagrange P0/P1, Raviart-Thomas
+RT1/RT2 and Brezzi-Douglas-Marini BDM1 on triangles and tetrahedra, and Nedelec of the
+first kind N1E (H(curl); the reference's "N2E" name maps to it too) of degree 1 on
+triangles and tetrahedra and degree 2 on triangles.
 
 Each element is built numerically from its Ciarlet definition: a monomial span
 and degree-of-freedom functionals (facet normal moments against Lagrange
 P_{k-1} (RT) / P_k (BDM) with the non-normalised facet normal of basix's
 sub-entity vertex order; RT2 interior moments against vector P0); the dual basis
-is ``span @ inv(D)`` with ``D[i, j] = l_i(span_j)`` evaluated by quadrature.
+is ``span @ inv(D)`` with ``D[i, j] = l_i(span_j)`` evaluated by quadrature. N1E:
+span (P_{k-1})^d plus the homogeneous degree-k vector polynomials p with p . x = 0;
+functionals the tangential edge moments against Lagrange P_{k-1} with the
+non-normalised edge tangent v1 - v0 of basix's sub-entity vertex order (edges of the
+tetrahedron (2,3), (1,3), (1,2), (0,3), (0,2), (0,1)), and for degree 2 on triangles
+the interior moments against vector P0. Its push-forward is the covariant Piola map
+phi = J^-T phihat (``covariant_piola``).
 
 ``entity_transformations`` restates basix's base transformations for these
 moment spaces (P0 moments: reflection = -1, rotation = 1; P1 moments: a cyclic
@@ -35,6 +43,10 @@ REF_VERTICES = {
 FACETS = {
     2: [(1, 2), (0, 2), (0, 1)],
     3: [(1, 2, 3), (0, 2, 3), (0, 1, 3), (0, 1, 2)],
+}
+EDGES = {
+    2: [(1, 2), (0, 2), (0, 1)],
+    3: [(2, 3), (1, 3), (1, 2), (0, 3), (0, 2), (0, 1)],
 }
 N_SUB = {2: (3, 3, 1), 3: (4, 6, 4, 1)}
 
@@ -111,6 +123,8 @@ class Element:
             exps = _exponents(d, self.degree)
             coeffs = [np.eye(len(exps))[j][None, :] for j in range(len(exps))]
             return exps, coeffs
+        if self.family == "N1E":
+            return self._span_nedelec()
         top = self.degree
         exps = _exponents(d, top)
         idx = {e: i for i, e in enumerate(exps)}
@@ -131,6 +145,36 @@ class Element:
                 coeffs.append(m)
         return exps, coeffs
 
+    def _span_nedelec(self):
+        """(P_{k-1})^d + {p in (homogeneous P_k)^d : p . x = 0} (a basis of the second part
+        from the null space of p -> p . x)."""
+        d, k = self.dim, self.degree
+        exps = _exponents(d, k)
+        idx = {e: i for i, e in enumerate(exps)}
+        coeffs = []
+        for c in range(d):
+            for e in _exponents(d, k - 1):
+                m = np.zeros((d, len(exps)))
+                m[c, idx[e]] = 1.0
+                coeffs.append(m)
+        hk = _exponents(d, k, homogeneous=True)
+        hk1 = {e: i for i, e in enumerate(_exponents(d, k + 1, homogeneous=True))}
+        M = np.zeros((len(hk1), d * len(hk)))
+        for c in range(d):
+            for j, e in enumerate(hk):
+                ee = list(e)
+                ee[c] += 1
+                M[hk1[tuple(ee)], c * len(hk) + j] = 1.0
+        _, sv, vt = np.linalg.svd(M)
+        rank = int(np.sum(sv > 1e-12))
+        for v in vt[rank:]:
+            m = np.zeros((d, len(exps)))
+            for c in range(d):
+                for j, e in enumerate(hk):
+                    m[c, idx[e]] = v[c * len(hk) + j]
+            coeffs.append(m)
+        return exps, coeffs
+
     def _build(self):
         d = self.dim
         exps, span = self._span()
@@ -149,6 +193,33 @@ class Element:
                 for vi in range(d + 1):
                     rows.append(vals[vi][None, :])
                     ent[0][vi] = [vi]
+        elif self.family == "N1E":
+            if d == 3 and self.degree > 1:
+                raise NotImplementedError("N1E of degree > 1 on tetrahedra (face moments)")
+            if self.degree > 2:
+                raise NotImplementedError("N1E of degree > 2")
+            s, ws = make_quadrature(1, 2 * self.degree + 2)
+            q = _lagrange_facet(1, self.degree - 1, s)
+            for ei, edge in enumerate(EDGES[d]):
+                V = REF_VERTICES[d][list(edge)]
+                x = V[0] + s @ (V[1:] - V[0])
+                vals = _eval_mono(exps, x)
+                tan = V[1] - V[0]
+                for j in range(q.shape[1]):
+                    wv = ws * q[:, j]
+                    r = np.zeros((d, len(exps)))
+                    for c in range(d):
+                        r[c] = tan[c] * (wv @ vals)
+                    ent[1][ei].append(len(rows))
+                    rows.append(r)
+            if self.degree == 2:
+                pts, wts = make_quadrature(d, 2 * self.degree + 2)
+                vals = _eval_mono(exps, pts)
+                for c in range(d):
+                    r = np.zeros((d, len(exps)))
+                    r[c] = wts @ vals
+                    ent[d][0].append(len(rows))
+                    rows.append(r)
         else:
             mdeg = self.degree - 1 if self.family == "RT" else self.degree
             s, ws = make_quadrature(d - 1, 2 * self.degree + 2)
@@ -201,12 +272,19 @@ class Element:
         return out
 
     def facet_moment_degree(self):
-        return self.degree - 1 if self.family == "RT" else self.degree
+        return self.degree - 1 if self.family in ("RT", "N1E") else self.degree
 
     def entity_transformations(self):
         if self.family == "P":
             return {}
         mdeg = self.facet_moment_degree()
+        if self.family == "N1E":
+            # tangential edge moments: a reversed edge flips the tangent (and, for P1 moments,
+            # swaps the two vertex functions)
+            iv = np.array([[-1.0]]) if mdeg == 0 else np.array([[0.0, -1.0], [-1.0, 0.0]])
+            if self.dim == 2:
+                return {"interval": [iv]}
+            return {"interval": [iv], "triangle": [np.zeros((0, 0)), np.zeros((0, 0))]}
         if self.dim == 2:
             if mdeg == 0:
                 return {"interval": [np.array([[-1.0]])]}
@@ -224,8 +302,20 @@ class Element:
 
 @functools.lru_cache(maxsize=None)
 def create_element(family: str, dim: int, degree: int, discontinuous: bool = False) -> Element:
-    fam = {"Lagrange": "P", "P": "P", "RT": "RT", "BDM": "BDM"}[family]
+    fam = {"Lagrange": "P", "P": "P", "RT": "RT", "BDM": "BDM", "N1E": "N1E", "N2E": "N1E"}[family]
     return Element(fam, dim, degree, discontinuous)
+
+
+def covariant_piola(phi_ref, jac):
+    """H(curl) push-forward phi = J^-T phihat: phi_ref (..., nb, dim) reference values at
+    points of one element, jac (dim, dim) its affine Jacobian."""
+    return np.einsum("dc,...bd->...bc", np.linalg.inv(jac), phi_ref)
+
+
+def curl_2d(grad_ref, jac):
+    """Scalar curl d phi_y / dx - d phi_x / dy of covariantly mapped 2-D fields from the
+    reference gradients grad_ref (2, nq, nb, 2): curl phi = curl phihat / det J."""
+    return (grad_ref[0, :, :, 1] - grad_ref[1, :, :, 0]) / np.linalg.det(jac)
 
 
 _POS_TO_ROT_REF = np.array([[0, 0], [0, 1], [2, 1], [1, 0], [2, 0], [1, 1]])

@@ -34,7 +34,7 @@ from itertools import permutations
 import numpy as np
 
 from .basis import create_element, facet_code_transform
-from .mesh import TET_FACES, TRI_EDGES, Mesh
+from .mesh import TET_EDGES, TET_FACES, TRI_EDGES, Mesh
 from .quadrature import make_quadrature
 
 _POS_TO_ROT_REF = np.array([[0, 0], [0, 1], [2, 1], [1, 0], [2, 0], [1, 1]])
@@ -106,7 +106,7 @@ class DiscreteSpace:
 
     @property
     def family(self):
-        return {"Lagrange": "P", "RT": "RT", "BDM": "BDM"}[self.family_name]
+        return {"Lagrange": "P", "RT": "RT", "BDM": "BDM", "N1E": "N1E", "N2E": "N1E"}[self.family_name]
 
     def make_discontinuous(self):
         self.discontinuous = True
@@ -158,6 +158,8 @@ class DiscreteSpace:
             sgn = np.ones((E, nb))
             for f in range(D + 1):
                 dofs = np.array(el.entity_dofs[D - 1][f])
+                if len(dofs) == 0:
+                    continue
                 for code in np.unique(self.codes[:, f]):
                     T = facet_code_transform(el, int(code))
                     rows = np.flatnonzero(self.codes[:, f] == code)
@@ -166,6 +168,19 @@ class DiscreteSpace:
                     s = T[np.arange(len(a)), a]
                     src[np.ix_(rows, dofs)] = dofs[a][None, :]
                     sgn[np.ix_(rows, dofs)] = s[None, :]
+            if D == 3 and any(len(e) for e in el.entity_dofs[1]):
+                # edge DoFs (N1E) after the face transformations: a local edge whose cell node
+                # order is not ascending is reflected (``permute_and_transform.py:37-55``,
+                # ``:129-139``)
+                nodes = self.mesh.entities[3].nodes[self.topology.cell_index]
+                refl = el.entity_transformations()["interval"][0]
+                a = np.argmax(np.abs(refl), axis=1)
+                sg = refl[np.arange(len(a)), a]
+                for k, (i, j) in enumerate(TET_EDGES):
+                    dofs = np.array(el.entity_dofs[1][k])
+                    rows = np.flatnonzero(nodes[:, i] > nodes[:, j])
+                    src[np.ix_(rows, dofs)] = dofs[a][None, :]
+                    sgn[np.ix_(rows, dofs)] = sg[None, :]
         # local dof i (basis i, comp c) -> reference basis src[i]
         self.local_src = src
         self.local_sign = sgn
