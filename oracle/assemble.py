@@ -83,6 +83,24 @@ def scatter_rhs(rg, dest, r_el):
         rg[d] += r_el[k]
 
 
+def coo_plan(n, rows, cols):
+    """The summation plan of ``csr_from_triplets`` without values: (indptr, indices, perm,
+    segptr) such that ``data[m] = sum(vals[perm[segptr[m]:segptr[m + 1]]])`` left to right
+    (same stable (row, col) order)."""
+    rows = np.asarray(rows, dtype=np.int64)
+    cols = np.asarray(cols, dtype=np.int64)
+    key = rows * n + cols
+    order = np.argsort(key, kind="stable")
+    key = key[order]
+    if len(key) == 0:
+        return np.zeros(n + 1, dtype=np.int64), np.zeros(0, dtype=np.int64), order, np.zeros(1, dtype=np.int64)
+    start = np.flatnonzero(np.r_[True, key[1:] != key[:-1]])
+    ukey = key[start]
+    indptr = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(indptr, ukey // n + 1, 1)
+    return np.cumsum(indptr), ukey % n, order.astype(np.int64), np.r_[start, len(key)].astype(np.int64)
+
+
 def assemble_darcy(ps, functions, bc_functions=None, element_range=None):
     """Darcy system ``(A, rg)`` exactly as ``md_elliptic_scalar.py:250-343``."""
     from .forms import darcy_bc_facet, darcy_element
