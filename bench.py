@@ -548,6 +548,28 @@ def bench_batch_solve(args, log):
            "preconditioner": "diagonal block (flux diagonal, Schur diagonal) per instance"}
     log(f"batch sample: {m} instances, {b.n_elements} tets: batched assembly + solve {t_batched * 1e3:.1f} ms "
         f"(MINRES {t_iter * 1e3:.1f} ms), one at a time {t_seq * 1e3:.1f} ms")
+    # the same batch with the batched two-grid Schur cycle on the pressure rows (block-Jacobi
+    # smoothers; "learned": the network smoother on the RT1-P0 instances)
+    for pc in ("twogrid", "learned"):
+        b.assemble()
+        b.solve_batched(rtol=1e-8, pc=pc)  # warm-up (cold structure, allocations)
+        torch.cuda.synchronize()
+        tb = time.perf_counter()
+        b.assemble()
+        _, inf, t_pc = b.solve_batched(rtol=1e-8, pc=pc)
+        torch.cuda.synchronize()
+        t_all = time.perf_counter() - tb
+        out[f"{pc}_assembly_plus_solve_s"] = round(t_all, 4)
+        out[f"{pc}_setup_plus_minres_s"] = round(t_pc, 4)
+        out[f"{pc}_converged"] = sum(1 for q in inf if q["converged"])
+        out[f"{pc}_max_residual"] = max(q["residual"] for q in inf)
+        out[f"{pc}_iterations_max"] = max(q["iterations"] for q in inf)
+        out[f"speedup_{pc}_vs_plain_batched"] = round(t_batched / t_all, 2)
+        out[f"speedup_{pc}_vs_sequential"] = round(t_seq / t_all, 2)
+        if pc == "learned":
+            out["learned_network_used"] = bool(any(g.network_used for g in b.last_tg.groups))
+        log(f"batch sample, {pc}: {t_all * 1e3:.1f} ms, iterations max {out[f'{pc}_iterations_max']}, "
+            f"{out[f'{pc}_converged']}/{m} converged")
     return out
 
 

@@ -450,6 +450,81 @@ int nfx_solve_minres_batch_ir(nfx_minres_batch* s, const double* d_b, double* d_
                               double inner_rtol, int max_inner, int max_outer, int* h_iters, int* h_outer,
                               double* h_resid);
 
+/* Two-grid Schur preconditioner of the pressure rows of a group of Darcy instances of one
+ * degree in the batched MINRES (RT1-P0: nb = 1 pressure unknown per cell; RT2-P1: nb = 4):
+ * the learned / two-grid replacement of the Schur diagonal of the block preconditioner
+ * (no reference counterpart; the solve it preconditions replaces
+ * examples/md_elliptic_scalar/md_elliptic_scalar.py:351-364 per instance). Cells of one
+ * instance are contiguous; cell c's unknowns sit at rows prow[c nb + i] of the stacked batch
+ * vector, cell_inst[c] is its instance (its index in the batch, for the mode gate); nbr[4c+k]
+ * the neighbour across local face k (-1 none); sd (n_cells, nb, nb) and so (n_cells, 4, nb,
+ * nb) the blocks of S = B diag(M)^-1 B^T; wd (n_cells, nb, nb) the smoother blocks, wo
+ * (n_cells, 4) its couplings (nb = 1 only, NULL = none); aggregates (agg_ptr / agg_mem /
+ * agg_of) never span two instances; ainv holds every instance's dense coarse inverse, row
+ * a of a's instance at ainv + arow[a] with alen[a] entries for the aggregates a0[a] ..
+ * a0[a] + alen[a] - 1. work: 3 n_cells nb + 2 n_agg floats. The cycle
+ * z = x2 + W (r - S x2), x2 = x1 + P Ac^-1 P^T (r - S x1), x1 = W r writes z on the
+ * group's pressure rows; with flux blocks (n_fb > 0) z = D_g^-1 r on their flux rows too. */
+typedef struct nfx_batch_tg {
+  int nb, n_cells, n_agg;
+  const int32_t* cell_inst;
+  const int32_t* prow;
+  const int32_t* nbr;
+  const float* sd;
+  const float* so;
+  const float* wd;
+  const float* wo;
+  const int32_t* agg_ptr;
+  const int32_t* agg_mem;
+  const int32_t* agg_of;
+  const int64_t* arow;
+  const int32_t* a0;
+  const int32_t* alen;
+  const float* ainv;
+  float* work;
+  int n_fb;                /* flux blocks (3 x 3) replacing the diagonal on flux rows, or 0 */
+  const int32_t* fb_inst;  /* (n_fb) instance of each block */
+  const int32_t* fb_rows;  /* (n_fb, 3) rows in the stacked vector */
+  const float* fb_inv;     /* (n_fb, 3, 3) inverse blocks */
+} nfx_batch_tg;
+/* S blocks from each instance's FP64 CSR values (d_vals[cell_inst[c]]): d_posb (n_cells,
+ * nbf, nb) positions of B(pressure unknown i, flux DoF a) of the cell's nbf flux DoFs, d_posd
+ * (n_cells, nbf) positions of their flux diagonals, d_fdofs (4, fd) the local flux DoFs of
+ * each local face, d_nloc (n_cells, 4, fd) their local index in the neighbour. FP64 and
+ * FP32 outputs in the layout of nfx_batch_tg (for nb = 1 the FP64 ones are the sdiag /
+ * soff of nfx_gnn_features). */
+int nfx_batch_tg_schur(int nb, int n_cells, int nbf, int fd, const int32_t* d_cell_inst, const double* const* d_vals,
+                       const int32_t* d_posb, const int32_t* d_posd, const int32_t* d_nbr, const int32_t* d_fdofs,
+                       const int32_t* d_nloc, double* d_sd64, double* d_so64, float* d_sd, float* d_so,
+                       void* stream);
+/* RT2-P1 variant of nfx_batch_tg_schur with the flux mass in 3 x 3 blocks (a face's three
+ * DoFs, a cell's three interior DoFs, local DoFs 12..14) instead of its diagonal: S = B D^-1
+ * B^T with d_posmf (n_cells, 4, 3, 3) / d_posmi (n_cells, 3, 3) the CSR positions of the blocks
+ * in the cell's local DoF order; bit 1 of *d_status when a block is not SPD. */
+int nfx_batch_tg_schur_blk(int n_cells, const int32_t* d_cell_inst, const double* const* d_vals,
+                           const int32_t* d_posb, const int32_t* d_posmf, const int32_t* d_posmi,
+                           const int32_t* d_nbr, const int32_t* d_fdofs, const int32_t* d_nloc, double* d_sd64,
+                           double* d_so64, float* d_sd, float* d_so, int32_t* d_status, void* stream);
+/* Inverses of the 3 x 3 flux blocks (FP32 out) from the CSR positions d_fb_pos (n_fb, 3, 3)
+ * of each block's instance values; bit 2 of *d_status when a block is not SPD. */
+int nfx_batch_tg_fluxinv(int n_fb, const int32_t* d_fb_inst, const double* const* d_vals, const int32_t* d_fb_pos,
+                         float* d_fb_inv, int32_t* d_status, void* stream);
+/* Block-Jacobi smoother wd = omega S_cc^-1 (FP64 inverse); bit 0 of *d_status when a block
+ * is not SPD (its smoother block is then zero). */
+int nfx_batch_tg_smoother(int nb, int n_cells, const double* d_sd64, double omega, float* d_wd, int32_t* d_status,
+                          void* stream);
+/* Per-instance coarse operators P^T S P (FP64, packed as ainv; d_ac must be zero). */
+int nfx_batch_tg_coarse(int nb, int n_agg, const int32_t* d_agg_ptr, const int32_t* d_agg_mem,
+                        const int32_t* d_agg_of, const int32_t* d_nbr, const double* d_sd64, const double* d_so64,
+                        const int64_t* d_arow, const int32_t* d_a0, double* d_ac, void* stream);
+/* One application z = TG(r) on the group's rows (d_mode: NULL, or only instances whose
+ * mode is 0 get z written). */
+int nfx_batch_tg_apply(const nfx_batch_tg* pc, const int32_t* d_mode, const float* d_r, float* d_z, void* stream);
+/* Attach (n_groups > 0) or detach (0) two-grid groups to a batched MINRES: each application
+ * of its preconditioner then runs the diagonal one and, on the groups' rows, the cycle. The
+ * descriptors are copied; the arrays they point to must outlive the solves. */
+int nfx_minres_batch_set_pc(nfx_minres_batch* s, int n_groups, const nfx_batch_tg* h_groups);
+
 /* ---- L2 projection ---------------------------------------------------------
  * Element mass matrix and right-hand side of one field of an L2 projection, as
  * L2ProjectorWeakForm.evaluate_form (src/weak_forms/l2_projector_weak_form.py:8-93)

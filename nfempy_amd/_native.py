@@ -114,6 +114,13 @@ SIGNATURES = {
     "nfx_minres_batch_create": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
     "nfx_minres_batch_destroy": (c_int, [c_void_p]),
     "nfx_minres_batch_rows": (c_i64, [c_void_p]),
+    "nfx_minres_batch_set_pc": (c_int, [c_void_p, c_int, c_void_p]),
+    "nfx_batch_tg_schur": (c_int, [c_int, c_int, c_int, c_int] + [c_void_p] * 11 + [c_void_p]),
+    "nfx_batch_tg_schur_blk": (c_int, [c_int] + [c_void_p] * 13 + [c_void_p]),
+    "nfx_batch_tg_fluxinv": (c_int, [c_int] + [c_void_p] * 5 + [c_void_p]),
+    "nfx_batch_tg_smoother": (c_int, [c_int, c_int, c_void_p, c_double, c_void_p, c_void_p, c_void_p]),
+    "nfx_batch_tg_coarse": (c_int, [c_int, c_int] + [c_void_p] * 9 + [c_void_p]),
+    "nfx_batch_tg_apply": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "nfx_solve_minres_batch_ir": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_double, c_int, c_int, c_void_p,
                                           c_void_p, c_void_p]),
     "nfx_elasticity_tg_blocks": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
@@ -161,6 +168,15 @@ class ElasticityTG(ctypes.Structure):
                 ("nbr", c_void_p), ("sblk", c_void_p), ("winv", c_void_p), ("agg_ptr", c_void_p),
                 ("agg_mem", c_void_p), ("agg_of", c_void_p), ("ainv", c_void_p), ("psign", c_float),
                 ("work", c_void_p)]
+
+
+class BatchTG(ctypes.Structure):
+    """Mirror of ``nfx_batch_tg`` (include/nfx.h)."""
+
+    _fields_ = [("nb", c_int), ("n_cells", c_int), ("n_agg", c_int)] + \
+        [(name, c_void_p) for name in ("cell_inst", "prow", "nbr", "sd", "so", "wd", "wo", "agg_ptr", "agg_mem",
+                                       "agg_of", "arow", "a0", "alen", "ainv", "work")] + \
+        [("n_fb", c_int), ("fb_inst", c_void_p), ("fb_rows", c_void_p), ("fb_inv", c_void_p)]
 
 
 # entry points whose return value is a count, not a status

@@ -240,11 +240,17 @@ class DarcyBatch:
         return {"rowptr": s.rowptr, "col": s.col, "vals32": inst["vals"], "vals64": s.vals64.clone(),
                 "sign": sign, "pinv": pinv, "b": -s.rhs64.clone()}
 
-    def solve_batched(self, indices=None, rtol=1e-8, inner_rtol=1e-5, max_inner=100000, max_outer=30):
+    def solve_batched(self, indices=None, rtol=1e-8, inner_rtol=1e-5, max_inner=100000, max_outer=30,
+                      pc="diag", theta=None):
         """Instances ``indices`` (default all) solved together by the batched MINRES: one set
         of launches per iteration, per-instance recurrences, stopping tests and FP64
-        refinement. The FP32 values are those of the last batched assembly. Returns
-        (list of x, list of infos, seconds of the batched solve alone)."""
+        refinement. The FP32 values are those of the last batched assembly. ``pc``: "diag"
+        (the block-diagonal preconditioner of every instance), "twogrid" (its pressure rows
+        replaced by the batched two-grid Schur cycle, block-Jacobi smoothers) or "learned"
+        (the same with the learned smoother on the RT1-P0 instances: ``theta``, default the
+        committed weights). Returns (list of x, list of infos, seconds of the preconditioner
+        set-up and the batched solve); the cold structure of the cycle is built once per
+        set of instances and kept."""
         from .solvers import MinresBatch
 
         t = torch()
@@ -252,12 +258,35 @@ class DarcyBatch:
         systems = [self.fp64_system(i) for i in idx]
         b = t.cat([sy["b"] for sy in systems])
         mb = MinresBatch(systems)
+        tg = None
+        if pc != "diag":
+            if pc not in ("twogrid", "learned"):
+                raise ValueError(f"unknown batched preconditioner {pc!r}")
+            from .batch_pc import BatchTwoGrid
+
+            if pc == "learned" and theta is None:
+                from .learned import default_weights_path, load_params
+
+                theta = load_params(default_weights_path())
+            key = (tuple(idx), pc)
+            cache = self.__dict__.setdefault("_tg_cache", {})
+            if key not in cache:
+                cache.clear()
+                cache[key] = BatchTwoGrid([self.instances[i]["system"] for i in idx], mb.offsets,
+                                          theta=theta if pc == "learned" else None, device=self.device)
+            tg = cache[key]
         t.cuda.synchronize()
         t0 = time.perf_counter()
+        if tg is not None:
+            mb.set_pc(tg.setup([sy["vals64"] for sy in systems]))
         x, infos = mb.solve(b, rtol=rtol, inner_rtol=inner_rtol, max_inner=max_inner, max_outer=max_outer)
         t.cuda.synchronize()
         dt = time.perf_counter() - t0
         mb.close()
+        if tg is not None:
+            for info in infos:
+                info["preconditioner"] = pc
+            self.last_tg = tg
         xs = [x[mb.offsets[j]:mb.offsets[j + 1]] for j in range(len(idx))]
         return xs, infos, dt
 

@@ -1069,6 +1069,27 @@ __global__ __launch_bounds__(kBlock) void k_bm_b(const nfx_minres_batch_item* __
   if (threadIdx.x == 0) partial_b[t] = bs;
 }
 
+// Tile partials of y'.z after a preconditioner that is not the fused diagonal (the
+// two-grid groups): same tiles and order as k_bm_b.
+__global__ __launch_bounds__(kBlock) void k_bm_dot(const nfx_minres_batch_item* __restrict__ items,
+                                                   const int32_t* __restrict__ tiles, const int32_t* __restrict__ act,
+                                                   const int32_t* __restrict__ mode, const float* __restrict__ y,
+                                                   const float* __restrict__ z, double* __restrict__ partial_b) {
+  __shared__ double sh[kBlock / 64];
+  const int t = act[blockIdx.x], inst = tiles[2 * t], row0 = tiles[2 * t + 1];
+  double acc = 0;
+  if (mode[inst] == 0) {
+    const nfx_minres_batch_item it = items[inst];
+    const int row = row0 + threadIdx.x;
+    if (row < it.n_rows) {
+      const int64_t g = it.row_off + row;
+      acc = (double)y[g] * (double)z[g];
+    }
+  }
+  const double bs = block_sum(acc, sh);
+  if (threadIdx.x == 0) partial_b[t] = bs;
+}
+
 // One block per instance: the sum of its tile partials in tile order. what = 0: alpha of
 // A(k) (and mode 1 -> 2: the final update has been applied); 1: the scalar step that A(k+1)
 // reads (st_out from st_in); 2: nothing but the sum (sum_out).
@@ -1199,6 +1220,7 @@ struct nfx_minres_batch {
   int32_t* h_mode = nullptr;
   double* h_sums = nullptr;
   double* h_scale = nullptr;
+  std::vector<nfx_batch_tg> pc;  // two-grid groups on the pressure rows (optional)
 };
 
 static void bm_free(nfx_minres_batch* s) {
@@ -1303,8 +1325,35 @@ extern "C" int nfx_minres_batch_destroy(nfx_minres_batch* s) {
 
 extern "C" int64_t nfx_minres_batch_rows(const nfx_minres_batch* s) { return s ? s->n_total : -1; }
 
+extern "C" int nfx_minres_batch_set_pc(nfx_minres_batch* s, int n_groups, const nfx_batch_tg* h_groups) {
+  NFX_REQUIRE(s && n_groups >= 0 && (n_groups == 0 || h_groups), "nfx_minres_batch_set_pc: bad arguments");
+  std::vector<nfx_batch_tg> g(h_groups, h_groups + n_groups);
+  for (const nfx_batch_tg& d : g) {
+    NFX_REQUIRE(d.nb == 1 || d.nb == 4, "nfx_minres_batch_set_pc: nb must be 1 or 4");
+    NFX_REQUIRE(d.n_cells > 0 && d.n_agg > 0 && d.cell_inst && d.prow && d.work,
+                "nfx_minres_batch_set_pc: incomplete group");
+  }
+  s->pc.swap(g);
+  return NFX_OK;
+}
+
+// The two-grid groups on top of the diagonal preconditioner k_bm_init / k_bm_b applied:
+// z on the groups' rows of active instances from y, then the tile partials of y'.z again.
+static int bm_apply_pc(nfx_minres_batch* s, const float* y, float* z) {
+  if (s->pc.empty() || !s->n_act) return NFX_OK;
+  for (const nfx_batch_tg& g : s->pc) {
+    const int rc = nfx::batch_tg_apply(&g, s->mode, y, z, s->stream);
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(k_bm_dot, dim3(s->n_act), dim3(kBmTile), 0, s->stream, s->items, s->tiles, s->act, s->mode, y,
+                     z, s->pb);
+  NFX_LAUNCH_CHECK();
+  return NFX_OK;
+}
+
 // One batched FP32 MINRES on every instance whose h_mode is 0 on entry (rhs already scaled);
 // iters[i] gets the instance's iteration count.
+
 static int bm_inner(nfx_minres_batch* s, const float* rhs, float* x, double tol, int maxit, int* iters) {
   const int64_t N = s->n_total;
   float* V = s->buf;
@@ -1318,9 +1367,12 @@ static int bm_inner(nfx_minres_batch* s, const float* rhs, float* x, double tol,
   NFX_CHECK_HIP(hipMemcpyAsync(s->mode, s->h_mode, sizeof(int32_t) * m, hipMemcpyHostToDevice, st));
   int rc = bm_set_active(s, 1);
   if (rc) return rc;
-  if (s->n_act)
+  if (s->n_act) {
     hipLaunchKernelGGL(k_bm_init, dim3(s->n_act), dim3(kBmTile), 0, st, s->items, s->tiles, s->act, s->mode, rhs,
                        P[0], P[1], Z[1], x, W[0], W[1], W[2], s->pb);
+    rc = bm_apply_pc(s, P[1], Z[1]);
+    if (rc) return rc;
+  }
   hipLaunchKernelGGL(k_bm_reduce, dim3(m), dim3(kBlock), 0, st, m, s->tile_ptr, s->pb, 1, 1, s->mode, S[0], S[1],
                      nullptr);
   NFX_LAUNCH_CHECK();
@@ -1347,6 +1399,8 @@ static int bm_inner(nfx_minres_batch* s, const float* rhs, float* x, double tol,
     if (active.empty()) break;  // the last final updates have been applied
     hipLaunchKernelGGL(k_bm_b, dim3(s->n_act), dim3(kBmTile), 0, st, s->items, s->tiles, s->act, s->mode, S[k & 1],
                        F, r2, r1, zprev, s->pb);
+    rc = bm_apply_pc(s, r1, zprev);
+    if (rc) return rc;
     hipLaunchKernelGGL(k_bm_reduce, dim3(m), dim3(kBlock), 0, st, m, s->tile_ptr, s->pb, 1, 0, s->mode, S[k & 1],
                        S[(k + 1) & 1], nullptr);
     NFX_LAUNCH_CHECK();

@@ -131,6 +131,12 @@ class MinresBatch:
         if nat.lib().nfx_minres_batch_rows(h) != self.offsets[-1]:
             raise RuntimeError("batch row count differs between host and library")
 
+    def set_pc(self, groups):
+        """Attach the two-grid groups (a ctypes array of ``nat.BatchTG``, or None to detach)."""
+        self._pc = groups
+        n = 0 if groups is None else len(groups)
+        nat.call("nfx_minres_batch_set_pc", self._h, n, groups if n else None)
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             nat.lib().nfx_minres_batch_destroy(self._h)
