@@ -780,6 +780,7 @@ def main():
     ap.add_argument("--md", action="store_true",
                     help="run the mixed-dimensional transient workload instead (24^3 x 6 matrix, 8 fractures)")
     ap.add_argument("--md-steps", type=int, default=200)
+    ap.add_argument("--no-md", action="store_true", help="leave the mixed-dimensional run out of the default line")
     ap.add_argument("--md-pc", choices=("learned", "twogrid", "diag"), default="learned")
     ap.add_argument("--md-compare-steps", type=int, default=10)
     ap.add_argument("--md-reference-exchange", action="store_true",
@@ -910,6 +911,9 @@ def main():
     batch_solve = None
     if args.batch_solve_instances > 0:
         batch_solve = bench_batch_solve(args, log)
+    md = None
+    if not args.no_md:
+        md = bench_md(args, log)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -949,6 +953,8 @@ def main():
         out["elasticity"] = elasticity
     if batch_solve:
         out["batch_solve"] = batch_solve
+    if md:
+        out["mixed_dimensional"] = md
     print(json.dumps(out), flush=True)
 
 
