@@ -525,6 +525,24 @@ int nfx_batch_tg_apply(const nfx_batch_tg* pc, const int32_t* d_mode, const floa
  * descriptors are copied; the arrays they point to must outlive the solves. */
 int nfx_minres_batch_set_pc(nfx_minres_batch* s, int n_groups, const nfx_batch_tg* h_groups);
 
+/* ---- sort-based numbering (cold path) ---------------------------------------
+ * The key step of the reference's mesh builder and of the PETSc pattern: entity ids in
+ * first-appearance order (src/mesh/mesh.py:239-339: edges, faces and volumes numbered as
+ * the cells are walked; src/mesh/mesh_topology.py:28-35: tag-filtered entity order) and
+ * the union of element (row, col) entries (the pattern setValue builds,
+ * src/assembly/ScatterFormData.py:5-11). Keys are non-negative int64 below 2^key_bits;
+ * K < 2^31. d_work: nfx_sort_workspace(K) bytes. */
+int nfx_sort_workspace(int64_t K, size_t* h_bytes);
+/* Stable ascending sort: d_sorted_keys, and d_perm[i] = input position of the i-th key
+ * (equal keys keep their input order). */
+int nfx_stable_sort64(int64_t K, int key_bits, const int64_t* d_keys, int64_t* d_sorted_keys, int32_t* d_perm,
+                      void* d_work, size_t work_bytes, void* stream);
+/* d_ids[i] = rank of keys[i] among the distinct keys ordered by first appearance,
+ * d_first[r] = input position of the first key of rank r (capacity K), *d_count = the
+ * number of distinct keys (device int). */
+int nfx_first_appearance(int64_t K, int key_bits, const int64_t* d_keys, int32_t* d_ids, int32_t* d_first,
+                         int32_t* d_count, void* d_work, size_t work_bytes, void* stream);
+
 /* ---- L2 projection ---------------------------------------------------------
  * Element mass matrix and right-hand side of one field of an L2 projection, as
  * L2ProjectorWeakForm.evaluate_form (src/weak_forms/l2_projector_weak_form.py:8-93)

@@ -30,6 +30,7 @@ _lib = None
 c_int = ctypes.c_int
 c_i64 = ctypes.c_int64
 c_double = ctypes.c_double
+c_size_t = ctypes.c_size_t
 c_float = ctypes.c_float
 c_void_p = ctypes.c_void_p
 c_char_p = ctypes.c_char_p
@@ -120,6 +121,10 @@ SIGNATURES = {
     "nfx_batch_tg_fluxinv": (c_int, [c_int] + [c_void_p] * 5 + [c_void_p]),
     "nfx_batch_tg_smoother": (c_int, [c_int, c_int, c_void_p, c_double, c_void_p, c_void_p, c_void_p]),
     "nfx_batch_tg_coarse": (c_int, [c_int, c_int] + [c_void_p] * 9 + [c_void_p]),
+    "nfx_sort_workspace": (c_int, [c_i64, c_void_p]),
+    "nfx_stable_sort64": (c_int, [c_i64, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
+    "nfx_first_appearance": (c_int, [c_i64, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_size_t,
+                                     c_void_p]),
     "nfx_batch_tg_apply": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "nfx_solve_minres_batch_ir": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_double, c_int, c_int, c_void_p,
                                           c_void_p, c_void_p]),

@@ -213,8 +213,13 @@ def unit_cube(n, lx=1.0, ly=1.0, lz=1.0):
 # numbering
 # ----------------------------------------------------------------------------
 
-def _first_appearance_ids(keys):
-    """keys (K,) int64 -> (ids (K,) in first-appearance order, first index per id)."""
+def _first_appearance_ids(keys, device=None):
+    """keys (K,) int64 -> (ids (K,) in first-appearance order, first index per id). With a
+    device the sort runs there (``numbering.first_appearance_ids``, same integers)."""
+    if device is not None:
+        from .numbering import first_appearance_ids
+
+        return first_appearance_ids(keys, device)
     uniq, first, inv = np.unique(keys, return_index=True, return_inverse=True)
     order = np.argsort(first, kind="stable")
     rank = np.empty(len(uniq), dtype=np.int64)
@@ -239,8 +244,10 @@ class EntityTable:
 class Mesh:
     """Reference ``Mesh`` (``src/mesh/mesh.py:13``) after ``build_conformal_mesh``."""
 
-    def __init__(self, dimension, file_name=None, points=None, blocks=None):
+    def __init__(self, dimension, file_name=None, points=None, blocks=None, device=None):
+        """device: run the numbering sorts there ("cuda"), else on the host."""
         self.dimension = dimension
+        self.device = device
         if file_name is not None:
             points, blocks = read_msh41(file_name)
         self.points = np.asarray(points, dtype=np.float64)
@@ -248,8 +255,8 @@ class Mesh:
         self.entities = None
 
     @classmethod
-    def from_blocks(cls, dimension, points, blocks):
-        m = cls(dimension, points=points, blocks=blocks)
+    def from_blocks(cls, dimension, points, blocks, device=None):
+        m = cls(dimension, points=points, blocks=blocks, device=device)
         m.build_conformal_mesh()
         return m
 
@@ -271,7 +278,8 @@ class Mesh:
               key2(tris[:, TRI_EDGES[:, 0]], tris[:, TRI_EDGES[:, 1]]).reshape(-1),
               key2(lines[:, 0], lines[:, 1]).reshape(-1)]
         all_ek = np.concatenate(ek)
-        eid, efirst = _first_appearance_ids(all_ek)
+        dev = self.device
+        eid, efirst = _first_appearance_ids(all_ek, dev)
         n_edges = len(efirst)
         tet_edge = eid[: ek[0].size].reshape(-1, 6) + V
         tri_edge = eid[ek[0].size: ek[0].size + ek[1].size].reshape(-1, 3) + V
@@ -318,7 +326,7 @@ class Mesh:
               key3(tris[:, 0], tris[:, 1], tris[:, 2]).reshape(-1)]
         all_fk = np.concatenate(fk)
         foff = V + n_edges
-        fid, ffirst = _first_appearance_ids(all_fk) if all_fk.size else (np.zeros(0, np.int64), np.zeros(0))
+        fid, ffirst = _first_appearance_ids(all_fk, dev) if all_fk.size else (np.zeros(0, np.int64), np.zeros(0))
         n_faces = len(ffirst)
         tet_face = fid[: fk[0].size].reshape(-1, 4) + foff
         tri_face = fid[fk[0].size:] + foff
@@ -356,7 +364,7 @@ class Mesh:
             if vk is None:
                 _, vinv = np.unique(s, axis=0, return_inverse=True)
                 vk = vinv.reshape(-1)
-            vid, vfirst = _first_appearance_ids(vk)
+            vid, vfirst = _first_appearance_ids(vk, dev)
             n_vol = len(vfirst)
             vol_nodes = np.zeros((n_vol, 4), dtype=np.int64)
             vol_nodes[vid[vfirst]] = tets[vfirst]
@@ -438,7 +446,7 @@ class Topology:
                 self.position[d] = np.full(self.n_total[d], -1, np.int64)
                 self.first_cell[d] = np.zeros(0, np.int64)
                 continue
-            pos, first = _first_appearance_ids(flat)
+            pos, first = _first_appearance_ids(flat, getattr(mesh, "device", None))
             ids = flat[first]
             self.entity_ids[d] = ids
             p = np.full(self.n_total[d], -1, np.int64)

@@ -43,7 +43,7 @@ class Pattern:
         return int(self.rowptr[-1])
 
 
-def build_pattern(n, ref_dest, ref_sign, keep=None, zero_diag_dest=None, upper=False):
+def build_pattern(n, ref_dest, ref_sign, keep=None, zero_diag_dest=None, upper=False, device=None):
     """Pattern of the union of element (row, col) entries.
 
     ref_dest, ref_sign: (E, nl) destination rows and signs in reference-basis order.
@@ -52,6 +52,7 @@ def build_pattern(n, ref_dest, ref_sign, keep=None, zero_diag_dest=None, upper=F
     zero_diag_dest: if given, (E, nl) rows that receive an explicit 0.0 diagonal
     (these contribute ZERO codes at the (a, a) entries; an entry kept by the mask
     keeps its value).
+    device: run the stable sort of the keys there (``numbering.stable_argsort``).
     """
     E, nl = ref_dest.shape
     rows = np.repeat(ref_dest[:, :, None], nl, axis=2)
@@ -70,8 +71,13 @@ def build_pattern(n, ref_dest, ref_sign, keep=None, zero_diag_dest=None, upper=F
     r = rows.reshape(-1)[flat_sel]
     c = cols.reshape(-1)[flat_sel]
     key = r.astype(np.int64) * n + c
-    order = np.argsort(key, kind="stable")
-    ks = key[order]
+    if device is not None:
+        from .numbering import stable_argsort
+
+        order, ks = stable_argsort(key, device)
+    else:
+        order = np.argsort(key, kind="stable")
+        ks = key[order]
     start = np.r_[True, ks[1:] != ks[:-1]] if len(ks) else np.zeros(0, bool)
     uid_sorted = np.cumsum(start) - 1
     ukeys = ks[start]
