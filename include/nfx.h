@@ -444,8 +444,10 @@ int nfx_minres_batch_create(nfx_minres_batch** out, int n_inst, const nfx_minres
 int nfx_minres_batch_destroy(nfx_minres_batch* s);
 /* Total rows of the batch (the length of d_b and d_x); a count, not a status. */
 int64_t nfx_minres_batch_rows(const nfx_minres_batch* s);
-/* Synchronous. Per instance: h_iters total MINRES iterations, h_outer refinement steps,
- * h_resid final relative FP64 residual. NFX_E_NOT_CONVERGED if any instance stays above rtol. */
+/* Synchronous. Each refinement step of an instance stops its inner MINRES at the relative
+ * reduction still missing, max(inner_rtol, rtol / (2 ||r|| / ||b||)) (at most 0.5). Per
+ * instance: h_iters total MINRES iterations, h_outer refinement steps, h_resid final relative
+ * FP64 residual. NFX_E_NOT_CONVERGED if any instance stays above rtol. */
 int nfx_solve_minres_batch_ir(nfx_minres_batch* s, const double* d_b, double* d_x, double rtol,
                               double inner_rtol, int max_inner, int max_outer, int* h_iters, int* h_outer,
                               double* h_resid);
@@ -458,11 +460,12 @@ int nfx_solve_minres_batch_ir(nfx_minres_batch* s, const double* d_b, double* d_
  * instance are contiguous; cell c's unknowns sit at rows prow[c nb + i] of the stacked batch
  * vector, cell_inst[c] is its instance (its index in the batch, for the mode gate); nbr[4c+k]
  * the neighbour across local face k (-1 none); sd (n_cells, nb, nb) and so (n_cells, 4, nb,
- * nb) the blocks of S = B diag(M)^-1 B^T; wd (n_cells, nb, nb) the smoother blocks, wo
+ * nb) the blocks of S = B diag(M)^-1 B^T, rs (n_cells, 5, nb) their row sums (own block,
+ * then neighbour k; nfx_batch_tg_rowsums); wd (n_cells, nb, nb) the smoother blocks, wo
  * (n_cells, 4) its couplings (nb = 1 only, NULL = none); aggregates (agg_ptr / agg_mem /
  * agg_of) never span two instances; ainv holds every instance's dense coarse inverse, row
  * a of a's instance at ainv + arow[a] with alen[a] entries for the aggregates a0[a] ..
- * a0[a] + alen[a] - 1. work: 3 n_cells nb + 2 n_agg floats. The cycle
+ * a0[a] + alen[a] - 1. work: 4 n_cells nb + 2 n_agg floats. The cycle
  * z = x2 + W (r - S x2), x2 = x1 + P Ac^-1 P^T (r - S x1), x1 = W r writes z on the
  * group's pressure rows; with flux blocks (n_fb > 0) z = D_g^-1 r on their flux rows too. */
 typedef struct nfx_batch_tg {
@@ -472,6 +475,7 @@ typedef struct nfx_batch_tg {
   const int32_t* nbr;
   const float* sd;
   const float* so;
+  const float* rs;
   const float* wd;
   const float* wo;
   const int32_t* agg_ptr;
@@ -517,10 +521,14 @@ int nfx_batch_tg_smoother(int nb, int n_cells, const double* d_sd64, double omeg
 int nfx_batch_tg_coarse(int nb, int n_agg, const int32_t* d_agg_ptr, const int32_t* d_agg_mem,
                         const int32_t* d_agg_of, const int32_t* d_nbr, const double* d_sd64, const double* d_so64,
                         const int64_t* d_arow, const int32_t* d_a0, double* d_ac, void* stream);
-/* One application z = TG(r) on the group's rows (d_mode: NULL, or only instances whose
- * mode is 0 get z written). */
-int nfx_batch_tg_apply(const nfx_batch_tg* pc, const int32_t* d_mode, const float* d_r, float* d_z, void* stream);
-/* Attach (n_groups > 0) or detach (0) two-grid groups to a batched MINRES: each application
+/* Row sums rs of the S blocks (FP64 in, FP32 out). */
+int nfx_batch_tg_rowsums(int nb, int n_cells, const double* d_sd64, const double* d_so64, float* d_rs,
+                         void* stream);
+/* One application z = TG(r) on the rows of one group, or of two groups of different nb in
+ * shared launches (d_mode: NULL, or only instances whose mode is 0 get z written). */
+int nfx_batch_tg_apply(int n_groups, const nfx_batch_tg* pc, const int32_t* d_mode, const float* d_r, float* d_z,
+                       void* stream);
+/* Attach (n_groups 1 or 2, of different nb) or detach (0) two-grid groups to a batched MINRES: each application
  * of its preconditioner then runs the diagonal one and, on the groups' rows, the cycle. The
  * descriptors are copied; the arrays they point to must outlive the solves. */
 int nfx_minres_batch_set_pc(nfx_minres_batch* s, int n_groups, const nfx_batch_tg* h_groups);

@@ -125,7 +125,8 @@ SIGNATURES = {
     "nfx_stable_sort64": (c_int, [c_i64, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
     "nfx_first_appearance": (c_int, [c_i64, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_size_t,
                                      c_void_p]),
-    "nfx_batch_tg_apply": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "nfx_batch_tg_apply": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "nfx_batch_tg_rowsums": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "nfx_solve_minres_batch_ir": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_double, c_int, c_int, c_void_p,
                                           c_void_p, c_void_p]),
     "nfx_elasticity_tg_blocks": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
@@ -179,7 +180,7 @@ class BatchTG(ctypes.Structure):
     """Mirror of ``nfx_batch_tg`` (include/nfx.h)."""
 
     _fields_ = [("nb", c_int), ("n_cells", c_int), ("n_agg", c_int)] + \
-        [(name, c_void_p) for name in ("cell_inst", "prow", "nbr", "sd", "so", "wd", "wo", "agg_ptr", "agg_mem",
+        [(name, c_void_p) for name in ("cell_inst", "prow", "nbr", "sd", "so", "rs", "wd", "wo", "agg_ptr", "agg_mem",
                                        "agg_of", "arow", "a0", "alen", "ainv", "work")] + \
         [("n_fb", c_int), ("fb_inst", c_void_p), ("fb_rows", c_void_p), ("fb_inv", c_void_p)]
 

@@ -180,17 +180,18 @@ class _Group:
         f32 = lambda k: t.empty(k, dtype=t.float32, device=device)
         self.sd64, self.so64 = f64(C * NB * NB), f64(C * 4 * NB * NB)
         self.sd, self.so = f32(C * NB * NB), f32(C * 4 * NB * NB)
+        self.rs = f32(C * 5 * NB)
         self.wd, self.wo = f32(C * NB * NB), None
         self.ac64 = f64(int(self.coff[-1]))
         self.ainv = f32(int(self.coff[-1]))
-        self.work = f32(3 * C * NB + 2 * A + 4)
+        self.work = f32(4 * C * NB + 2 * A + 4)
         self.status = t.zeros(1, dtype=t.int32, device=device)
         self.network_used = False
 
     def desc(self):
         d = self.d
         return nat.BatchTG(self.nb, self.C, self.A, nat.ptr(d["cell_inst"]), nat.ptr(d["prow"]), nat.ptr(d["nbr"]),
-                           nat.ptr(self.sd), nat.ptr(self.so), nat.ptr(self.wd),
+                           nat.ptr(self.sd), nat.ptr(self.so), nat.ptr(self.rs), nat.ptr(self.wd),
                            None if self.wo is None else nat.ptr(self.wo), nat.ptr(d["agg_ptr"]),
                            nat.ptr(d["agg_mem"]), nat.ptr(d["agg_of"]), nat.ptr(d["arow"]), nat.ptr(d["a0"]),
                            nat.ptr(d["alen"]), nat.ptr(self.ainv), nat.ptr(self.work), self.n_fb,
@@ -255,6 +256,7 @@ class BatchTwoGrid:
                 nat.call("nfx_batch_tg_schur", g.nb, g.C, g.nbf, g.fd, nat.ptr(d["cell_inst"]), nat.ptr(self.vptr),
                          nat.ptr(d["posb"]), nat.ptr(d["posd"]), nat.ptr(d["nbr"]), nat.ptr(d["fdofs"]),
                          nat.ptr(d["nloc"]), nat.ptr(g.sd64), nat.ptr(g.so64), nat.ptr(g.sd), nat.ptr(g.so), s)
+            nat.call("nfx_batch_tg_rowsums", g.nb, g.C, nat.ptr(g.sd64), nat.ptr(g.so64), nat.ptr(g.rs), s)
             g.network_used = False
             if g.nb == 1 and self._gnn is not None and network:
                 q = self._gnn
@@ -288,6 +290,6 @@ class BatchTwoGrid:
         """z = the preconditioner's two-grid part on the groups' rows of r (tests)."""
         t = torch()
         z = t.zeros_like(r) if z is None else z
-        for d in self._descs:
-            nat.call("nfx_batch_tg_apply", ctypes.byref(d), None, nat.ptr(r), nat.ptr(z), nat.stream_handle(stream))
+        nat.call("nfx_batch_tg_apply", len(self._descs), self._descs, None, nat.ptr(r), nat.ptr(z),
+                 nat.stream_handle(stream))
         return z

@@ -212,25 +212,21 @@ __global__ __launch_bounds__(kTB) void k_btg_fluxinv(int n_fb, const int32_t* __
     for (int j = 0; j < 3; ++j) fb_inv[(int64_t)g * 9 + i * 3 + j] = (float)A[i][j];
 }
 
-// z = D_g^-1 y on the flux rows of each block of an active instance
-__global__ __launch_bounds__(kTB) void k_btg_flux(const nfx_batch_tg pc, const int32_t* __restrict__ mode,
-                                                  const float* __restrict__ y, float* __restrict__ z) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= pc.n_fb) return;
-  if (mode && mode[pc.fb_inst[g]] != 0) return;
-  int r[3];
-  float yv[3];
+// rs[c][b][i] = sum_j S_b[i][j] over the cell's own block (b = 0) and neighbour blocks
+template <int NB>
+__global__ __launch_bounds__(kTB) void k_btg_rowsums(int n_cells, const double* __restrict__ sd64,
+                                                     const double* __restrict__ so64, float* __restrict__ rs) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_cells) return;
+  for (int b = 0; b < 5; ++b) {
+    const double* S = b == 0 ? sd64 + (int64_t)c * NB * NB : so64 + ((int64_t)c * 4 + (b - 1)) * NB * NB;
 #pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    r[i] = pc.fb_rows[(int64_t)g * 3 + i];
-    yv[i] = y[r[i]];
-  }
+    for (int i = 0; i < NB; ++i) {
+      double v = 0.0;
 #pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    float s = 0.f;
-#pragma unroll
-    for (int j = 0; j < 3; ++j) s += pc.fb_inv[(int64_t)g * 9 + i * 3 + j] * yv[j];
-    z[r[i]] = s;
+      for (int j = 0; j < NB; ++j) v += S[i * NB + j];
+      rs[((int64_t)c * 5 + b) * NB + i] = (float)v;
+    }
   }
 }
 
@@ -348,102 +344,201 @@ __device__ __forceinline__ void w_row(int c, const int32_t* __restrict__ nbr, co
   }
 }
 
+// Work of one group: x1 | x2 | t2 | t1 (n_cells nb each) | cv | y (n_agg each).
+struct GroupWork {
+  float *x1, *x2, *t2, *t1, *cv, *y;
+};
+
+__device__ __forceinline__ GroupWork group_work(const nfx_batch_tg& pc) {
+  GroupWork w;
+  const int64_t cn = (int64_t)pc.n_cells * pc.nb;
+  w.x1 = pc.work;
+  w.x2 = w.x1 + cn;
+  w.t2 = w.x2 + cn;
+  w.t1 = w.t2 + cn;
+  w.cv = w.t1 + cn;
+  w.y = w.cv + pc.n_agg;
+  return w;
+}
+
+// Stage bodies, one work item (cell, aggregate wave or flux block) each; the kernels below
+// run two groups (RT1-P0 and RT2-P1) in one launch: blocks [0, split) serve the first,
+// the rest the second.
+
 // x1 = W r
 template <int NB>
-__global__ __launch_bounds__(kTB) void k_btg_pre(const nfx_batch_tg pc, const float* __restrict__ r,
-                                                 float* __restrict__ x1) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void st_pre(const nfx_batch_tg& pc, int c, const float* __restrict__ r) {
   if (c >= pc.n_cells) return;
+  const GroupWork w = group_work(pc);
   const int32_t* prow = pc.prow;
   float o[NB];
   w_row<NB>(c, pc.nbr, pc.wd, pc.wo, [&](int q, int i) { return r[prow[(int64_t)q * NB + i]]; }, o);
 #pragma unroll
-  for (int i = 0; i < NB; ++i) x1[(int64_t)c * NB + i] = o[i];
+  for (int i = 0; i < NB; ++i) w.x1[(int64_t)c * NB + i] = o[i];
 }
 
-// cv[a] = sum over the cells p of a and their unknowns of (r - S x1)_p (one wave per aggregate)
+// t1 = r - S x1 on the cells of aggregate `wave` (kept for the post step) and
+// cv[wave] = the sum of t1 over them (one wave per aggregate, lanes over its cells)
 template <int NB>
-__global__ __launch_bounds__(kTB) void k_btg_restrict(const nfx_batch_tg pc, const float* __restrict__ r,
-                                                      const float* __restrict__ x1, float* __restrict__ cv) {
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
+__device__ __forceinline__ void st_restrict(const nfx_batch_tg& pc, int wave, int lane, const float* __restrict__ r) {
   if (wave >= pc.n_agg) return;
+  const GroupWork w = group_work(pc);
   float acc = 0.f;
   for (int k = pc.agg_ptr[wave] + lane; k < pc.agg_ptr[wave + 1]; k += 64) {
     const int p = pc.agg_mem[k];
     float rp[NB], s[NB];
     load_r<NB>(p, pc.prow, r, rp);
-    s_row<NB>(p, pc.nbr, pc.sd, pc.so, [&](int q, int i) { return x1[(int64_t)q * NB + i]; }, s);
+    s_row<NB>(p, pc.nbr, pc.sd, pc.so, [&](int q, int i) { return w.x1[(int64_t)q * NB + i]; }, s);
 #pragma unroll
-    for (int i = 0; i < NB; ++i) acc += rp[i] - s[i];
+    for (int i = 0; i < NB; ++i) {
+      const float t = rp[i] - s[i];
+      w.t1[(int64_t)p * NB + i] = t;
+      acc += t;
+    }
   }
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-  if (lane == 0) cv[wave] = acc;
+  if (lane == 0) w.cv[wave] = acc;
 }
 
 // y[a] = sum_t ainv[arow[a] + t] cv[a0[a] + t] over the aggregates of a's instance
-__global__ __launch_bounds__(kTB) void k_btg_gemv(const nfx_batch_tg pc, const float* __restrict__ cv,
-                                                  float* __restrict__ y) {
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
+__device__ __forceinline__ void st_gemv(const nfx_batch_tg& pc, int wave, int lane) {
   if (wave >= pc.n_agg) return;
+  const GroupWork w = group_work(pc);
   const float* a = pc.ainv + pc.arow[wave];
-  const float* x = cv + pc.a0[wave];
+  const float* x = w.cv + pc.a0[wave];
   const int n = pc.alen[wave];
   float acc = 0.f;
   for (int t = lane; t < n; t += 64) acc += a[t] * x[t];
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-  if (lane == 0) y[wave] = acc;
+  if (lane == 0) w.y[wave] = acc;
 }
 
-// x2 = x1 + y[aggregate]; t2 = r - S x2 (the neighbours' x2 recomputed from x1 and y)
+// x2 = x1 + P y; t2 = r - S x2 = t1 - S P y, with S P y from the row sums of the blocks
+// (rs[c][0][i] for the cell's own block, rs[c][1 + k][i] for neighbour k)
 template <int NB>
-__global__ __launch_bounds__(kTB) void k_btg_post1(const nfx_batch_tg pc, const float* __restrict__ r,
-                                                   const float* __restrict__ x1, const float* __restrict__ y,
-                                                   float* __restrict__ x2, float* __restrict__ t2) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void st_post1(const nfx_batch_tg& pc, int c) {
   if (c >= pc.n_cells) return;
-  const int32_t* agg_of = pc.agg_of;
-  auto xv = [&](int q, int i) { return x1[(int64_t)q * NB + i] + y[agg_of[q]]; };
-  float rc[NB], s[NB];
-  load_r<NB>(c, pc.prow, r, rc);
-  s_row<NB>(c, pc.nbr, pc.sd, pc.so, xv, s);
+  const GroupWork w = group_work(pc);
+  const float yc = w.y[pc.agg_of[c]];
+  float spy[NB];
+#pragma unroll
+  for (int i = 0; i < NB; ++i) spy[i] = pc.rs[((int64_t)c * 5) * NB + i] * yc;
+  for (int k = 0; k < 4; ++k) {
+    const int q = pc.nbr[4 * c + k];
+    if (q < 0) continue;
+    const float yq = w.y[pc.agg_of[q]];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) spy[i] += pc.rs[((int64_t)c * 5 + 1 + k) * NB + i] * yq;
+  }
 #pragma unroll
   for (int i = 0; i < NB; ++i) {
-    x2[(int64_t)c * NB + i] = xv(c, i);
-    t2[(int64_t)c * NB + i] = rc[i] - s[i];
+    w.x2[(int64_t)c * NB + i] = w.x1[(int64_t)c * NB + i] + yc;
+    w.t2[(int64_t)c * NB + i] = w.t1[(int64_t)c * NB + i] - spy[i];
   }
 }
 
 // z = x2 + W t2 on the rows of cells whose instance is active (mode 0, or all without mode)
 template <int NB>
-__global__ __launch_bounds__(kTB) void k_btg_post2(const nfx_batch_tg pc, const int32_t* __restrict__ mode,
-                                                   const float* __restrict__ x2, const float* __restrict__ t2,
-                                                   float* __restrict__ z) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void st_post2(const nfx_batch_tg& pc, int c, const int32_t* __restrict__ mode,
+                                         float* __restrict__ z) {
   if (c >= pc.n_cells) return;
   if (mode && mode[pc.cell_inst[c]] != 0) return;
+  const GroupWork w = group_work(pc);
   float o[NB];
-  w_row<NB>(c, pc.nbr, pc.wd, pc.wo, [&](int q, int i) { return t2[(int64_t)q * NB + i]; }, o);
+  w_row<NB>(c, pc.nbr, pc.wd, pc.wo, [&](int q, int i) { return w.t2[(int64_t)q * NB + i]; }, o);
 #pragma unroll
-  for (int i = 0; i < NB; ++i) z[pc.prow[(int64_t)c * NB + i]] = x2[(int64_t)c * NB + i] + o[i];
+  for (int i = 0; i < NB; ++i) z[pc.prow[(int64_t)c * NB + i]] = w.x2[(int64_t)c * NB + i] + o[i];
 }
 
-template <int NB>
-int apply_nb(const nfx_batch_tg& pc, const int32_t* mode, const float* r, float* z, hipStream_t st) {
-  const int C = pc.n_cells, A = pc.n_agg;
-  float* x1 = pc.work;
-  float* x2 = x1 + (int64_t)C * NB;
-  float* t2 = x2 + (int64_t)C * NB;
-  float* cv = t2 + (int64_t)C * NB;
-  float* y = cv + A;
-  hipLaunchKernelGGL(k_btg_pre<NB>, dim3(grid_for(C, kTB)), dim3(kTB), 0, st, pc, r, x1);
-  hipLaunchKernelGGL(k_btg_restrict<NB>, dim3(grid_for((int64_t)A * 64, kTB)), dim3(kTB), 0, st, pc, r, x1, cv);
-  hipLaunchKernelGGL(k_btg_gemv, dim3(grid_for((int64_t)A * 64, kTB)), dim3(kTB), 0, st, pc, cv, y);
-  hipLaunchKernelGGL(k_btg_post1<NB>, dim3(grid_for(C, kTB)), dim3(kTB), 0, st, pc, r, x1, y, x2, t2);
-  hipLaunchKernelGGL(k_btg_post2<NB>, dim3(grid_for(C, kTB)), dim3(kTB), 0, st, pc, mode, x2, t2, z);
-  if (pc.n_fb > 0)
-    hipLaunchKernelGGL(k_btg_flux, dim3(grid_for(pc.n_fb, kTB)), dim3(kTB), 0, st, pc, mode, r, z);
+// z = D_g^-1 y on the flux rows of block g of an active instance
+__device__ __forceinline__ void st_flux(const nfx_batch_tg& pc, int g, const int32_t* __restrict__ mode,
+                                        const float* __restrict__ y, float* __restrict__ z) {
+  if (g >= pc.n_fb) return;
+  if (mode && mode[pc.fb_inst[g]] != 0) return;
+  int r[3];
+  float yv[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    r[i] = pc.fb_rows[(int64_t)g * 3 + i];
+    yv[i] = y[r[i]];
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    float s = 0.f;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) s += pc.fb_inv[(int64_t)g * 9 + i * 3 + j] * yv[j];
+    z[r[i]] = s;
+  }
+}
+
+template <int NA, int NBB>
+__global__ __launch_bounds__(kTB) void k_btg_pre(const nfx_batch_tg pa, const nfx_batch_tg pb, int split,
+                                                 const float* __restrict__ r) {
+  const int b = blockIdx.x;
+  if (b < split)
+    st_pre<NA>(pa, b * kTB + threadIdx.x, r);
+  else
+    st_pre<NBB>(pb, (b - split) * kTB + threadIdx.x, r);
+}
+
+template <int NA, int NBB>
+__global__ __launch_bounds__(kTB) void k_btg_restrict(const nfx_batch_tg pa, const nfx_batch_tg pb, int split,
+                                                      const float* __restrict__ r) {
+  const int b = blockIdx.x, lane = threadIdx.x & 63;
+  if (b < split)
+    st_restrict<NA>(pa, (b * kTB + threadIdx.x) >> 6, lane, r);
+  else
+    st_restrict<NBB>(pb, ((b - split) * kTB + threadIdx.x) >> 6, lane, r);
+}
+
+__global__ __launch_bounds__(kTB) void k_btg_gemv(const nfx_batch_tg pa, const nfx_batch_tg pb, int split) {
+  const int b = blockIdx.x, lane = threadIdx.x & 63;
+  if (b < split)
+    st_gemv(pa, (b * kTB + threadIdx.x) >> 6, lane);
+  else
+    st_gemv(pb, ((b - split) * kTB + threadIdx.x) >> 6, lane);
+}
+
+template <int NA, int NBB>
+__global__ __launch_bounds__(kTB) void k_btg_post1(const nfx_batch_tg pa, const nfx_batch_tg pb, int split) {
+  const int b = blockIdx.x;
+  if (b < split)
+    st_post1<NA>(pa, b * kTB + threadIdx.x);
+  else
+    st_post1<NBB>(pb, (b - split) * kTB + threadIdx.x);
+}
+
+// post2 of both groups, then the flux blocks of either group (blocks [split2, grid))
+template <int NA, int NBB>
+__global__ __launch_bounds__(kTB) void k_btg_post2(const nfx_batch_tg pa, const nfx_batch_tg pb, int split,
+                                                   int split2, int split3, const int32_t* __restrict__ mode,
+                                                   const float* __restrict__ y, float* __restrict__ z) {
+  const int b = blockIdx.x;
+  if (b < split)
+    st_post2<NA>(pa, b * kTB + threadIdx.x, mode, z);
+  else if (b < split2)
+    st_post2<NBB>(pb, (b - split) * kTB + threadIdx.x, mode, z);
+  else if (b < split3)
+    st_flux(pa, (b - split2) * kTB + threadIdx.x, mode, y, z);
+  else
+    st_flux(pb, (b - split3) * kTB + threadIdx.x, mode, y, z);
+}
+
+template <int NA, int NBB>
+int apply_pair(const nfx_batch_tg& pa, const nfx_batch_tg* pbp, const int32_t* mode, const float* r, float* z,
+               hipStream_t st) {
+  // a single group runs as the pair (pa, pa) with no blocks for the second member
+  const nfx_batch_tg& pb = pbp ? *pbp : pa;
+  const bool two = pbp != nullptr;
+  const int ca = grid_for(pa.n_cells, kTB), cb = two ? grid_for(pb.n_cells, kTB) : 0;
+  const int aa = grid_for((int64_t)pa.n_agg * 64, kTB), ab = two ? grid_for((int64_t)pb.n_agg * 64, kTB) : 0;
+  const int fa = grid_for(pa.n_fb, kTB), fb = two ? grid_for(pb.n_fb, kTB) : 0;
+  hipLaunchKernelGGL((k_btg_pre<NA, NBB>), dim3(ca + cb), dim3(kTB), 0, st, pa, pb, ca, r);
+  hipLaunchKernelGGL((k_btg_restrict<NA, NBB>), dim3(aa + ab), dim3(kTB), 0, st, pa, pb, aa, r);
+  hipLaunchKernelGGL(k_btg_gemv, dim3(aa + ab), dim3(kTB), 0, st, pa, pb, aa);
+  hipLaunchKernelGGL((k_btg_post1<NA, NBB>), dim3(ca + cb), dim3(kTB), 0, st, pa, pb, ca);
+  hipLaunchKernelGGL((k_btg_post2<NA, NBB>), dim3(ca + cb + fa + fb), dim3(kTB), 0, st, pa, pb, ca, ca + cb,
+                     ca + cb + fa, mode, r, z);
   NFX_LAUNCH_CHECK();
   return NFX_OK;
 }
@@ -452,8 +547,8 @@ int check_desc(const nfx_batch_tg* pc) {
   NFX_REQUIRE(pc, "nfx_batch_tg: null descriptor");
   NFX_REQUIRE(pc->nb == 1 || pc->nb == 4, "nfx_batch_tg: nb must be 1 or 4");
   NFX_REQUIRE(pc->n_cells > 0 && pc->n_agg > 0, "nfx_batch_tg: empty group");
-  NFX_REQUIRE(pc->cell_inst && pc->prow && pc->nbr && pc->sd && pc->so && pc->wd && pc->agg_ptr && pc->agg_mem &&
-                  pc->agg_of && pc->arow && pc->a0 && pc->alen && pc->ainv && pc->work,
+  NFX_REQUIRE(pc->cell_inst && pc->prow && pc->nbr && pc->sd && pc->so && pc->rs && pc->wd && pc->agg_ptr &&
+                  pc->agg_mem && pc->agg_of && pc->arow && pc->a0 && pc->alen && pc->ainv && pc->work,
               "nfx_batch_tg: null pointer");
   NFX_REQUIRE(pc->nb == 1 || !pc->wo, "nfx_batch_tg: smoother couplings need nb = 1");
   NFX_REQUIRE(pc->n_fb >= 0 && (pc->n_fb == 0 || (pc->fb_inst && pc->fb_rows && pc->fb_inv)),
@@ -463,10 +558,17 @@ int check_desc(const nfx_batch_tg* pc) {
 
 }  // namespace
 
-int batch_tg_apply(const nfx_batch_tg* pc, const int32_t* mode, const float* r, float* z, hipStream_t st) {
-  const int rc = check_desc(pc);
-  if (rc) return rc;
-  return pc->nb == 1 ? apply_nb<1>(*pc, mode, r, z, st) : apply_nb<4>(*pc, mode, r, z, st);
+int batch_tg_apply(const nfx_batch_tg* pc, int n_groups, const int32_t* mode, const float* r, float* z,
+                   hipStream_t st) {
+  NFX_REQUIRE(pc && (n_groups == 1 || n_groups == 2), "nfx_batch_tg_apply: one or two groups");
+  for (int g = 0; g < n_groups; ++g) {
+    const int rc = check_desc(pc + g);
+    if (rc) return rc;
+  }
+  if (n_groups == 1)
+    return pc->nb == 1 ? apply_pair<1, 1>(pc[0], nullptr, mode, r, z, st) : apply_pair<4, 4>(pc[0], nullptr, mode, r, z, st);
+  NFX_REQUIRE(pc[0].nb != pc[1].nb, "nfx_batch_tg_apply: two groups need different nb");
+  return pc[0].nb == 1 ? apply_pair<1, 4>(pc[0], pc + 1, mode, r, z, st) : apply_pair<1, 4>(pc[1], pc, mode, r, z, st);
 }
 
 }  // namespace nfx
@@ -559,9 +661,25 @@ int nfx_batch_tg_coarse(int nb, int n_agg, const int32_t* d_agg_ptr, const int32
   return NFX_OK;
 }
 
-int nfx_batch_tg_apply(const nfx_batch_tg* pc, const int32_t* d_mode, const float* d_r, float* d_z, void* stream) {
+int nfx_batch_tg_rowsums(int nb, int n_cells, const double* d_sd64, const double* d_so64, float* d_rs,
+                         void* stream) {
+  NFX_REQUIRE(nb == 1 || nb == 4, "nfx_batch_tg_rowsums: nb must be 1 or 4");
+  NFX_REQUIRE(n_cells >= 0, "nfx_batch_tg_rowsums: n_cells < 0");
+  if (n_cells == 0) return NFX_OK;
+  NFX_REQUIRE(d_sd64 && d_so64 && d_rs, "nfx_batch_tg_rowsums: null pointer");
+  hipStream_t st = (hipStream_t)stream;
+  if (nb == 1)
+    hipLaunchKernelGGL(k_btg_rowsums<1>, dim3(grid_for(n_cells, kTB)), dim3(kTB), 0, st, n_cells, d_sd64, d_so64, d_rs);
+  else
+    hipLaunchKernelGGL(k_btg_rowsums<4>, dim3(grid_for(n_cells, kTB)), dim3(kTB), 0, st, n_cells, d_sd64, d_so64, d_rs);
+  NFX_LAUNCH_CHECK();
+  return NFX_OK;
+}
+
+int nfx_batch_tg_apply(int n_groups, const nfx_batch_tg* pc, const int32_t* d_mode, const float* d_r, float* d_z,
+                       void* stream) {
   NFX_REQUIRE(d_r && d_z, "nfx_batch_tg_apply: null vector");
-  return nfx::batch_tg_apply(pc, d_mode, d_r, d_z, (hipStream_t)stream);
+  return nfx::batch_tg_apply(pc, n_groups, d_mode, d_r, d_z, (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -85,7 +85,9 @@ int twogrid_apply_scaled(const nfx_twogrid_pc* pc, const float* r, float* z, con
 int el_twogrid_apply_scaled(const nfx_elasticity_tg* pc, const float* r, float* z, const double* dscale,
                             hipStream_t stream);
 
-// Batched two-grid Schur cycle on a group's rows (nfx_batch_tg_apply); mode as there.
-int batch_tg_apply(const nfx_batch_tg* pc, const int32_t* mode, const float* r, float* z, hipStream_t stream);
+// Batched two-grid Schur cycle on the rows of one group or of two groups of different nb
+// in shared launches (nfx_batch_tg_apply); mode as there.
+int batch_tg_apply(const nfx_batch_tg* pc, int n_groups, const int32_t* mode, const float* r, float* z,
+                   hipStream_t stream);
 
 }  // namespace nfx

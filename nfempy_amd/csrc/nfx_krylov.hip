@@ -1327,6 +1327,8 @@ extern "C" int64_t nfx_minres_batch_rows(const nfx_minres_batch* s) { return s ?
 
 extern "C" int nfx_minres_batch_set_pc(nfx_minres_batch* s, int n_groups, const nfx_batch_tg* h_groups) {
   NFX_REQUIRE(s && n_groups >= 0 && (n_groups == 0 || h_groups), "nfx_minres_batch_set_pc: bad arguments");
+  NFX_REQUIRE(n_groups <= 2 && (n_groups < 2 || h_groups[0].nb != h_groups[1].nb),
+              "nfx_minres_batch_set_pc: at most two groups, of different nb");
   std::vector<nfx_batch_tg> g(h_groups, h_groups + n_groups);
   for (const nfx_batch_tg& d : g) {
     NFX_REQUIRE(d.nb == 1 || d.nb == 4, "nfx_minres_batch_set_pc: nb must be 1 or 4");
@@ -1341,10 +1343,8 @@ extern "C" int nfx_minres_batch_set_pc(nfx_minres_batch* s, int n_groups, const 
 // z on the groups' rows of active instances from y, then the tile partials of y'.z again.
 static int bm_apply_pc(nfx_minres_batch* s, const float* y, float* z) {
   if (s->pc.empty() || !s->n_act) return NFX_OK;
-  for (const nfx_batch_tg& g : s->pc) {
-    const int rc = nfx::batch_tg_apply(&g, s->mode, y, z, s->stream);
-    if (rc) return rc;
-  }
+  const int rc = nfx::batch_tg_apply(s->pc.data(), (int)s->pc.size(), s->mode, y, z, s->stream);
+  if (rc) return rc;
   hipLaunchKernelGGL(k_bm_dot, dim3(s->n_act), dim3(kBmTile), 0, s->stream, s->items, s->tiles, s->act, s->mode, y,
                      z, s->pb);
   NFX_LAUNCH_CHECK();
@@ -1354,7 +1354,7 @@ static int bm_apply_pc(nfx_minres_batch* s, const float* y, float* z) {
 // One batched FP32 MINRES on every instance whose h_mode is 0 on entry (rhs already scaled);
 // iters[i] gets the instance's iteration count.
 
-static int bm_inner(nfx_minres_batch* s, const float* rhs, float* x, double tol, int maxit, int* iters) {
+static int bm_inner(nfx_minres_batch* s, const float* rhs, float* x, const double* tol, int maxit, int* iters) {
   const int64_t N = s->n_total;
   float* V = s->buf;
   float* P[2] = {V + N * 0, V + N * 1};
@@ -1414,7 +1414,7 @@ static int bm_inner(nfx_minres_batch* s, const float* rhs, float* x, double tol,
       bool changed = false;
       for (int i : active) {
         const MinresState& q = s->h_st[i];
-        const bool done = q.stop || !(q.beta1 > 0) || q.phibar <= tol * q.beta1 || k >= maxit;
+        const bool done = q.stop || !(q.beta1 > 0) || q.phibar <= tol[i] * q.beta1 || k >= maxit;
         if (done) {
           s->h_mode[i] = 1;
           iters[i] = k;
@@ -1449,6 +1449,7 @@ extern "C" int nfx_solve_minres_batch_ir(nfx_minres_batch* s, const double* d_b,
   float* rf = s->buf + N * 8;
   float* dx = s->buf + N * 9;
   std::vector<int> outer_active(m, 1), it(m);
+  std::vector<double> tol(m, inner_rtol);
   for (int i = 0; i < m; ++i) {
     h_iters[i] = 0;
     h_outer[i] = 0;
@@ -1487,6 +1488,10 @@ extern "C" int nfx_solve_minres_batch_ir(nfx_minres_batch* s, const double* d_b,
       } else {
         s->h_mode[i] = 0;
         ++n_act;
+        // the correction needs only the reduction still missing (the recurrence's
+        // estimate is in the preconditioner's norm: half of it as a margin), never less
+        // than inner_rtol
+        tol[i] = std::min(0.5, std::max(inner_rtol, 0.5 * rtol / rel));
       }
     }
     if (!n_act) break;
@@ -1499,7 +1504,7 @@ extern "C" int nfx_solve_minres_batch_ir(nfx_minres_batch* s, const double* d_b,
     NFX_LAUNCH_CHECK();
     // the inner solve changes h_mode: keep the outer set for the update that follows
     std::vector<int32_t> mode_outer(s->h_mode, s->h_mode + m);
-    rc = bm_inner(s, rf, dx, inner_rtol, max_inner, it.data());
+    rc = bm_inner(s, rf, dx, tol.data(), max_inner, it.data());
     if (rc) return rc;
     for (int i = 0; i < m; ++i) {
       h_iters[i] += it[i];

@@ -3,8 +3,9 @@
 Every instance of a mixed RT1-P0 / RT2-P1 batch reaches an FP64 relative residual of
 1e-8 against the oracle-assembled system (scipy's sparse LU of that system agrees to
 1e-6); an instance's result does not depend on the other instances of the batch
-(bitwise equal when it is solved alone); the per-instance iteration counts match the
-single-system MINRES to within a few iterations (same recurrence, other reduction order).
+(bitwise equal when it is solved alone); the per-instance iteration counts stay within a
+check interval of the single-system MINRES's or below (same recurrence, other reduction
+order, refinement steps that stop at the reduction still missing).
 """
 
 import numpy as np
@@ -41,9 +42,10 @@ def test_batched_minres_against_oracle_and_alone():
         err = np.linalg.norm(x - xr) / np.linalg.norm(xr)
         print(f"[batch solve] instance {i} (n={n}, k={k}): residual vs oracle {res:.1e}, vs LU {err:.1e}", flush=True)
         assert res <= 1.2e-8 and err <= 1e-6, (i, res, err)
-        # the single-system MINRES: same recurrence, iteration counts within a check interval
+        # the single-system MINRES: same recurrence; the batched refinement steps stop at the
+        # reduction still missing, so never more iterations than it (within a check interval)
         _, single = b.solve(i)
-        assert abs(single["iterations"] - info["iterations"]) <= 32, (i, single["iterations"], info["iterations"])
+        assert info["iterations"] <= single["iterations"] + 32, (i, single["iterations"], info["iterations"])
     # instance 2 alone: bitwise the same result
     xa, ia, _ = b.solve_batched([2])
     assert ia[0]["iterations"] == infos[2]["iterations"]
