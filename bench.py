@@ -467,6 +467,24 @@ def bench_batch(args, log):
                "iterations_mean": round(float(np.mean([q["iterations"] for q in binfos])), 1)}
     log(f"batched solve of all {m} instances: {t_batched:.2f} s ({t_iter:.2f} s MINRES), "
         f"{batched['converged']} converged, max residual {batched['max_residual']:.2e}")
+    # the same with the batched two-grid Schur preconditioner (learned smoother on the RT1-P0
+    # instances, RT2-P1 flux blocks): the first call also builds the per-mesh structure
+    if not args.no_batch_twogrid:
+        torch.cuda.synchronize()
+        tb = time.perf_counter()
+        _, tinfos, t_tg = b.solve_batched(rtol=1e-8, pc="learned")
+        torch.cuda.synchronize()
+        t_all = time.perf_counter() - tb
+        batched.update({
+            "learned_twogrid_time_s": round(t_all, 3), "learned_twogrid_setup_plus_minres_s": round(t_tg, 3),
+            "learned_twogrid_converged": sum(1 for q in tinfos if q["converged"]),
+            "learned_twogrid_max_residual": max(q["residual"] for q in tinfos),
+            "learned_twogrid_iterations_max": max(q["iterations"] for q in tinfos),
+            "learned_twogrid_iterations_mean": round(float(np.mean([q["iterations"] for q in tinfos])), 1),
+            "speedup_learned_twogrid_vs_plain_batched_minres": round(t_iter / t_tg, 2)})
+        log(f"batched two-grid solve of all {m} instances: {t_all:.2f} s (set-up + MINRES {t_tg:.2f} s), "
+            f"{batched['learned_twogrid_converged']} converged, iterations max "
+            f"{batched['learned_twogrid_iterations_max']}")
     # solve a stratified sample one instance at a time: smallest and largest of each degree
     sample = []
     for k in (1, 2):
@@ -773,6 +791,7 @@ def main():
                     help="run the batched workload instead: 256 instances, n ~ U{8..32}, k in {1, 2}")
     ap.add_argument("--batch-instances", type=int, default=256)
     ap.add_argument("--batch-solves", type=int, default=8, help="instances solved after the batched assembly")
+    ap.add_argument("--no-batch-twogrid", action="store_true", help="--batch: skip the batched two-grid solve")
     ap.add_argument("--batch-solve-instances", type=int, default=32,
                     help="instances of the default run's batched assembly + solve sample (0: skip)")
     ap.add_argument("--batch-solve-nmin", type=int, default=4)
