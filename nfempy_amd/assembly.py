@@ -85,6 +85,8 @@ class Mat:
         from .device import torch
 
         t = torch()
+        if self.type == "sbaij" and self._sys is not None:
+            raise NotImplementedError("generic forms into an SBAIJ matrix that holds a device elasticity operator")
         blocks = []
         dev_vals = None
         if self._sys is not None:
@@ -95,6 +97,10 @@ class Mat:
         gr = np.concatenate([b[0] for b in self._gen])
         gc = np.concatenate([b[1] for b in self._gen])
         gv = np.concatenate([b[2] for b in self._gen])
+        if self.type == "sbaij":
+            # SBAIJ stores the upper triangle; lower entries are ignored
+            up = gr <= gc
+            gr, gc, gv = gr[up], gc[up], gv[up]
         blocks.append((gr, gc))
         rows = np.concatenate([b[0] for b in blocks])
         cols = np.concatenate([b[1] for b in blocks])

@@ -218,8 +218,6 @@ class BatchTwoGrid:
         self._gnn = None
         g1 = [g for g in self.groups if g.nb == 1]
         if theta is not None and g1:
-            from .learned import load_params  # noqa: F401  (weights come from the caller)
-
             g = g1[0]
             E = g.C
             th = t.as_tensor(np.asarray(theta, dtype=np.float32), device=device)
