@@ -289,6 +289,99 @@ def cpu_solve_baseline(n: int, seed: int, log):
     }
 
 
+def bench_c1(log, n=32):
+    """The reference's own CPU-runnable case (C1): 2-D Darcy RT1-P0 on the unit square, n x n
+    squares split into 2 triangles, K = 1, f = 2 pi^2 sin(pi x) sin(pi y), p_D = 0 (exact
+    p = sin(pi x) sin(pi y)). GPU: FP64 assembly (pattern reused) and the FP32 MINRES + FP64
+    refinement solve to 1e-8; CPU (1 core): the oracle's per-element assembly + COO scatter
+    and scipy's sparse LU (the reference's direct solve). Reports both times, the CSR
+    difference and the L2 errors of both solutions against the exact fields."""
+    import scipy.sparse as sps
+    import scipy.sparse.linalg as spla
+    import torch
+
+    import oracle.mesh as om
+    import oracle.norms as onorms
+    import oracle.spaces as osp
+    from nfempy_amd.device import DarcySystem
+    from nfempy_amd.mesh import Mesh, unit_square
+    from nfempy_amd.postprocess import l2_error
+    from nfempy_amd.solvers import MinresIR, darcy_preconditioner
+    from nfempy_amd.spaces import ProductSpace
+    from oracle.assemble import assemble_darcy
+
+    PI = np.pi
+    fun = {"rhs": lambda x, y, z: np.array([2 * PI * PI * np.sin(PI * x) * np.sin(PI * y)]),
+           "kappa": lambda x, y, z: 1.0}
+    ex = {"p": lambda x, y, z: np.array([np.sin(PI * x) * np.sin(PI * y)]),
+          "u": lambda x, y, z: np.array([[-PI * np.cos(PI * x) * np.sin(PI * y), -PI * np.sin(PI * x) * np.cos(PI * y)]])}
+    pD = lambda x, y, z: np.zeros((1,) + np.shape(x))
+    mesh = Mesh.from_blocks(2, *unit_square(n))
+    ps = ProductSpace({"u": (2, 1, "RT", 1, mesh), "p": (2, 1, "Lagrange", 0, mesh)})
+    ps.make_subspaces_discontinuous({"p": True})
+    ps.build_structures({"u": [1], "p": [1]}, {"u": [2, 3, 4, 5]})
+    sysd = DarcySystem(ps)
+    for _ in range(2):
+        sysd.assemble(fun["kappa"], fun["rhs"], pD, fp64=True)
+        sysd.assemble(fun["kappa"], fun["rhs"], pD, fp64=False)
+    torch.cuda.synchronize()
+    # the device values from host callables: the timed assembly includes their evaluation and
+    # upload, as a drop-in caller's would
+    t0 = time.perf_counter()
+    sysd.assemble(fun["kappa"], fun["rhs"], pD, fp64=True)
+    sysd.assemble(fun["kappa"], fun["rhs"], pD, fp64=False)
+    torch.cuda.synchronize()
+    t_asm = time.perf_counter() - t0
+
+    def gsolve():
+        sign, pinv = darcy_preconditioner(sysd.rowptr, sysd.col, sysd.vals64, sysd.n, sysd.n_u)
+        sol = MinresIR(sysd.rowptr, sysd.col, sysd.vals32, sysd.vals64, sign, pinv)
+        x, info = sol.solve(-sysd.rhs64, rtol=1e-8, max_inner=100000)
+        sol.close()
+        return x, info
+
+    gsolve()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    xg, ig = gsolve()
+    torch.cuda.synchronize()
+    t_sol = time.perf_counter() - t0
+    xg = xg.cpu().numpy()
+    # CPU, one core: the oracle
+    pts, bl = om.structured_square(n)
+    m = om.ConformalMesh(2, pts, bl)
+    ops = osp.ProductSpace({"u": (2, 1, "RT", 1, m), "p": (2, 1, "Lagrange", 0, m)}, {"p": True})
+    ops.build_structures({"u": [1], "p": [1]}, {"u": [2, 3, 4, 5]})
+    ctx = _one_core()
+    try:
+        t0 = time.perf_counter()
+        A, rg = assemble_darcy(ops, fun, {"p": pD})
+        t_casm = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        xc = spla.spsolve(A.tocsc(), -rg)
+        t_csol = time.perf_counter() - t0
+    finally:
+        if ctx is not None:
+            ctx.unregister()
+    G = sps.csr_matrix((sysd.vals64.cpu().numpy(), sysd.pattern.col, sysd.pattern.rowptr), shape=A.shape)
+    same_pattern = bool(np.array_equal(sysd.pattern.rowptr, A.indptr) and np.array_equal(sysd.pattern.col, A.indices))
+    e_g = [float(v) for v in l2_error(2, ps, ex, xg)]
+    e_c = [float(v) for v in onorms.l2_error(ops, ex, xc)]
+    out = {
+        "workload": f"darcy2d-rt1p0-square{n}x{n}x2-K1-manufactured", "elements": int(sysd.E), "dofs": int(sysd.n),
+        "nnz": int(sysd.nnz), "gpu_assembly_fp64_fp32_s": round(t_asm, 5), "gpu_solve_s": round(t_sol, 5),
+        "gpu_iterations": ig["iterations"], "gpu_residual": ig["residual"],
+        "cpu_1core_assembly_s": round(t_casm, 4), "cpu_1core_lu_s": round(t_csol, 4),
+        "speedup_assembly_plus_solve": round((t_casm + t_csol) / (t_asm + t_sol), 1),
+        "pattern_bit_exact": same_pattern,
+        "max_rel_value_diff_fp64": float(abs(G - A).max() / abs(A).max()),
+        "l2_errors_gpu_u_p": e_g, "l2_errors_cpu_u_p": e_c,
+    }
+    log(f"C1 {n}x{n}x2: GPU assembly {t_asm * 1e3:.2f} ms + solve {t_sol * 1e3:.2f} ms; CPU 1 core "
+        f"{t_casm:.2f} s + LU {t_csol * 1e3:.1f} ms; errors GPU {e_g}, CPU {e_c}")
+    return out
+
+
 def bench_elasticity(args, log):
     """Dual-mixed elasticity, 16^3 x 6 Kuhn tets, BDM1-P0-P0, Lame parameters log-normal per element."""
     import torch
@@ -349,10 +442,19 @@ def bench_elasticity(args, log):
                 "frac": round(achieved / 8000.0, 4), "traffic": profiled_traffic(kname, n),
                 "kernel": kname, "kernel_ms": round(k_ms, 5), "alg_bytes_per_tet": round(alg, 2)}
     log(f"elasticity: {kname} {k_ms * 1e3:.2f} us/launch, {achieved:.1f} GB/s algorithmic")
+    # matrix-core utilisation of the element contraction: the useful flops of the element
+    # matrix, 2 nq (12 * 12 * 3 + 36 * 36 + 2 * 36 + 36) per tet, against the FP32 matrix peak;
+    # the issued 16 x 16 x 4 tiles (6 for the D block and 2 per needed facet block, per
+    # incidence, 2,048 flop each) in "issued_tflops"
+    useful = 2 * 8 * (12 * 12 * 3 + 36 * 36 + 2 * 36 + 36)
+    issued = 4 * E * (6 + 2 * 2.5) * 2048
+    mfma = {"useful_flop_per_tet": useful, "useful_tflops": round(useful * E / (k_ms * 1e-3) / 1e12, 2),
+            "issued_tflops": round(issued / (k_ms * 1e-3) / 1e12, 2), "peak_fp32_matrix_tflops": 157.3,
+            "useful_frac": round(useful * E / (k_ms * 1e-3) / 1e12 / 157.3, 4)}
     out = {"workload": f"elasticity3d-bdm1p0p0-kuhn{n}x6-lognormal-lame(sigma=0.5)", "elements": E, "dofs": sysd.n,
            "nnz_upper": nnz_u, "cold_setup_s": round(t1 - t0, 3),
            "ms_per_assembly": round(ms, 4), "tets_per_s": round(E / (ms * 1e-3), 1),
-           "kernel_tets_per_s": round(E / (k_ms * 1e-3), 1), "roofline": roofline}
+           "kernel_tets_per_s": round(E / (k_ms * 1e-3), 1), "roofline": roofline, "mfma": mfma}
     if getattr(args, "no_elasticity_solve", False):
         return out
     sysd.assemble("A", fp64=True, u_dirichlet=uD)
@@ -801,6 +903,7 @@ def main():
     ap.add_argument("--md-steps", type=int, default=200)
     ap.add_argument("--device-numbering", action="store_true",
                     help="cold path sorts (numbering, pattern) on the device instead of the host")
+    ap.add_argument("--no-c1", action="store_true", help="leave the 2-D reference case out of the default line")
     ap.add_argument("--no-md", action="store_true", help="leave the mixed-dimensional run out of the default line")
     ap.add_argument("--md-pc", choices=("learned", "twogrid", "diag"), default="learned")
     ap.add_argument("--md-compare-steps", type=int, default=10)
@@ -940,6 +1043,9 @@ def main():
     md = None
     if not args.no_md:
         md = bench_md(args, log)
+    c1 = None
+    if not args.no_c1:
+        c1 = bench_c1(log)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -982,6 +1088,8 @@ def main():
         out["batch_solve"] = batch_solve
     if md:
         out["mixed_dimensional"] = md
+    if c1:
+        out["c1_reference_case"] = c1
     print(json.dumps(out), flush=True)
 
 
