@@ -290,7 +290,7 @@ def cpu_solve_baseline(n: int, seed: int, log):
 
 
 def bench_c1(log, n=32):
-    """The reference's own CPU-runnable case (C1): 2-D Darcy RT1-P0 on the unit square, n x n
+    """The reference's own CPU-runnable case: 2-D Darcy RT1-P0 on the unit square, n x n
     squares split into 2 triangles, K = 1, f = 2 pi^2 sin(pi x) sin(pi y), p_D = 0 (exact
     p = sin(pi x) sin(pi y)). GPU: FP64 assembly (pattern reused) and the FP32 MINRES + FP64
     refinement solve to 1e-8; CPU (1 core): the oracle's per-element assembly + COO scatter
@@ -377,7 +377,7 @@ def bench_c1(log, n=32):
         "max_rel_value_diff_fp64": float(abs(G - A).max() / abs(A).max()),
         "l2_errors_gpu_u_p": e_g, "l2_errors_cpu_u_p": e_c,
     }
-    log(f"C1 {n}x{n}x2: GPU assembly {t_asm * 1e3:.2f} ms + solve {t_sol * 1e3:.2f} ms; CPU 1 core "
+    log(f"2-D reference case {n}x{n}x2: GPU assembly {t_asm * 1e3:.2f} ms + solve {t_sol * 1e3:.2f} ms; CPU 1 core "
         f"{t_casm:.2f} s + LU {t_csol * 1e3:.1f} ms; errors GPU {e_g}, CPU {e_c}")
     return out
 
@@ -1089,7 +1089,7 @@ def main():
     if md:
         out["mixed_dimensional"] = md
     if c1:
-        out["c1_reference_case"] = c1
+        out["reference_2d_case"] = c1
     print(json.dumps(out), flush=True)
 
 

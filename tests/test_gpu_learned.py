@@ -174,7 +174,7 @@ def test_coarse_inverse_and_graph_replays():
         assert torch.allclose(pc.Ainv, pc.Ainv.T, rtol=0, atol=1e-6 * float(eager.abs().max()))
         _ = (sysd.vals32 * 2).sum()  # eager work between replays
         torch.cuda.synchronize()
-    # full C4 size: 4096 aggregates, banded recursion with fill (eager and graph)
+    # the 32^3 x 6 size: 4096 aggregates, banded recursion with fill (eager and graph)
     print("[learned] coarse inverse n=32", flush=True)
     sysd, _, f = _system(32, seed=10000, ell=0.05)
     pc = LearnedSchurPC(sysd, None)
