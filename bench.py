@@ -636,6 +636,16 @@ def bench_batch_solve(args, log):
     b.capture()
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t0
+    # the batched assembly alone (captured graph replays of the degree-specialised launches)
+    for _ in range(5):
+        b.assemble()
+    torch.cuda.synchronize()
+    ta = time.perf_counter()
+    reps = 50
+    for _ in range(reps):
+        b.assemble()
+    torch.cuda.synchronize()
+    asm_ms = (time.perf_counter() - ta) * 1e3 / reps
     b.assemble()
     b.solve_batched(rtol=1e-8)  # warm-up (allocations, first launches)
     torch.cuda.synchronize()
@@ -658,6 +668,7 @@ def bench_batch_solve(args, log):
                        "lognormal(l=0.1,sigma=1)",
            "instances": m, "elements": b.n_elements, "rt1_instances": sum(1 for _, k in specs if k == 1),
            "setup_s": round(t_setup, 2),
+           "batched_assembly_ms": round(asm_ms, 4), "batched_assembly_tets_per_s": round(b.n_elements / (asm_ms * 1e-3), 1),
            "batched_assembly_plus_solve_s": round(t_batched, 4), "batched_minres_s": round(t_iter, 4),
            "sequential_assembly_plus_solve_s": round(t_seq, 4),
            "speedup_batched_vs_sequential": round(t_seq / t_batched, 2),
