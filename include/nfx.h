@@ -666,6 +666,10 @@ int nfx_spd_inverse_small(int n, float* d_A, int lda, void* stream);
  * and tail_dinv = 1 / S_ff; then r_p <- r_p - sum slot_coef r_f before the cycle (S in
  * ELL form then holds the cells' Schur complement S - S_pf S_ff^-1 S_fq) and
  * z_f = tail_dinv (r_f - sum tail_coef z_p) after it.
+ * Flux rows: when flux_rowptr is not NULL, two damped Jacobi sweeps on the flux mass
+ * block M replace the diagonal scaling, z_u = flux_omega dMinv (2 r_u - flux_omega
+ * M (dMinv r_u)), with M the entries of rows < n_u and columns < n_u of the FP32 CSR
+ * (flux_rowptr, flux_col, flux_val), read at every apply (NULL: z_u = dMinv r_u).
  * work: 3 n_cells + 2 n_agg floats (4 n_cells + 2 n_agg with slot_tail). */
 typedef struct nfx_twogrid_pc {
   int n_u, n_cells, n_agg;
@@ -689,6 +693,10 @@ typedef struct nfx_twogrid_pc {
   const float* slot_coef;
   const int32_t* tail_cells;
   const float* tail_coef;
+  const int32_t* flux_rowptr;
+  const int32_t* flux_col;
+  const float* flux_val;
+  float flux_omega;
 } nfx_twogrid_pc;
 int nfx_twogrid_apply(const nfx_twogrid_pc* pc, const float* d_r, float* d_z, void* stream);
 /* C = beta C + alpha op(A) op(B) (FP32, row-major, leading dimensions lda/ldb/ldc;

@@ -157,7 +157,8 @@ class TwoGridPC(ctypes.Structure):
                 ("Ascale", c_void_p), ("dMinv", c_void_p), ("psign", c_float), ("work", c_void_p),
                 ("n_tail", c_int), ("tail_dinv", c_void_p),
                 ("slot_tail", c_void_p), ("slot_coef", c_void_p), ("tail_cells", c_void_p),
-                ("tail_coef", c_void_p)]
+                ("tail_coef", c_void_p), ("flux_rowptr", c_void_p), ("flux_col", c_void_p),
+                ("flux_val", c_void_p), ("flux_omega", c_float)]
 
 
 class MinresBatchItem(ctypes.Structure):

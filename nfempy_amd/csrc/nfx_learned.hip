@@ -535,6 +535,48 @@ __global__ void k_tg_pre(int E, const int32_t* __restrict__ nbr, const float* __
   x1[p] = ell_row(p, nbr, Wd, Wo, r);
 }
 
+// Blocks [0, cell_blocks): x1 = W r on the cells, as k_tg_pre. Blocks after them: the
+// flux rows as two damped Jacobi sweeps on the flux mass block M (the CSR entries of
+// rows < n_u with columns < n_u), in closed form
+// z_u = omega dMinv (2 r_u - omega M (dMinv r_u)), four lanes per row; scaled by
+// 1 / *dscale like the pressure rows.
+constexpr int kFluxLanes = 4;
+__global__ __launch_bounds__(256) void k_tg_pre_flux(int E, int cell_blocks, const int32_t* __restrict__ nbr,
+                                                     const float* __restrict__ Wd, const float* __restrict__ Wo,
+                                                     const float* __restrict__ rp, float* __restrict__ x1, int n_u,
+                                                     const int32_t* __restrict__ rowptr,
+                                                     const int32_t* __restrict__ col,
+                                                     const float* __restrict__ val,
+                                                     const float* __restrict__ dMinv, const float* __restrict__ r,
+                                                     float omega, const double* __restrict__ dscale,
+                                                     float* __restrict__ z) {
+  if ((int)blockIdx.x < cell_blocks) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < E) x1[p] = ell_row(p, nbr, Wd, Wo, rp);
+    return;
+  }
+  const int64_t t = (int64_t)(blockIdx.x - cell_blocks) * blockDim.x + threadIdx.x;
+  const int64_t i = t / kFluxLanes;
+  const int sub = (int)(t % kFluxLanes);
+  float acc = 0.f;
+  if (i < n_u) {
+    const int e = rowptr[i + 1];
+    for (int k = rowptr[i] + sub; k < e; k += kFluxLanes) {
+      const int c = col[k];
+      if (c < n_u) acc += val[k] * (dMinv[c] * r[c]);
+    }
+  }
+#pragma unroll
+  for (int off = kFluxLanes / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, kFluxLanes);
+  if (i >= n_u || sub != 0) return;
+  float sc = 1.f;
+  if (dscale) {
+    const double d = *dscale;
+    sc = d > 0 ? (float)(1.0 / d) : 0.f;
+  }
+  z[i] = sc * omega * dMinv[i] * (2.f * r[i] - omega * acc);
+}
+
 // c[a] = sum_{p in a} (r - S x1)_p   (one wave per aggregate)
 __global__ __launch_bounds__(256) void k_tg_restrict(int n_agg, const int32_t* __restrict__ agg_ptr,
                                                      const int32_t* __restrict__ agg_mem,
@@ -837,13 +879,14 @@ __global__ void k_tg_post1(int E, const int32_t* __restrict__ nbr, const int32_t
 // z_f = tail_dinv (r_f - sum_k tail_coef z_p(k)) with z of the two cells recomputed here
 // (same arithmetic as their own rows), or tail_dinv r_f when tail_cells is null
 // (dscale: optional device scalar; the output is divided by it, 0 gives 0)
-__global__ void k_tg_post2(int n_u, int E, const int32_t* __restrict__ nbr, const float* __restrict__ Wd,
+// (rows from i0 on: i0 = n_u when the flux rows were written by k_tg_pre_flux, else 0)
+__global__ void k_tg_post2(int i0, int n_u, int E, const int32_t* __restrict__ nbr, const float* __restrict__ Wd,
                            const float* __restrict__ Wo, const float* __restrict__ x2,
                            const float* __restrict__ t2, const float* __restrict__ dMinv,
                            const float* __restrict__ r, float psign, const double* __restrict__ dscale,
                            int n_tail, const float* __restrict__ tail_dinv, const int32_t* __restrict__ tail_cells,
                            const float* __restrict__ tail_coef, float* __restrict__ z) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = i0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   float sc = 1.f;
   if (dscale) {
     const double d = *dscale;
@@ -1130,6 +1173,8 @@ int nfx::twogrid_apply_scaled(const nfx_twogrid_pc* pc, const float* r, float* z
   NFX_REQUIRE(pc->nbr && pc->Wd && pc->Wo && pc->sd && pc->so && pc->agg_ptr && pc->agg_mem && pc->agg_of &&
                   (pc->Ainv || (pc->Ainv16 && pc->Ascale)) && r && z && pc->work && (n_u == 0 || pc->dMinv),
               "nfx_twogrid_apply: null pointer");
+  NFX_REQUIRE(!pc->flux_rowptr || (pc->flux_col && pc->flux_val),
+              "nfx_twogrid_apply: incomplete flux-row CSR");
   const bool lu = pc->slot_tail != nullptr;
   NFX_REQUIRE(!lu || (pc->n_tail > 0 && pc->slot_coef && pc->tail_cells && pc->tail_coef),
               "nfx_twogrid_apply: incomplete tail coupling");
@@ -1148,7 +1193,15 @@ int nfx::twogrid_apply_scaled(const nfx_twogrid_pc* pc, const float* r, float* z
   }
   // (a fused x1 + restriction kernel, recomputing x1 at the members' neighbours, measured
   // 18.6 us against 4.8 + 5.9 us for the two launches at 196,608 cells)
-  hipLaunchKernelGGL(k_tg_pre, dim3(grid_for(E, 256)), dim3(256), 0, st, E, pc->nbr, pc->Wd, pc->Wo, rp, x1);
+  const bool sweeps = pc->flux_rowptr != nullptr && n_u > 0;
+  if (sweeps) {
+    const int cb = grid_for(E, 256);
+    const int fb = grid_for((int64_t)n_u * kFluxLanes, 256);
+    hipLaunchKernelGGL(k_tg_pre_flux, dim3(cb + fb), dim3(256), 0, st, E, cb, pc->nbr, pc->Wd, pc->Wo, rp, x1, n_u,
+                       pc->flux_rowptr, pc->flux_col, pc->flux_val, pc->dMinv, r, pc->flux_omega, dscale, z);
+  } else {
+    hipLaunchKernelGGL(k_tg_pre, dim3(grid_for(E, 256)), dim3(256), 0, st, E, pc->nbr, pc->Wd, pc->Wo, rp, x1);
+  }
   hipLaunchKernelGGL(k_tg_restrict, dim3(grid_for((int64_t)n_agg * 64, 256)), dim3(256), 0, st, n_agg, pc->agg_ptr,
                      pc->agg_mem, pc->nbr, pc->sd, pc->so, rp, x1, c);
   if (pc->Ainv16)
@@ -1158,7 +1211,8 @@ int nfx::twogrid_apply_scaled(const nfx_twogrid_pc* pc, const float* r, float* z
     hipLaunchKernelGGL(k_gemv, dim3(grid_for((int64_t)n_agg * 64, 256)), dim3(256), 0, st, n_agg, pc->Ainv, c, y);
   hipLaunchKernelGGL(k_tg_post1, dim3(grid_for(E, 256)), dim3(256), 0, st, E, pc->nbr, pc->agg_of, pc->sd, pc->so,
                      rp, x1, y, x2, t2);
-  hipLaunchKernelGGL(k_tg_post2, dim3(grid_for((int64_t)n_u + E + pc->n_tail, 256)), dim3(256), 0, st, n_u, E,
+  const int i0 = sweeps ? n_u : 0;
+  hipLaunchKernelGGL(k_tg_post2, dim3(grid_for((int64_t)n_u - i0 + E + pc->n_tail, 256)), dim3(256), 0, st, i0, n_u, E,
                      pc->nbr, pc->Wd, pc->Wo, x2, t2, pc->dMinv, r, pc->psign, dscale, pc->n_tail, pc->tail_dinv,
                      lu ? pc->tail_cells : nullptr, lu ? pc->tail_coef : nullptr, z);
   NFX_LAUNCH_CHECK();

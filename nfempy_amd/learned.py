@@ -43,6 +43,10 @@ F_IN = 8
 WIDTH = 64
 LAYERS = 4
 OMEGA0 = 0.65
+# weight of the two Jacobi sweeps on the flux mass block: 2 / (lmin + lmax) for the
+# spectrum [0.5, 2] of diag(M)^-1 M of the RT1 flux mass on the Kuhn meshes (the
+# sweeps' error factor (1 - w l)^2 is then at most 0.36 over the spectrum)
+FLUX_OMEGA = 0.8
 
 
 def param_layout():
@@ -298,10 +302,14 @@ class LearnedSchurPC:
     """Two-grid Schur block preconditioner with the network smoother (callable z = M^-1 r)."""
 
     def __init__(self, sysd, theta=None, agg_h=None, psign=-1.0, graph=None, coarse_fp16=True, extra_diag=None,
-                 extra_off=None):
+                 extra_off=None, flux_sweeps=2, flux_omega=FLUX_OMEGA):
         """extra_diag / extra_off: optional FP64 (n_cells) / (n_cells * 4) added to S (diagonal,
         ELL off-diagonals) before the network inputs and the coarse operator are formed (a
-        storage term; the Schur complement of an eliminated block)."""
+        storage term; the Schur complement of an eliminated block).
+        flux_sweeps: 2 applies two damped Jacobi sweeps (weight flux_omega) on the flux mass
+        block, read from the system's FP32 CSR values at every application; 1 the diagonal
+        scaling alone. The sweeps apply to the system's own flux rows only (not with the
+        descriptor overrides of a larger system) and need a positive mass diagonal."""
         t = torch()
         self.sys = sysd
         self.g = graph if graph is not None else SchurGraph(sysd, agg_h, device=sysd.device)
@@ -330,6 +338,11 @@ class LearnedSchurPC:
         # diagonal block of rows after the cells, and its block-LU coupling to the cells
         # (slot_tail, slot_coef, tail_cells, tail_coef; see nfx_twogrid_pc)
         self.flux_dinv = None
+        if flux_sweeps not in (1, 2):
+            raise ValueError("flux_sweeps must be 1 or 2")
+        self.flux_sweeps = int(flux_sweeps)
+        self.flux_omega = float(flux_omega)
+        self._mass_positive = False
         self.tail_dinv = None
         self.tail_lu = None
         self.h = self.m = self.z = None
@@ -371,7 +384,10 @@ class LearnedSchurPC:
             self.soff.add_(self.extra_off)
         nat.call("nfx_gnn_features", g.E, nat.ptr(g.nbr), nat.ptr(self.sdiag), nat.ptr(self.soff), nat.ptr(self.a),
                  nat.ptr(self.x), nat.ptr(self.sd32), nat.ptr(self.so32), s)
-        self.dMinv.copy_(1.0 / vals64[self.g.diag_pos_u].abs())
+        dm = vals64[self.g.diag_pos_u]
+        self.dMinv.copy_(1.0 / dm.abs())
+        if self.flux_sweeps == 2:
+            self._mass_positive = bool((dm > 0).all())
 
     def forward(self, stream=None):
         g = self.g
@@ -456,13 +472,23 @@ class LearnedSchurPC:
             if st.numel() != 4 * g.E or sc.numel() != 4 * g.E or tc.numel() != 2 * tail.numel() \
                     or tk.numel() != 2 * tail.numel():
                 raise ValueError("tail coupling arrays do not match the cell and tail counts")
+        sweeps = self.sweeps_active
+        sys = self.sys
         return nat.TwoGridPC(int(dminv.numel()) if self.flux_dinv is not None else g.n_u, g.E, g.n_agg,
                              nat.ptr(g.nbr), nat.ptr(self.Wd), nat.ptr(self.Wo),
                              nat.ptr(self.sd32), nat.ptr(self.so32), nat.ptr(g.agg_ptr), nat.ptr(g.agg_mem),
                              nat.ptr(g.agg_of), nat.ptr(self.Ainv), nat.ptr(self.Ainv16), nat.ptr(self.Ascale),
                              nat.ptr(dminv), self.psign, nat.ptr(self.work), 0 if tail is None else int(tail.numel()),
                              None if tail is None else nat.ptr(tail),
-                             *(nat.ptr(a) for a in lu) if lu is not None else (None,) * 4)
+                             *(nat.ptr(a) for a in lu) if lu is not None else (None,) * 4,
+                             nat.ptr(sys.rowptr) if sweeps else None, nat.ptr(sys.col) if sweeps else None,
+                             nat.ptr(sys.vals32) if sweeps else None, self.flux_omega if sweeps else 0.0)
+
+    @property
+    def sweeps_active(self):
+        """Whether the flux rows get the two mass sweeps (else the diagonal scaling)."""
+        return (self.flux_sweeps == 2 and self.flux_dinv is None and self.tail_dinv is None
+                and self._mass_positive)
 
     def smoother_numpy(self):
         """(Wd, Wo (E,4)) on the host (tests)."""

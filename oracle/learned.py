@@ -16,12 +16,16 @@ the device implementation of the learned correction of the block preconditioner:
   with ``Pi`` the S-orthogonal projector on piecewise constants over aggregates,
   and its gradient with respect to the entries of ``W``;
 - ``twogrid_apply``: one symmetric two-grid cycle (pre-smooth W, exact coarse
-  solve, post-smooth W).
+  solve, post-smooth W);
+- ``flux_sweeps``: the flux rows of the block preconditioner, damped Jacobi sweeps
+  on the flux mass block ``M`` from a zero start (one sweep is the diagonal scaling
+  ``omega diag(M)^-1``).
 
 Pinned by: central finite differences of ``forward``+``twogrid_loss`` against
 ``backward`` (FP64, no rounding), ``schur_ell`` against a dense ``B diag(M)^-1 B^T``,
-and the two-grid cycle against its closed form on a small dense problem
-(``tests/test_oracle_learned.py``).
+the two-grid cycle against its closed form on a small dense problem, and
+``flux_sweeps`` against ``omega (2 - omega) diag(M)^-1`` for a diagonal ``M`` and
+against ``M^-1`` in the limit of many sweeps (``tests/test_oracle_learned.py``).
 """
 
 from __future__ import annotations
@@ -243,3 +247,16 @@ def twogrid_apply(Wd, Wo, nbr, sdiag, soff, agg, n_agg, r):
     x2 = x1 + np.linalg.solve(Ac, c)[agg]
     t2 = r - ell_mul(nbr, sdiag, soff, x2)
     return x2 + ell_mul(nbr, Wd, Wo, t2)
+
+
+def flux_sweeps(indptr, indices, data, n_u, r, omega, sweeps=2):
+    """``sweeps`` damped Jacobi sweeps on ``M z = r`` from z = 0, ``M`` the rows and
+    columns < n_u of the CSR matrix: z <- z + omega diag(M)^-1 (r - M z)."""
+    A = sps.csr_matrix((data, indices, indptr))
+    M = A[:n_u, :n_u].tocsr()
+    d = M.diagonal()
+    r = np.asarray(r, dtype=np.float64)
+    z = np.zeros_like(r)
+    for _ in range(sweeps):
+        z = z + omega * (r - M @ z) / d
+    return z

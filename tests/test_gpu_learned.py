@@ -131,8 +131,19 @@ def test_learned_parity_with_oracle():
                              rn[sysd.n_u:])
     zn = z.double().cpu().numpy()
     assert np.abs(zn[sysd.n_u:] - pc.psign * zp_or).max() <= 1e-4 * np.abs(zp_or).max()
+    # flux rows: two damped Jacobi sweeps on the flux mass block (FP32 values on the device)
+    assert pc.sweeps_active
+    zu_or = ol.flux_sweeps(rp, ci, v64, sysd.n_u, rn[:sysd.n_u], pc.flux_omega)
+    assert np.abs(zn[:sysd.n_u] - zu_or).max() <= 1e-5 * np.abs(zu_or).max()
+    # with flux_sweeps=1 the flux rows are the diagonal scaling and the pressure rows unchanged
+    pc1 = LearnedSchurPC(sysd, th, graph=g, coarse_fp16=False, flux_sweeps=1)
+    assert pc1.setup() and not pc1.sweeps_active
+    z1 = pc1(r)
+    torch.cuda.synchronize()
+    z1n = z1.double().cpu().numpy()
     dM = np.array([v64[rp[i]:rp[i + 1]][ci[rp[i]:rp[i + 1]] == i][0] for i in range(sysd.n_u)])
-    assert np.allclose(zn[:sysd.n_u], rn[:sysd.n_u] / np.abs(dM), rtol=1e-5)
+    assert np.allclose(z1n[:sysd.n_u], rn[:sysd.n_u] / np.abs(dM), rtol=1e-5)
+    assert np.array_equal(z1n[sysd.n_u:], zn[sysd.n_u:])
     # row-scaled FP16 coarse inverse: same application within 3e-3
     pc16 = LearnedSchurPC(sysd, th, graph=g, coarse_fp16=True)
     assert pc16.setup()

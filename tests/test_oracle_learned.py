@@ -231,3 +231,45 @@ def test_aggregates_of_kuhn_cube_are_hex_blocks():
     expect = (i // 2) + (n // 2) * ((j // 2) + (n // 2) * (k // 2))
     assert np.array_equal(agg, expect)
     assert np.all(np.bincount(agg) == 48)
+
+
+def test_flux_sweeps_pins():
+    """Two sweeps on a diagonal M are omega (2 - omega) diag(M)^-1; many sweeps converge
+    to M^-1 r on the assembled flux mass; one sweep is the diagonal scaling. A planted
+    sign slip in the correction (z - omega D^-1 (r - M z)) fails the limit check."""
+    import scipy.sparse as sps
+    import scipy.sparse.linalg as spla
+
+    from oracle.learned import flux_sweeps
+
+    A, n_u, *_ = small_system(2, seed=1)
+    rng = np.random.default_rng(4)
+    r = rng.standard_normal(n_u)
+    d = A.diagonal()[:n_u]
+    assert np.allclose(flux_sweeps(A.indptr, A.indices, A.data, n_u, r, 0.8, sweeps=1), 0.8 * r / d, rtol=1e-14)
+    D = sps.csr_matrix(sps.diags(np.r_[d, np.ones(A.shape[0] - n_u)]))
+    got = flux_sweeps(D.indptr, D.indices, D.data, n_u, r, 0.8)
+    assert np.allclose(got, 0.8 * 1.2 * r / d, rtol=1e-14)
+    M = A[:n_u, :n_u].tocsc()
+    exact = spla.spsolve(M, r)
+    lim = flux_sweeps(A.indptr, A.indices, A.data, n_u, r, 0.8, sweeps=200)
+    assert np.linalg.norm(lim - exact) <= 1e-10 * np.linalg.norm(exact)
+    z = np.zeros(n_u)
+    for _ in range(200):
+        z = z - 0.8 * (r - M @ z) / d
+    assert not np.linalg.norm(z - exact) <= 1e-2 * np.linalg.norm(exact)
+
+
+def test_flux_mass_spectrum_of_kuhn_cube():
+    """diag(M)^-1 M of the RT1 flux mass on the Kuhn cube has its spectrum in [0.5, 2]
+    (for any element-wise constant permeability): the two flux sweeps' weight 0.8 is
+    2 / (0.5 + 2), with error factor (1 - 0.8 l)^2 <= 0.36 over the spectrum."""
+    from nfempy_amd.learned import FLUX_OMEGA
+
+    for seed in (0, 1):
+        A, n_u, *_ = small_system(2, seed=seed)
+        M = A[:n_u, :n_u].toarray()
+        s = 1 / np.sqrt(np.diag(M))
+        ev = np.linalg.eigvalsh(s[:, None] * M * s[None, :])
+        assert ev.min() >= 0.5 - 1e-10 and ev.max() <= 2 + 1e-10, (ev.min(), ev.max())
+    assert FLUX_OMEGA == 2 / (0.5 + 2)
