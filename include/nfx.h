@@ -666,10 +666,11 @@ int nfx_spd_inverse_small(int n, float* d_A, int lda, void* stream);
  * and tail_dinv = 1 / S_ff; then r_p <- r_p - sum slot_coef r_f before the cycle (S in
  * ELL form then holds the cells' Schur complement S - S_pf S_ff^-1 S_fq) and
  * z_f = tail_dinv (r_f - sum tail_coef z_p) after it.
- * Flux rows: when flux_rowptr is not NULL, two damped Jacobi sweeps on the flux mass
- * block M replace the diagonal scaling, z_u = flux_omega dMinv (2 r_u - flux_omega
- * M (dMinv r_u)), with M the entries of rows < n_u and columns < n_u of the FP32 CSR
- * (flux_rowptr, flux_col, flux_val), read at every apply (NULL: z_u = dMinv r_u).
+ * Flux rows: when flux_coef is not NULL, z_u = C r_u with C a sparse n_u x n_u operator
+ * in slot-major ELL form, flux_width slots per row: z_i = sum_s flux_coef[s n_u + i]
+ * r[flux_col[s n_u + i]] (padding slots: coefficient 0, any column < n_u); for example the
+ * two damped Jacobi sweeps on the flux mass block of nfx_flux_sweep_coef. NULL:
+ * z_u = dMinv r_u.
  * work: 3 n_cells + 2 n_agg floats (4 n_cells + 2 n_agg with slot_tail). */
 typedef struct nfx_twogrid_pc {
   int n_u, n_cells, n_agg;
@@ -693,12 +694,19 @@ typedef struct nfx_twogrid_pc {
   const float* slot_coef;
   const int32_t* tail_cells;
   const float* tail_coef;
-  const int32_t* flux_rowptr;
+  int flux_width;
   const int32_t* flux_col;
-  const float* flux_val;
-  float flux_omega;
+  const float* flux_coef;
 } nfx_twogrid_pc;
 int nfx_twogrid_apply(const nfx_twogrid_pc* pc, const float* d_r, float* d_z, void* stream);
+/* Flux-row operator of nfx_twogrid_pc for two damped Jacobi sweeps (weight omega) on the
+ * flux mass block M from a zero start: C = 2 omega D^-1 - omega^2 D^-1 M D^-1, D = diag(M)
+ * (positive), from the FP64 CSR values. d_pos / d_col (width x n_u, slot-major): CSR
+ * position and column of the s-th entry of flux row i with column < n_u (d_pos -1 for a
+ * padding slot); d_diag_pos[i]: CSR position of M_ii. Writes d_coef (width x n_u). */
+int nfx_flux_sweep_coef(int n_u, int width, const int32_t* d_pos, const int32_t* d_col,
+                        const int32_t* d_diag_pos, const double* d_vals64, double omega, float* d_coef,
+                        void* stream);
 /* C = beta C + alpha op(A) op(B) (FP32, row-major, leading dimensions lda/ldb/ldc;
  * op(A) = A (M x K) or A^T when transA (A stored K x M); op(B) = B (K x N) or B^T when
  * transB); when d_C2 is not null the result is also stored transposed into it

@@ -140,6 +140,8 @@ SIGNATURES = {
     "nfx_sgemm": (c_int, [c_int, c_int, c_int, c_float, c_void_p, c_int, c_int, c_void_p, c_int, c_int, c_float,
                           c_void_p, c_int, c_void_p, c_int, c_void_p]),
     "nfx_spd_inverse_small": (c_int, [c_int, c_void_p, c_int, c_void_p]),
+    "nfx_flux_sweep_coef": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p,
+                                    c_void_p]),
     "nfx_fgmres_create": (c_int, [c_void_p, c_int, c_int, c_void_p]),
     "nfx_fgmres_destroy": (c_int, [c_void_p]),
     "nfx_solve_fgmres_ir": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
@@ -157,8 +159,8 @@ class TwoGridPC(ctypes.Structure):
                 ("Ascale", c_void_p), ("dMinv", c_void_p), ("psign", c_float), ("work", c_void_p),
                 ("n_tail", c_int), ("tail_dinv", c_void_p),
                 ("slot_tail", c_void_p), ("slot_coef", c_void_p), ("tail_cells", c_void_p),
-                ("tail_coef", c_void_p), ("flux_rowptr", c_void_p), ("flux_col", c_void_p),
-                ("flux_val", c_void_p), ("flux_omega", c_float)]
+                ("tail_coef", c_void_p), ("flux_width", c_int), ("flux_col", c_void_p),
+                ("flux_coef", c_void_p)]
 
 
 class MinresBatchItem(ctypes.Structure):

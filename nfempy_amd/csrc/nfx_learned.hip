@@ -536,45 +536,52 @@ __global__ void k_tg_pre(int E, const int32_t* __restrict__ nbr, const float* __
 }
 
 // Blocks [0, cell_blocks): x1 = W r on the cells, as k_tg_pre. Blocks after them: the
-// flux rows as two damped Jacobi sweeps on the flux mass block M (the CSR entries of
-// rows < n_u with columns < n_u), in closed form
-// z_u = omega dMinv (2 r_u - omega M (dMinv r_u)), four lanes per row; scaled by
-// 1 / *dscale like the pressure rows.
-constexpr int kFluxLanes = 4;
+// flux rows z_u = C r_u, C in slot-major ELL form (width slots per row, one thread per
+// row, coalesced over rows), scaled by 1 / *dscale like the pressure rows.
 __global__ __launch_bounds__(256) void k_tg_pre_flux(int E, int cell_blocks, const int32_t* __restrict__ nbr,
                                                      const float* __restrict__ Wd, const float* __restrict__ Wo,
                                                      const float* __restrict__ rp, float* __restrict__ x1, int n_u,
-                                                     const int32_t* __restrict__ rowptr,
-                                                     const int32_t* __restrict__ col,
-                                                     const float* __restrict__ val,
-                                                     const float* __restrict__ dMinv, const float* __restrict__ r,
-                                                     float omega, const double* __restrict__ dscale,
-                                                     float* __restrict__ z) {
+                                                     int width, const int32_t* __restrict__ fcol,
+                                                     const float* __restrict__ fcoef, const float* __restrict__ r,
+                                                     const double* __restrict__ dscale, float* __restrict__ z) {
   if ((int)blockIdx.x < cell_blocks) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p < E) x1[p] = ell_row(p, nbr, Wd, Wo, rp);
     return;
   }
-  const int64_t t = (int64_t)(blockIdx.x - cell_blocks) * blockDim.x + threadIdx.x;
-  const int64_t i = t / kFluxLanes;
-  const int sub = (int)(t % kFluxLanes);
+  const int i = (blockIdx.x - cell_blocks) * blockDim.x + threadIdx.x;
+  if (i >= n_u) return;
   float acc = 0.f;
-  if (i < n_u) {
-    const int e = rowptr[i + 1];
-    for (int k = rowptr[i] + sub; k < e; k += kFluxLanes) {
-      const int c = col[k];
-      if (c < n_u) acc += val[k] * (dMinv[c] * r[c]);
-    }
+  for (int s = 0; s < width; ++s) {
+    const int64_t k = (int64_t)s * n_u + i;
+    acc += fcoef[k] * r[fcol[k]];
   }
-#pragma unroll
-  for (int off = kFluxLanes / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, kFluxLanes);
-  if (i >= n_u || sub != 0) return;
   float sc = 1.f;
   if (dscale) {
     const double d = *dscale;
     sc = d > 0 ? (float)(1.0 / d) : 0.f;
   }
-  z[i] = sc * omega * dMinv[i] * (2.f * r[i] - omega * acc);
+  z[i] = sc * acc;
+}
+
+// C = 2 omega D^-1 - omega^2 D^-1 M D^-1 on the flux-row ELL slots (FP64 arithmetic,
+// FP32 store); one thread per (slot, row)
+__global__ void k_flux_sweep_coef(int n_u, int width, const int32_t* __restrict__ pos,
+                                  const int32_t* __restrict__ fcol, const int32_t* __restrict__ diag_pos,
+                                  const double* __restrict__ vals, double omega, float* __restrict__ coef) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (int64_t)width * n_u) return;
+  const int i = (int)(k % n_u);
+  const int q = pos[k];
+  if (q < 0) {
+    coef[k] = 0.f;
+    return;
+  }
+  const int j = fcol[k];
+  const double di = 1.0 / vals[diag_pos[i]], dj = 1.0 / vals[diag_pos[j]];
+  double c = -omega * omega * di * vals[q] * dj;
+  if (j == i) c += 2.0 * omega * di;
+  coef[k] = (float)c;
 }
 
 // c[a] = sum_{p in a} (r - S x1)_p   (one wave per aggregate)
@@ -990,6 +997,17 @@ extern "C" {
 
 int nfx_gnn_num_params(void) { return kNumParams; }
 
+int nfx_flux_sweep_coef(int n_u, int width, const int32_t* pos, const int32_t* fcol, const int32_t* diag_pos,
+                        const double* vals64, double omega, float* coef, void* stream) {
+  NFX_REQUIRE(n_u >= 0 && width > 0, "nfx_flux_sweep_coef: bad sizes");
+  if (n_u == 0) return NFX_OK;
+  NFX_REQUIRE(pos && fcol && diag_pos && vals64 && coef, "nfx_flux_sweep_coef: null pointer");
+  hipLaunchKernelGGL(k_flux_sweep_coef, dim3(grid_for((int64_t)width * n_u, 256)), dim3(256), 0,
+                     (hipStream_t)stream, n_u, width, pos, fcol, diag_pos, vals64, omega, coef);
+  NFX_LAUNCH_CHECK();
+  return NFX_OK;
+}
+
 int nfx_schur_ell(int E, const int32_t* pos_b, const int32_t* pos_qb, const int32_t* pos_m,
                   const double* vals64, double* sdiag, double* soff, void* stream) {
   NFX_REQUIRE(E >= 0, "nfx_schur_ell: E < 0");
@@ -1173,8 +1191,8 @@ int nfx::twogrid_apply_scaled(const nfx_twogrid_pc* pc, const float* r, float* z
   NFX_REQUIRE(pc->nbr && pc->Wd && pc->Wo && pc->sd && pc->so && pc->agg_ptr && pc->agg_mem && pc->agg_of &&
                   (pc->Ainv || (pc->Ainv16 && pc->Ascale)) && r && z && pc->work && (n_u == 0 || pc->dMinv),
               "nfx_twogrid_apply: null pointer");
-  NFX_REQUIRE(!pc->flux_rowptr || (pc->flux_col && pc->flux_val),
-              "nfx_twogrid_apply: incomplete flux-row CSR");
+  NFX_REQUIRE(!pc->flux_coef || (pc->flux_col && pc->flux_width > 0),
+              "nfx_twogrid_apply: incomplete flux-row operator");
   const bool lu = pc->slot_tail != nullptr;
   NFX_REQUIRE(!lu || (pc->n_tail > 0 && pc->slot_coef && pc->tail_cells && pc->tail_coef),
               "nfx_twogrid_apply: incomplete tail coupling");
@@ -1193,12 +1211,12 @@ int nfx::twogrid_apply_scaled(const nfx_twogrid_pc* pc, const float* r, float* z
   }
   // (a fused x1 + restriction kernel, recomputing x1 at the members' neighbours, measured
   // 18.6 us against 4.8 + 5.9 us for the two launches at 196,608 cells)
-  const bool sweeps = pc->flux_rowptr != nullptr && n_u > 0;
+  const bool sweeps = pc->flux_coef != nullptr && n_u > 0;
   if (sweeps) {
     const int cb = grid_for(E, 256);
-    const int fb = grid_for((int64_t)n_u * kFluxLanes, 256);
+    const int fb = grid_for(n_u, 256);
     hipLaunchKernelGGL(k_tg_pre_flux, dim3(cb + fb), dim3(256), 0, st, E, cb, pc->nbr, pc->Wd, pc->Wo, rp, x1, n_u,
-                       pc->flux_rowptr, pc->flux_col, pc->flux_val, pc->dMinv, r, pc->flux_omega, dscale, z);
+                       pc->flux_width, pc->flux_col, pc->flux_coef, r, dscale, z);
   } else {
     hipLaunchKernelGGL(k_tg_pre, dim3(grid_for(E, 256)), dim3(256), 0, st, E, pc->nbr, pc->Wd, pc->Wo, rp, x1);
   }

@@ -198,6 +198,26 @@ class SchurGraph:
         self.device = device
 
 
+def flux_ell(rowptr, col, n_u):
+    """(pos, col), each (width, n_u) int32 slot-major: the CSR positions and columns of the
+    entries of flux row i with column < n_u, in CSR order; padding slots pos -1, column i."""
+    rowptr = np.asarray(rowptr, dtype=np.int64)
+    col = np.asarray(col, dtype=np.int64)
+    end = int(rowptr[n_u])
+    rows = np.repeat(np.arange(n_u), np.diff(rowptr[: n_u + 1]))
+    k = np.flatnonzero(col[:end] < n_u)
+    r = rows[k]
+    cnt = np.bincount(r, minlength=n_u)
+    width = int(cnt.max()) if n_u else 0
+    first = np.concatenate([[0], np.cumsum(cnt)[:-1]])
+    slot = np.arange(len(k)) - first[r]
+    pos = np.full((width, n_u), -1, dtype=np.int32)
+    fc = np.broadcast_to(np.arange(n_u, dtype=np.int32), (width, n_u)).copy()
+    pos[slot, r] = k
+    fc[slot, r] = col[k]
+    return pos, fc
+
+
 def aggregate_cells(centroids, h):
     """Aggregate per cell: boxes of side h anchored at the smallest centroid
     coordinate (shifted by h/16 so that box faces fall between cell centroids of a
@@ -307,9 +327,10 @@ class LearnedSchurPC:
         ELL off-diagonals) before the network inputs and the coarse operator are formed (a
         storage term; the Schur complement of an eliminated block).
         flux_sweeps: 2 applies two damped Jacobi sweeps (weight flux_omega) on the flux mass
-        block, read from the system's FP32 CSR values at every application; 1 the diagonal
-        scaling alone. The sweeps apply to the system's own flux rows only (not with the
-        descriptor overrides of a larger system) and need a positive mass diagonal."""
+        block, as the operator 2 w D^-1 - w^2 D^-1 M D^-1 built at set-up from the FP64
+        values; 1 the diagonal scaling alone. The sweeps apply to the system's own flux rows
+        only (not with the descriptor overrides of a larger system) and need a positive mass
+        diagonal."""
         t = torch()
         self.sys = sysd
         self.g = graph if graph is not None else SchurGraph(sysd, agg_h, device=sysd.device)
@@ -343,6 +364,14 @@ class LearnedSchurPC:
         self.flux_sweeps = int(flux_sweeps)
         self.flux_omega = float(flux_omega)
         self._mass_positive = False
+        self.flux_width = 0
+        self.flux_pos = self.flux_col = self.flux_coef = None
+        if self.flux_sweeps == 2 and g.n_u > 0:
+            pos, col = flux_ell(sysd.pattern.rowptr, sysd.pattern.col, g.n_u)
+            self.flux_width = pos.shape[0]
+            self.flux_pos, self.flux_col = to_dev(pos, device=dev), to_dev(col, device=dev)
+            self._diag_pos32 = self.g.diag_pos_u.to(t.int32).contiguous()
+            self.flux_coef = f32(pos.size)
         self.tail_dinv = None
         self.tail_lu = None
         self.h = self.m = self.z = None
@@ -386,8 +415,12 @@ class LearnedSchurPC:
                  nat.ptr(self.x), nat.ptr(self.sd32), nat.ptr(self.so32), s)
         dm = vals64[self.g.diag_pos_u]
         self.dMinv.copy_(1.0 / dm.abs())
-        if self.flux_sweeps == 2:
+        if self.flux_coef is not None:
             self._mass_positive = bool((dm > 0).all())
+            if self._mass_positive:
+                nat.call("nfx_flux_sweep_coef", g.n_u, self.flux_width, nat.ptr(self.flux_pos),
+                         nat.ptr(self.flux_col), nat.ptr(self._diag_pos32), nat.ptr(vals64), self.flux_omega,
+                         nat.ptr(self.flux_coef), s)
 
     def forward(self, stream=None):
         g = self.g
@@ -473,7 +506,6 @@ class LearnedSchurPC:
                     or tk.numel() != 2 * tail.numel():
                 raise ValueError("tail coupling arrays do not match the cell and tail counts")
         sweeps = self.sweeps_active
-        sys = self.sys
         return nat.TwoGridPC(int(dminv.numel()) if self.flux_dinv is not None else g.n_u, g.E, g.n_agg,
                              nat.ptr(g.nbr), nat.ptr(self.Wd), nat.ptr(self.Wo),
                              nat.ptr(self.sd32), nat.ptr(self.so32), nat.ptr(g.agg_ptr), nat.ptr(g.agg_mem),
@@ -481,8 +513,8 @@ class LearnedSchurPC:
                              nat.ptr(dminv), self.psign, nat.ptr(self.work), 0 if tail is None else int(tail.numel()),
                              None if tail is None else nat.ptr(tail),
                              *(nat.ptr(a) for a in lu) if lu is not None else (None,) * 4,
-                             nat.ptr(sys.rowptr) if sweeps else None, nat.ptr(sys.col) if sweeps else None,
-                             nat.ptr(sys.vals32) if sweeps else None, self.flux_omega if sweeps else 0.0)
+                             self.flux_width if sweeps else 0, nat.ptr(self.flux_col) if sweeps else None,
+                             nat.ptr(self.flux_coef) if sweeps else None)
 
     @property
     def sweeps_active(self):

@@ -221,3 +221,31 @@ def test_packed_offsets_required_and_validated():
     with pytest.raises(ValueError):
         sysd._validate(sysd.u_sp.mesh)
     sysd.row_pos = good
+
+
+def test_flux_ell_layout():
+    """The slot-major flux-row ELL of the two-grid preconditioner holds exactly the entries
+    of rows < n_u with columns < n_u, in CSR order; padding slots point at the row itself."""
+    import scipy.sparse as sps
+
+    from nfempy_amd.learned import flux_ell
+
+    A = (sps.random(40, 40, density=0.15, random_state=3, format="csr") + sps.eye(40)).tocsr()
+    A.sort_indices()
+    n_u = 29
+    pos, col = flux_ell(A.indptr, A.indices, n_u)
+    blk = A[:n_u, :n_u].tocsr()
+    assert pos.shape == col.shape and pos.shape[0] == int(np.diff(blk.indptr).max())
+    got = np.zeros((n_u, n_u))
+    for s in range(pos.shape[0]):
+        for i in range(n_u):
+            if pos[s, i] >= 0:
+                assert A.indices[pos[s, i]] == col[s, i] and col[s, i] < n_u
+                got[i, col[s, i]] += A.data[pos[s, i]]
+            else:
+                assert col[s, i] == i
+    assert np.array_equal(got, blk.toarray())
+    # slots in CSR order within each row
+    for i in range(n_u):
+        p = pos[:, i][pos[:, i] >= 0]
+        assert np.all(np.diff(p) > 0)
