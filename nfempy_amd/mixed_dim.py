@@ -430,8 +430,9 @@ class MixedDimDarcy:
         from .learned import LearnedSchurPC
 
         t = self.coupled_pc_terms()
+        # (the coupled system's flux rows keep its own diagonal, flux_dinv below: no flux sweeps)
         pc = LearnedSchurPC(self.m, theta, psign=1.0, graph=self._schur_graph, extra_diag=t["extra_diag"],
-                            extra_off=t["extra_off"])
+                            extra_off=t["extra_off"], flux_sweeps=1)
         pc.setup(vals64=self.vals64, network=theta is not None)
         pc.flux_dinv = self.pinv[: self.n_u].contiguous()
         pc.tail_dinv = t["tail_dinv"]
