@@ -873,7 +873,7 @@ def bench_solve(args, sysd, cent, fsrc, pd, log):
         out[f"{key}_residual"] = il["residual"]
         out[f"{key}_fallback"] = il["fallback"]
         out[f"{key}_method"] = il["method"]
-    out["learned_flux_block"] = (f"two damped Jacobi sweeps on the flux mass (w={pc.flux_omega})"
+    out["learned_flux_block"] = (f"two damped Jacobi sweeps on the flux mass (w={pc.flux_weight:.4f})"
                                  if pc.sweeps_active else "diag(M)^-1")
     out["speedup_learned_vs_plain"] = round(out["time_to_solve_plain_s"] / out["time_to_solve_learned_s"], 2)
     out["speedup_learned_vs_twogrid_jacobi"] = round(

@@ -703,10 +703,13 @@ int nfx_twogrid_apply(const nfx_twogrid_pc* pc, const float* d_r, float* d_z, vo
  * flux mass block M from a zero start: C = 2 omega D^-1 - omega^2 D^-1 M D^-1, D = diag(M)
  * (positive), from the FP64 CSR values. d_pos / d_col (width x n_u, slot-major): CSR
  * position and column of the s-th entry of flux row i with column < n_u (d_pos -1 for a
- * padding slot); d_diag_pos[i]: CSR position of M_ii. Writes d_coef (width x n_u). */
+ * padding slot); d_diag_pos[i]: CSR position of M_ii. Writes d_coef (width x n_u).
+ * d_bound (one float, may be NULL when omega > 0): receives b = max_i sum_j |M_ij| / M_ii,
+ * an upper bound of the spectral radius of D^-1 M; omega <= 0 selects omega = 2 / (0.5 + b)
+ * (0.8 on meshes with b = 2, and omega b < 2 always, so C stays positive definite). */
 int nfx_flux_sweep_coef(int n_u, int width, const int32_t* d_pos, const int32_t* d_col,
                         const int32_t* d_diag_pos, const double* d_vals64, double omega, float* d_coef,
-                        void* stream);
+                        float* d_bound, void* stream);
 /* C = beta C + alpha op(A) op(B) (FP32, row-major, leading dimensions lda/ldb/ldc;
  * op(A) = A (M x K) or A^T when transA (A stored K x M); op(B) = B (K x N) or B^T when
  * transB); when d_C2 is not null the result is also stored transposed into it

@@ -141,7 +141,7 @@ SIGNATURES = {
                           c_void_p, c_int, c_void_p, c_int, c_void_p]),
     "nfx_spd_inverse_small": (c_int, [c_int, c_void_p, c_int, c_void_p]),
     "nfx_flux_sweep_coef": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p,
-                                    c_void_p]),
+                                    c_void_p, c_void_p]),
     "nfx_fgmres_create": (c_int, [c_void_p, c_int, c_int, c_void_p]),
     "nfx_fgmres_destroy": (c_int, [c_void_p]),
     "nfx_solve_fgmres_ir": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,

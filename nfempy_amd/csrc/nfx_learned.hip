@@ -564,11 +564,36 @@ __global__ __launch_bounds__(256) void k_tg_pre_flux(int E, int cell_blocks, con
   z[i] = sc * acc;
 }
 
+__global__ void k_flux_bound_init(float* __restrict__ bound) {
+  if (threadIdx.x == 0) bound[0] = 0.f;
+}
+
+// bound = max_i sum_j |M_ij| / M_ii over the flux-row ELL slots (the row-sum norm of
+// D^-1 M, an upper bound of its spectral radius); positive floats order as their bits
+__global__ __launch_bounds__(256) void k_flux_bound(int n_u, int width, const int32_t* __restrict__ pos,
+                                                    const int32_t* __restrict__ diag_pos,
+                                                    const double* __restrict__ vals, float* __restrict__ bound) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  float b = 0.f;
+  if (i < n_u) {
+    double s = 0.0;
+    for (int k = 0; k < width; ++k) {
+      const int q = pos[(int64_t)k * n_u + i];
+      if (q >= 0) s += fabs(vals[q]);
+    }
+    b = (float)(s / vals[diag_pos[i]]);
+  }
+  for (int off = 32; off > 0; off >>= 1) b = fmaxf(b, __shfl_xor(b, off, 64));
+  if ((threadIdx.x & 63) == 0 && b > 0.f)
+    atomicMax(reinterpret_cast<unsigned int*>(bound), __float_as_uint(b));
+}
+
 // C = 2 omega D^-1 - omega^2 D^-1 M D^-1 on the flux-row ELL slots (FP64 arithmetic,
-// FP32 store); one thread per (slot, row)
+// FP32 store); one thread per (slot, row). omega <= 0: omega = 2 / (0.5 + *bound).
 __global__ void k_flux_sweep_coef(int n_u, int width, const int32_t* __restrict__ pos,
                                   const int32_t* __restrict__ fcol, const int32_t* __restrict__ diag_pos,
-                                  const double* __restrict__ vals, double omega, float* __restrict__ coef) {
+                                  const double* __restrict__ vals, double omega, const float* __restrict__ bound,
+                                  float* __restrict__ coef) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= (int64_t)width * n_u) return;
   const int i = (int)(k % n_u);
@@ -577,10 +602,11 @@ __global__ void k_flux_sweep_coef(int n_u, int width, const int32_t* __restrict_
     coef[k] = 0.f;
     return;
   }
+  const double w = omega > 0.0 ? omega : 2.0 / (0.5 + (double)bound[0]);
   const int j = fcol[k];
   const double di = 1.0 / vals[diag_pos[i]], dj = 1.0 / vals[diag_pos[j]];
-  double c = -omega * omega * di * vals[q] * dj;
-  if (j == i) c += 2.0 * omega * di;
+  double c = -w * w * di * vals[q] * dj;
+  if (j == i) c += 2.0 * w * di;
   coef[k] = (float)c;
 }
 
@@ -998,12 +1024,19 @@ extern "C" {
 int nfx_gnn_num_params(void) { return kNumParams; }
 
 int nfx_flux_sweep_coef(int n_u, int width, const int32_t* pos, const int32_t* fcol, const int32_t* diag_pos,
-                        const double* vals64, double omega, float* coef, void* stream) {
+                        const double* vals64, double omega, float* coef, float* bound, void* stream) {
   NFX_REQUIRE(n_u >= 0 && width > 0, "nfx_flux_sweep_coef: bad sizes");
   if (n_u == 0) return NFX_OK;
   NFX_REQUIRE(pos && fcol && diag_pos && vals64 && coef, "nfx_flux_sweep_coef: null pointer");
-  hipLaunchKernelGGL(k_flux_sweep_coef, dim3(grid_for((int64_t)width * n_u, 256)), dim3(256), 0,
-                     (hipStream_t)stream, n_u, width, pos, fcol, diag_pos, vals64, omega, coef);
+  NFX_REQUIRE(omega > 0.0 || bound, "nfx_flux_sweep_coef: the automatic weight needs the bound workspace");
+  hipStream_t st = (hipStream_t)stream;
+  if (bound) {
+    hipLaunchKernelGGL(k_flux_bound_init, dim3(1), dim3(64), 0, st, bound);
+    hipLaunchKernelGGL(k_flux_bound, dim3(grid_for(n_u, 256)), dim3(256), 0, st, n_u, width, pos, diag_pos, vals64,
+                       bound);
+  }
+  hipLaunchKernelGGL(k_flux_sweep_coef, dim3(grid_for((int64_t)width * n_u, 256)), dim3(256), 0, st, n_u, width,
+                     pos, fcol, diag_pos, vals64, omega, bound, coef);
   NFX_LAUNCH_CHECK();
   return NFX_OK;
 }

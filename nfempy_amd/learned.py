@@ -322,13 +322,15 @@ class LearnedSchurPC:
     """Two-grid Schur block preconditioner with the network smoother (callable z = M^-1 r)."""
 
     def __init__(self, sysd, theta=None, agg_h=None, psign=-1.0, graph=None, coarse_fp16=True, extra_diag=None,
-                 extra_off=None, flux_sweeps=2, flux_omega=FLUX_OMEGA):
+                 extra_off=None, flux_sweeps=2, flux_omega=None):
         """extra_diag / extra_off: optional FP64 (n_cells) / (n_cells * 4) added to S (diagonal,
         ELL off-diagonals) before the network inputs and the coarse operator are formed (a
         storage term; the Schur complement of an eliminated block).
         flux_sweeps: 2 applies two damped Jacobi sweeps (weight flux_omega) on the flux mass
         block, as the operator 2 w D^-1 - w^2 D^-1 M D^-1 built at set-up from the FP64
-        values; 1 the diagonal scaling alone. The sweeps apply to the system's own flux rows
+        values; 1 the diagonal scaling alone. flux_omega None: w = 2 / (0.5 + b) with b the
+        row-sum bound of the spectral radius of D^-1 M, computed at set-up (0.8 on the Kuhn
+        meshes, where b = 2 is the spectral radius). The sweeps apply to the system's own flux rows
         only (not with the descriptor overrides of a larger system) and need a positive mass
         diagonal."""
         t = torch()
@@ -362,7 +364,7 @@ class LearnedSchurPC:
         if flux_sweeps not in (1, 2):
             raise ValueError("flux_sweeps must be 1 or 2")
         self.flux_sweeps = int(flux_sweeps)
-        self.flux_omega = float(flux_omega)
+        self.flux_omega = None if flux_omega is None else float(flux_omega)
         self._mass_positive = False
         self.flux_width = 0
         self.flux_pos = self.flux_col = self.flux_coef = None
@@ -372,6 +374,7 @@ class LearnedSchurPC:
             self.flux_pos, self.flux_col = to_dev(pos, device=dev), to_dev(col, device=dev)
             self._diag_pos32 = self.g.diag_pos_u.to(t.int32).contiguous()
             self.flux_coef = f32(pos.size)
+            self._flux_bound = f32(1)
         self.tail_dinv = None
         self.tail_lu = None
         self.h = self.m = self.z = None
@@ -419,8 +422,8 @@ class LearnedSchurPC:
             self._mass_positive = bool((dm > 0).all())
             if self._mass_positive:
                 nat.call("nfx_flux_sweep_coef", g.n_u, self.flux_width, nat.ptr(self.flux_pos),
-                         nat.ptr(self.flux_col), nat.ptr(self._diag_pos32), nat.ptr(vals64), self.flux_omega,
-                         nat.ptr(self.flux_coef), s)
+                         nat.ptr(self.flux_col), nat.ptr(self._diag_pos32), nat.ptr(vals64),
+                         self.flux_omega or 0.0, nat.ptr(self.flux_coef), nat.ptr(self._flux_bound), s)
 
     def forward(self, stream=None):
         g = self.g
@@ -515,6 +518,20 @@ class LearnedSchurPC:
                              *(nat.ptr(a) for a in lu) if lu is not None else (None,) * 4,
                              self.flux_width if sweeps else 0, nat.ptr(self.flux_col) if sweeps else None,
                              nat.ptr(self.flux_coef) if sweeps else None)
+
+    @property
+    def flux_weight(self):
+        """The sweeps' weight in use (reads the device bound when it is automatic)."""
+        if self.flux_omega is not None:
+            return self.flux_omega
+        if self.flux_coef is None:
+            return None
+        return 2.0 / (0.5 + float(self._flux_bound.item()))
+
+    @property
+    def flux_bound(self):
+        """Row-sum bound of the spectral radius of D^-1 M from the last set-up."""
+        return None if self.flux_coef is None else float(self._flux_bound.item())
 
     @property
     def sweeps_active(self):

@@ -52,6 +52,7 @@ def _bf(t):
 
 
 def test_learned_parity_with_oracle():
+    import scipy.sparse as sps
     import torch
     import oracle.learned as ol
     from nfempy_amd.learned import LearnedSchurPC, SmootherTrainer
@@ -133,8 +134,21 @@ def test_learned_parity_with_oracle():
     assert np.abs(zn[sysd.n_u:] - pc.psign * zp_or).max() <= 1e-4 * np.abs(zp_or).max()
     # flux rows: two damped Jacobi sweeps on the flux mass block (FP32 values on the device)
     assert pc.sweeps_active
-    zu_or = ol.flux_sweeps(rp, ci, v64, sysd.n_u, rn[:sysd.n_u], pc.flux_omega)
+    A64 = sps.csr_matrix((v64, ci, rp))
+    Mh = A64[:sysd.n_u, :sysd.n_u]
+    bound = float((abs(Mh).sum(axis=1).A1 / Mh.diagonal()).max())
+    assert abs(pc.flux_bound - bound) <= 1e-6 * bound
+    w = pc.flux_weight
+    assert abs(w - 2 / (0.5 + bound)) <= 1e-6 and w * bound < 2
+    zu_or = ol.flux_sweeps(rp, ci, v64, sysd.n_u, rn[:sysd.n_u], w)
     assert np.abs(zn[:sysd.n_u] - zu_or).max() <= 1e-5 * np.abs(zu_or).max()
+    # a fixed weight gives the same operator as the oracle with that weight
+    pcw = LearnedSchurPC(sysd, th, graph=g, coarse_fp16=False, flux_omega=0.7)
+    assert pcw.setup() and pcw.flux_weight == 0.7
+    zw = pcw(r)
+    torch.cuda.synchronize()
+    zu_w = ol.flux_sweeps(rp, ci, v64, sysd.n_u, rn[:sysd.n_u], 0.7)
+    assert np.abs(zw.double().cpu().numpy()[:sysd.n_u] - zu_w).max() <= 1e-5 * np.abs(zu_w).max()
     # with flux_sweeps=1 the flux rows are the diagonal scaling and the pressure rows unchanged
     pc1 = LearnedSchurPC(sysd, th, graph=g, coarse_fp16=False, flux_sweeps=1)
     assert pc1.setup() and not pc1.sweeps_active
