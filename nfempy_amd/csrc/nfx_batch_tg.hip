@@ -363,12 +363,23 @@ __device__ __forceinline__ GroupWork group_work(const nfx_batch_tg& pc) {
 
 // Stage bodies, one work item (cell, aggregate wave or flux block) each; the kernels below
 // run two groups (RT1-P0 and RT2-P1) in one launch: blocks [0, split) serve the first,
-// the rest the second.
+// the rest the second. Items of an instance whose mode is not 0 (converged or not
+// selected) return at once: every stage reads only its own instance's cells, aggregates
+// and coarse block, and each application writes its work arrays before reading them.
+
+__device__ __forceinline__ bool cell_idle(const nfx_batch_tg& pc, const int32_t* __restrict__ mode, int c) {
+  return mode && mode[pc.cell_inst[c]] != 0;
+}
+
+__device__ __forceinline__ bool agg_idle(const nfx_batch_tg& pc, const int32_t* __restrict__ mode, int a) {
+  return mode && mode[pc.cell_inst[pc.agg_mem[pc.agg_ptr[a]]]] != 0;
+}
 
 // x1 = W r
 template <int NB>
-__device__ __forceinline__ void st_pre(const nfx_batch_tg& pc, int c, const float* __restrict__ r) {
-  if (c >= pc.n_cells) return;
+__device__ __forceinline__ void st_pre(const nfx_batch_tg& pc, int c, const int32_t* __restrict__ mode,
+                                       const float* __restrict__ r) {
+  if (c >= pc.n_cells || cell_idle(pc, mode, c)) return;
   const GroupWork w = group_work(pc);
   const int32_t* prow = pc.prow;
   float o[NB];
@@ -380,8 +391,9 @@ __device__ __forceinline__ void st_pre(const nfx_batch_tg& pc, int c, const floa
 // t1 = r - S x1 on the cells of aggregate `wave` (kept for the post step) and
 // cv[wave] = the sum of t1 over them (one wave per aggregate, lanes over its cells)
 template <int NB>
-__device__ __forceinline__ void st_restrict(const nfx_batch_tg& pc, int wave, int lane, const float* __restrict__ r) {
-  if (wave >= pc.n_agg) return;
+__device__ __forceinline__ void st_restrict(const nfx_batch_tg& pc, int wave, int lane,
+                                            const int32_t* __restrict__ mode, const float* __restrict__ r) {
+  if (wave >= pc.n_agg || agg_idle(pc, mode, wave)) return;
   const GroupWork w = group_work(pc);
   float acc = 0.f;
   for (int k = pc.agg_ptr[wave] + lane; k < pc.agg_ptr[wave + 1]; k += 64) {
@@ -401,8 +413,8 @@ __device__ __forceinline__ void st_restrict(const nfx_batch_tg& pc, int wave, in
 }
 
 // y[a] = sum_t ainv[arow[a] + t] cv[a0[a] + t] over the aggregates of a's instance
-__device__ __forceinline__ void st_gemv(const nfx_batch_tg& pc, int wave, int lane) {
-  if (wave >= pc.n_agg) return;
+__device__ __forceinline__ void st_gemv(const nfx_batch_tg& pc, int wave, int lane, const int32_t* __restrict__ mode) {
+  if (wave >= pc.n_agg || agg_idle(pc, mode, wave)) return;
   const GroupWork w = group_work(pc);
   const float* a = pc.ainv + pc.arow[wave];
   const float* x = w.cv + pc.a0[wave];
@@ -416,8 +428,8 @@ __device__ __forceinline__ void st_gemv(const nfx_batch_tg& pc, int wave, int la
 // x2 = x1 + P y; t2 = r - S x2 = t1 - S P y, with S P y from the row sums of the blocks
 // (rs[c][0][i] for the cell's own block, rs[c][1 + k][i] for neighbour k)
 template <int NB>
-__device__ __forceinline__ void st_post1(const nfx_batch_tg& pc, int c) {
-  if (c >= pc.n_cells) return;
+__device__ __forceinline__ void st_post1(const nfx_batch_tg& pc, int c, const int32_t* __restrict__ mode) {
+  if (c >= pc.n_cells || cell_idle(pc, mode, c)) return;
   const GroupWork w = group_work(pc);
   const float yc = w.y[pc.agg_of[c]];
   float spy[NB];
@@ -441,8 +453,7 @@ __device__ __forceinline__ void st_post1(const nfx_batch_tg& pc, int c) {
 template <int NB>
 __device__ __forceinline__ void st_post2(const nfx_batch_tg& pc, int c, const int32_t* __restrict__ mode,
                                          float* __restrict__ z) {
-  if (c >= pc.n_cells) return;
-  if (mode && mode[pc.cell_inst[c]] != 0) return;
+  if (c >= pc.n_cells || cell_idle(pc, mode, c)) return;
   const GroupWork w = group_work(pc);
   float o[NB];
   w_row<NB>(c, pc.nbr, pc.wd, pc.wo, [&](int q, int i) { return w.t2[(int64_t)q * NB + i]; }, o);
@@ -473,39 +484,42 @@ __device__ __forceinline__ void st_flux(const nfx_batch_tg& pc, int g, const int
 
 template <int NA, int NBB>
 __global__ __launch_bounds__(kTB) void k_btg_pre(const nfx_batch_tg pa, const nfx_batch_tg pb, int split,
-                                                 const float* __restrict__ r) {
+                                                 const int32_t* __restrict__ mode, const float* __restrict__ r) {
   const int b = blockIdx.x;
   if (b < split)
-    st_pre<NA>(pa, b * kTB + threadIdx.x, r);
+    st_pre<NA>(pa, b * kTB + threadIdx.x, mode, r);
   else
-    st_pre<NBB>(pb, (b - split) * kTB + threadIdx.x, r);
+    st_pre<NBB>(pb, (b - split) * kTB + threadIdx.x, mode, r);
 }
 
 template <int NA, int NBB>
 __global__ __launch_bounds__(kTB) void k_btg_restrict(const nfx_batch_tg pa, const nfx_batch_tg pb, int split,
+                                                      const int32_t* __restrict__ mode,
                                                       const float* __restrict__ r) {
   const int b = blockIdx.x, lane = threadIdx.x & 63;
   if (b < split)
-    st_restrict<NA>(pa, (b * kTB + threadIdx.x) >> 6, lane, r);
+    st_restrict<NA>(pa, (b * kTB + threadIdx.x) >> 6, lane, mode, r);
   else
-    st_restrict<NBB>(pb, ((b - split) * kTB + threadIdx.x) >> 6, lane, r);
+    st_restrict<NBB>(pb, ((b - split) * kTB + threadIdx.x) >> 6, lane, mode, r);
 }
 
-__global__ __launch_bounds__(kTB) void k_btg_gemv(const nfx_batch_tg pa, const nfx_batch_tg pb, int split) {
+__global__ __launch_bounds__(kTB) void k_btg_gemv(const nfx_batch_tg pa, const nfx_batch_tg pb, int split,
+                                                  const int32_t* __restrict__ mode) {
   const int b = blockIdx.x, lane = threadIdx.x & 63;
   if (b < split)
-    st_gemv(pa, (b * kTB + threadIdx.x) >> 6, lane);
+    st_gemv(pa, (b * kTB + threadIdx.x) >> 6, lane, mode);
   else
-    st_gemv(pb, ((b - split) * kTB + threadIdx.x) >> 6, lane);
+    st_gemv(pb, ((b - split) * kTB + threadIdx.x) >> 6, lane, mode);
 }
 
 template <int NA, int NBB>
-__global__ __launch_bounds__(kTB) void k_btg_post1(const nfx_batch_tg pa, const nfx_batch_tg pb, int split) {
+__global__ __launch_bounds__(kTB) void k_btg_post1(const nfx_batch_tg pa, const nfx_batch_tg pb, int split,
+                                                   const int32_t* __restrict__ mode) {
   const int b = blockIdx.x;
   if (b < split)
-    st_post1<NA>(pa, b * kTB + threadIdx.x);
+    st_post1<NA>(pa, b * kTB + threadIdx.x, mode);
   else
-    st_post1<NBB>(pb, (b - split) * kTB + threadIdx.x);
+    st_post1<NBB>(pb, (b - split) * kTB + threadIdx.x, mode);
 }
 
 // post2 of both groups, then the flux blocks of either group (blocks [split2, grid))
@@ -533,10 +547,10 @@ int apply_pair(const nfx_batch_tg& pa, const nfx_batch_tg* pbp, const int32_t* m
   const int ca = grid_for(pa.n_cells, kTB), cb = two ? grid_for(pb.n_cells, kTB) : 0;
   const int aa = grid_for((int64_t)pa.n_agg * 64, kTB), ab = two ? grid_for((int64_t)pb.n_agg * 64, kTB) : 0;
   const int fa = grid_for(pa.n_fb, kTB), fb = two ? grid_for(pb.n_fb, kTB) : 0;
-  hipLaunchKernelGGL((k_btg_pre<NA, NBB>), dim3(ca + cb), dim3(kTB), 0, st, pa, pb, ca, r);
-  hipLaunchKernelGGL((k_btg_restrict<NA, NBB>), dim3(aa + ab), dim3(kTB), 0, st, pa, pb, aa, r);
-  hipLaunchKernelGGL(k_btg_gemv, dim3(aa + ab), dim3(kTB), 0, st, pa, pb, aa);
-  hipLaunchKernelGGL((k_btg_post1<NA, NBB>), dim3(ca + cb), dim3(kTB), 0, st, pa, pb, ca);
+  hipLaunchKernelGGL((k_btg_pre<NA, NBB>), dim3(ca + cb), dim3(kTB), 0, st, pa, pb, ca, mode, r);
+  hipLaunchKernelGGL((k_btg_restrict<NA, NBB>), dim3(aa + ab), dim3(kTB), 0, st, pa, pb, aa, mode, r);
+  hipLaunchKernelGGL(k_btg_gemv, dim3(aa + ab), dim3(kTB), 0, st, pa, pb, aa, mode);
+  hipLaunchKernelGGL((k_btg_post1<NA, NBB>), dim3(ca + cb), dim3(kTB), 0, st, pa, pb, ca, mode);
   hipLaunchKernelGGL((k_btg_post2<NA, NBB>), dim3(ca + cb + fa + fb), dim3(kTB), 0, st, pa, pb, ca, ca + cb,
                      ca + cb + fa, mode, r, z);
   NFX_LAUNCH_CHECK();
