@@ -368,13 +368,11 @@ class LearnedSchurPC:
         self._mass_positive = False
         self.flux_width = 0
         self.flux_pos = self.flux_col = self.flux_coef = None
+        self._flux_rows = 0
+        self._flux_external = False
         if self.flux_sweeps == 2 and g.n_u > 0:
             pos, col = flux_ell(sysd.pattern.rowptr, sysd.pattern.col, g.n_u)
-            self.flux_width = pos.shape[0]
-            self.flux_pos, self.flux_col = to_dev(pos, device=dev), to_dev(col, device=dev)
-            self._diag_pos32 = self.g.diag_pos_u.to(t.int32).contiguous()
-            self.flux_coef = f32(pos.size)
-            self._flux_bound = f32(1)
+            self._flux_alloc(pos, col, self.g.diag_pos_u.to(t.int32).contiguous())
         self.tail_dinv = None
         self.tail_lu = None
         self.h = self.m = self.z = None
@@ -418,12 +416,47 @@ class LearnedSchurPC:
                  nat.ptr(self.x), nat.ptr(self.sd32), nat.ptr(self.so32), s)
         dm = vals64[self.g.diag_pos_u]
         self.dMinv.copy_(1.0 / dm.abs())
-        if self.flux_coef is not None:
-            self._mass_positive = bool((dm > 0).all())
-            if self._mass_positive:
-                nat.call("nfx_flux_sweep_coef", g.n_u, self.flux_width, nat.ptr(self.flux_pos),
-                         nat.ptr(self.flux_col), nat.ptr(self._diag_pos32), nat.ptr(vals64),
-                         self.flux_omega or 0.0, nat.ptr(self.flux_coef), nat.ptr(self._flux_bound), s)
+        if self.flux_coef is not None and not self._flux_external:
+            self._flux_build(vals64, stream)
+
+    def _flux_alloc(self, pos, col, diag_pos32):
+        t = torch()
+        dev = self.sys.device
+        self.flux_width = pos.shape[0]
+        self._flux_rows = pos.shape[1]
+        self.flux_pos, self.flux_col = to_dev(pos, device=dev), to_dev(col, device=dev)
+        self._diag_pos32 = diag_pos32
+        self.flux_coef = t.empty(pos.size, dtype=t.float32, device=dev)
+        self._flux_bound = t.zeros(1, dtype=t.float32, device=dev)
+
+    def _flux_build(self, vals64, stream=None):
+        dm = vals64[self._diag_pos32.long()]
+        self._mass_positive = bool((dm > 0).all())
+        if self._mass_positive:
+            nat.call("nfx_flux_sweep_coef", self._flux_rows, self.flux_width, nat.ptr(self.flux_pos),
+                     nat.ptr(self.flux_col), nat.ptr(self._diag_pos32), nat.ptr(vals64), self.flux_omega or 0.0,
+                     nat.ptr(self.flux_coef), nat.ptr(self._flux_bound), nat.stream_handle(stream))
+
+    def use_flux_rows(self, rowptr, col, n_u, vals64, stream=None):
+        """Flux sweeps on the flux rows [0, n_u) of a larger system that owns the descriptor's
+        flux rows (with ``flux_dinv``): host CSR pattern (rowptr, col) and its FP64 device
+        values. Builds the flux ELL once and its coefficients from vals64 (call again after
+        the values change). Inactive when the mass diagonal is not positive."""
+        t = torch()
+        rp = np.asarray(rowptr, dtype=np.int64)
+        cc = np.asarray(col, dtype=np.int64)
+        rows = np.repeat(np.arange(n_u), np.diff(rp[: n_u + 1]))
+        d = np.flatnonzero(rows == cc[: rp[n_u]])
+        diag = np.full(n_u, -1, dtype=np.int64)
+        diag[rows[d]] = d
+        if n_u == 0 or (diag < 0).any():
+            raise ValueError("every flux row needs a diagonal entry")
+        pos, fcol = flux_ell(rp, cc, n_u)
+        self._flux_alloc(pos, fcol, to_dev(diag.astype(np.int32), device=self.sys.device))
+        self._flux_external = True
+        self.flux_sweeps = 2
+        self._flux_build(vals64, stream)
+        return self.sweeps_active
 
     def forward(self, stream=None):
         g = self.g
@@ -536,8 +569,8 @@ class LearnedSchurPC:
     @property
     def sweeps_active(self):
         """Whether the flux rows get the two mass sweeps (else the diagonal scaling)."""
-        return (self.flux_sweeps == 2 and self.flux_dinv is None and self.tail_dinv is None
-                and self._mass_positive)
+        n_u = int(self.flux_dinv.numel()) if self.flux_dinv is not None else self.g.n_u
+        return self.flux_coef is not None and self._mass_positive and self._flux_rows == n_u
 
     def smoother_numpy(self):
         """(Wd, Wo (E,4)) on the host (tests)."""

@@ -200,12 +200,15 @@ class MixedDimDarcy:
     the diagonal block preconditioner (``nfx_md_block_diag``), set up once."""
 
     def __init__(self, net, dt=1e-3, storage=1.0, kappa_matrix=1.0, kappa_normal=1.0, p_boundary=None,
-                 device="cuda", conservative=True):
+                 device="cuda", conservative=True, flux_sweeps=True):
         from .device import p1_bary  # noqa: F401
         from .pattern import row_codes
 
         t = torch()
         self.net, self.dt, self.storage, self.device = net, float(dt), float(storage), device
+        # the two-grid preconditioner's flux rows: two damped Jacobi sweeps on the flux mass
+        # block (True) or its diagonal
+        self.flux_sweeps = bool(flux_sweeps)
         # +1: the reference's coupling form; -1: the matrix outflow is a fracture source
         self.exchange_sign = -1 if conservative else 1
         mesh = net.mesh
@@ -430,7 +433,8 @@ class MixedDimDarcy:
         from .learned import LearnedSchurPC
 
         t = self.coupled_pc_terms()
-        # (the coupled system's flux rows keep its own diagonal, flux_dinv below: no flux sweeps)
+        # (the flux rows are the coupled system's, matrix and fracture: its diagonal in
+        # flux_dinv, and the two mass sweeps built from its own CSR, not the matrix system's)
         pc = LearnedSchurPC(self.m, theta, psign=1.0, graph=self._schur_graph, extra_diag=t["extra_diag"],
                             extra_off=t["extra_off"], flux_sweeps=1)
         pc.setup(vals64=self.vals64, network=theta is not None)
@@ -438,6 +442,8 @@ class MixedDimDarcy:
         pc.tail_dinv = t["tail_dinv"]
         if "slot_tail" in t:
             pc.tail_lu = (t["slot_tail"], t["slot_coef"], t["tail_cells"], t["tail_coef"])
+        if self.flux_sweeps:
+            pc.use_flux_rows(self.rowptr_np, self.col_np, self.n_u, self.vals64)
         return pc
 
     def coupled_pc_terms(self):

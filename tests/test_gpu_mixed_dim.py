@@ -81,11 +81,14 @@ def test_md_transient_steps_vs_oracle(small, conservative, pc):
 
 @pytest.mark.parametrize("conservative", [False, True])
 def test_md_coupled_preconditioner_rows(small, conservative):
-    """The two-grid descriptor over the coupled vector: flux rows 1/|a_kk| r; matrix cells
+    """The two-grid descriptor over the coupled vector: flux rows two damped Jacobi sweeps on
+    the coupled flux mass block (against the oracle's sweeps on the coupled CSR), or
+    1/|a_kk| r without the sweeps; matrix cells
     the two-grid cycle on r_m - sum slot_coef r_f (against the same cycle applied to a
     vector laid out as the matrix system alone); fracture pressures by back-substitution,
     z_f = tail_dinv (r_f - sum tail_coef z_m) (conservative exchange), or tail_dinv r_f."""
     import torch
+    import oracle.learned as ol
     from nfempy_amd.mixed_dim import MixedDimDarcy
 
     net = small[0]
@@ -99,8 +102,17 @@ def test_md_coupled_preconditioner_rows(small, conservative):
     z = pc(r)
     torch.cuda.synchronize()
     zn, rn = z.cpu().numpy(), r.cpu().numpy()
-    fl = md.pinv[:n_u].cpu().numpy()
-    assert np.allclose(zn[:n_u], fl * rn[:n_u], rtol=1e-6, atol=0)
+    assert pc.sweeps_active
+    A = md.csr_numpy(fp64=True)
+    zu = ol.flux_sweeps(A.indptr, A.indices, A.data, n_u, rn[:n_u].astype(np.float64), pc.flux_weight)
+    assert np.abs(zn[:n_u] - zu).max() <= 1e-5 * np.abs(zu).max()
+    md0 = MixedDimDarcy(net, kappa_normal=2.0, conservative=conservative, flux_sweeps=False).setup(
+        preconditioner="twogrid")
+    assert not md0.pc.sweeps_active
+    z0 = md0.pc(r).cpu().numpy()
+    fl = md0.pinv[:n_u].cpu().numpy()
+    assert np.allclose(z0[:n_u], fl * rn[:n_u], rtol=1e-6, atol=0)
+    assert np.array_equal(z0[n_u:], zn[n_u:])
     tail = pc.tail_dinv.cpu().numpy()
     rf = rn[n_u + E:]
     rq = rn[n_u: n_u + E].astype(np.float64)
