@@ -600,7 +600,8 @@ int nfx_field_values(int kind, int dim, int nb, int n_points, int n_comp, int n_
  * entry points build the pressure Schur approximation S = B diag(M)^-1 B^T of the
  * assembled matrix, run the message-passing network that maps S to a symmetric
  * sparse smoother W (pattern of S), train it, and apply the two-grid cycle that
- * replaces diag(S)^-1 in the block preconditioner. Layouts:
+ * replaces diag(S)^-1 in the block preconditioner (and, with nfx_flux_sweep_coef, two
+ * damped Jacobi sweeps on the flux mass block that replace diag(M)^-1). Layouts:
  *   ELL: row p has 4 slots (the cells across its faces in ascending face order),
  *        d_nbr (E,4) int32 (-1 = boundary face), values (E,4);
  *   sample blocks: (ns, E) row-major (one sample per row);
