@@ -194,6 +194,8 @@ class SchurGraph:
         i32 = lambda a: to_dev(np.ascontiguousarray(a, dtype=np.int32), device=device)
         self.nbr, self.pos_b, self.pos_qb, self.pos_m = i32(nbr), i32(pos_b), i32(pos_qb), i32(pos_m)
         self.agg_ptr, self.agg_mem, self.agg_of = i32(ptr), i32(order), i32(agg_of)
+        if n_u and (pos_diag[:n_u] < 0).any():
+            raise ValueError("a flux row without its diagonal entry in the pattern")
         self.diag_pos_u = to_dev(pos_diag[:n_u], device=device).long()
         self.device = device
 
