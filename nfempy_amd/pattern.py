@@ -55,22 +55,27 @@ def build_pattern(n, ref_dest, ref_sign, keep=None, zero_diag_dest=None, upper=F
     device: run the stable sort of the keys there (``numbering.stable_argsort``).
     """
     E, nl = ref_dest.shape
-    rows = np.repeat(ref_dest[:, :, None], nl, axis=2)
-    cols = np.repeat(ref_dest[:, None, :], nl, axis=1)
-    sel = np.ones((E, nl, nl), dtype=bool) if keep is None else keep.copy()
-    if upper:
-        sel &= rows <= cols
-    zero = np.zeros((E, nl, nl), dtype=bool)
-    if zero_diag_dest is not None:
-        diag = np.zeros((E, nl, nl), dtype=bool)
-        idx = np.arange(nl)
-        diag[:, idx, idx] = True
-        zero = diag & ~sel
-        sel = sel | diag
-    flat_sel = sel.reshape(-1)
-    r = rows.reshape(-1)[flat_sel]
-    c = cols.reshape(-1)[flat_sel]
-    key = r.astype(np.int64) * n + c
+    rd = np.asarray(ref_dest, dtype=np.int64)
+    # keys of every element entry (a, b): row ref_dest[e, a], column ref_dest[e, b]
+    key = (rd[:, :, None] * n + rd[:, None, :]).reshape(-1)
+    plain = keep is None and not upper and zero_diag_dest is None
+    if plain:
+        flat_sel = None
+        zero_flat = None
+    else:
+        sel = np.ones((E, nl, nl), dtype=bool) if keep is None else keep.copy()
+        if upper:
+            sel &= rd[:, :, None] <= rd[:, None, :]
+        zero = np.zeros((E, nl, nl), dtype=bool)
+        if zero_diag_dest is not None:
+            diag = np.zeros((E, nl, nl), dtype=bool)
+            idx = np.arange(nl)
+            diag[:, idx, idx] = True
+            zero = diag & ~sel
+            sel = sel | diag
+        flat_sel = sel.reshape(-1)
+        key = key[flat_sel]
+        zero_flat = zero.reshape(-1)[flat_sel]
     if device is not None:
         from .numbering import stable_argsort
 
@@ -78,17 +83,15 @@ def build_pattern(n, ref_dest, ref_sign, keep=None, zero_diag_dest=None, upper=F
     else:
         order = np.argsort(key, kind="stable")
         ks = key[order]
+    del key
     start = np.r_[True, ks[1:] != ks[:-1]] if len(ks) else np.zeros(0, bool)
     uid_sorted = np.cumsum(start) - 1
     ukeys = ks[start]
+    del ks
     nnz = len(ukeys)
     counts = np.bincount(uid_sorted, minlength=nnz)
     seg_first = np.flatnonzero(start)
-    rank_sorted = np.arange(len(ks)) - seg_first[uid_sorted]
-    uid = np.empty(len(ks), np.int64)
-    uid[order] = uid_sorted
-    rank = np.empty(len(ks), np.int64)
-    rank[order] = rank_sorted
+    del start
     multi = counts > 1
     seg_ids = np.flatnonzero(multi)
     seg_len = counts[seg_ids]
@@ -98,12 +101,27 @@ def build_pattern(n, ref_dest, ref_sign, keep=None, zero_diag_dest=None, upper=F
     n_stage = int(segptr[-1])
     if max(nnz, n_stage) > SLOT_INDEX_MASK:
         raise ValueError("pattern too large for 28-bit slot codes")
-    code = np.where(multi[uid], SLOT_STAGE | (segptr[np.maximum(seg_of_uid[uid], 0)] + rank), uid)
-    sgn = (ref_sign[:, :, None] * ref_sign[:, None, :]).reshape(-1)[flat_sel]
-    code = code | np.where(sgn < 0, SLOT_NEG, 0)
-    code = code | np.where(zero.reshape(-1)[flat_sel], SLOT_ZERO, 0)
-    full = np.full(E * nl * nl, -1, np.int64)
-    full[flat_sel] = code
+    # codes in sorted order (rank = position inside the entry's run, i.e. element order),
+    # then one scatter back to entry order
+    m_sorted = multi[uid_sorted]
+    rank_sorted = np.arange(len(uid_sorted)) - seg_first[uid_sorted]
+    code_sorted = np.where(m_sorted, SLOT_STAGE | (segptr[np.maximum(seg_of_uid[uid_sorted], 0)] + rank_sorted),
+                           uid_sorted)
+    del m_sorted, rank_sorted, uid_sorted
+    code = np.empty(len(code_sorted), np.int64)
+    code[order] = code_sorted
+    del code_sorted, order
+    sgn = (ref_sign[:, :, None] * ref_sign[:, None, :]).reshape(-1)
+    if flat_sel is not None:
+        sgn = sgn[flat_sel]
+    code |= np.where(sgn < 0, SLOT_NEG, 0)
+    if zero_flat is not None:
+        code |= np.where(zero_flat, SLOT_ZERO, 0)
+    if flat_sel is None:
+        full = code
+    else:
+        full = np.full(E * nl * nl, -1, np.int64)
+        full[flat_sel] = code
     slot = np.ascontiguousarray(full.reshape(E, nl * nl).T).astype(np.int32)
     urow = ukeys // n
     ucol = ukeys % n
