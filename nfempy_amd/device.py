@@ -365,22 +365,33 @@ class DarcySystem:
             if np.any(e_of[0, bc_r] != np.repeat(self.bc_neigh_np, self.bc_rows_np.shape[1])):
                 raise ValueError("a Dirichlet facet row's element must be the facet's neighbour")
             packed[1, bc_r] = -2 - np.arange(bc_r.size)
-        # column positions of every reference DoF b of each incident element, inside row r
-        rowptr = self.pattern.rowptr.astype(np.int64)
-        keys = np.repeat(np.arange(n, dtype=np.int64), np.diff(rowptr)) * n + self.pattern.col
+        # column positions of every reference DoF b of each incident element, inside row r:
+        # the CSR index of entry (a, b) of element e is its slot code's value index, or for
+        # a staged entry the target of its stage segment
+        pat = self.pattern
+        rowptr = pat.rowptr.astype(np.int64)
         # bytes per incidence: nl, padded to whole words for the larger elements; the
         # device copy packs nl <= 7 into 4-bit fields (row_pos_device)
         nlp = nl if nl <= 8 else (nl + 3) // 4 * 4
         pos = np.full((n, 2, nlp), 255, np.uint8)
         if (np.diff(rowptr).max(initial=0)) >= 255:
             raise ValueError("row too long for uint8 column positions")
+        segptr = pat.segptr.astype(np.int64)
+        target = pat.target.astype(np.int64)
+        b_idx = np.arange(nl)
         for j in range(2):
             live = inc[j] >= 0
             r = np.flatnonzero(live)
-            cols = self.ref_dest[e_of[j, live]]  # (m, nl)
-            q = r[:, None] * n + cols
-            idx = np.searchsorted(keys, q)
-            if not np.array_equal(keys[np.minimum(idx, len(keys) - 1)], q):
+            e, a = e_of[j, live], a_of[j, live]
+            code = pat.slot[a[:, None] * nl + b_idx[None, :], e[:, None]].astype(np.int64)  # (m, nl)
+            if (code < 0).any():
+                raise ValueError("an incident entry is missing from the pattern")
+            idx = code & nat.SLOT_INDEX_MASK
+            staged = (code & nat.SLOT_STAGE) != 0
+            if staged.any():
+                idx[staged] = target[np.searchsorted(segptr, idx[staged], side="right") - 1]
+            cols = self.ref_dest[e]
+            if not np.array_equal(pat.col[idx], cols):
                 raise ValueError("incident column missing from the pattern")
             pos[r, j, :nl] = (idx - rowptr[r][:, None]).astype(np.uint8)
         signs = ((self.ref_sign < 0).astype(np.uint64) << np.arange(nl, dtype=np.uint64)[None, :]).sum(axis=1)
