@@ -51,6 +51,36 @@ def darcy_space(n, k):
     return ps
 
 
+def _cold_system(key):
+    """Host half of one batch system (runs in a worker process, no device work): the
+    product space of (n, k) and its CSR pattern."""
+    from .pattern import build_pattern
+
+    ps = darcy_space(*key)
+    rd, rs = ps.reference_destinations()
+    return key, ps, build_pattern(ps.n_dof, rd, rs)
+
+
+def _cold_work(key):
+    """Relative host cost of building system (n, k): cells times the RT2 factor."""
+    return key[0] ** 3 * (6 if key[1] == 2 else 1)
+
+
+# below this total (four 16^3 RT2-P1 systems) the systems are built in this process: the
+# workers' start-up (a fresh interpreter and its imports each) would cost more
+COLD_POOL_MIN_WORK = 4 * _cold_work((16, 2))
+
+
+def cold_workers(keys):
+    """Worker processes for the host half of a batch's distinct systems: one per system, at
+    most 16 and at most the host's cores; 1 (build them in this process) for a small batch."""
+    import os
+
+    if sum(_cold_work(k) for k in keys) < COLD_POOL_MIN_WORK:
+        return 1
+    return max(1, min(16, os.cpu_count() or 1, len(keys)))
+
+
 RT2_SPAN_PER_ROW = 33  # staging entries per tile row of the compact RT2 launch (kRt2SpanPerRow)
 
 
@@ -72,21 +102,43 @@ def span_tiles(rowptr, tile_rows, span_per_row=RT2_SPAN_PER_ROW):
 
 
 class DarcyBatch:
-    def __init__(self, specs, seed=0, ell=0.1, sigma=1.0, device="cuda", log=None, split=True):
+    def __init__(self, specs, seed=0, ell=0.1, sigma=1.0, device="cuda", log=None, split=True, workers=None):
         """specs: list of (n, k) per instance; log: optional progress callable; split: one
-        launch per degree with a degree-specialised kernel (else one launch over both)."""
+        launch per degree with a degree-specialised kernel (else one launch over both);
+        workers: processes that build the distinct systems' spaces and patterns (host work,
+        largest first) while this process uploads each one as it arrives (default
+        ``cold_workers``; 1 builds them here)."""
         t = torch()
         if nat.call("nfx_darcy_batch_item_size") != ctypes.sizeof(_Item):
             raise RuntimeError("batch descriptor layout differs between host and library")
         self.device = device
         self.specs = [(int(n), int(k)) for n, k in specs]
-        self.systems = {}
         keys = sorted(set(self.specs))
-        for j, key in enumerate(keys):
-            self.systems[key] = DarcySystem(darcy_space(*key), device=device)
+        workers = cold_workers(keys) if workers is None else max(1, int(workers))
+        built = {}
+
+        def add(key, ps, pattern=None):
+            built[key] = DarcySystem(ps, device=device, pattern=pattern)
             if log is not None:
-                log(f"batch system {j + 1}/{len(keys)}: n={key[0]} k={key[1]} E={self.systems[key].E} "
-                    f"nnz={self.systems[key].nnz}")
+                log(f"batch system {len(built)}/{len(keys)}: n={key[0]} k={key[1]} E={built[key].E} "
+                    f"nnz={built[key].nnz}")
+
+        if workers > 1 and len(keys) > 1:
+            import concurrent.futures
+            import multiprocessing
+
+            # spawn: fresh interpreters that never touch the device; largest systems first
+            order = sorted(keys, key=lambda kk: -_cold_work(kk))
+            with concurrent.futures.ProcessPoolExecutor(max_workers=workers,
+                                                        mp_context=multiprocessing.get_context("spawn")) as ex:
+                futs = [ex.submit(_cold_system, key) for key in order]
+                for fut in concurrent.futures.as_completed(futs):
+                    key, ps, pattern = fut.result()
+                    add(key, ps, pattern)
+        else:
+            for key in keys:
+                add(key, darcy_space(*key))
+        self.systems = {key: built[key] for key in keys}
         self.instances = []
         for i, (n, k) in enumerate(self.specs):
             sysd = self.systems[(n, k)]

@@ -250,8 +250,11 @@ def xcd_tile_map(first_elem, n_rows, tile=ROW_TILE, n_xcd=N_XCD):
 class DarcySystem:
     """Mixed Darcy RT_k - P_{k-1} on one product space, resident on the GPU."""
 
-    def __init__(self, space, flux="u", potential="p", device="cuda", split=None, cold_device=None):
+    def __init__(self, space, flux="u", potential="p", device="cuda", split=None, cold_device=None,
+                 pattern=None):
         """cold_device: run the pattern's key sort there ("cuda"; same pattern as the host).
+        pattern: the space's pattern built beforehand (``build_pattern`` of its reference
+        destinations and signs, e.g. in a worker process; not with split).
         split: optional (m, 2) (element index, local flux DoF) pairs whose flux DoF is
         replaced by a new one of their own (appended after the flux DoFs, pressure DoFs
         shifted): the facet DoF of a face cut by a fracture, on one side of it, so that
@@ -321,7 +324,14 @@ class DarcySystem:
             self.n_split = m
         self.split = split
         self.ref_dest, self.ref_sign = ref_dest, ref_sign
-        self.pattern = build_pattern(self.n, ref_dest, ref_sign, device=cold_device)
+        if pattern is not None:
+            if split is not None and len(split):
+                raise ValueError("a prebuilt pattern cannot be combined with split DoFs")
+            if pattern.n != self.n or pattern.slot.shape != (self.nl * self.nl, self.E):
+                raise ValueError("the prebuilt pattern does not belong to this space")
+            self.pattern = pattern
+        else:
+            self.pattern = build_pattern(self.n, ref_dest, ref_sign, device=cold_device)
         pat = self.pattern
         self.rowptr = to_dev(pat.rowptr, device=device)
         self.col = to_dev(pat.col, device=device)

@@ -249,3 +249,53 @@ def test_flux_ell_layout():
     for i in range(n_u):
         p = pos[:, i][pos[:, i] >= 0]
         assert np.all(np.diff(p) > 0)
+
+
+def test_batch_cold_systems_in_worker_processes():
+    """The host half of a batch's systems built in spawned worker processes (space and
+    pattern) equals the one built in this process; the small-batch rule keeps tiny batches
+    in-process."""
+    import concurrent.futures
+    import multiprocessing
+
+    from nfempy_amd.batch import _cold_system, cold_workers, darcy_space
+    from nfempy_amd.pattern import build_pattern
+
+    keys = [(3, 1), (4, 2), (5, 1)]
+    assert cold_workers(keys) == 1
+    assert cold_workers([(32, 2), (30, 2), (28, 1)]) > 1 or multiprocessing.cpu_count() == 1
+    with concurrent.futures.ProcessPoolExecutor(max_workers=2, mp_context=multiprocessing.get_context("spawn")) as ex:
+        got = {k: (ps, pat) for k, ps, pat in ex.map(_cold_system, keys)}
+    for key in keys:
+        ps = darcy_space(*key)
+        rd, rs = ps.reference_destinations()
+        ref = build_pattern(ps.n_dof, rd, rs)
+        ps2, pat = got[key]
+        rd2, rs2 = ps2.reference_destinations()
+        assert ps2.n_dof == ps.n_dof and np.array_equal(rd2, rd) and np.array_equal(rs2, rs)
+        for f in ("rowptr", "col", "slot", "segptr", "target"):
+            assert np.array_equal(getattr(pat, f), getattr(ref, f)), (key, f)
+        assert pat.n_stage == ref.n_stage
+
+
+def test_darcy_system_with_prebuilt_pattern():
+    """DarcySystem given the pattern built beforehand has the same pattern and row tables
+    as the one that builds it; a pattern of another space is refused."""
+    from nfempy_amd.batch import darcy_space
+    from nfempy_amd.device import DarcySystem
+    from nfempy_amd.pattern import build_pattern
+
+    ps = darcy_space(3, 2)
+    rd, rs = ps.reference_destinations()
+    pat = build_pattern(ps.n_dof, rd, rs)
+    a = DarcySystem(ps, device="cpu")
+    b = DarcySystem(ps, device="cpu", pattern=pat)
+    assert b.pattern is pat
+    for f in ("rowptr", "col", "slot", "segptr", "target"):
+        assert np.array_equal(getattr(a.pattern, f), getattr(b.pattern, f))
+    for f in ("row_inc_np", "row_pos_np", "row_sign_mask_np", "row_tile_map_np"):
+        assert np.array_equal(getattr(a, f), getattr(b, f))
+    other = darcy_space(2, 2)
+    rdo, rso = other.reference_destinations()
+    with pytest.raises(ValueError):
+        DarcySystem(ps, device="cpu", pattern=build_pattern(other.n_dof, rdo, rso))
