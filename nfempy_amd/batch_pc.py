@@ -52,16 +52,11 @@ def cell_structure(sysd):
     nbf, nb, E, n, n_u = sysd.nb, sysd.np_, sysd.E, sysd.n, sysd.n_u
     rd = np.asarray(sysd.ref_dest, dtype=np.int64)
     fl, pr = rd[:, :nbf], rd[:, nbf:]
-    rp = sysd.pattern.rowptr.astype(np.int64)
-    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(rp))
-    keys = rows * n + sysd.pattern.col.astype(np.int64)
-    q = pr[:, None, :] * n + fl[:, :, None]  # (E, nbf, nb): pressure row, flux column
-    posb = np.searchsorted(keys, q)
-    qd = fl * n + fl
-    posd = np.searchsorted(keys, qd)
-    if not (np.array_equal(keys[np.minimum(posb, len(keys) - 1)], q)
-            and np.array_equal(keys[np.minimum(posd, len(keys) - 1)], qd)):
-        raise ValueError("pattern lacks a B entry or a flux diagonal")
+    # CSR positions of element entries (from the slot codes): B (pressure row i, flux
+    # column j) as (E, nbf, nb), and the flux diagonals (E, nbf)
+    jf, ip = np.arange(nbf)[:, None], nbf + np.arange(nb)[None, :]
+    posb = sysd.entry_positions(ip, jf)
+    posd = sysd.entry_positions(np.arange(nbf), np.arange(nbf))
     fdofs = np.asarray(sysd.u_el.entity_dofs[2], dtype=np.int64)  # (4, fd)
     fd = fdofs.shape[1]
     # incidences of every flux DoF: (element, local index), at most two
@@ -99,13 +94,11 @@ def cell_structure(sysd):
             raise ValueError("unexpected RT2 DoF layout")
         gf = fl[:, fdofs]  # (E, 4, 3)
         gi = fl[:, inter]  # (E, 3)
-        qf = gf[:, :, :, None] * n + gf[:, :, None, :]
-        qi = gi[:, :, None] * n + gi[:, None, :]
-        posmf, posmi = np.searchsorted(keys, qf), np.searchsorted(keys, qi)
-        if not (np.array_equal(keys[np.minimum(posmf, len(keys) - 1)], qf)
-                and np.array_equal(keys[np.minimum(posmi, len(keys) - 1)], qi)):
-            raise ValueError("pattern lacks a flux block entry")
-        _, first = np.unique(np.sort(gf.reshape(-1, 3), axis=1), axis=0, return_index=True)
+        posmf = sysd.entry_positions(fdofs[:, :, None], fdofs[:, None, :])  # (E, 4, 3, 3)
+        posmi = sysd.entry_positions(inter[:, None], inter[None, :])  # (E, 3, 3)
+        # first occurrence of each face in (element, local face) order; a face DoF belongs to
+        # one face only, so its smallest DoF names it
+        _, first = np.unique(gf.reshape(-1, 3).min(axis=1), return_index=True)
         first = np.sort(first)
         blk = {"posmf": posmf.astype(np.int32), "posmi": posmi.astype(np.int32),
                "fb_rows": np.concatenate([gf.reshape(-1, 3)[first], gi]),

@@ -247,6 +247,24 @@ def xcd_tile_map(first_elem, n_rows, tile=ROW_TILE, n_xcd=N_XCD):
     return slots.reshape(-1)
 
 
+def slot_positions(pattern, code):
+    """CSR value positions of slot codes (any shape): the value index of a direct code, the
+    target of its stage segment for a staged one. A dropped entry (code < 0) is an error."""
+    code = np.asarray(code, dtype=np.int64)
+    if (code < 0).any():
+        raise ValueError("an element entry is missing from the pattern")
+    idx = code & nat.SLOT_INDEX_MASK
+    staged = (code & nat.SLOT_STAGE) != 0
+    if staged.any():
+        # stage index -> value index of its segment, built once per pattern
+        st = getattr(pattern, "_stage_target", None)
+        if st is None:
+            st = np.repeat(pattern.target.astype(np.int64), np.diff(pattern.segptr.astype(np.int64)))
+            pattern._stage_target = st
+        idx[staged] = st[idx[staged]]
+    return idx
+
+
 class DarcySystem:
     """Mixed Darcy RT_k - P_{k-1} on one product space, resident on the GPU."""
 
@@ -386,20 +404,12 @@ class DarcySystem:
         pos = np.full((n, 2, nlp), 255, np.uint8)
         if (np.diff(rowptr).max(initial=0)) >= 255:
             raise ValueError("row too long for uint8 column positions")
-        segptr = pat.segptr.astype(np.int64)
-        target = pat.target.astype(np.int64)
         b_idx = np.arange(nl)
         for j in range(2):
             live = inc[j] >= 0
             r = np.flatnonzero(live)
             e, a = e_of[j, live], a_of[j, live]
-            code = pat.slot[a[:, None] * nl + b_idx[None, :], e[:, None]].astype(np.int64)  # (m, nl)
-            if (code < 0).any():
-                raise ValueError("an incident entry is missing from the pattern")
-            idx = code & nat.SLOT_INDEX_MASK
-            staged = (code & nat.SLOT_STAGE) != 0
-            if staged.any():
-                idx[staged] = target[np.searchsorted(segptr, idx[staged], side="right") - 1]
+            idx = slot_positions(pat, pat.slot[a[:, None] * nl + b_idx[None, :], e[:, None]])  # (m, nl)
             cols = self.ref_dest[e]
             if not np.array_equal(pat.col[idx], cols):
                 raise ValueError("incident column missing from the pattern")
@@ -420,6 +430,18 @@ class DarcySystem:
         self.row_inc = to_dev(np.ascontiguousarray(packed.astype(np.int32)), device=device)
         self.row_pos = to_dev(row_pos_device(pos, nl, sign_mask, self.pattern.rowptr, ROW_TILE), device=device)
         self.row_signs = to_dev(signs.astype(np.uint32).view(np.int32), device=device)
+
+    def entry_positions(self, a, b):
+        """CSR value positions of the element entries (a, b) (local reference DoFs, integer
+        arrays of one shape S) of every element: (E,) + S, from the pattern's slot codes;
+        checked against the pattern's columns."""
+        a, b = np.broadcast_arrays(np.asarray(a, dtype=np.int64), np.asarray(b, dtype=np.int64))
+        flat = (a * self.nl + b).ravel()
+        pos = slot_positions(self.pattern, self.pattern.slot[flat].T).reshape((self.E,) + a.shape)
+        want = self.ref_dest[:, b.ravel()].reshape((self.E,) + a.shape)
+        if not np.array_equal(self.pattern.col[pos], want):
+            raise ValueError("slot codes do not point at the entries' columns")
+        return pos
 
     def _validate(self, mesh):
         """Host-side checks that every kernel index stays inside its buffer."""
