@@ -50,7 +50,9 @@ extern "C" {
  * Darcy row assembly, slot_pairs of the elasticity assembly, the block-LU tail of
  * nfx_twogrid_pc, block operator of the MINRES solver; 0.3.0: tile launch map of the Darcy
  * row assembly, packed positions and entry signs of the lowest-order rows; 0.4.0: in-tile
- * row offsets in the lowest-order position words, compact RT2 batched launch, degree 3). */
+ * row offsets in the lowest-order position words, compact RT2 batched launch, degree 3;
+ * 0.5.0: flux-row operator fields of nfx_twogrid_pc (flux_width, flux_col, flux_coef, appended
+ * after tail_coef) and nfx_flux_sweep_coef). */
 int nfx_version(void);
 /* Thread-local description of the last error on this thread. */
 const char* nfx_last_error(void);

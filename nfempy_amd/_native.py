@@ -12,6 +12,8 @@ import os
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "lib" / "libnfx.so"
+# library version (10000 major + 100 minor + patch) whose struct layouts the mirrors below follow
+MIN_VERSION = 500
 
 NFX_OK = 0
 NFX_E_INVALID = -1
@@ -215,6 +217,11 @@ def lib():
         fn = getattr(handle, name)
         fn.restype = res
         fn.argtypes = args
+    version = handle.nfx_version()
+    if version < MIN_VERSION:
+        # the descriptor structs mirrored here have the layout of MIN_VERSION
+        raise NfxError(f"{LIB_PATH} is version {version}, these bindings need {MIN_VERSION} or later "
+                       f"(rebuild with `python -m nfempy_amd.build_lib`)")
     _lib = handle
     return _lib
 

@@ -10,7 +10,7 @@ static thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
 }  // namespace nfx
 
-extern "C" int nfx_version(void) { return 400; }
+extern "C" int nfx_version(void) { return 500; }
 
 extern "C" const char* nfx_last_error(void) { return nfx::g_last_error.c_str(); }
 

@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
     lib = _native.lib()
     for name in declared_functions():
         assert hasattr(lib, name), name
-    assert lib.nfx_version() >= 400
+    assert lib.nfx_version() >= _native.MIN_VERSION == 500
 
 
 def test_errors_are_reported_without_a_gpu():
