@@ -261,7 +261,7 @@ def cpu_solve_baseline(n: int, seed: int, log):
 
     def learned():
         pc.setup(network=True)
-        return solve_learned(sysd, pc, bd, rtol=1e-8, restart=60, reorth=False, check=4)
+        return solve_learned(sysd, pc, bd, rtol=1e-8, restart=40, reorth=False, check=4)
 
     out = {}
     for name, fn in (("plain", plain), ("learned", learned)):
@@ -848,14 +848,14 @@ def bench_solve(args, sysd, cent, fsrc, pd, log):
         return out
     theta = load_params(default_weights_path())
     pc = LearnedSchurPC(sysd, theta)
-    fg = Fgmres(N, 60)
+    fg = Fgmres(N, 40)
 
     def learned(network):
         t0 = time.perf_counter()
         pc.setup(network=network)
         torch.cuda.synchronize()
         t_setup = time.perf_counter() - t0
-        x, info = solve_learned(sysd, pc, b, rtol=1e-8, restart=60, reorth=False, check=4, solver=fg)
+        x, info = solve_learned(sysd, pc, b, rtol=1e-8, restart=40, reorth=False, check=4, solver=fg)
         info["setup_s"] = t_setup
         return x, info
 
